@@ -1,0 +1,1582 @@
+// The simulation step of the tripartite neuron-astrocyte network on gfx950.
+//
+// One engine steps a set of independent networks (one for asg_net_*, many for
+// asg_batch_*) through "windows" of W consecutive steps, where W is the
+// smallest synaptic delay in steps (10 in every shipped configuration). No
+// signal emitted inside a window can arrive inside it, so within a window
+// every cell is independent of every other cell and the per-cell state stays
+// in registers for W steps. A window is four launches:
+//
+//   k_cells   one lane per neuron / astrocyte, W steps each:
+//             arrivals from the ring slot, Poisson drive, noise, RK4, spike
+//             test (a wavefront ballot writes one 64-bit spike word per 64
+//             neurons and step), astrocyte update and its SIC factor.
+//   k_spikes  one workgroup per network: prefix scan of the spike words into
+//             a source-ordered spike list per step, Tsodyks-Markram update
+//             per spiking source in time order, raster append.
+//   k_or      one workgroup per spike: marks bit j of the step's spike list
+//             in a per-target 64-bit mask for every synapse of the row.
+//   k_sum     one lane per target: adds the marked spikes' weights into the
+//             ring slot in ascending spike (= source) order.
+//
+// The last two replace the reference's sequential scatter (network.cpp:
+// 249-279) with an order-independent OR followed by an ordered sum, so the
+// floating-point accumulation order per target is the reference's (source
+// ascending, then CSR order) without atomics on doubles.
+//
+// Exact reformulations (each gives the reference's bits, see below):
+//  * STP state is kept per source, not per synapse: every synapse of a row is
+//    initialised alike (network.cpp:43-52) and updated at the same instants
+//    with the same t_now (network.cpp:255-277), so all of a source's exc and
+//    n2a synapses carry identical (x, u, t_last).
+//  * SIC delivery is a gather at arrival time: target t adds w_a2n*factor(a)
+//    once per (a, t) synapse, astrocytes ascending, which is exactly the
+//    sequence of += the reference's scatter (network.cpp:281-299) applies to
+//    ring_sic[slot][t]; the pending flag of a slot is a function of the step.
+//  * n2a rows keep (astrocyte, multiplicity) pairs sorted by astrocyte; a
+//    source's duplicates hit the same accumulator consecutively either way.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "asg.h"
+#include "amath.h"
+#include "cell_math.h"
+#include "rng.h"
+
+namespace asg {
+
+// --------------------------------------------------------------- errors ----
+
+namespace {
+thread_local std::string g_last_error;
+
+bool abort_on_error() {
+    static const bool value = [] {
+        const char* v = std::getenv("ASG_ABORT_ON_ERROR");
+        return v != nullptr && v[0] == '1';
+    }();
+    return value;
+}
+
+asg_status fail(asg_status code, const std::string& what) {
+    g_last_error = what;
+    if (abort_on_error()) {
+        std::fprintf(stderr, "asg: %s\n", what.c_str());
+        std::abort();
+    }
+    return code;
+}
+
+struct HipError {
+    hipError_t err;
+    std::string what;
+};
+
+void hip_check(hipError_t err, const char* call, const char* file, int line) {
+    if (err != hipSuccess) {
+        char buf[512];
+        std::snprintf(buf, sizeof(buf), "HIP error at %s:%d during %s: %s", file, line, call,
+                      hipGetErrorString(err));
+        throw HipError{err, buf};
+    }
+}
+#define HIP_CHECK(call) ::asg::hip_check((call), #call, __FILE__, __LINE__)
+
+template <typename F>
+asg_status guarded(F&& f) {
+    try {
+        return f();
+    } catch (const HipError& e) {
+        return fail(e.err == hipErrorOutOfMemory ? ASG_ERR_NOMEM : ASG_ERR_DEVICE, e.what);
+    } catch (const std::bad_alloc&) {
+        return fail(ASG_ERR_NOMEM, "host allocation failed");
+    }
+}
+}  // namespace
+
+// --------------------------------------------------------------- layout ----
+
+constexpr int kCellBlock = 64;     // one wavefront per k_cells / k_sum block
+constexpr int kSpikeBlock = 256;   // k_spikes workgroup
+constexpr int kOrBlock = 256;      // k_or workgroup
+constexpr int kWindowCap = 16;     // longest window (bounds scratch memory)
+
+struct InputK {
+    double poiss_rate, poiss_weight, lambda, limit;
+    double noise_std;
+    int32_t independent;
+    int32_t steps_per_noise;
+};
+
+struct AstroPopK {
+    AstroK k;
+    double SIC_th, SIC_scale;
+    InputK in;
+};
+
+// Device-resident description of one network: sizes, constants, and the
+// addresses of its state, connectivity and scratch arrays.
+struct DevNet {
+    uint32_t n_exc, n_inh, n_neur, n_astro;
+    uint32_t words;        // ceil(n_neur / 64)
+    uint32_t cE, cI;       // 64-spike chunks needed for all exc / inh sources
+    int32_t S, Sf;         // ring slots, SIC-factor ring slots
+    int32_t d_exc, d_inh, d_n2a, d_a2n;
+    int32_t sic_interval, substeps;
+    int32_t win_W;         // length of the window k_spikes just closed
+    int32_t overflow;      // raster buffer overflowed
+    int64_t step;          // first step of the next window
+    int64_t target;        // step this call stops at
+    int64_t win_T0;        // first step of the window k_spikes just closed
+    uint64_t seed;
+    double dt, h, half_h, h_sixth;
+    NeuronK pk[2];
+    InputK in[2];
+    AstroPopK ap;
+    int32_t stp_on, fac_on;
+    double U, one_minus_U, tau_rec, tau_fac;
+    double w_exc, w_inh, w_n2a, w_a2n;
+    // neuron state
+    double *V, *w, *gex, *dgex, *gin, *dgin, *Isic;
+    int32_t* refr;
+    const double *V_reset, *b;
+    // astrocyte state and parameters
+    double *Ca, *IP3, *hh;
+    const double *Ca_tot, *IP3_0, *tau_IP3, *delta_IP3;
+    // rings: [S][n_neur] / [S][n_astro]; factors [Sf][n_astro]
+    double *ring_exc, *ring_inh, *ring_astro, *fac;
+    double *stp_x, *stp_u, *stp_tl;
+    // connectivity
+    const uint32_t *exc_row, *exc_tgt, *inh_row, *inh_tgt;
+    const uint32_t *n2a_row, *n2a_tgt, *n2a_mult, *n2a_raw_row;
+    const uint32_t *sic_row, *sic_src, *sic_mult;
+    // window scratch
+    uint64_t* spk_bits;    // [Wmax][words]
+    uint32_t* spk_list;    // [Wmax][n_neur]
+    uint32_t* spk_n;       // [Wmax][2]: total, excitatory
+    double* spk_c;         // [Wmax][n_neur][2]: exc-or-inh weight, n2a weight
+    uint64_t *mask_exc, *mask_inh, *mask_astro;  // [Wmax][cE|cI][targets]
+    // raster
+    int64_t* r_step;
+    uint32_t* r_src;
+    int64_t r_cap, r_head;
+    // counters
+    int64_t c_steps, c_spikes, c_syn, c_sic;
+};
+
+struct CellBlock {
+    int32_t net;
+    int32_t kind;   // 0 neurons, 1 astrocytes
+    uint32_t first;
+    uint32_t pad;
+};
+
+struct WorkItem {
+    int32_t net;
+    int32_t k;      // step within the window
+    uint32_t j;     // index in that step's spike list
+    uint32_t pad;
+};
+
+__device__ inline int window_len(const DevNet& n, int32_t Wg) {
+    const int64_t rem = n.target - n.step;
+    return rem <= 0 ? 0 : (rem < Wg ? static_cast<int>(rem) : Wg);
+}
+
+// --------------------------------------------------------------- kernels ---
+
+__device__ void neuron_window(DevNet& n, uint32_t i, int64_t T0, int W) {
+    const bool valid = i < n.n_neur;
+    const int pi = (i < n.n_exc) ? 0 : 1;
+    const NeuronK& p = n.pk[pi];
+    const InputK& in = n.in[pi];
+    const uint32_t N = n.n_neur;
+    double V = 0, w = 0, gex = 0, dgex = 0, gin = 0, dgin = 0, Isic = 0, Vr = 0, b = 0;
+    int refr = 0;
+    if (valid) {
+        V = n.V[i];
+        w = n.w[i];
+        gex = n.gex[i];
+        dgex = n.dgex[i];
+        gin = n.gin[i];
+        dgin = n.dgin[i];
+        Isic = n.Isic[i];
+        refr = n.refr[i];
+        Vr = n.V_reset[i];
+        b = n.b[i];
+    }
+    // The independent noise of one noise interval is one draw; keep it.
+    int64_t noise_idx = -1;
+    double noise_val = 0.0;
+    const uint32_t lane = threadIdx.x & 63;
+    for (int k = 0; k < W; ++k) {
+        const int64_t T = T0 + k;
+        bool spiked = false;
+        if (valid) {
+            const size_t slot = static_cast<size_t>(T % n.S) * N + i;
+            double exc_in = 0.0, inh_in = 0.0;
+            // apply_arrivals, network.cpp:301-327
+            const double ex = n.ring_exc[slot];
+            if (ex != 0.0) {
+                if (ex > 0.0) {
+                    exc_in += ex;
+                } else if (ex < 0.0) {
+                    inh_in -= ex;
+                }
+                n.ring_exc[slot] = 0.0;
+            }
+            const double ih = n.ring_inh[slot];
+            if (ih != 0.0) {
+                const double wgt = -ih;
+                if (wgt > 0.0) {
+                    exc_in += wgt;
+                } else if (wgt < 0.0) {
+                    inh_in -= wgt;
+                }
+                n.ring_inh[slot] = 0.0;
+            }
+            if (T >= n.d_a2n && (T - n.d_a2n) % n.sic_interval == 0) {
+                const double* f = n.fac + static_cast<size_t>((T - n.d_a2n) % n.Sf) * n.n_astro;
+                double acc = 0.0;
+                for (uint32_t q = n.sic_row[i]; q < n.sic_row[i + 1]; ++q) {
+                    const double fa = f[n.sic_src[q]];
+                    if (fa != 0.0) {
+                        const double v = n.w_a2n * fa;
+                        for (uint32_t r = n.sic_mult[q]; r > 0; --r) {
+                            acc += v;
+                        }
+                    }
+                }
+                Isic = acc;
+            }
+            // NeuronPopulation::update, neuron.cpp:143-170
+            if (in.poiss_rate > 0.0) {
+                const int ev = rng_poisson_fresh(n.seed ^ kSaltNeuronPoisson,
+                                                 static_cast<uint64_t>(T) * kStreamStride + i,
+                                                 in.lambda, in.limit);
+                if (ev > 0) {
+                    exc_in += in.poiss_weight * static_cast<double>(ev);
+                }
+            }
+            double I_ext = Isic;
+            if (in.noise_std > 0.0) {
+                const int64_t idx = T / in.steps_per_noise;
+                if (idx != noise_idx) {
+                    noise_idx = idx;
+                    uint64_t ctr = 0;
+                    if (in.independent) {
+                        noise_val = 0.0 + in.noise_std *
+                                              rng_normal_fresh(n.seed ^ kSaltNeuronNoise,
+                                                               static_cast<uint64_t>(idx) *
+                                                                       kNoiseStride + i,
+                                                               ctr);
+                    } else {
+                        const uint64_t salt = pi == 0 ? kSaltNoiseSharedExc : kSaltNoiseSharedInh;
+                        noise_val = 0.0 + in.noise_std *
+                                              rng_normal_fresh(n.seed ^ salt,
+                                                               static_cast<uint64_t>(idx), ctr);
+                    }
+                }
+                I_ext += noise_val;
+            }
+            if (exc_in != 0.0) {
+                dgex += exc_in * p.psc_ex;
+            }
+            if (inh_in != 0.0) {
+                dgin += inh_in * p.psc_in;
+            }
+            spiked = adex_step(p, Vr, b, I_ext, n.h, n.half_h, n.h_sixth, n.substeps, V, w, gex,
+                               dgex, gin, dgin, refr);
+        }
+        const uint64_t mask = __ballot(spiked);
+        if (lane == 0 && (i & ~63u) < N) {
+            n.spk_bits[static_cast<size_t>(k) * n.words + (i >> 6)] = mask;
+        }
+    }
+    if (valid) {
+        n.V[i] = V;
+        n.w[i] = w;
+        n.gex[i] = gex;
+        n.dgex[i] = dgex;
+        n.gin[i] = gin;
+        n.dgin[i] = dgin;
+        n.Isic[i] = Isic;
+        n.refr[i] = refr;
+    }
+}
+
+__device__ void astro_window(DevNet& n, uint32_t a, int64_t T0, int W) {
+    if (a >= n.n_astro) {
+        return;
+    }
+    const uint32_t NA = n.n_astro;
+    const AstroPopK& ap = n.ap;
+    double Ca = n.Ca[a], IP3 = n.IP3[a], hh = n.hh[a];
+    const double Ca_tot = n.Ca_tot[a], IP3_0 = n.IP3_0[a], tau = n.tau_IP3[a],
+                 delta = n.delta_IP3[a];
+    int64_t noise_idx = -1;
+    double noise = 0.0;
+    for (int k = 0; k < W; ++k) {
+        const int64_t T = T0 + k;
+        const size_t slot = static_cast<size_t>(T % n.S) * NA + a;
+        double ip3_in = 0.0;
+        const double v = n.ring_astro[slot];
+        if (v != 0.0) {
+            ip3_in += v;
+            n.ring_astro[slot] = 0.0;
+        }
+        // drive_astrocytes, network.cpp:340-355
+        if (ap.in.poiss_rate > 0.0) {
+            const int ev = rng_poisson_fresh(n.seed ^ kSaltAstroPoisson,
+                                             static_cast<uint64_t>(T) * kStreamStride + a,
+                                             ap.in.lambda, ap.in.limit);
+            if (ev > 0) {
+                ip3_in += ap.in.poiss_weight * static_cast<double>(ev);
+            }
+        }
+        // AstrocytePopulation::update noise, astrocyte.cpp:239-248, 310-314
+        if (ap.in.noise_std > 0.0) {
+            const int64_t idx = T / ap.in.steps_per_noise;
+            if (idx != noise_idx) {
+                noise_idx = idx;
+                const uint64_t nseed = n.seed ^ kSaltAstroNoise;
+                const uint64_t stream = ap.in.independent
+                                            ? static_cast<uint64_t>(idx) * kStreamStride + a
+                                            : static_cast<uint64_t>(idx);
+                noise = ap.in.noise_std * rng_normal_clamped(nseed, stream);
+            }
+        }
+        astro_step(ap.k, Ca_tot, IP3_0, tau, delta, ip3_in, noise, n.h, n.half_h, n.h_sixth,
+                   n.substeps, Ca, IP3, hh);
+        if (T % n.sic_interval == 0) {
+            n.fac[static_cast<size_t>(T % n.Sf) * NA + a] = sic_factor(Ca, ap.SIC_th, ap.SIC_scale);
+        }
+    }
+    n.Ca[a] = Ca;
+    n.IP3[a] = IP3;
+    n.hh[a] = hh;
+}
+
+__global__ __launch_bounds__(kCellBlock) void k_cells(DevNet* nets, const CellBlock* blocks,
+                                                      int32_t Wg, uint32_t* work_count) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *work_count = 0;
+    }
+    const CellBlock cb = blocks[blockIdx.x];
+    DevNet& n = nets[cb.net];
+    const int W = window_len(n, Wg);
+    if (W == 0) {
+        return;
+    }
+    const int64_t T0 = n.step;
+    const uint32_t i = cb.first + threadIdx.x;
+    if (cb.kind == 0) {
+        neuron_window(n, i, T0, W);
+    } else {
+        astro_window(n, i, T0, W);
+    }
+}
+
+// Block-wide exclusive scan of one uint32 per thread (kSpikeBlock threads).
+__device__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_tmp, uint32_t& total) {
+    const uint32_t tid = threadIdx.x;
+    s_tmp[tid] = v;
+    __syncthreads();
+    for (uint32_t off = 1; off < kSpikeBlock; off <<= 1) {
+        const uint32_t add = tid >= off ? s_tmp[tid - off] : 0u;
+        __syncthreads();
+        s_tmp[tid] += add;
+        __syncthreads();
+    }
+    total = s_tmp[kSpikeBlock - 1];
+    const uint32_t incl = s_tmp[tid];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ __launch_bounds__(kSpikeBlock) void k_spikes(DevNet* nets, const int32_t* active,
+                                                        int32_t Wg, WorkItem* work,
+                                                        uint32_t* work_count) {
+    __shared__ uint32_t s_tmp[kSpikeBlock];
+    __shared__ uint64_t s_syn[kSpikeBlock];
+    const int32_t net = active[blockIdx.x];
+    DevNet& n = nets[net];
+    const int W = window_len(n, Wg);
+    const int64_t T0 = n.step;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t words = n.words;
+    const uint32_t per = (words + kSpikeBlock - 1) / kSpikeBlock;
+    const uint32_t w_lo = min(words, tid * per), w_hi = min(words, w_lo + per);
+    const uint32_t N = n.n_neur;
+    int64_t head = n.r_head;
+    int64_t tot_spikes = 0;
+    uint64_t syn_local = 0;
+    for (int k = 0; k < W; ++k) {
+        const int64_t T = T0 + k;
+        const uint64_t* bits = n.spk_bits + static_cast<size_t>(k) * words;
+        uint32_t cnt = 0, cntE = 0;
+        for (uint32_t q = w_lo; q < w_hi; ++q) {
+            const uint64_t m = bits[q];
+            cnt += __popcll(m);
+            const uint32_t base = q * 64;
+            if (base + 64 <= n.n_exc) {
+                cntE += __popcll(m);
+            } else if (base < n.n_exc) {
+                cntE += __popcll(m & ((1ULL << (n.n_exc - base)) - 1ULL));
+            }
+        }
+        uint32_t total, totalE;
+        uint32_t off = block_exclusive_scan(cnt, s_tmp, total);
+        block_exclusive_scan(cntE, s_tmp, totalE);
+        uint32_t* list = n.spk_list + static_cast<size_t>(k) * N;
+        for (uint32_t q = w_lo; q < w_hi; ++q) {
+            uint64_t m = bits[q];
+            while (m) {
+                const uint32_t bit = static_cast<uint32_t>(__ffsll(static_cast<long long>(m)) - 1);
+                list[off++] = q * 64 + bit;
+                m &= m - 1;
+            }
+        }
+        __syncthreads();
+        // Tsodyks-Markram per spiking source (network.cpp:230-247), then the
+        // weights every synapse of its rows receives this step.
+        const double t_now = static_cast<double>(T) * n.dt;
+        double* cvals = n.spk_c + static_cast<size_t>(k) * N * 2;
+        const bool room = head + static_cast<int64_t>(total) <= n.r_cap;
+        for (uint32_t j = tid; j < total; j += kSpikeBlock) {
+            const uint32_t s = list[j];
+            const bool exc = s < n.n_exc;
+            double c0 = exc ? n.w_exc : n.w_inh;
+            double c1 = n.w_n2a;
+            if (n.stp_on) {
+                const double dts = t_now - n.stp_tl[s];
+                const double x_decay = asg_exp(-dts / n.tau_rec);
+                const double u_decay = n.fac_on ? asg_exp(-dts / n.tau_fac) : 0.0;
+                double x = n.stp_x[s];
+                double u = n.stp_u[s];
+                x = 1.0 + (x - x * u - 1.0) * x_decay;
+                u = n.U + u * n.one_minus_U * u_decay;
+                n.stp_x[s] = x;
+                n.stp_u[s] = u;
+                n.stp_tl[s] = t_now;
+                c0 = c0 * x * u;
+                c1 = c1 * x * u;
+            }
+            cvals[2 * j] = c0;
+            cvals[2 * j + 1] = c1;
+            if (exc) {
+                syn_local += (n.exc_row[s + 1] - n.exc_row[s]) +
+                             (n.n2a_raw_row[s + 1] - n.n2a_raw_row[s]);
+            } else {
+                syn_local += n.inh_row[s - n.n_exc + 1] - n.inh_row[s - n.n_exc];
+            }
+            if (room) {
+                n.r_step[head + j] = T;
+                n.r_src[head + j] = s;
+            }
+        }
+        if (tid == 0) {
+            n.spk_n[2 * k] = total;
+            n.spk_n[2 * k + 1] = totalE;
+            if (!room) {
+                n.overflow = 1;
+            }
+        }
+        if (total > 0) {
+            uint32_t base = 0;
+            if (tid == 0) {
+                base = atomicAdd(work_count, total);
+                s_tmp[0] = base;
+            }
+            __syncthreads();
+            base = s_tmp[0];
+            for (uint32_t j = tid; j < total; j += kSpikeBlock) {
+                work[base + j] = WorkItem{net, k, j, 0u};
+            }
+        }
+        if (room) {
+            head += total;
+        }
+        tot_spikes += total;
+        __syncthreads();
+    }
+    s_syn[tid] = syn_local;
+    __syncthreads();
+    if (tid == 0) {
+        uint64_t syn = 0;
+        for (int q = 0; q < kSpikeBlock; ++q) {
+            syn += s_syn[q];
+        }
+        n.c_syn += static_cast<int64_t>(syn);
+        n.c_spikes += tot_spikes;
+        n.c_steps += W;
+        n.r_head = head;
+        n.win_T0 = T0;
+        n.win_W = W;
+        n.step = T0 + W;
+    }
+}
+
+__global__ __launch_bounds__(kOrBlock) void k_or(DevNet* nets, const WorkItem* work,
+                                                 const uint32_t* work_count) {
+    const uint32_t total = *work_count;
+    for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
+        const WorkItem it = work[item];
+        DevNet& n = nets[it.net];
+        const uint32_t N = n.n_neur;
+        const uint32_t s = n.spk_list[static_cast<size_t>(it.k) * N + it.j];
+        const uint32_t nE = n.spk_n[2 * it.k + 1];
+        if (s < n.n_exc) {
+            const uint64_t bit = 1ULL << (it.j & 63);
+            const size_t chunk = static_cast<size_t>(it.k) * n.cE + (it.j >> 6);
+            uint64_t* me = n.mask_exc + chunk * N;
+            for (uint32_t q = n.exc_row[s] + threadIdx.x; q < n.exc_row[s + 1]; q += kOrBlock) {
+                atomicOr(reinterpret_cast<unsigned long long*>(me + n.exc_tgt[q]), bit);
+            }
+            uint64_t* ma = n.mask_astro + chunk * n.n_astro;
+            for (uint32_t q = n.n2a_row[s] + threadIdx.x; q < n.n2a_row[s + 1]; q += kOrBlock) {
+                atomicOr(reinterpret_cast<unsigned long long*>(ma + n.n2a_tgt[q]), bit);
+            }
+        } else {
+            const uint32_t jj = it.j - nE;
+            const uint64_t bit = 1ULL << (jj & 63);
+            const size_t chunk = static_cast<size_t>(it.k) * n.cI + (jj >> 6);
+            uint64_t* mi = n.mask_inh + chunk * N;
+            const uint32_t r = s - n.n_exc;
+            for (uint32_t q = n.inh_row[r] + threadIdx.x; q < n.inh_row[r + 1]; q += kOrBlock) {
+                atomicOr(reinterpret_cast<unsigned long long*>(mi + n.inh_tgt[q]), bit);
+            }
+        }
+    }
+}
+
+// Multiplicity of astrocyte a in source s's deduplicated n2a row.
+__device__ inline uint32_t n2a_multiplicity(const DevNet& n, uint32_t s, uint32_t a) {
+    uint32_t lo = n.n2a_row[s], hi = n.n2a_row[s + 1];
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (n.n2a_tgt[mid] < a) {
+            lo = mid + 1;
+        } else {
+            hi = mid;
+        }
+    }
+    return (lo < n.n2a_row[s + 1] && n.n2a_tgt[lo] == a) ? n.n2a_mult[lo] : 0u;
+}
+
+__global__ __launch_bounds__(kCellBlock) void k_sum(DevNet* nets, const CellBlock* blocks) {
+    const CellBlock cb = blocks[blockIdx.x];
+    DevNet& n = nets[cb.net];
+    const int W = n.win_W;
+    const int64_t T0 = n.win_T0;
+    const uint32_t i = cb.first + threadIdx.x;
+    const uint32_t N = n.n_neur;
+    if (cb.kind == 0) {
+        if (i >= N) {
+            return;
+        }
+        for (int k = 0; k < W; ++k) {
+            const uint32_t tot = n.spk_n[2 * k], nE = n.spk_n[2 * k + 1], nI = tot - nE;
+            const double* cvals = n.spk_c + static_cast<size_t>(k) * N * 2;
+            if (nE > 0) {
+                const size_t slot = static_cast<size_t>((T0 + k + n.d_exc) % n.S) * N + i;
+                double acc = n.ring_exc[slot];
+                bool any = false;
+                for (uint32_t c = 0; c < (nE + 63) / 64; ++c) {
+                    uint64_t* mp = n.mask_exc + (static_cast<size_t>(k) * n.cE + c) * N + i;
+                    uint64_t m = *mp;
+                    if (m) {
+                        *mp = 0;
+                        any = true;
+                        while (m) {
+                            const uint32_t j = c * 64 + (__ffsll(static_cast<long long>(m)) - 1);
+                            acc += cvals[2 * j];
+                            m &= m - 1;
+                        }
+                    }
+                }
+                if (any) {
+                    n.ring_exc[slot] = acc;
+                }
+            }
+            if (nI > 0) {
+                const size_t slot = static_cast<size_t>((T0 + k + n.d_inh) % n.S) * N + i;
+                double acc = n.ring_inh[slot];
+                bool any = false;
+                for (uint32_t c = 0; c < (nI + 63) / 64; ++c) {
+                    uint64_t* mp = n.mask_inh + (static_cast<size_t>(k) * n.cI + c) * N + i;
+                    uint64_t m = *mp;
+                    if (m) {
+                        *mp = 0;
+                        any = true;
+                        while (m) {
+                            const uint32_t j = nE + c * 64 + (__ffsll(static_cast<long long>(m)) - 1);
+                            acc -= cvals[2 * j];
+                            m &= m - 1;
+                        }
+                    }
+                }
+                if (any) {
+                    n.ring_inh[slot] = acc;
+                }
+            }
+        }
+    } else {
+        const uint32_t NA = n.n_astro;
+        if (i >= NA) {
+            return;
+        }
+        for (int k = 0; k < W; ++k) {
+            const uint32_t nE = n.spk_n[2 * k + 1];
+            if (nE == 0) {
+                continue;
+            }
+            const double* cvals = n.spk_c + static_cast<size_t>(k) * N * 2;
+            const uint32_t* list = n.spk_list + static_cast<size_t>(k) * N;
+            const size_t slot = static_cast<size_t>((T0 + k + n.d_n2a) % n.S) * NA + i;
+            double acc = n.ring_astro[slot];
+            bool any = false;
+            for (uint32_t c = 0; c < (nE + 63) / 64; ++c) {
+                uint64_t* mp = n.mask_astro + (static_cast<size_t>(k) * n.cE + c) * NA + i;
+                uint64_t m = *mp;
+                if (m) {
+                    *mp = 0;
+                    any = true;
+                    while (m) {
+                        const uint32_t j = c * 64 + (__ffsll(static_cast<long long>(m)) - 1);
+                        const double cv = cvals[2 * j + 1];
+                        for (uint32_t r = n2a_multiplicity(n, list[j], i); r > 0; --r) {
+                            acc += cv;
+                        }
+                        m &= m - 1;
+                    }
+                }
+            }
+            if (any) {
+                n.ring_astro[slot] = acc;
+            }
+        }
+    }
+}
+
+// Element-wise evaluation of the step's FP64 functions (diagnostics).
+__global__ void k_math(int32_t fn, const double* x, double* y, int64_t n) {
+    for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const double v = x[i];
+        double r;
+        switch (fn) {
+            case ASG_MATH_EXP: r = asg_exp(v); break;
+            case ASG_MATH_LOG: r = asg_log(v); break;
+            case ASG_MATH_SIN: r = asg_sin(v); break;
+            case ASG_MATH_COS: r = asg_cos(v); break;
+            default: r = asg_sqrt(v); break;
+        }
+        y[i] = r;
+    }
+}
+
+// ------------------------------------------------------- host: one net -----
+
+namespace {
+
+int steps_per_noise(double noise_dt, double dt) {
+    const long long r = std::llround(noise_dt / dt);
+    return std::max(1, static_cast<int>(r));
+}
+
+NeuronK make_neuron_k(const asg_neuron_params& s, double h) {
+    NeuronK p{};
+    p.C_m = s.C_m;
+    p.neg_g_L = -s.g_L;
+    p.E_L = s.E_L;
+    p.V_th = s.V_th;
+    p.Delta_T = s.Delta_T;
+    p.a = s.a;
+    p.tau_w = s.tau_w;
+    p.V_peak = s.V_peak;
+    p.E_ex = s.E_ex;
+    p.E_in = s.E_in;
+    p.I_e = s.I_e;
+    p.gL_DeltaT = s.g_L * s.Delta_T;
+    constexpr double kE = 2.7182818284590452353602874713527;
+    p.psc_ex = kE / s.tau_syn_ex;
+    p.psc_in = kE / s.tau_syn_in;
+    p.ex_decay = asg_exp(-h / s.tau_syn_ex);
+    p.in_decay = asg_exp(-h / s.tau_syn_in);
+    p.refr_steps = static_cast<int>(s.t_ref / h + 0.5);
+    return p;
+}
+
+InputK make_input_k(const asg_input_params& in, double dt) {
+    InputK k{};
+    k.poiss_rate = in.poiss_rate;
+    k.poiss_weight = in.poiss_weight;
+    k.lambda = in.poiss_rate * dt * 1e-3;
+    k.limit = asg_exp(-k.lambda);
+    k.noise_std = in.gauss_noise_std;
+    k.independent = in.independent_noise != 0;
+    k.steps_per_noise = steps_per_noise(in.noise_dt, dt);
+    return k;
+}
+
+AstroK make_astro_k(const asg_astro_constants& c) {
+    AstroK k{};
+    k.ratio_ER_cyt = c.ratio_ER_cyt;
+    k.Kd_IP3_1 = c.Kd_IP3_1;
+    k.Kd_IP3_2 = c.Kd_IP3_2;
+    k.Kd_act = c.Kd_act;
+    k.ratio_rate_IP3R = c.ratio_ER_cyt * c.rate_IP3R;
+    k.ratio_rate_L = c.ratio_ER_cyt * c.rate_L;
+    k.rate_SERCA = c.rate_SERCA;
+    k.Km_SERCA_sq = c.Km_SERCA * c.Km_SERCA;
+    k.k_Kd_inh = c.k_IP3R * c.Kd_inh;
+    k.k_IP3R = c.k_IP3R;
+    return k;
+}
+
+std::string check_csr(const char* name, const asg_csr& c, uint32_t rows, uint32_t n_targets) {
+    if (c.n_rows != rows) {
+        return std::string(name) + ": n_rows does not match the population size";
+    }
+    if (c.row_start == nullptr) {
+        return std::string(name) + ": row_start is NULL";
+    }
+    if (c.row_start[0] != 0) {
+        return std::string(name) + ": row_start[0] must be 0";
+    }
+    for (uint32_t r = 0; r < rows; ++r) {
+        if (c.row_start[r + 1] < c.row_start[r]) {
+            return std::string(name) + ": row_start is not monotone";
+        }
+    }
+    const uint32_t nnz = c.row_start[rows];
+    if (nnz > 0 && c.target == nullptr) {
+        return std::string(name) + ": target is NULL";
+    }
+    for (uint32_t q = 0; q < nnz; ++q) {
+        if (c.target[q] >= n_targets) {
+            return std::string(name) + ": target index out of range";
+        }
+    }
+    if (c.delay_steps < 1) {
+        return std::string(name) + ": delay_steps must be >= 1";
+    }
+    return "";
+}
+
+std::string validate(const asg_net_desc& d) {
+    const uint32_t N = d.n_exc + d.n_inh;
+    if (d.substeps < 1) return "substeps must be at least 1";
+    if (!(d.dt > 0.0)) return "dt must be positive";
+    if (d.sic_interval < 1) return "sic_interval must be at least 1";
+    if (d.precision != ASG_FP64 && d.precision != ASG_FP32) return "unknown precision";
+    if (N > 0 && (d.V_reset == nullptr || d.b == nullptr)) return "V_reset / b arrays are NULL";
+    if (d.n_astro > 0 && (d.Ca_tot == nullptr || d.IP3_0 == nullptr || d.tau_IP3 == nullptr ||
+                          d.delta_IP3 == nullptr)) {
+        return "astrocyte parameter arrays are NULL";
+    }
+    std::string e;
+    if (!(e = check_csr("exc", d.exc, d.n_exc, N)).empty()) return e;
+    if (!(e = check_csr("inh", d.inh, d.n_inh, N)).empty()) return e;
+    if (!(e = check_csr("n2a", d.n2a, d.n_exc, d.n_astro)).empty()) return e;
+    if (!(e = check_csr("a2n", d.a2n, d.n_astro, N)).empty()) return e;
+    return "";
+}
+
+// Host-side derived connectivity: deduplicated n2a rows and the SIC gather
+// lists (a2n transposed to targets, astrocytes ascending, with multiplicity).
+struct DerivedConn {
+    std::vector<uint32_t> n2a_row, n2a_tgt, n2a_mult;
+    std::vector<uint32_t> sic_row, sic_src, sic_mult;
+};
+
+DerivedConn derive(const asg_net_desc& d) {
+    DerivedConn out;
+    const uint32_t N = d.n_exc + d.n_inh;
+    out.n2a_row.assign(d.n_exc + 1, 0);
+    std::vector<uint32_t> row;
+    for (uint32_t s = 0; s < d.n_exc; ++s) {
+        row.assign(d.n2a.target + d.n2a.row_start[s], d.n2a.target + d.n2a.row_start[s + 1]);
+        std::sort(row.begin(), row.end());
+        for (size_t q = 0; q < row.size();) {
+            size_t e = q;
+            while (e < row.size() && row[e] == row[q]) ++e;
+            out.n2a_tgt.push_back(row[q]);
+            out.n2a_mult.push_back(static_cast<uint32_t>(e - q));
+            q = e;
+        }
+        out.n2a_row[s + 1] = static_cast<uint32_t>(out.n2a_tgt.size());
+    }
+    // Count (target, astro) pairs; astrocytes are visited ascending, so
+    // each target's list comes out sorted by astrocyte.
+    std::vector<uint32_t> count(N + 1, 0);
+    std::vector<uint32_t> last(N, UINT32_MAX);
+    for (uint32_t a = 0; a < d.n_astro; ++a) {
+        for (uint32_t q = d.a2n.row_start[a]; q < d.a2n.row_start[a + 1]; ++q) {
+            const uint32_t t = d.a2n.target[q];
+            if (last[t] != a) {
+                last[t] = a;
+                ++count[t + 1];
+            }
+        }
+    }
+    out.sic_row.assign(N + 1, 0);
+    for (uint32_t t = 0; t < N; ++t) out.sic_row[t + 1] = out.sic_row[t] + count[t + 1];
+    out.sic_src.assign(out.sic_row[N], 0);
+    out.sic_mult.assign(out.sic_row[N], 0);
+    std::vector<uint32_t> fill(out.sic_row.begin(), out.sic_row.end() - 1);
+    std::fill(last.begin(), last.end(), UINT32_MAX);
+    for (uint32_t a = 0; a < d.n_astro; ++a) {
+        for (uint32_t q = d.a2n.row_start[a]; q < d.a2n.row_start[a + 1]; ++q) {
+            const uint32_t t = d.a2n.target[q];
+            if (last[t] != a) {
+                last[t] = a;
+                out.sic_src[fill[t]] = a;
+                out.sic_mult[fill[t]] = 1;
+                ++fill[t];
+            } else {
+                ++out.sic_mult[fill[t] - 1];
+            }
+        }
+    }
+    return out;
+}
+
+int window_of(const asg_net_desc& d) {
+    int w = std::min(std::min(d.exc.delay_steps, d.inh.delay_steps),
+                     std::min(d.n2a.delay_steps, d.a2n.delay_steps));
+    return std::max(1, std::min(w, kWindowCap));
+}
+
+// Byte plan of one network's device arrays, so the footprint query and the
+// allocation agree.
+struct Plan {
+    size_t N, NA, words, S, Sf, Wmax, cE, cI, r_cap;
+    size_t n_exc_syn, n_inh_syn, n2a_dedup, sic_pairs;
+    size_t bytes;
+};
+
+Plan plan_of(const asg_net_desc& d, const DerivedConn* dc) {
+    Plan p{};
+    p.N = d.n_exc + d.n_inh;
+    p.NA = d.n_astro;
+    p.words = (p.N + 63) / 64;
+    p.Wmax = static_cast<size_t>(window_of(d));
+    const int dmax = std::max(std::max(d.exc.delay_steps, d.inh.delay_steps),
+                              std::max(d.n2a.delay_steps, d.a2n.delay_steps));
+    p.S = static_cast<size_t>(dmax) + p.Wmax;
+    p.Sf = static_cast<size_t>(d.a2n.delay_steps) + p.Wmax;
+    p.cE = (d.n_exc + 63) / 64;
+    p.cI = (d.n_inh + 63) / 64;
+    p.r_cap = std::max<size_t>(size_t(1) << 20, 16 * p.N);
+    p.n_exc_syn = d.exc.row_start[d.n_exc];
+    p.n_inh_syn = d.inh.row_start[d.n_inh];
+    const size_t n2a_raw = d.n2a.row_start[d.n_exc];
+    const size_t a2n_raw = d.a2n.row_start[d.n_astro];
+    p.n2a_dedup = dc ? dc->n2a_tgt.size() : n2a_raw;
+    p.sic_pairs = dc ? dc->sic_src.size() : a2n_raw;
+    size_t b = 0;
+    b += p.N * (7 * 8 + 4 + 2 * 8 + 3 * 8);        // neuron state, V_reset/b, STP
+    b += p.NA * (3 * 8 + 4 * 8);                    // astrocyte state and parameters
+    b += p.S * p.N * 2 * 8 + p.S * p.NA * 8 + p.Sf * p.NA * 8;
+    b += (d.n_exc + 1 + p.n_exc_syn + d.n_inh + 1 + p.n_inh_syn) * 4;
+    b += (2 * (d.n_exc + 1) + 2 * p.n2a_dedup) * 4;
+    b += (p.N + 1 + 2 * p.sic_pairs) * 4;
+    b += p.Wmax * (p.words * 8 + p.N * 4 + 8 + p.N * 16);
+    b += p.Wmax * (p.cE * p.N + p.cI * p.N + p.cE * p.NA) * 8;
+    b += p.r_cap * 12;
+    p.bytes = b;
+    return p;
+}
+
+// One network's device allocation and its host mirror.
+struct NetSlot {
+    bool used = false;
+    DevNet dn{};                  // host copy (device addresses)
+    std::vector<void*> allocs;
+    size_t bytes = 0;
+    int W = 1;
+    int64_t step = 0;
+    int64_t total = 0;            // run length; INT64_MAX for asg_net
+    std::vector<int64_t> q_step;  // drained raster not yet handed out
+    std::vector<uint32_t> q_src;
+    size_t q_pos = 0;
+};
+
+template <typename T>
+T* dev_alloc(NetSlot& s, size_t count) {
+    void* p = nullptr;
+    if (count == 0) count = 1;
+    HIP_CHECK(hipMalloc(&p, count * sizeof(T)));
+    s.allocs.push_back(p);
+    return static_cast<T*>(p);
+}
+
+template <typename T>
+T* dev_copy(NetSlot& s, const T* host, size_t count, hipStream_t st) {
+    T* p = dev_alloc<T>(s, count);
+    if (count > 0) {
+        HIP_CHECK(hipMemcpyAsync(p, host, count * sizeof(T), hipMemcpyHostToDevice, st));
+    }
+    return p;
+}
+
+template <typename T>
+T* dev_fill(NetSlot& s, T value, size_t count, hipStream_t st) {
+    std::vector<T> h(std::max<size_t>(count, 1), value);
+    T* p = dev_alloc<T>(s, count);
+    HIP_CHECK(hipMemcpyAsync(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
+    HIP_CHECK(hipStreamSynchronize(st));
+    return p;
+}
+
+template <typename T>
+T* dev_zero(NetSlot& s, size_t count, hipStream_t st) {
+    T* p = dev_alloc<T>(s, count);
+    HIP_CHECK(hipMemsetAsync(p, 0, std::max<size_t>(count, 1) * sizeof(T), st));
+    return p;
+}
+
+void release(NetSlot& s) {
+    for (void* p : s.allocs) {
+        (void)hipFree(p);
+    }
+    s.allocs.clear();
+    s.used = false;
+    s.q_step.clear();
+    s.q_src.clear();
+    s.q_pos = 0;
+}
+
+void build_slot(NetSlot& s, const asg_net_desc& d, int64_t total, hipStream_t st) {
+    const DerivedConn dc = derive(d);
+    const Plan pl = plan_of(d, &dc);
+    DevNet& n = s.dn;
+    n = DevNet{};
+    n.n_exc = d.n_exc;
+    n.n_inh = d.n_inh;
+    n.n_neur = static_cast<uint32_t>(pl.N);
+    n.n_astro = d.n_astro;
+    n.words = static_cast<uint32_t>(pl.words);
+    n.cE = static_cast<uint32_t>(pl.cE);
+    n.cI = static_cast<uint32_t>(pl.cI);
+    n.S = static_cast<int32_t>(pl.S);
+    n.Sf = static_cast<int32_t>(pl.Sf);
+    n.d_exc = d.exc.delay_steps;
+    n.d_inh = d.inh.delay_steps;
+    n.d_n2a = d.n2a.delay_steps;
+    n.d_a2n = d.a2n.delay_steps;
+    n.sic_interval = d.sic_interval;
+    n.substeps = d.substeps;
+    n.seed = d.seed;
+    n.dt = d.dt;
+    n.h = d.dt / static_cast<double>(d.substeps);
+    n.half_h = 0.5 * n.h;
+    n.h_sixth = n.h / 6.0;
+    n.pk[0] = make_neuron_k(d.neuron[0], n.h);
+    n.pk[1] = make_neuron_k(d.neuron[1], n.h);
+    n.in[0] = make_input_k(d.input_exc, d.dt);
+    n.in[1] = make_input_k(d.input_inh, d.dt);
+    n.ap.k = make_astro_k(d.astro.c);
+    n.ap.SIC_th = d.astro.SIC_th;
+    n.ap.SIC_scale = d.astro.SIC_scale;
+    n.ap.in = make_input_k(d.input_astro, d.dt);
+    n.stp_on = d.stp.enabled != 0;
+    n.fac_on = !(d.stp.tau_fac < 1e-10);
+    n.U = d.stp.U;
+    n.one_minus_U = 1.0 - d.stp.U;
+    n.tau_rec = d.stp.tau_rec;
+    n.tau_fac = d.stp.tau_fac;
+    n.w_exc = d.exc.weight;
+    n.w_inh = d.inh.weight;
+    n.w_n2a = d.n2a.weight;
+    n.w_a2n = d.a2n.weight;
+
+    const size_t N = pl.N, NA = pl.NA;
+    std::vector<double> V0(N);
+    for (uint32_t i = 0; i < N; ++i) {
+        V0[i] = d.neuron[i < d.n_exc ? 0 : 1].V_m_init;
+    }
+    n.V = dev_copy(s, V0.data(), N, st);
+    n.w = dev_zero<double>(s, N, st);
+    n.gex = dev_zero<double>(s, N, st);
+    n.dgex = dev_zero<double>(s, N, st);
+    n.gin = dev_zero<double>(s, N, st);
+    n.dgin = dev_zero<double>(s, N, st);
+    n.Isic = dev_zero<double>(s, N, st);
+    n.refr = dev_zero<int32_t>(s, N, st);
+    n.V_reset = dev_copy(s, d.V_reset, N, st);
+    n.b = dev_copy(s, d.b, N, st);
+    n.Ca = dev_fill(s, d.astro.Ca_init, NA, st);
+    n.IP3 = dev_fill(s, d.astro.IP3_init, NA, st);
+    n.hh = dev_fill(s, d.astro.h_init, NA, st);
+    n.Ca_tot = dev_copy(s, d.Ca_tot, NA, st);
+    n.IP3_0 = dev_copy(s, d.IP3_0, NA, st);
+    n.tau_IP3 = dev_copy(s, d.tau_IP3, NA, st);
+    n.delta_IP3 = dev_copy(s, d.delta_IP3, NA, st);
+    n.ring_exc = dev_zero<double>(s, pl.S * N, st);
+    n.ring_inh = dev_zero<double>(s, pl.S * N, st);
+    n.ring_astro = dev_zero<double>(s, pl.S * NA, st);
+    n.fac = dev_zero<double>(s, pl.Sf * NA, st);
+    n.stp_x = dev_fill(s, 1.0, N, st);
+    n.stp_u = dev_fill(s, d.stp.U, N, st);
+    n.stp_tl = dev_fill(s, -1e12, N, st);
+    n.exc_row = dev_copy(s, d.exc.row_start, d.n_exc + 1, st);
+    n.exc_tgt = dev_copy(s, d.exc.target, pl.n_exc_syn, st);
+    n.inh_row = dev_copy(s, d.inh.row_start, d.n_inh + 1, st);
+    n.inh_tgt = dev_copy(s, d.inh.target, pl.n_inh_syn, st);
+    n.n2a_row = dev_copy(s, dc.n2a_row.data(), dc.n2a_row.size(), st);
+    n.n2a_tgt = dev_copy(s, dc.n2a_tgt.data(), dc.n2a_tgt.size(), st);
+    n.n2a_mult = dev_copy(s, dc.n2a_mult.data(), dc.n2a_mult.size(), st);
+    n.n2a_raw_row = dev_copy(s, d.n2a.row_start, d.n_exc + 1, st);
+    n.sic_row = dev_copy(s, dc.sic_row.data(), dc.sic_row.size(), st);
+    n.sic_src = dev_copy(s, dc.sic_src.data(), dc.sic_src.size(), st);
+    n.sic_mult = dev_copy(s, dc.sic_mult.data(), dc.sic_mult.size(), st);
+    n.spk_bits = dev_zero<uint64_t>(s, pl.Wmax * pl.words, st);
+    n.spk_list = dev_zero<uint32_t>(s, pl.Wmax * N, st);
+    n.spk_n = dev_zero<uint32_t>(s, pl.Wmax * 2, st);
+    n.spk_c = dev_zero<double>(s, pl.Wmax * N * 2, st);
+    n.mask_exc = dev_zero<uint64_t>(s, pl.Wmax * pl.cE * N, st);
+    n.mask_inh = dev_zero<uint64_t>(s, pl.Wmax * pl.cI * N, st);
+    n.mask_astro = dev_zero<uint64_t>(s, pl.Wmax * pl.cE * NA, st);
+    n.r_cap = static_cast<int64_t>(pl.r_cap);
+    n.r_step = dev_alloc<int64_t>(s, pl.r_cap);
+    n.r_src = dev_alloc<uint32_t>(s, pl.r_cap);
+    HIP_CHECK(hipStreamSynchronize(st));
+    s.bytes = pl.bytes;
+    s.W = static_cast<int>(pl.Wmax);
+    s.step = 0;
+    s.total = total;
+    s.used = true;
+}
+
+}  // namespace
+
+// --------------------------------------------------------------- engine ----
+
+// A set of networks stepped together. asg_net wraps an engine with one slot.
+struct Engine {
+    hipStream_t stream = nullptr;
+    int device = 0;
+    int max_nets = 0;
+    int64_t budget = 0;
+    std::vector<NetSlot> slots;
+    DevNet* d_nets = nullptr;
+    CellBlock* d_blocks = nullptr;
+    int32_t* d_active = nullptr;
+    WorkItem* d_work = nullptr;
+    uint32_t* d_work_count = nullptr;
+    size_t blocks_cap = 0, active_cap = 0, work_cap = 0;
+    int n_blocks = 0, n_active = 0, Wg = 1;
+    bool dirty = true;
+    bool pending = false;  // async work in flight
+
+    explicit Engine(int max_n, int64_t budget_bytes) : max_nets(max_n), budget(budget_bytes) {
+        HIP_CHECK(hipGetDevice(&device));
+        HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        slots.resize(static_cast<size_t>(max_n));
+        HIP_CHECK(hipMalloc(&d_nets, sizeof(DevNet) * static_cast<size_t>(max_n)));
+        HIP_CHECK(hipMalloc(&d_work_count, sizeof(uint32_t)));
+        HIP_CHECK(hipMemset(d_work_count, 0, sizeof(uint32_t)));
+    }
+
+    ~Engine() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (NetSlot& s : slots) release(s);
+        (void)hipFree(d_nets);
+        (void)hipFree(d_blocks);
+        (void)hipFree(d_active);
+        (void)hipFree(d_work);
+        (void)hipFree(d_work_count);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    int64_t bytes_in_use() const {
+        int64_t b = 0;
+        for (const NetSlot& s : slots) {
+            if (s.used) b += static_cast<int64_t>(s.bytes);
+        }
+        return b;
+    }
+
+    void wait() {
+        if (pending) {
+            HIP_CHECK(hipStreamSynchronize(stream));
+            pending = false;
+        }
+    }
+
+    void upload(int slot) {
+        HIP_CHECK(hipMemcpyAsync(d_nets + slot, &slots[slot].dn, sizeof(DevNet),
+                                 hipMemcpyHostToDevice, stream));
+    }
+
+    // Pull the device copy of a slot's descriptor (its step, raster head and
+    // counters move on the device).
+    void download(int slot) {
+        HIP_CHECK(hipMemcpyAsync(&slots[slot].dn, d_nets + slot, sizeof(DevNet),
+                                 hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+    }
+
+    template <typename T>
+    static void grow(T*& p, size_t& cap, size_t need) {
+        if (need <= cap) return;
+        (void)hipFree(p);
+        p = nullptr;
+        cap = std::max(need, cap * 2);
+        HIP_CHECK(hipMalloc(&p, cap * sizeof(T)));
+    }
+
+    void rebuild_tables() {
+        std::vector<CellBlock> blocks;
+        std::vector<int32_t> active;
+        size_t work_items = 0;
+        Wg = kWindowCap;
+        for (int sidx = 0; sidx < max_nets; ++sidx) {
+            const NetSlot& s = slots[sidx];
+            if (!s.used) continue;
+            active.push_back(sidx);
+            Wg = std::min(Wg, s.W);
+            for (uint32_t f = 0; f < s.dn.n_neur; f += kCellBlock) {
+                blocks.push_back(CellBlock{sidx, 0, f, 0});
+            }
+            for (uint32_t f = 0; f < s.dn.n_astro; f += kCellBlock) {
+                blocks.push_back(CellBlock{sidx, 1, f, 0});
+            }
+            work_items += static_cast<size_t>(s.W) * s.dn.n_neur;
+        }
+        n_blocks = static_cast<int>(blocks.size());
+        n_active = static_cast<int>(active.size());
+        if (n_blocks > 0) {
+            grow(d_blocks, blocks_cap, blocks.size());
+            HIP_CHECK(hipMemcpyAsync(d_blocks, blocks.data(), blocks.size() * sizeof(CellBlock),
+                                     hipMemcpyHostToDevice, stream));
+        }
+        if (n_active > 0) {
+            grow(d_active, active_cap, active.size());
+            HIP_CHECK(hipMemcpyAsync(d_active, active.data(), active.size() * sizeof(int32_t),
+                                     hipMemcpyHostToDevice, stream));
+        }
+        grow(d_work, work_cap, std::max<size_t>(work_items, 1));
+        HIP_CHECK(hipStreamSynchronize(stream));
+        dirty = false;
+    }
+
+    int add(const asg_net_desc& d, int64_t total) {
+        int free_slot = -1;
+        for (int i = 0; i < max_nets; ++i) {
+            if (!slots[i].used) {
+                free_slot = i;
+                break;
+            }
+        }
+        if (free_slot < 0) return -1;
+        const DerivedConn* none = nullptr;
+        const Plan pl = plan_of(d, none);
+        if (budget > 0 && bytes_in_use() + static_cast<int64_t>(pl.bytes) > budget) return -2;
+        wait();
+        NetSlot& s = slots[free_slot];
+        try {
+            build_slot(s, d, total, stream);
+        } catch (...) {
+            release(s);
+            throw;
+        }
+        upload(free_slot);
+        dirty = true;
+        return free_slot;
+    }
+
+    void remove(int slot) {
+        wait();
+        release(slots[slot]);
+        dirty = true;
+    }
+
+    // Advance every used slot by up to n steps (to its total).
+    void step(int64_t n, bool async) {
+        wait();
+        if (n <= 0) return;
+        if (dirty) rebuild_tables();
+        if (n_active == 0) return;
+        int64_t windows = 0;
+        for (int sidx = 0; sidx < max_nets; ++sidx) {
+            NetSlot& s = slots[sidx];
+            if (!s.used) continue;
+            const int64_t target = std::min(s.total, s.step + n);
+            if (target != s.dn.target) {
+                s.dn.target = target;
+                // Only the target field changes on the host side.
+                HIP_CHECK(hipMemcpyAsync(&d_nets[sidx].target, &s.dn.target, sizeof(int64_t),
+                                         hipMemcpyHostToDevice, stream));
+            }
+            const int64_t rem = target - s.step;
+            windows = std::max(windows, (rem + Wg - 1) / Wg);
+            s.step = target;
+        }
+        const int or_grid = 2048;
+        for (int64_t wdx = 0; wdx < windows; ++wdx) {
+            hipLaunchKernelGGL(k_cells, dim3(n_blocks), dim3(kCellBlock), 0, stream, d_nets,
+                               d_blocks, Wg, d_work_count);
+            hipLaunchKernelGGL(k_spikes, dim3(n_active), dim3(kSpikeBlock), 0, stream, d_nets,
+                               d_active, Wg, d_work, d_work_count);
+            hipLaunchKernelGGL(k_or, dim3(or_grid), dim3(kOrBlock), 0, stream, d_nets, d_work,
+                               d_work_count);
+            hipLaunchKernelGGL(k_sum, dim3(n_blocks), dim3(kCellBlock), 0, stream, d_nets,
+                               d_blocks);
+        }
+        HIP_CHECK(hipGetLastError());
+        pending = true;
+        if (!async) wait();
+    }
+
+    // Move the device raster of a slot into its host queue.
+    void drain(int slot) {
+        wait();
+        NetSlot& s = slots[slot];
+        download(slot);
+        const int64_t nr = s.dn.r_head;
+        if (nr > 0) {
+            const size_t old = s.q_step.size();
+            s.q_step.resize(old + static_cast<size_t>(nr));
+            s.q_src.resize(old + static_cast<size_t>(nr));
+            HIP_CHECK(hipMemcpyAsync(s.q_step.data() + old, s.dn.r_step, nr * sizeof(int64_t),
+                                     hipMemcpyDeviceToHost, stream));
+            HIP_CHECK(hipMemcpyAsync(s.q_src.data() + old, s.dn.r_src, nr * sizeof(uint32_t),
+                                     hipMemcpyDeviceToHost, stream));
+            const int64_t zero = 0;
+            HIP_CHECK(hipMemcpyAsync(&d_nets[slot].r_head, &zero, sizeof(int64_t),
+                                     hipMemcpyHostToDevice, stream));
+            HIP_CHECK(hipStreamSynchronize(stream));
+            s.dn.r_head = 0;
+        }
+    }
+
+    bool overflowed(int slot) {
+        int32_t flag = 0;
+        HIP_CHECK(hipMemcpyAsync(&flag, &d_nets[slot].overflow, sizeof(int32_t),
+                                 hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+        return flag != 0;
+    }
+
+    int64_t spikes_out(int slot, int64_t* step, uint32_t* src, int64_t cap) {
+        drain(slot);
+        NetSlot& s = slots[slot];
+        const int64_t avail = static_cast<int64_t>(s.q_step.size() - s.q_pos);
+        const int64_t m = std::min(avail, cap);
+        if (m > 0) {
+            std::memcpy(step, s.q_step.data() + s.q_pos, m * sizeof(int64_t));
+            std::memcpy(src, s.q_src.data() + s.q_pos, m * sizeof(uint32_t));
+            s.q_pos += static_cast<size_t>(m);
+        }
+        if (s.q_pos == s.q_step.size()) {
+            s.q_step.clear();
+            s.q_src.clear();
+            s.q_pos = 0;
+        }
+        return m;
+    }
+
+    void read_state(int slot, int field, int64_t off, int64_t cnt, double* out) {
+        wait();
+        const DevNet& n = slots[slot].dn;
+        const double* src = nullptr;
+        switch (field) {
+            case ASG_FIELD_V: src = n.V; break;
+            case ASG_FIELD_W: src = n.w; break;
+            case ASG_FIELD_G_EX: src = n.gex; break;
+            case ASG_FIELD_DG_EX: src = n.dgex; break;
+            case ASG_FIELD_G_IN: src = n.gin; break;
+            case ASG_FIELD_DG_IN: src = n.dgin; break;
+            case ASG_FIELD_I_SIC: src = n.Isic; break;
+            case ASG_FIELD_CA: src = n.Ca; break;
+            case ASG_FIELD_IP3: src = n.IP3; break;
+            case ASG_FIELD_H: src = n.hh; break;
+            case ASG_FIELD_STP_X: src = n.stp_x; break;
+            case ASG_FIELD_STP_U: src = n.stp_u; break;
+            case ASG_FIELD_STP_TLAST: src = n.stp_tl; break;
+            default: break;
+        }
+        if (cnt > 0) {
+            HIP_CHECK(hipMemcpyAsync(out, src + off, cnt * sizeof(double), hipMemcpyDeviceToHost,
+                                     stream));
+            HIP_CHECK(hipStreamSynchronize(stream));
+        }
+    }
+};
+
+}  // namespace asg
+
+// ------------------------------------------------------------------ C ABI --
+
+using asg::Engine;
+using asg::fail;
+using asg::guarded;
+
+struct asg_net {
+    std::unique_ptr<Engine> eng;
+};
+
+struct asg_batch {
+    std::unique_ptr<Engine> eng;
+};
+
+namespace {
+
+int64_t field_size(const asg::NetSlot& s, int32_t field) {
+    if (field >= ASG_FIELD_CA && field <= ASG_FIELD_H) return s.dn.n_astro;
+    return s.dn.n_neur;
+}
+
+asg_status read_state_common(Engine& e, int32_t slot, int32_t field, int64_t offset,
+                             int64_t count, double* out) {
+    if (slot < 0 || slot >= e.max_nets || !e.slots[slot].used) {
+        return fail(ASG_ERR_INVALID, "no network in this slot");
+    }
+    if (field < 0 || field >= ASG_FIELD_COUNT) return fail(ASG_ERR_INVALID, "unknown state field");
+    const int64_t size = field_size(e.slots[slot], field);
+    if (offset < 0 || count < 0 || offset + count > size || (count > 0 && out == nullptr)) {
+        return fail(ASG_ERR_INVALID, "state range out of bounds");
+    }
+    e.read_state(slot, field, offset, count, out);
+    return ASG_OK;
+}
+
+asg_status counters_common(Engine& e, int32_t slot, asg_counters* out) {
+    if (slot < 0 || slot >= e.max_nets || !e.slots[slot].used || out == nullptr) {
+        return fail(ASG_ERR_INVALID, "no network in this slot");
+    }
+    e.wait();
+    e.download(slot);
+    const asg::DevNet& n = e.slots[slot].dn;
+    out->steps = n.c_steps;
+    out->spikes = n.c_spikes;
+    out->synapse_events = n.c_syn;
+    // SIC gathers: one per (astrocyte, target) pair on every arrival step.
+    int64_t pend = 0;
+    if (n.c_steps > n.d_a2n) {
+        pend = (n.c_steps - 1 - n.d_a2n) / n.sic_interval + 1;
+    }
+    int64_t pairs = 0;
+    if (n.n_neur > 0) {
+        uint32_t last = 0;
+        HIP_CHECK(hipMemcpy(&last, n.sic_row + n.n_neur, sizeof(uint32_t), hipMemcpyDeviceToHost));
+        pairs = last;
+    }
+    out->sic_entries = pend * pairs;
+    return ASG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* asg_last_error(void) { return asg::g_last_error.c_str(); }
+
+asg_status asg_host_fail(asg_status code, const char* what) { return fail(code, what); }
+
+const char* asg_version(void) { return "astrosimgpu-amd 0.1 (gfx950)"; }
+
+int64_t asg_query_footprint(const asg_net_desc* desc) {
+    if (desc == nullptr) return -1;
+    return static_cast<int64_t>(asg::plan_of(*desc, nullptr).bytes);
+}
+
+asg_status asg_net_create(const asg_net_desc* desc, asg_net** out) {
+    if (desc == nullptr || out == nullptr) return fail(ASG_ERR_INVALID, "NULL argument");
+    const std::string err = asg::validate(*desc);
+    if (!err.empty()) return fail(ASG_ERR_INVALID, err);
+    if (desc->precision != ASG_FP64) {
+        return fail(ASG_ERR_INVALID, "asg_net_create: only ASG_FP64 is available here");
+    }
+    return guarded([&]() -> asg_status {
+        auto net = std::make_unique<asg_net>();
+        net->eng = std::make_unique<Engine>(1, 0);
+        if (net->eng->add(*desc, INT64_MAX) != 0) {
+            return fail(ASG_ERR_CAPACITY, "could not place the network");
+        }
+        *out = net.release();
+        return ASG_OK;
+    });
+}
+
+asg_status asg_net_destroy(asg_net* net) {
+    delete net;
+    return ASG_OK;
+}
+
+asg_status asg_net_step(asg_net* net, int64_t n_steps) {
+    if (net == nullptr || n_steps < 0) return fail(ASG_ERR_INVALID, "bad argument");
+    return guarded([&]() -> asg_status {
+        net->eng->step(n_steps, false);
+        if (net->eng->overflowed(0)) {
+            return fail(ASG_ERR_OVERFLOW, "raster buffer overflowed; drain spikes more often");
+        }
+        return ASG_OK;
+    });
+}
+
+int64_t asg_net_current_step(const asg_net* net) {
+    return net == nullptr ? -1 : net->eng->slots[0].step;
+}
+
+asg_status asg_net_spikes(asg_net* net, int64_t* step, uint32_t* source, int64_t capacity,
+                          int64_t* n_out) {
+    if (net == nullptr || n_out == nullptr || capacity < 0 ||
+        (capacity > 0 && (step == nullptr || source == nullptr))) {
+        return fail(ASG_ERR_INVALID, "bad argument");
+    }
+    return guarded([&]() -> asg_status {
+        *n_out = net->eng->spikes_out(0, step, source, capacity);
+        return ASG_OK;
+    });
+}
+
+asg_status asg_net_read_state(asg_net* net, int32_t field, int64_t offset, int64_t count,
+                              double* out) {
+    if (net == nullptr) return fail(ASG_ERR_INVALID, "NULL handle");
+    return guarded([&] { return read_state_common(*net->eng, 0, field, offset, count, out); });
+}
+
+asg_status asg_net_counters(asg_net* net, asg_counters* out) {
+    if (net == nullptr) return fail(ASG_ERR_INVALID, "NULL handle");
+    return guarded([&] { return counters_common(*net->eng, 0, out); });
+}
+
+asg_status asg_batch_create(int32_t max_nets, int64_t budget_bytes, asg_batch** out) {
+    if (max_nets < 1 || out == nullptr || budget_bytes < 0) {
+        return fail(ASG_ERR_INVALID, "bad argument");
+    }
+    return guarded([&]() -> asg_status {
+        auto b = std::make_unique<asg_batch>();
+        b->eng = std::make_unique<Engine>(max_nets, budget_bytes);
+        *out = b.release();
+        return ASG_OK;
+    });
+}
+
+asg_status asg_batch_destroy(asg_batch* batch) {
+    delete batch;
+    return ASG_OK;
+}
+
+asg_status asg_batch_add(asg_batch* batch, const asg_net_desc* desc, int64_t total_steps,
+                         int32_t* slot) {
+    if (batch == nullptr || desc == nullptr || slot == nullptr || total_steps < 0) {
+        return fail(ASG_ERR_INVALID, "bad argument");
+    }
+    const std::string err = asg::validate(*desc);
+    if (!err.empty()) return fail(ASG_ERR_INVALID, err);
+    if (desc->precision != ASG_FP64) {
+        return fail(ASG_ERR_INVALID, "asg_batch_add: only ASG_FP64 is available here");
+    }
+    return guarded([&]() -> asg_status {
+        const int s = batch->eng->add(*desc, total_steps);
+        if (s == -1) return fail(ASG_ERR_CAPACITY, "no free slot");
+        if (s == -2) return fail(ASG_ERR_CAPACITY, "device memory budget exceeded");
+        *slot = s;
+        return ASG_OK;
+    });
+}
+
+asg_status asg_batch_remove(asg_batch* batch, int32_t slot) {
+    if (batch == nullptr || slot < 0 || slot >= batch->eng->max_nets ||
+        !batch->eng->slots[slot].used) {
+        return fail(ASG_ERR_INVALID, "no network in this slot");
+    }
+    return guarded([&]() -> asg_status {
+        batch->eng->remove(slot);
+        return ASG_OK;
+    });
+}
+
+asg_status asg_batch_step(asg_batch* batch, int64_t n_steps) {
+    if (batch == nullptr || n_steps < 0) return fail(ASG_ERR_INVALID, "bad argument");
+    return guarded([&]() -> asg_status {
+        batch->eng->step(n_steps, false);
+        return ASG_OK;
+    });
+}
+
+asg_status asg_batch_step_async(asg_batch* batch, int64_t n_steps) {
+    if (batch == nullptr || n_steps < 0) return fail(ASG_ERR_INVALID, "bad argument");
+    return guarded([&]() -> asg_status {
+        batch->eng->step(n_steps, true);
+        return ASG_OK;
+    });
+}
+
+int64_t asg_batch_net_step(asg_batch* batch, int32_t slot) {
+    if (batch == nullptr || slot < 0 || slot >= batch->eng->max_nets ||
+        !batch->eng->slots[slot].used) {
+        return -1;
+    }
+    return batch->eng->slots[slot].step;
+}
+
+asg_status asg_batch_spikes(asg_batch* batch, int32_t slot, int64_t* step, uint32_t* source,
+                            int64_t capacity, int64_t* n_out) {
+    if (batch == nullptr || slot < 0 || slot >= batch->eng->max_nets ||
+        !batch->eng->slots[slot].used || n_out == nullptr || capacity < 0 ||
+        (capacity > 0 && (step == nullptr || source == nullptr))) {
+        return fail(ASG_ERR_INVALID, "bad argument");
+    }
+    return guarded([&]() -> asg_status {
+        if (batch->eng->overflowed(slot)) {
+            return fail(ASG_ERR_OVERFLOW, "raster buffer overflowed; drain spikes more often");
+        }
+        *n_out = batch->eng->spikes_out(slot, step, source, capacity);
+        return ASG_OK;
+    });
+}
+
+asg_status asg_batch_read_state(asg_batch* batch, int32_t slot, int32_t field, int64_t offset,
+                                int64_t count, double* out) {
+    if (batch == nullptr) return fail(ASG_ERR_INVALID, "NULL handle");
+    return guarded([&] { return read_state_common(*batch->eng, slot, field, offset, count, out); });
+}
+
+asg_status asg_batch_counters(asg_batch* batch, int32_t slot, asg_counters* out) {
+    if (batch == nullptr) return fail(ASG_ERR_INVALID, "NULL handle");
+    return guarded([&] { return counters_common(*batch->eng, slot, out); });
+}
+
+int64_t asg_batch_bytes_in_use(const asg_batch* batch) {
+    return batch == nullptr ? -1 : batch->eng->bytes_in_use();
+}
+
+asg_status asg_device_math(int32_t fn, const double* x, double* y, int64_t n) {
+    if (fn < ASG_MATH_EXP || fn > ASG_MATH_SQRT || n < 0 || (n > 0 && (x == nullptr || y == nullptr))) {
+        return fail(ASG_ERR_INVALID, "bad argument");
+    }
+    if (n == 0) return ASG_OK;
+    return guarded([&]() -> asg_status {
+        double* d = nullptr;
+        HIP_CHECK(hipMalloc(&d, 2 * static_cast<size_t>(n) * sizeof(double)));
+        struct Free {
+            double* p;
+            ~Free() { (void)hipFree(p); }
+        } guard{d};
+        HIP_CHECK(hipMemcpy(d, x, static_cast<size_t>(n) * sizeof(double), hipMemcpyHostToDevice));
+        const int block = 256;
+        const int64_t grid = std::min<int64_t>((n + block - 1) / block, 65536);
+        hipLaunchKernelGGL(asg::k_math, dim3(static_cast<unsigned>(grid)), dim3(block), 0, nullptr,
+                           fn, d, d + n, n);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipMemcpy(y, d + n, static_cast<size_t>(n) * sizeof(double), hipMemcpyDeviceToHost));
+        return ASG_OK;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,244 @@
+"""Python face of the C ABI: build a network from a reference JSON config and
+step it on the device.
+
+Mirrors the reference's host interface (src/network.cpp, network.hpp:58-67):
+``Model`` is ``load_config`` + ``Network::build`` (host side), ``DeviceNetwork``
+is the device-resident step loop of ``Network::run`` (network.cpp:385-468)
+with the spike raster and state read back on demand, ``Batch`` holds many
+independent networks stepped in shared launches.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import _capi as capi
+from ._capi import check, lib
+
+
+def _overrides(seed=-1, sim_time=-1.0, pre_time=-1.0, n_astro=-1, assignments=None):
+    ov = capi.ModelOverrides()
+    ov.seed = int(seed)
+    ov.sim_time = float(sim_time)
+    ov.pre_time = float(pre_time)
+    ov.n_astro = int(n_astro)
+    ov.assignments = assignments.encode() if assignments else None
+    return ov
+
+
+class Model:
+    """A configured and built network on the host.
+
+    ``config`` is a path to a reference-format JSON file, or pass ``text``.
+    ``assignments`` is a dict or "a.b=v;c.d=w" string of JSON overrides applied
+    before the file is read (e.g. {"synapse.w_n2a": 0.3, "N.N_E": 800}).
+    """
+
+    def __init__(self, config: Optional[str] = None, *, text: Optional[str] = None,
+                 seed: int = -1, sim_time: float = -1.0, pre_time: float = -1.0,
+                 n_astro: int = -1, assignments=None):
+        if isinstance(assignments, dict):
+            assignments = ";".join(f"{k}={_json_literal(v)}" for k, v in assignments.items())
+        ov = _overrides(seed, sim_time, pre_time, n_astro, assignments)
+        handle = C.c_void_p()
+        if text is not None:
+            st = lib().asg_model_load_text(text.encode(), C.byref(ov), C.byref(handle))
+        else:
+            if config is None:
+                raise ValueError("give a config path or text")
+            st = lib().asg_model_load(str(config).encode(), C.byref(ov), C.byref(handle))
+        check(st)
+        self._h = handle
+        info = capi.ModelInfo()
+        check(lib().asg_model_get_info(self._h, C.byref(info)))
+        self.info = {name: getattr(info, name) for name, _ in capi.ModelInfo._fields_}
+        self.info["output_dir"] = info.output_dir.decode()
+
+    @property
+    def desc(self):
+        """Pointer to the asg_net_desc (valid while this object lives)."""
+        return lib().asg_model_desc(self._h)
+
+    @property
+    def n_neurons(self) -> int:
+        return self.info["n_exc"] + self.info["n_inh"]
+
+    @property
+    def n_astro(self) -> int:
+        return self.info["n_astro"]
+
+    def stats(self) -> dict:
+        keys = ("n_astro", "n_exc", "n_inh", "n_primary_exc", "n_primary_inh",
+                "n_neuron_to_astro", "n_astro_to_neuron")
+        return {k: self.info[k] for k in keys}
+
+    def csr(self, name: str):
+        """(row_start, target, weight, delay_steps) of one set as numpy copies."""
+        d = self.desc.contents
+        c = getattr(d, name)
+        rows = np.ctypeslib.as_array(c.row_start, shape=(c.n_rows + 1,)).copy()
+        nnz = int(rows[-1])
+        tgt = (np.ctypeslib.as_array(c.target, shape=(nnz,)).copy() if nnz
+               else np.zeros(0, np.uint32))
+        return rows, tgt, c.weight, c.delay_steps
+
+    def footprint(self) -> int:
+        return int(lib().asg_query_footprint(self.desc))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().asg_model_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+def _json_literal(v) -> str:
+    if isinstance(v, bool):
+        return "true" if v else "false"
+    if isinstance(v, str):
+        return '"' + v + '"'
+    return repr(float(v)) if isinstance(v, float) else str(v)
+
+
+class DeviceNetwork:
+    """One network whose whole step runs on the device (FP64)."""
+
+    def __init__(self, model: Model):
+        self.model = model  # keeps the host arrays alive while the desc is read
+        self._h = C.c_void_p()
+        check(lib().asg_net_create(model.desc, C.byref(self._h)))
+
+    def step(self, n_steps: int) -> None:
+        check(lib().asg_net_step(self._h, int(n_steps)))
+
+    @property
+    def current_step(self) -> int:
+        return int(lib().asg_net_current_step(self._h))
+
+    def spikes(self, capacity: int = 1 << 20):
+        """Drain the recorded raster: (steps int64, sources uint32)."""
+        out_s, out_n = [], []
+        step = np.empty(capacity, np.int64)
+        src = np.empty(capacity, np.uint32)
+        n = C.c_int64()
+        while True:
+            check(lib().asg_net_spikes(self._h, step.ctypes.data_as(capi.I64P),
+                                       src.ctypes.data_as(capi.U32P), capacity, C.byref(n)))
+            out_s.append(step[:n.value].copy())
+            out_n.append(src[:n.value].copy())
+            if n.value < capacity:
+                break
+        return np.concatenate(out_s), np.concatenate(out_n)
+
+    def state(self, field: str, offset: int = 0, count: Optional[int] = None) -> np.ndarray:
+        size = self.model.n_astro if field in capi.ASTRO_FIELDS else self.model.n_neurons
+        if count is None:
+            count = size - offset
+        out = np.empty(count, np.float64)
+        check(lib().asg_net_read_state(self._h, capi.FIELDS[field], offset, count,
+                                       out.ctypes.data_as(capi.DBLP)))
+        return out
+
+    def counters(self) -> dict:
+        c = capi.Counters()
+        check(lib().asg_net_counters(self._h, C.byref(c)))
+        return {name: getattr(c, name) for name, _ in capi.Counters._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().asg_net_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Batch:
+    """Independent networks stepped together; slots can be added and removed."""
+
+    def __init__(self, max_nets: int, budget_bytes: int = 0):
+        self._h = C.c_void_p()
+        check(lib().asg_batch_create(int(max_nets), int(budget_bytes), C.byref(self._h)))
+        self.models = {}
+
+    def add(self, model: Model, total_steps: int) -> int:
+        slot = C.c_int32()
+        check(lib().asg_batch_add(self._h, model.desc, int(total_steps), C.byref(slot)))
+        self.models[slot.value] = model
+        return slot.value
+
+    def try_add(self, model: Model, total_steps: int) -> Optional[int]:
+        """Like add, but returns None when the batch has no room now."""
+        slot = C.c_int32()
+        st = lib().asg_batch_add(self._h, model.desc, int(total_steps), C.byref(slot))
+        if st == capi.ASG_ERR_CAPACITY:
+            return None
+        check(st)
+        self.models[slot.value] = model
+        return slot.value
+
+    def remove(self, slot: int) -> None:
+        check(lib().asg_batch_remove(self._h, int(slot)))
+        self.models.pop(slot, None)
+
+    def step(self, n_steps: int, wait: bool = True) -> None:
+        fn = lib().asg_batch_step if wait else lib().asg_batch_step_async
+        check(fn(self._h, int(n_steps)))
+
+    def net_step(self, slot: int) -> int:
+        return int(lib().asg_batch_net_step(self._h, int(slot)))
+
+    def spikes(self, slot: int, capacity: int = 1 << 20):
+        out_s, out_n = [], []
+        step = np.empty(capacity, np.int64)
+        src = np.empty(capacity, np.uint32)
+        n = C.c_int64()
+        while True:
+            check(lib().asg_batch_spikes(self._h, int(slot), step.ctypes.data_as(capi.I64P),
+                                         src.ctypes.data_as(capi.U32P), capacity, C.byref(n)))
+            out_s.append(step[:n.value].copy())
+            out_n.append(src[:n.value].copy())
+            if n.value < capacity:
+                break
+        return np.concatenate(out_s), np.concatenate(out_n)
+
+    def state(self, slot: int, field: str, offset: int = 0, count: Optional[int] = None):
+        m = self.models[slot]
+        size = m.n_astro if field in capi.ASTRO_FIELDS else m.n_neurons
+        if count is None:
+            count = size - offset
+        out = np.empty(count, np.float64)
+        check(lib().asg_batch_read_state(self._h, int(slot), capi.FIELDS[field], offset, count,
+                                         out.ctypes.data_as(capi.DBLP)))
+        return out
+
+    def counters(self, slot: int) -> dict:
+        c = capi.Counters()
+        check(lib().asg_batch_counters(self._h, int(slot), C.byref(c)))
+        return {name: getattr(c, name) for name, _ in capi.Counters._fields_}
+
+    def bytes_in_use(self) -> int:
+        return int(lib().asg_batch_bytes_in_use(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().asg_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+def device_math(fn: str, x: np.ndarray) -> np.ndarray:
+    """Evaluate exp/log/sin/cos/sqrt of the device step on the device."""
+    codes = {"exp": 0, "log": 1, "sin": 2, "cos": 3, "sqrt": 4}
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.empty_like(x)
+    check(lib().asg_device_math(codes[fn], x.ctypes.data_as(capi.DBLP),
+                                y.ctypes.data_as(capi.DBLP), x.size))
+    return y

@@ -1,0 +1,155 @@
+"""The C ABI library loads, exports what include/asg.h declares, and the host
+side (config loading, network construction, argument validation) behaves as
+the reference's host code does. No device calls."""
+
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import REPO, config_path
+
+import astrosimgpu_amd as A
+from astrosimgpu_amd import _capi as capi
+
+
+def declared_functions():
+    with open(os.path.join(REPO, "include", "asg.h")) as f:
+        text = f.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(asg_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_function():
+    names = declared_functions()
+    assert len(names) >= 30
+    out = subprocess.run(["nm", "-D", "--defined-only", A.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (asg_[a-z0-9_]+)$", out, flags=re.M))
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+    lib = A.lib()
+    for n in names:
+        assert hasattr(lib, n)
+
+
+def test_prototype_table_covers_header():
+    assert set(declared_functions()) <= set(capi.PROTOTYPES)
+
+
+def test_struct_layout_matches_c():
+    # sizes computed by the C compiler for the same declarations
+    assert C.sizeof(capi.NeuronParams) == 17 * 8
+    assert C.sizeof(capi.InputParams) == 40
+    assert C.sizeof(capi.Csr) == 40
+    assert C.sizeof(capi.Stp) == 32
+
+
+def test_use_case_stats_match_published_counts():
+    # docs/gpu-port.md:13-15 of the reference: seed 1 synapse counts
+    m = A.Model(config_path("use_case"))
+    s = m.stats()
+    assert (s["n_primary_exc"], s["n_primary_inh"], s["n_neuron_to_astro"]) == (39790, 9944, 8097)
+    assert s["n_astro_to_neuron"] == s["n_neuron_to_astro"]
+    assert m.info["ring_slots"] == 11
+    assert m.info["pre_steps"] == 200000 and m.info["sim_steps"] == 3000000
+
+
+def test_same_seed_same_network_and_different_seed_differs():
+    a = A.Model(config_path("quick"), seed=7)
+    b = A.Model(config_path("quick"), seed=7)
+    c = A.Model(config_path("quick"), seed=8)
+    for name in ("exc", "inh", "n2a", "a2n"):
+        ra, ta, _, _ = a.csr(name)
+        rb, tb, _, _ = b.csr(name)
+        assert np.array_equal(ra, rb) and np.array_equal(ta, tb)
+    assert a.stats() != c.stats()
+
+
+def test_unique_third_out_collapses_pairs():
+    base = A.Model(config_path("quick"))
+    uniq = A.Model(config_path("quick"), assignments={"connectivity.unique_third_out": True})
+    assert uniq.stats()["n_astro_to_neuron"] < base.stats()["n_astro_to_neuron"]
+    assert uniq.stats()["n_neuron_to_astro"] == base.stats()["n_neuron_to_astro"]
+
+
+def test_overrides_apply_after_file():
+    m = A.Model(config_path("quick"), seed=5, sim_time=123.0, pre_time=4.0, n_astro=30)
+    assert m.info["seed"] == 5 and m.info["sim_time"] == 123.0 and m.info["pre_sim_time"] == 4.0
+    assert m.info["n_astro"] == 30
+
+
+def _text(**edits):
+    import json
+    with open(config_path("quick")) as f:
+        cfg = json.load(f)
+    for path, value in edits.items():
+        node = cfg
+        keys = path.split("__")
+        for k in keys[:-1]:
+            node = node.setdefault(k, {})
+        node[keys[-1]] = value
+    return json.dumps(cfg)
+
+
+@pytest.mark.parametrize("edit,message", [
+    ({"simulation__bogus": 1}, "unrecognised keys"),
+    ({"simulation__substeps": 0}, "substeps must be at least 1"),
+    ({"simulation__dt": 0}, "dt must be positive"),
+    ({"simulation__record_every": 0}, "record_every must be at least 1"),
+    ({"synapse__sic_interval": 0}, "sic_interval must be at least 1"),
+    ({"connectivity__pool_size": 0}, "pool_size must be at least 1"),
+    ({"connectivity__pool_type": "ring"}, "unknown pool_type"),
+])
+def test_config_errors_are_reported(edit, message):
+    with pytest.raises(A.AsgError) as e:
+        A.Model(text=_text(**edit))
+    assert e.value.status == capi.ASG_ERR_CONFIG
+    assert message in str(e.value)
+
+
+def test_malformed_json_and_missing_file():
+    with pytest.raises(A.AsgError) as e:
+        A.Model(text="{ oops }")
+    assert "JSON parse error" in str(e.value)
+    with pytest.raises(A.AsgError) as e:
+        A.Model("/nonexistent/config.json")
+    assert "cannot open configuration file" in str(e.value)
+
+
+def test_comments_and_aliases():
+    text = """// leading comment
+    { "N": {"N_A": 4, "N_E": 8, "N_I": 2}, // trailing
+      "astrocyte": {"IP3": 0.3, "Ca_init": 0.1},
+      "input_exc": {"gauss_noise_var": 7.0},
+      "input_inh": {"gauss_noise_var": 7.0, "gauss_noise_std": 3.0} }"""
+    m = A.Model(text=text)
+    d = m.desc.contents
+    assert d.astro.IP3_init == 0.3
+    assert d.input_exc.gauss_noise_std == 7.0
+    assert d.input_inh.gauss_noise_std == 3.0
+    # noise_dt <= 0 resolves to 10 * dt (parameters.cpp:169-173)
+    assert d.input_exc.noise_dt == 10.0 * d.dt
+
+
+def test_invalid_descriptions_rejected_before_the_device():
+    m = A.Model(config_path("quick"))
+    d = capi.NetDesc.from_buffer_copy(m.desc.contents)
+    d.substeps = 0
+    h = C.c_void_p()
+    st = A.lib().asg_net_create(C.byref(d), C.byref(h))
+    assert st == capi.ASG_ERR_INVALID
+    assert b"substeps" in A.lib().asg_last_error()
+    d = capi.NetDesc.from_buffer_copy(m.desc.contents)
+    d.exc.delay_steps = 0
+    assert A.lib().asg_net_create(C.byref(d), C.byref(h)) == capi.ASG_ERR_INVALID
+    assert A.lib().asg_net_create(None, C.byref(h)) == capi.ASG_ERR_INVALID
+
+
+def test_footprint_of_sparse_network_fits_one_gpu():
+    m = A.Model(config_path("paper_sparse"))
+    fp = m.footprint()
+    assert 100e6 < fp < 2e9

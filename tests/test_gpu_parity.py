@@ -1,0 +1,212 @@
+"""Device step against the oracle and the reference build's rasters.
+
+Bar: the spike raster is identical (integer (step, neuron) pairs) and every
+state array is bitwise equal to the oracle's after the same number of steps
+(that is stricter than the 1e-12 relative tolerance asked of FP64 state).
+"""
+
+import numpy as np
+import pytest
+
+from _cases import model_for, raster_hash, recorded
+from conftest import config_path, golden, golden_exists
+
+import astrosimgpu_amd as A
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+STATE_FIELDS = ("V", "w", "g_ex", "dg_ex", "g_in", "dg_in", "I_sic", "Ca", "IP3", "h")
+
+
+def _announce(what):
+    print(f"[gpu case] {what}", flush=True)
+
+
+@pytest.mark.parametrize("fn,lo,hi", [
+    ("exp", -745.0, 709.0), ("exp", -20.0, 30.0), ("log", 1e-300, 1.0), ("log", 1.0, 3000.0),
+    ("sin", 0.0, 6.3), ("cos", 0.0, 6.3), ("sqrt", 0.0, 1e6),
+])
+def test_device_math_is_host_amath_bit_for_bit(fn, lo, hi):
+    _announce(f"device math {fn}")
+    rng = np.random.default_rng(20261203)
+    x = rng.uniform(lo, hi, 1 << 20)
+    if fn == "log":
+        x = np.concatenate([x, np.exp(rng.uniform(-700, 700, 1 << 18))])
+    host = oracle.math(fn, x, oracle.LIBM_AMATH)
+    dev = A.device_math(fn, x)
+    assert np.array_equal(host.view(np.uint64), dev.view(np.uint64))
+
+
+def test_device_exp_log_equal_c_library():
+    rng = np.random.default_rng(7)
+    x = rng.uniform(-30, 30, 1 << 20)
+    assert np.array_equal(A.device_math("exp", x).view(np.uint64),
+                          oracle.math("exp", x, oracle.LIBM_GLIBC).view(np.uint64))
+    y = np.exp(rng.uniform(-60, 10, 1 << 20))
+    assert np.array_equal(A.device_math("log", y).view(np.uint64),
+                          oracle.math("log", y, oracle.LIBM_GLIBC).view(np.uint64))
+
+
+def _run_device(m, steps, chunk):
+    net = A.DeviceNetwork(m)
+    done = 0
+    parts_s, parts_n = [], []
+    while done < steps:
+        k = min(chunk, steps - done)
+        net.step(k)
+        done += k
+        s, n = net.spikes()
+        parts_s.append(s)
+        parts_n.append(n)
+    return net, np.concatenate(parts_s), np.concatenate(parts_n)
+
+
+CASES = ["quick_s1", "quick_s7", "use_case_2s", "bursting_2s", "random_pool_1s", "sparse_200ms"]
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_device_equals_oracle_bitwise(case):
+    _announce(case)
+    m, g = model_for(case)
+    steps = g["total_steps"]
+    net, s, n = _run_device(m, steps, chunk=steps // 3 + 7)
+    o = oracle.OracleRun(m.desc, oracle.LIBM_AMATH)
+    o.step(steps)
+    os_, on = o.spikes()
+    assert np.array_equal(s, os_) and np.array_equal(n, on)
+    for f in STATE_FIELDS:
+        assert np.array_equal(net.state(f).view(np.uint64), o.state(f).view(np.uint64)), f
+    c = net.counters()
+    assert c["steps"] == steps and c["spikes"] == len(s)
+
+
+@pytest.mark.parametrize("case", CASES + ["sparse_1s_s2"])
+def test_device_equals_reference_raster(case):
+    _announce(case)
+    m, g = model_for(case)
+    _, s, n = _run_device(m, g["total_steps"], chunk=997)
+    s, n = recorded(s, n, g["first_recorded_step"])
+    assert np.array_equal(s, np.array(g["steps"], np.int64))
+    assert np.array_equal(n, np.array(g["neurons"], np.uint32))
+
+
+@pytest.mark.parametrize("case", ["sparse_10s_s1", "sparse_10s_s2", "sparse_10s_s3"])
+def test_sparse_10s_raster_equals_reference_build(case):
+    """The 10,000-neuron Sparse network, 1 s transient + 10 s recorded:
+    the device raster hashes to the reference build's."""
+    if not golden_exists(case):
+        pytest.skip("full-length fixture not generated")
+    _announce(case)
+    m, g = model_for(case)
+    _, s, n = _run_device(m, g["total_steps"], chunk=10000)
+    s, n = recorded(s, n, g["first_recorded_step"])
+    assert len(s) == g["n_spikes"]
+    counts = np.bincount((s - g["first_recorded_step"]) // 1000,
+                         minlength=len(g["counts_per_1000"]))
+    assert counts.tolist() == g["counts_per_1000"]
+    assert raster_hash(s, n) == g["raster_sha256"]
+
+
+def test_windowed_and_unwindowed_stepping_agree():
+    """Chunk boundaries that cut windows (odd step counts) change nothing."""
+    _announce("chunking")
+    m = A.Model(config_path("quick"), seed=3)
+    a, sa, na = _run_device(m, 3000, chunk=3000)
+    b, sb, nb = _run_device(m, 3000, chunk=7)
+    c, sc, nc = _run_device(m, 300, chunk=1)
+    assert np.array_equal(sc, sa[sa < 300]) and np.array_equal(nc, na[sa < 300])
+    assert np.array_equal(sa, sb) and np.array_equal(na, nb)
+    for f in STATE_FIELDS:
+        assert np.array_equal(a.state(f), b.state(f)), f
+
+
+def test_batch_members_equal_solo_runs():
+    """Networks of different size and seed stepped in one batch give the
+    same bits as each run alone."""
+    _announce("batch")
+    specs = [("quick", 1), ("quick", 9), ("use_case", 2), ("random_pool", 3)]
+    models = [A.Model(config_path(c), seed=s) for c, s in specs]
+    steps = [1500, 2200, 1800, 900]
+    batch = A.Batch(8)
+    slots = [batch.add(m, t) for m, t in zip(models, steps)]
+    batch.step(700)
+    batch.step(3000)
+    for m, slot, t in zip(models, slots, steps):
+        assert batch.net_step(slot) == t
+        bs, bn = batch.spikes(slot)
+        o = oracle.OracleRun(m.desc, oracle.LIBM_AMATH)
+        o.step(t)
+        os_, on = o.spikes()
+        assert np.array_equal(bs, os_) and np.array_equal(bn, on)
+        for f in ("V", "Ca", "IP3", "I_sic"):
+            assert np.array_equal(batch.state(slot, f), o.state(f)), f
+
+
+def test_astro_dropin_matches_engine_cell_math():
+    """asg_astro_* (the astrocyte_cuda.hpp replacement) against the oracle's
+    astrocyte update: an astrocyte-only network driven by its Poisson input,
+    replayed through the drop-in with the same per-step inputs."""
+    _announce("astro drop-in")
+    text = """{ "N": {"N_A": 3000, "N_E": 0, "N_I": 0},
+      "astrocyte": {"IP3": 0.3, "Ca_tot": 1.8958264782990153, "IP3_0": 0.010925583871830699,
+                    "tau_IP3": 1058.7589568400845, "delta_IP3": 0.04911592657233997},
+      "randomize_astro": {"enabled": true, "lower": 0.9, "upper": 1.05, "var": 0.1},
+      "input_astro": {"poiss_rate": 300.0, "poiss_weight": 1.0, "gauss_noise_std": 0.0} }"""
+    m = A.Model(text=text)
+    d = m.desc.contents
+    n = d.n_astro
+    steps = 400
+    o = oracle.OracleRun(m.desc, oracle.LIBM_AMATH)
+    import ctypes as C
+    from astrosimgpu_amd import _capi as capi
+    lib = A.lib()
+    arr = lambda ptr: np.ctypeslib.as_array(ptr, shape=(n,)).copy()
+    Ca = np.full(n, d.astro.Ca_init)
+    IP3 = np.full(n, d.astro.IP3_init)
+    hh = np.full(n, d.astro.h_init)
+    Ca_tot, IP3_0, tau, delta = arr(d.Ca_tot), arr(d.IP3_0), arr(d.tau_IP3), arr(d.delta_IP3)
+    P = lambda a: a.ctypes.data_as(capi.DBLP)
+    h = lib.asg_astro_create(n, P(Ca), P(IP3), P(hh), P(Ca_tot), P(IP3_0), P(tau), P(delta))
+    lam = d.input_astro.poiss_rate * d.dt * 1e-3
+    seed = d.seed ^ 0xC2B2AE3D27D4EB4F
+    for step in range(steps):
+        ev = _poisson_counts(seed, step, n, lam)
+        ip3_in = d.input_astro.poiss_weight * ev.astype(np.float64)
+        ip3_in[ev == 0] = 0.0
+        lib.asg_astro_push_input(h, P(ip3_in))
+        lib.asg_astro_update(h, C.byref(d.astro.c), d.dt, 1, 0.0, 1, 0.0, 0, 0)
+    lib.asg_astro_destroy(h, P(Ca), P(IP3), P(hh))
+    o.step(steps)
+    assert np.array_equal(Ca, o.state("Ca"))
+    assert np.array_equal(IP3, o.state("IP3"))
+    assert np.array_equal(hh, o.state("h"))
+
+
+def _poisson_counts(seed, step, n, lam):
+    """Knuth Poisson of fresh counter generators (reference rng.hpp:107-124),
+    vectorised in numpy with the splitmix64 finaliser (rng.hpp:26-33)."""
+    M = np.uint64(0xFFFFFFFFFFFFFFFF)
+    stream = np.uint64(step) * np.uint64(1000003) + np.arange(n, dtype=np.uint64)
+    limit = oracle.math("exp", np.array([-lam]), oracle.LIBM_AMATH)[0]
+    counts = np.zeros(n, np.int64)
+    prod = None
+    alive = np.ones(n, bool)
+    c = 0
+    with np.errstate(over="ignore"):
+        while alive.any():
+            z = (np.uint64(seed) + stream * np.uint64(0x9E3779B97F4A7C15)
+                 + np.uint64(c) * np.uint64(0xBF58476D1CE4E5B9)) & M
+            z = ((z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)) & M
+            z = ((z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)) & M
+            z = z ^ (z >> np.uint64(31))
+            u = (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+            if prod is None:
+                prod = u
+            else:
+                prod = np.where(alive, prod * u, prod)
+            if c > 0:
+                counts += alive
+            alive = alive & (prod > limit)
+            c += 1
+    return counts
