@@ -86,6 +86,10 @@ class Counters(C.Structure):
                 ("sic_entries", C.c_int64)]
 
 
+class KernelTimes(C.Structure):
+    _fields_ = [("ms", C.c_double * 4), ("windows", C.c_int64)]
+
+
 class ModelOverrides(C.Structure):
     _fields_ = [("seed", C.c_int64), ("sim_time", C.c_double), ("pre_time", C.c_double),
                 ("n_astro", C.c_int64), ("assignments", C.c_char_p)]
@@ -149,6 +153,10 @@ PROTOTYPES = {
     "asg_batch_counters": (C.c_int, [P, C.c_int32, C.POINTER(Counters)]),
     "asg_batch_bytes_in_use": (C.c_int64, [P]),
     "asg_device_math": (C.c_int, [C.c_int32, DBLP, DBLP, C.c_int64]),
+    "asg_net_set_profiling": (C.c_int, [P, C.c_int32]),
+    "asg_net_kernel_times": (C.c_int, [P, C.POINTER(KernelTimes)]),
+    "asg_batch_set_profiling": (C.c_int, [P, C.c_int32]),
+    "asg_batch_kernel_times": (C.c_int, [P, C.POINTER(KernelTimes)]),
 }
 
 

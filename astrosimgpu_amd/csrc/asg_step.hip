@@ -7,22 +7,25 @@
 // every cell is independent of every other cell and the per-cell state stays
 // in registers for W steps. A window is four launches:
 //
-//   k_cells   one lane per neuron / astrocyte, W steps each:
-//             arrivals from the ring slot, Poisson drive, noise, RK4, spike
-//             test (a wavefront ballot writes one 64-bit spike word per 64
-//             neurons and step), astrocyte update and its SIC factor.
-//   k_spikes  one workgroup per network: prefix scan of the spike words into
-//             a source-ordered spike list per step, Tsodyks-Markram update
-//             per spiking source in time order, raster append.
-//   k_or      one workgroup per spike: marks bit j of the step's spike list
-//             in a per-target 64-bit mask for every synapse of the row.
-//   k_sum     one lane per target: adds the marked spikes' weights into the
-//             ring slot in ascending spike (= source) order.
+//   k_cells    one lane per neuron / astrocyte, W steps each: arrivals from
+//              the ring slots, Poisson drive, noise, RK4, spike test (a
+//              wavefront ballot writes one 64-bit spike word per 64 neurons
+//              and step), astrocyte update and its SIC factor.
+//   k_compact  one workgroup per (network, step): prefix scan of the spike
+//              words into a source-ordered spike list, one work item per
+//              spike.
+//   k_deliver  lanes per (neuron, step): the SIC each neuron receives from
+//              that step's exchange, and each spiking source's
+//              Tsodyks-Markram update in time order and raster entries;
+//              workgroups per spike: set the spike's bit in the per-target
+//              64-bit mask of every synapse of its rows.
+//   k_sum      one lane per (target, step): adds the marked spikes' weights
+//              into the ring slot in ascending spike (= source) order.
 //
-// The last two replace the reference's sequential scatter (network.cpp:
-// 249-279) with an order-independent OR followed by an ordered sum, so the
-// floating-point accumulation order per target is the reference's (source
-// ascending, then CSR order) without atomics on doubles.
+// Marking + ordered sum replace the reference's sequential scatter
+// (network.cpp:249-279) with an order-independent OR followed by a sum in the
+// reference's floating-point order (source ascending, then CSR order),
+// without atomics on doubles.
 //
 // Exact reformulations (each gives the reference's bits, see below):
 //  * STP state is kept per source, not per synapse: every synapse of a row is
@@ -104,9 +107,7 @@ asg_status guarded(F&& f) {
 
 // --------------------------------------------------------------- layout ----
 
-constexpr int kCellBlock = 64;     // one wavefront per k_cells / k_sum block
-constexpr int kSpikeBlock = 256;   // k_spikes workgroup
-constexpr int kOrBlock = 256;      // k_or workgroup
+constexpr int kCellBlock = 64;     // one wavefront per block of the per-cell kernels
 constexpr int kWindowCap = 16;     // longest window (bounds scratch memory)
 
 struct InputK {
@@ -128,14 +129,15 @@ struct DevNet {
     uint32_t n_exc, n_inh, n_neur, n_astro;
     uint32_t words;        // ceil(n_neur / 64)
     uint32_t cE, cI;       // 64-spike chunks needed for all exc / inh sources
-    int32_t S, Sf;         // ring slots, SIC-factor ring slots
+    int32_t S;             // ring slots: longest delay + longest window
+    int32_t Wmax;          // longest window this network's scratch holds
     int32_t d_exc, d_inh, d_n2a, d_a2n;
     int32_t sic_interval, substeps;
-    int32_t win_W;         // length of the window k_spikes just closed
+    int32_t win_W;         // length of the current window
     int32_t overflow;      // raster buffer overflowed
     int64_t step;          // first step of the next window
     int64_t target;        // step this call stops at
-    int64_t win_T0;        // first step of the window k_spikes just closed
+    int64_t win_T0;        // first step of the current window
     uint64_t seed;
     double dt, h, half_h, h_sixth;
     NeuronK pk[2];
@@ -151,15 +153,22 @@ struct DevNet {
     // astrocyte state and parameters
     double *Ca, *IP3, *hh;
     const double *Ca_tot, *IP3_0, *tau_IP3, *delta_IP3;
-    // rings: [S][n_neur] / [S][n_astro]; factors [Sf][n_astro]
-    double *ring_exc, *ring_inh, *ring_astro, *fac;
+    // rings [S][targets]: exc, inh and SIC per neuron, n2a per astrocyte
+    double *ring_exc, *ring_inh, *ring_sic, *ring_astro;
+    double* fac;           // [Wmax][n_astro]: SIC factor of each step of the window
     double *stp_x, *stp_u, *stp_tl;
     // connectivity
     const uint32_t *exc_row, *exc_tgt, *inh_row, *inh_tgt;
     const uint32_t *n2a_row, *n2a_tgt, *n2a_mult, *n2a_raw_row;
-    const uint32_t *sic_row, *sic_src, *sic_mult;
+    // SIC gather, column-major: pair q of target t at [q * n_neur + t];
+    // multiplicity 0 marks padding (a target's pairs come first, astrocytes
+    // ascending)
+    uint32_t sic_width;
+    const uint32_t *sic_src, *sic_mult;
+    int64_t sic_pairs;
     // window scratch
     uint64_t* spk_bits;    // [Wmax][words]
+    uint32_t* spk_woff;    // [Wmax][words]: spikes before each word in its step
     uint32_t* spk_list;    // [Wmax][n_neur]
     uint32_t* spk_n;       // [Wmax][2]: total, excitatory
     double* spk_c;         // [Wmax][n_neur][2]: exc-or-inh weight, n2a weight
@@ -176,14 +185,7 @@ struct CellBlock {
     int32_t net;
     int32_t kind;   // 0 neurons, 1 astrocytes
     uint32_t first;
-    uint32_t pad;
-};
-
-struct WorkItem {
-    int32_t net;
-    int32_t k;      // step within the window
-    uint32_t j;     // index in that step's spike list
-    uint32_t pad;
+    uint32_t lead;  // 1 for exactly one block per network
 };
 
 __device__ inline int window_len(const DevNet& n, int32_t Wg) {
@@ -191,16 +193,36 @@ __device__ inline int window_len(const DevNet& n, int32_t Wg) {
     return rem <= 0 ? 0 : (rem < Wg ? static_cast<int>(rem) : Wg);
 }
 
+__device__ inline void add_i64(int64_t* p, int64_t v) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(p), static_cast<unsigned long long>(v));
+}
+
 // --------------------------------------------------------------- kernels ---
 
-__device__ void neuron_window(DevNet& n, uint32_t i, int64_t T0, int W) {
-    const bool valid = i < n.n_neur;
-    const int pi = (i < n.n_exc) ? 0 : 1;
-    const NeuronK& p = n.pk[pi];
-    const InputK& in = n.in[pi];
+__device__ void neuron_window(const DevNet& n, uint32_t i, int64_t T0, int W) {
+    // Everything the loop reads is copied into registers once; the rings are
+    // reached through restrict pointers so their stores cannot force reloads.
     const uint32_t N = n.n_neur;
+    const bool valid = i < N;
+    const int pi = (i < n.n_exc) ? 0 : 1;
+    const NeuronK p = n.pk[pi];
+    const InputK in = n.in[pi];
+    const int S = n.S;
+    const int d_a2n = n.d_a2n;
+    const int sic_interval = n.sic_interval;
+    const int substeps = n.substeps;
+    const uint64_t seed = n.seed;
+    const double hs = n.h, half_h = n.half_h, h_sixth = n.h_sixth;
+    const uint32_t words = n.words;
+    double* __restrict__ ring_exc = n.ring_exc;
+    double* __restrict__ ring_inh = n.ring_inh;
+    const double* __restrict__ ring_sic = n.ring_sic;
+    uint64_t* __restrict__ spk_bits = n.spk_bits;
+    const uint64_t salt_shared = pi == 0 ? kSaltNoiseSharedExc : kSaltNoiseSharedInh;
+
     double V = 0, w = 0, gex = 0, dgex = 0, gin = 0, dgin = 0, Isic = 0, Vr = 0, b = 0;
     int refr = 0;
+    double ex_n = 0, ih_n = 0, sic_n = 0;  // ring values of the next step
     if (valid) {
         V = n.V[i];
         w = n.w[i];
@@ -212,6 +234,10 @@ __device__ void neuron_window(DevNet& n, uint32_t i, int64_t T0, int W) {
         refr = n.refr[i];
         Vr = n.V_reset[i];
         b = n.b[i];
+        const size_t s0 = static_cast<size_t>(T0 % S) * N + i;
+        ex_n = ring_exc[s0];
+        ih_n = ring_inh[s0];
+        sic_n = ring_sic[s0];
     }
     // The independent noise of one noise interval is one draw; keep it.
     int64_t noise_idx = -1;
@@ -221,19 +247,24 @@ __device__ void neuron_window(DevNet& n, uint32_t i, int64_t T0, int W) {
         const int64_t T = T0 + k;
         bool spiked = false;
         if (valid) {
-            const size_t slot = static_cast<size_t>(T % n.S) * N + i;
+            const size_t slot = static_cast<size_t>(T % S) * N + i;
+            const double ex = ex_n, ih = ih_n, sic = sic_n;
+            if (k + 1 < W) {
+                const size_t nx = static_cast<size_t>((T + 1) % S) * N + i;
+                ex_n = ring_exc[nx];
+                ih_n = ring_inh[nx];
+                sic_n = ring_sic[nx];
+            }
             double exc_in = 0.0, inh_in = 0.0;
             // apply_arrivals, network.cpp:301-327
-            const double ex = n.ring_exc[slot];
             if (ex != 0.0) {
                 if (ex > 0.0) {
                     exc_in += ex;
                 } else if (ex < 0.0) {
                     inh_in -= ex;
                 }
-                n.ring_exc[slot] = 0.0;
+                ring_exc[slot] = 0.0;
             }
-            const double ih = n.ring_inh[slot];
             if (ih != 0.0) {
                 const double wgt = -ih;
                 if (wgt > 0.0) {
@@ -241,25 +272,16 @@ __device__ void neuron_window(DevNet& n, uint32_t i, int64_t T0, int W) {
                 } else if (wgt < 0.0) {
                     inh_in -= wgt;
                 }
-                n.ring_inh[slot] = 0.0;
+                ring_inh[slot] = 0.0;
             }
-            if (T >= n.d_a2n && (T - n.d_a2n) % n.sic_interval == 0) {
-                const double* f = n.fac + static_cast<size_t>((T - n.d_a2n) % n.Sf) * n.n_astro;
-                double acc = 0.0;
-                for (uint32_t q = n.sic_row[i]; q < n.sic_row[i + 1]; ++q) {
-                    const double fa = f[n.sic_src[q]];
-                    if (fa != 0.0) {
-                        const double v = n.w_a2n * fa;
-                        for (uint32_t r = n.sic_mult[q]; r > 0; --r) {
-                            acc += v;
-                        }
-                    }
-                }
-                Isic = acc;
+            // A slot carries a new SIC exactly when an exchange was sent
+            // d_a2n steps earlier; otherwise the last value is held.
+            if (T >= d_a2n && (T - d_a2n) % sic_interval == 0) {
+                Isic = sic;
             }
             // NeuronPopulation::update, neuron.cpp:143-170
             if (in.poiss_rate > 0.0) {
-                const int ev = rng_poisson_fresh(n.seed ^ kSaltNeuronPoisson,
+                const int ev = rng_poisson_fresh(seed ^ kSaltNeuronPoisson,
                                                  static_cast<uint64_t>(T) * kStreamStride + i,
                                                  in.lambda, in.limit);
                 if (ev > 0) {
@@ -274,14 +296,13 @@ __device__ void neuron_window(DevNet& n, uint32_t i, int64_t T0, int W) {
                     uint64_t ctr = 0;
                     if (in.independent) {
                         noise_val = 0.0 + in.noise_std *
-                                              rng_normal_fresh(n.seed ^ kSaltNeuronNoise,
+                                              rng_normal_fresh(seed ^ kSaltNeuronNoise,
                                                                static_cast<uint64_t>(idx) *
                                                                        kNoiseStride + i,
                                                                ctr);
                     } else {
-                        const uint64_t salt = pi == 0 ? kSaltNoiseSharedExc : kSaltNoiseSharedInh;
                         noise_val = 0.0 + in.noise_std *
-                                              rng_normal_fresh(n.seed ^ salt,
+                                              rng_normal_fresh(seed ^ salt_shared,
                                                                static_cast<uint64_t>(idx), ctr);
                     }
                 }
@@ -293,12 +314,12 @@ __device__ void neuron_window(DevNet& n, uint32_t i, int64_t T0, int W) {
             if (inh_in != 0.0) {
                 dgin += inh_in * p.psc_in;
             }
-            spiked = adex_step(p, Vr, b, I_ext, n.h, n.half_h, n.h_sixth, n.substeps, V, w, gex,
-                               dgex, gin, dgin, refr);
+            spiked = adex_step(p, Vr, b, I_ext, hs, half_h, h_sixth, substeps, V, w, gex, dgex,
+                               gin, dgin, refr);
         }
         const uint64_t mask = __ballot(spiked);
         if (lane == 0 && (i & ~63u) < N) {
-            n.spk_bits[static_cast<size_t>(k) * n.words + (i >> 6)] = mask;
+            spk_bits[static_cast<size_t>(k) * words + (i >> 6)] = mask;
         }
     }
     if (valid) {
@@ -313,29 +334,40 @@ __device__ void neuron_window(DevNet& n, uint32_t i, int64_t T0, int W) {
     }
 }
 
-__device__ void astro_window(DevNet& n, uint32_t a, int64_t T0, int W) {
-    if (a >= n.n_astro) {
+__device__ void astro_window(const DevNet& n, uint32_t a, int64_t T0, int W) {
+    const uint32_t NA = n.n_astro;
+    if (a >= NA) {
         return;
     }
-    const uint32_t NA = n.n_astro;
-    const AstroPopK& ap = n.ap;
+    const AstroPopK ap = n.ap;
+    const int S = n.S;
+    const int sic_interval = n.sic_interval;
+    const int substeps = n.substeps;
+    const uint64_t seed = n.seed;
+    const double hs = n.h, half_h = n.half_h, h_sixth = n.h_sixth;
+    double* __restrict__ ring_astro = n.ring_astro;
+    double* __restrict__ fac = n.fac;
     double Ca = n.Ca[a], IP3 = n.IP3[a], hh = n.hh[a];
     const double Ca_tot = n.Ca_tot[a], IP3_0 = n.IP3_0[a], tau = n.tau_IP3[a],
                  delta = n.delta_IP3[a];
+    double v_n = ring_astro[static_cast<size_t>(T0 % S) * NA + a];
     int64_t noise_idx = -1;
     double noise = 0.0;
     for (int k = 0; k < W; ++k) {
         const int64_t T = T0 + k;
-        const size_t slot = static_cast<size_t>(T % n.S) * NA + a;
+        const size_t slot = static_cast<size_t>(T % S) * NA + a;
+        const double v = v_n;
+        if (k + 1 < W) {
+            v_n = ring_astro[static_cast<size_t>((T + 1) % S) * NA + a];
+        }
         double ip3_in = 0.0;
-        const double v = n.ring_astro[slot];
         if (v != 0.0) {
             ip3_in += v;
-            n.ring_astro[slot] = 0.0;
+            ring_astro[slot] = 0.0;
         }
         // drive_astrocytes, network.cpp:340-355
         if (ap.in.poiss_rate > 0.0) {
-            const int ev = rng_poisson_fresh(n.seed ^ kSaltAstroPoisson,
+            const int ev = rng_poisson_fresh(seed ^ kSaltAstroPoisson,
                                              static_cast<uint64_t>(T) * kStreamStride + a,
                                              ap.in.lambda, ap.in.limit);
             if (ev > 0) {
@@ -347,17 +379,17 @@ __device__ void astro_window(DevNet& n, uint32_t a, int64_t T0, int W) {
             const int64_t idx = T / ap.in.steps_per_noise;
             if (idx != noise_idx) {
                 noise_idx = idx;
-                const uint64_t nseed = n.seed ^ kSaltAstroNoise;
+                const uint64_t nseed = seed ^ kSaltAstroNoise;
                 const uint64_t stream = ap.in.independent
                                             ? static_cast<uint64_t>(idx) * kStreamStride + a
                                             : static_cast<uint64_t>(idx);
                 noise = ap.in.noise_std * rng_normal_clamped(nseed, stream);
             }
         }
-        astro_step(ap.k, Ca_tot, IP3_0, tau, delta, ip3_in, noise, n.h, n.half_h, n.h_sixth,
-                   n.substeps, Ca, IP3, hh);
-        if (T % n.sic_interval == 0) {
-            n.fac[static_cast<size_t>(T % n.Sf) * NA + a] = sic_factor(Ca, ap.SIC_th, ap.SIC_scale);
+        astro_step(ap.k, Ca_tot, IP3_0, tau, delta, ip3_in, noise, hs, half_h, h_sixth, substeps,
+                   Ca, IP3, hh);
+        if (T % sic_interval == 0) {
+            fac[static_cast<size_t>(k) * NA + a] = sic_factor(Ca, ap.SIC_th, ap.SIC_scale);
         }
     }
     n.Ca[a] = Ca;
@@ -365,18 +397,20 @@ __device__ void astro_window(DevNet& n, uint32_t a, int64_t T0, int W) {
     n.hh[a] = hh;
 }
 
+// Window pass 1: every cell of every network advances W steps.
 __global__ __launch_bounds__(kCellBlock) void k_cells(DevNet* nets, const CellBlock* blocks,
-                                                      int32_t Wg, uint32_t* work_count) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        *work_count = 0;
-    }
+                                                      int32_t Wg) {
     const CellBlock cb = blocks[blockIdx.x];
     DevNet& n = nets[cb.net];
     const int W = window_len(n, Wg);
+    const int64_t T0 = n.step;
+    if (cb.lead && threadIdx.x == 0) {
+        n.win_T0 = T0;
+        n.win_W = W;
+    }
     if (W == 0) {
         return;
     }
-    const int64_t T0 = n.step;
     const uint32_t i = cb.first + threadIdx.x;
     if (cb.kind == 0) {
         neuron_window(n, i, T0, W);
@@ -385,176 +419,252 @@ __global__ __launch_bounds__(kCellBlock) void k_cells(DevNet* nets, const CellBl
     }
 }
 
-// Block-wide exclusive scan of one uint32 per thread (kSpikeBlock threads).
-__device__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_tmp, uint32_t& total) {
-    const uint32_t tid = threadIdx.x;
-    s_tmp[tid] = v;
-    __syncthreads();
-    for (uint32_t off = 1; off < kSpikeBlock; off <<= 1) {
-        const uint32_t add = tid >= off ? s_tmp[tid - off] : 0u;
-        __syncthreads();
-        s_tmp[tid] += add;
-        __syncthreads();
+// Exclusive prefix sum over the 64 lanes of a wavefront (no barriers).
+__device__ inline uint32_t wave_exclusive_scan(uint32_t v, uint32_t& total) {
+    const int lane = static_cast<int>(threadIdx.x & 63);
+    uint32_t x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) {
+            x += y;
+        }
     }
-    total = s_tmp[kSpikeBlock - 1];
-    const uint32_t incl = s_tmp[tid];
-    __syncthreads();
-    return incl - v;
+    total = __shfl(x, 63, 64);
+    return x - v;
 }
 
-__global__ __launch_bounds__(kSpikeBlock) void k_spikes(DevNet* nets, const int32_t* active,
-                                                        int32_t Wg, WorkItem* work,
-                                                        uint32_t* work_count) {
-    __shared__ uint32_t s_tmp[kSpikeBlock];
-    __shared__ uint64_t s_syn[kSpikeBlock];
-    const int32_t net = active[blockIdx.x];
-    DevNet& n = nets[net];
-    const int W = window_len(n, Wg);
-    const int64_t T0 = n.step;
-    const uint32_t tid = threadIdx.x;
+// Window pass 2: one wavefront per (network, step) turns that step's spike
+// words into the source-ordered spike list (the reference sorts each step's
+// spikes by source, neuron.cpp:243-248) and each word's spike offset.
+__global__ __launch_bounds__(64) void k_compact(DevNet* nets, const int32_t* active) {
+    DevNet& n = nets[active[blockIdx.x]];
+    const int k = static_cast<int>(blockIdx.y);
+    if (k >= n.win_W) {
+        return;
+    }
+    const uint32_t lane = threadIdx.x;
     const uint32_t words = n.words;
-    const uint32_t per = (words + kSpikeBlock - 1) / kSpikeBlock;
-    const uint32_t w_lo = min(words, tid * per), w_hi = min(words, w_lo + per);
-    const uint32_t N = n.n_neur;
-    int64_t head = n.r_head;
-    int64_t tot_spikes = 0;
-    uint64_t syn_local = 0;
-    for (int k = 0; k < W; ++k) {
-        const int64_t T = T0 + k;
-        const uint64_t* bits = n.spk_bits + static_cast<size_t>(k) * words;
-        uint32_t cnt = 0, cntE = 0;
-        for (uint32_t q = w_lo; q < w_hi; ++q) {
-            const uint64_t m = bits[q];
-            cnt += __popcll(m);
-            const uint32_t base = q * 64;
-            if (base + 64 <= n.n_exc) {
-                cntE += __popcll(m);
-            } else if (base < n.n_exc) {
-                cntE += __popcll(m & ((1ULL << (n.n_exc - base)) - 1ULL));
-            }
+    const uint32_t per = (words + 63) / 64;
+    const uint32_t w_lo = min(words, lane * per), w_hi = min(words, w_lo + per);
+    const uint64_t* bits = n.spk_bits + static_cast<size_t>(k) * words;
+    uint32_t cnt = 0, cntE = 0;
+    for (uint32_t q = w_lo; q < w_hi; ++q) {
+        const uint64_t m = bits[q];
+        cnt += __popcll(m);
+        const uint32_t base = q * 64;
+        if (base + 64 <= n.n_exc) {
+            cntE += __popcll(m);
+        } else if (base < n.n_exc) {
+            cntE += __popcll(m & ((1ULL << (n.n_exc - base)) - 1ULL));
         }
-        uint32_t total, totalE;
-        uint32_t off = block_exclusive_scan(cnt, s_tmp, total);
-        block_exclusive_scan(cntE, s_tmp, totalE);
-        uint32_t* list = n.spk_list + static_cast<size_t>(k) * N;
-        for (uint32_t q = w_lo; q < w_hi; ++q) {
-            uint64_t m = bits[q];
-            while (m) {
-                const uint32_t bit = static_cast<uint32_t>(__ffsll(static_cast<long long>(m)) - 1);
-                list[off++] = q * 64 + bit;
-                m &= m - 1;
-            }
-        }
-        __syncthreads();
-        // Tsodyks-Markram per spiking source (network.cpp:230-247), then the
-        // weights every synapse of its rows receives this step.
-        const double t_now = static_cast<double>(T) * n.dt;
-        double* cvals = n.spk_c + static_cast<size_t>(k) * N * 2;
-        const bool room = head + static_cast<int64_t>(total) <= n.r_cap;
-        for (uint32_t j = tid; j < total; j += kSpikeBlock) {
-            const uint32_t s = list[j];
-            const bool exc = s < n.n_exc;
-            double c0 = exc ? n.w_exc : n.w_inh;
-            double c1 = n.w_n2a;
-            if (n.stp_on) {
-                const double dts = t_now - n.stp_tl[s];
-                const double x_decay = asg_exp(-dts / n.tau_rec);
-                const double u_decay = n.fac_on ? asg_exp(-dts / n.tau_fac) : 0.0;
-                double x = n.stp_x[s];
-                double u = n.stp_u[s];
-                x = 1.0 + (x - x * u - 1.0) * x_decay;
-                u = n.U + u * n.one_minus_U * u_decay;
-                n.stp_x[s] = x;
-                n.stp_u[s] = u;
-                n.stp_tl[s] = t_now;
-                c0 = c0 * x * u;
-                c1 = c1 * x * u;
-            }
-            cvals[2 * j] = c0;
-            cvals[2 * j + 1] = c1;
-            if (exc) {
-                syn_local += (n.exc_row[s + 1] - n.exc_row[s]) +
-                             (n.n2a_raw_row[s + 1] - n.n2a_raw_row[s]);
-            } else {
-                syn_local += n.inh_row[s - n.n_exc + 1] - n.inh_row[s - n.n_exc];
-            }
-            if (room) {
-                n.r_step[head + j] = T;
-                n.r_src[head + j] = s;
-            }
-        }
-        if (tid == 0) {
-            n.spk_n[2 * k] = total;
-            n.spk_n[2 * k + 1] = totalE;
-            if (!room) {
-                n.overflow = 1;
-            }
-        }
-        if (total > 0) {
-            uint32_t base = 0;
-            if (tid == 0) {
-                base = atomicAdd(work_count, total);
-                s_tmp[0] = base;
-            }
-            __syncthreads();
-            base = s_tmp[0];
-            for (uint32_t j = tid; j < total; j += kSpikeBlock) {
-                work[base + j] = WorkItem{net, k, j, 0u};
-            }
-        }
-        if (room) {
-            head += total;
-        }
-        tot_spikes += total;
-        __syncthreads();
     }
-    s_syn[tid] = syn_local;
-    __syncthreads();
-    if (tid == 0) {
-        uint64_t syn = 0;
-        for (int q = 0; q < kSpikeBlock; ++q) {
-            syn += s_syn[q];
+    uint32_t total, totalE;
+    uint32_t off = wave_exclusive_scan(cnt, total);
+    wave_exclusive_scan(cntE, totalE);
+    uint32_t* list = n.spk_list + static_cast<size_t>(k) * n.n_neur;
+    uint32_t* woff = n.spk_woff + static_cast<size_t>(k) * words;
+    for (uint32_t q = w_lo; q < w_hi; ++q) {
+        woff[q] = off;
+        uint64_t m = bits[q];
+        while (m) {
+            const uint32_t bit = static_cast<uint32_t>(__ffsll(static_cast<long long>(m)) - 1);
+            list[off++] = q * 64 + bit;
+            m &= m - 1;
         }
-        n.c_syn += static_cast<int64_t>(syn);
-        n.c_spikes += tot_spikes;
-        n.c_steps += W;
-        n.r_head = head;
-        n.win_T0 = T0;
-        n.win_W = W;
-        n.step = T0 + W;
+    }
+    if (lane == 0) {
+        n.spk_n[2 * k] = total;
+        n.spk_n[2 * k + 1] = totalE;
+        if (total > 0) {
+            add_i64(&n.c_spikes, total);
+        }
     }
 }
 
-__global__ __launch_bounds__(kOrBlock) void k_or(DevNet* nets, const WorkItem* work,
-                                                 const uint32_t* work_count) {
-    const uint32_t total = *work_count;
-    for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
-        const WorkItem it = work[item];
-        DevNet& n = nets[it.net];
-        const uint32_t N = n.n_neur;
-        const uint32_t s = n.spk_list[static_cast<size_t>(it.k) * N + it.j];
-        const uint32_t nE = n.spk_n[2 * it.k + 1];
-        if (s < n.n_exc) {
-            const uint64_t bit = 1ULL << (it.j & 63);
-            const size_t chunk = static_cast<size_t>(it.k) * n.cE + (it.j >> 6);
-            uint64_t* me = n.mask_exc + chunk * N;
-            for (uint32_t q = n.exc_row[s] + threadIdx.x; q < n.exc_row[s + 1]; q += kOrBlock) {
-                atomicOr(reinterpret_cast<unsigned long long*>(me + n.exc_tgt[q]), bit);
+// The SIC neuron t receives from the exchange sent at window step k
+// (network.cpp:281-299): w_a2n * factor(a) added once per (a, t) synapse,
+// astrocytes ascending, written to ring_sic at the arrival slot.
+__device__ void deliver_sic(const DevNet& n, uint32_t t, int k) {
+    const uint32_t N = n.n_neur;
+    const int64_t e = n.win_T0 + k;
+    if (e % n.sic_interval != 0) {
+        return;
+    }
+    const double* __restrict__ f = n.fac + static_cast<size_t>(k) * n.n_astro;
+    const uint32_t* __restrict__ src = n.sic_src;
+    const uint32_t* __restrict__ mult = n.sic_mult;
+    const uint32_t width = n.sic_width;
+    const double w_a2n = n.w_a2n;
+    double acc = 0.0;
+    for (uint32_t q = 0; q < width; ++q) {
+        const uint32_t m = mult[static_cast<size_t>(q) * N + t];
+        if (m == 0) {
+            break;
+        }
+        const double fa = f[src[static_cast<size_t>(q) * N + t]];
+        if (fa != 0.0) {
+            const double v = w_a2n * fa;
+            uint32_t r = m;
+            for (; r >= 4; r -= 4) {
+                acc += v;
+                acc += v;
+                acc += v;
+                acc += v;
             }
-            uint64_t* ma = n.mask_astro + chunk * n.n_astro;
-            for (uint32_t q = n.n2a_row[s] + threadIdx.x; q < n.n2a_row[s + 1]; q += kOrBlock) {
-                atomicOr(reinterpret_cast<unsigned long long*>(ma + n.n2a_tgt[q]), bit);
-            }
-        } else {
-            const uint32_t jj = it.j - nE;
-            const uint64_t bit = 1ULL << (jj & 63);
-            const size_t chunk = static_cast<size_t>(it.k) * n.cI + (jj >> 6);
-            uint64_t* mi = n.mask_inh + chunk * N;
-            const uint32_t r = s - n.n_exc;
-            for (uint32_t q = n.inh_row[r] + threadIdx.x; q < n.inh_row[r + 1]; q += kOrBlock) {
-                atomicOr(reinterpret_cast<unsigned long long*>(mi + n.inh_tgt[q]), bit);
+            for (; r > 0; --r) {
+                acc += v;
             }
         }
+    }
+    n.ring_sic[static_cast<size_t>((e + n.d_a2n) % n.S) * N + t] = acc;
+}
+
+// Set bit (j mod 64) of chunk j / 64 in the mask of every target of one row,
+// the whole wavefront together, with 8 target loads in flight per lane.
+__device__ inline void mark_row(const uint32_t* __restrict__ tgt, uint32_t q0, uint32_t q1,
+                                uint64_t* base, uint64_t bit, uint32_t lane) {
+    constexpr int U = 8;
+    for (uint32_t q = q0 + lane; q < q1; q += 64 * U) {
+        uint32_t tt[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t qq = q + static_cast<uint32_t>(u) * 64;
+            tt[u] = qq < q1 ? tgt[qq] : UINT32_MAX;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (tt[u] != UINT32_MAX) {
+                atomicOr(reinterpret_cast<unsigned long long*>(base + tt[u]), bit);
+            }
+        }
+    }
+}
+
+// The spikes one wavefront's 64 neurons emitted at window step k: each
+// spike's bit (its index in the step's spike list) is set in the per-target
+// mask of every synapse of its rows. OR is order-independent, so
+// wavefronts may mark in any order.
+__device__ void mark_step(const DevNet& n, uint32_t first, uint32_t lane, int k) {
+    const uint32_t N = n.n_neur;
+    const uint32_t word_i = first >> 6;
+    const uint64_t word = n.spk_bits[static_cast<size_t>(k) * n.words + word_i];
+    if (word == 0) {
+        return;
+    }
+    const uint32_t off = n.spk_woff[static_cast<size_t>(k) * n.words + word_i];
+    const uint32_t nE = n.spk_n[2 * k + 1];
+    uint64_t m = word;
+    while (m) {
+        const uint32_t L = static_cast<uint32_t>(__ffsll(static_cast<long long>(m)) - 1);
+        m &= m - 1;
+        const uint32_t s = first + L;
+        const uint32_t js = off + static_cast<uint32_t>(__popcll(word & ((1ULL << L) - 1ULL)));
+        if (s < n.n_exc) {
+            const uint64_t bit = 1ULL << (js & 63);
+            const size_t chunk = static_cast<size_t>(k) * n.cE + (js >> 6);
+            mark_row(n.exc_tgt, n.exc_row[s], n.exc_row[s + 1], n.mask_exc + chunk * N, bit, lane);
+            mark_row(n.n2a_tgt, n.n2a_row[s], n.n2a_row[s + 1], n.mask_astro + chunk * n.n_astro,
+                     bit, lane);
+        } else {
+            const uint32_t jj = js - nE;
+            const uint64_t bit = 1ULL << (jj & 63);
+            const size_t chunk = static_cast<size_t>(k) * n.cI + (jj >> 6);
+            const uint32_t r = s - n.n_exc;
+            mark_row(n.inh_tgt, n.inh_row[r], n.inh_row[r + 1], n.mask_inh + chunk * N, bit, lane);
+        }
+    }
+}
+
+// For the 64 neurons of one wavefront, over the whole window in time order:
+// each spike's Tsodyks-Markram update (network.cpp:230-247), the weights its
+// rows deliver and its raster entry.
+__device__ void deliver_sources(DevNet& n, uint32_t first, uint32_t lane) {
+    const uint32_t N = n.n_neur;
+    const uint32_t t = first + lane;
+    const bool valid = t < N;
+    const int W = n.win_W;
+    const int64_t T0 = n.win_T0;
+    const uint32_t words = n.words;
+    const uint32_t word_i = first >> 6;
+    int64_t window_total = 0;
+    for (int k2 = 0; k2 < W; ++k2) {
+        window_total += n.spk_n[2 * k2];
+    }
+    const bool room = n.r_head + window_total <= n.r_cap;
+    const uint64_t below = (1ULL << lane) - 1ULL;
+    int64_t before = 0;
+    int64_t syn = 0;
+    for (int k2 = 0; k2 < W; ++k2) {
+        const uint64_t word = n.spk_bits[static_cast<size_t>(k2) * words + word_i];
+        const uint32_t tot = n.spk_n[2 * k2];
+        if (word != 0) {
+            const uint32_t off = n.spk_woff[static_cast<size_t>(k2) * words + word_i];
+            if (valid && ((word >> lane) & 1ULL)) {
+                const uint32_t j = off + static_cast<uint32_t>(__popcll(word & below));
+                const bool exc = t < n.n_exc;
+                double c0 = exc ? n.w_exc : n.w_inh;
+                double c1 = n.w_n2a;
+                const double t_now = static_cast<double>(T0 + k2) * n.dt;
+                if (n.stp_on) {
+                    const double dts = t_now - n.stp_tl[t];
+                    const double x_decay = asg_exp(-dts / n.tau_rec);
+                    const double u_decay = n.fac_on ? asg_exp(-dts / n.tau_fac) : 0.0;
+                    double x = n.stp_x[t];
+                    double u = n.stp_u[t];
+                    x = 1.0 + (x - x * u - 1.0) * x_decay;
+                    u = n.U + u * n.one_minus_U * u_decay;
+                    n.stp_x[t] = x;
+                    n.stp_u[t] = u;
+                    n.stp_tl[t] = t_now;
+                    c0 = c0 * x * u;
+                    c1 = c1 * x * u;
+                }
+                double* cv = n.spk_c + (static_cast<size_t>(k2) * N + j) * 2;
+                cv[0] = c0;
+                cv[1] = c1;
+                if (exc) {
+                    syn += (n.exc_row[t + 1] - n.exc_row[t]) +
+                           (n.n2a_raw_row[t + 1] - n.n2a_raw_row[t]);
+                } else {
+                    syn += n.inh_row[t - n.n_exc + 1] - n.inh_row[t - n.n_exc];
+                }
+                if (room) {
+                    n.r_step[n.r_head + before + j] = T0 + k2;
+                    n.r_src[n.r_head + before + j] = t;
+                }
+            }
+        }
+        before += tot;
+    }
+    if (syn > 0) {
+        add_i64(&n.c_syn, syn);
+    }
+    if (!room && window_total > 0 && t == 0) {
+        n.overflow = 1;
+    }
+}
+
+// Window pass 3: one wavefront per (64 neurons, window step); four per block.
+__global__ __launch_bounds__(256) void k_deliver(DevNet* nets, const CellBlock* nblocks,
+                                                 int32_t n_items, int32_t Wg) {
+    const int e = static_cast<int>(blockIdx.x) * 4 + static_cast<int>(threadIdx.x >> 6);
+    if (e >= n_items) {
+        return;
+    }
+    const CellBlock cb = nblocks[e / Wg];
+    const int k = e % Wg;
+    DevNet& n = nets[cb.net];
+    if (k >= n.win_W) {
+        return;
+    }
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t t = cb.first + lane;
+    if (t < n.n_neur) {
+        deliver_sic(n, t, k);
+    }
+    mark_step(n, cb.first, lane, k);
+    if (k == 0) {
+        deliver_sources(n, cb.first, lane);
     }
 }
 
@@ -572,97 +682,113 @@ __device__ inline uint32_t n2a_multiplicity(const DevNet& n, uint32_t s, uint32_
     return (lo < n.n2a_row[s + 1] && n.n2a_tgt[lo] == a) ? n.n2a_mult[lo] : 0u;
 }
 
-__global__ __launch_bounds__(kCellBlock) void k_sum(DevNet* nets, const CellBlock* blocks) {
-    const CellBlock cb = blocks[blockIdx.x];
+// Window pass 4, one lane per (target, step), four wavefronts per block: add
+// the marked spikes' weights into the arrival slot in ascending spike order,
+// i.e. ascending source as the reference's scatter (network.cpp:249-279).
+// The lead wavefront then closes the window.
+__global__ __launch_bounds__(256) void k_sum(DevNet* nets, const CellBlock* blocks,
+                                             int32_t n_items, int32_t Wg) {
+    const int e = static_cast<int>(blockIdx.x) * 4 + static_cast<int>(threadIdx.x >> 6);
+    if (e >= n_items) {
+        return;
+    }
+    const CellBlock cb = blocks[e / Wg];
+    const int k = e % Wg;
+    const uint32_t lane = threadIdx.x & 63;
     DevNet& n = nets[cb.net];
     const int W = n.win_W;
     const int64_t T0 = n.win_T0;
-    const uint32_t i = cb.first + threadIdx.x;
+    if (cb.lead && k == 0 && lane == 0) {
+        int64_t tot = 0;
+        for (int k2 = 0; k2 < W; ++k2) {
+            tot += n.spk_n[2 * k2];
+        }
+        if (n.r_head + tot <= n.r_cap) {
+            n.r_head += tot;
+        }
+        n.c_steps += W;
+        n.step = T0 + W;
+    }
+    if (k >= W) {
+        return;
+    }
+    const uint32_t i = cb.first + lane;
     const uint32_t N = n.n_neur;
+    const uint32_t tot = n.spk_n[2 * k], nE = n.spk_n[2 * k + 1], nI = tot - nE;
+    const double* cvals = n.spk_c + static_cast<size_t>(k) * N * 2;
     if (cb.kind == 0) {
         if (i >= N) {
             return;
         }
-        for (int k = 0; k < W; ++k) {
-            const uint32_t tot = n.spk_n[2 * k], nE = n.spk_n[2 * k + 1], nI = tot - nE;
-            const double* cvals = n.spk_c + static_cast<size_t>(k) * N * 2;
-            if (nE > 0) {
-                const size_t slot = static_cast<size_t>((T0 + k + n.d_exc) % n.S) * N + i;
-                double acc = n.ring_exc[slot];
-                bool any = false;
-                for (uint32_t c = 0; c < (nE + 63) / 64; ++c) {
-                    uint64_t* mp = n.mask_exc + (static_cast<size_t>(k) * n.cE + c) * N + i;
-                    uint64_t m = *mp;
-                    if (m) {
-                        *mp = 0;
-                        any = true;
-                        while (m) {
-                            const uint32_t j = c * 64 + (__ffsll(static_cast<long long>(m)) - 1);
-                            acc += cvals[2 * j];
-                            m &= m - 1;
-                        }
-                    }
-                }
-                if (any) {
-                    n.ring_exc[slot] = acc;
-                }
-            }
-            if (nI > 0) {
-                const size_t slot = static_cast<size_t>((T0 + k + n.d_inh) % n.S) * N + i;
-                double acc = n.ring_inh[slot];
-                bool any = false;
-                for (uint32_t c = 0; c < (nI + 63) / 64; ++c) {
-                    uint64_t* mp = n.mask_inh + (static_cast<size_t>(k) * n.cI + c) * N + i;
-                    uint64_t m = *mp;
-                    if (m) {
-                        *mp = 0;
-                        any = true;
-                        while (m) {
-                            const uint32_t j = nE + c * 64 + (__ffsll(static_cast<long long>(m)) - 1);
-                            acc -= cvals[2 * j];
-                            m &= m - 1;
-                        }
-                    }
-                }
-                if (any) {
-                    n.ring_inh[slot] = acc;
-                }
-            }
-        }
-    } else {
-        const uint32_t NA = n.n_astro;
-        if (i >= NA) {
-            return;
-        }
-        for (int k = 0; k < W; ++k) {
-            const uint32_t nE = n.spk_n[2 * k + 1];
-            if (nE == 0) {
-                continue;
-            }
-            const double* cvals = n.spk_c + static_cast<size_t>(k) * N * 2;
-            const uint32_t* list = n.spk_list + static_cast<size_t>(k) * N;
-            const size_t slot = static_cast<size_t>((T0 + k + n.d_n2a) % n.S) * NA + i;
-            double acc = n.ring_astro[slot];
+        if (nE > 0) {
+            const size_t slot = static_cast<size_t>((T0 + k + n.d_exc) % n.S) * N + i;
+            double acc = n.ring_exc[slot];
             bool any = false;
             for (uint32_t c = 0; c < (nE + 63) / 64; ++c) {
-                uint64_t* mp = n.mask_astro + (static_cast<size_t>(k) * n.cE + c) * NA + i;
+                uint64_t* mp = n.mask_exc + (static_cast<size_t>(k) * n.cE + c) * N + i;
                 uint64_t m = *mp;
                 if (m) {
                     *mp = 0;
                     any = true;
                     while (m) {
                         const uint32_t j = c * 64 + (__ffsll(static_cast<long long>(m)) - 1);
-                        const double cv = cvals[2 * j + 1];
-                        for (uint32_t r = n2a_multiplicity(n, list[j], i); r > 0; --r) {
-                            acc += cv;
-                        }
+                        acc += cvals[2 * j];
                         m &= m - 1;
                     }
                 }
             }
             if (any) {
-                n.ring_astro[slot] = acc;
+                n.ring_exc[slot] = acc;
             }
+        }
+        if (nI > 0) {
+            const size_t slot = static_cast<size_t>((T0 + k + n.d_inh) % n.S) * N + i;
+            double acc = n.ring_inh[slot];
+            bool any = false;
+            for (uint32_t c = 0; c < (nI + 63) / 64; ++c) {
+                uint64_t* mp = n.mask_inh + (static_cast<size_t>(k) * n.cI + c) * N + i;
+                uint64_t m = *mp;
+                if (m) {
+                    *mp = 0;
+                    any = true;
+                    while (m) {
+                        const uint32_t j = nE + c * 64 + (__ffsll(static_cast<long long>(m)) - 1);
+                        acc -= cvals[2 * j];
+                        m &= m - 1;
+                    }
+                }
+            }
+            if (any) {
+                n.ring_inh[slot] = acc;
+            }
+        }
+    } else {
+        const uint32_t NA = n.n_astro;
+        if (i >= NA || nE == 0) {
+            return;
+        }
+        const uint32_t* list = n.spk_list + static_cast<size_t>(k) * N;
+        const size_t slot = static_cast<size_t>((T0 + k + n.d_n2a) % n.S) * NA + i;
+        double acc = n.ring_astro[slot];
+        bool any = false;
+        for (uint32_t c = 0; c < (nE + 63) / 64; ++c) {
+            uint64_t* mp = n.mask_astro + (static_cast<size_t>(k) * n.cE + c) * NA + i;
+            uint64_t m = *mp;
+            if (m) {
+                *mp = 0;
+                any = true;
+                while (m) {
+                    const uint32_t j = c * 64 + (__ffsll(static_cast<long long>(m)) - 1);
+                    const double cv = cvals[2 * j + 1];
+                    for (uint32_t r = n2a_multiplicity(n, list[j], i); r > 0; --r) {
+                        acc += cv;
+                    }
+                    m &= m - 1;
+                }
+            }
+        }
+        if (any) {
+            n.ring_astro[slot] = acc;
         }
     }
 }
@@ -775,6 +901,7 @@ std::string check_csr(const char* name, const asg_csr& c, uint32_t rows, uint32_
 
 std::string validate(const asg_net_desc& d) {
     const uint32_t N = d.n_exc + d.n_inh;
+    if (N + d.n_astro == 0) return "the network has no cells";
     if (d.substeps < 1) return "substeps must be at least 1";
     if (!(d.dt > 0.0)) return "dt must be positive";
     if (d.sic_interval < 1) return "sic_interval must be at least 1";
@@ -796,7 +923,9 @@ std::string validate(const asg_net_desc& d) {
 // lists (a2n transposed to targets, astrocytes ascending, with multiplicity).
 struct DerivedConn {
     std::vector<uint32_t> n2a_row, n2a_tgt, n2a_mult;
-    std::vector<uint32_t> sic_row, sic_src, sic_mult;
+    uint32_t sic_width = 0;
+    int64_t sic_pairs = 0;
+    std::vector<uint32_t> sic_src, sic_mult;  // [width][N]
 };
 
 DerivedConn derive(const asg_net_desc& d) {
@@ -829,22 +958,27 @@ DerivedConn derive(const asg_net_desc& d) {
             }
         }
     }
-    out.sic_row.assign(N + 1, 0);
-    for (uint32_t t = 0; t < N; ++t) out.sic_row[t + 1] = out.sic_row[t] + count[t + 1];
-    out.sic_src.assign(out.sic_row[N], 0);
-    out.sic_mult.assign(out.sic_row[N], 0);
-    std::vector<uint32_t> fill(out.sic_row.begin(), out.sic_row.end() - 1);
+    // Column-major (ELL) layout: width = most pairs of any target.
+    uint32_t width = 0;
+    for (uint32_t t = 0; t < N; ++t) width = std::max(width, count[t + 1]);
+    out.sic_width = width;
+    out.sic_pairs = 0;
+    out.sic_src.assign(static_cast<size_t>(width) * N, 0);
+    out.sic_mult.assign(static_cast<size_t>(width) * N, 0);
+    std::vector<uint32_t> fill(N, 0);
     std::fill(last.begin(), last.end(), UINT32_MAX);
     for (uint32_t a = 0; a < d.n_astro; ++a) {
         for (uint32_t q = d.a2n.row_start[a]; q < d.a2n.row_start[a + 1]; ++q) {
             const uint32_t t = d.a2n.target[q];
             if (last[t] != a) {
                 last[t] = a;
-                out.sic_src[fill[t]] = a;
-                out.sic_mult[fill[t]] = 1;
+                const size_t at = static_cast<size_t>(fill[t]) * N + t;
+                out.sic_src[at] = a;
+                out.sic_mult[at] = 1;
                 ++fill[t];
+                ++out.sic_pairs;
             } else {
-                ++out.sic_mult[fill[t] - 1];
+                ++out.sic_mult[static_cast<size_t>(fill[t] - 1) * N + t];
             }
         }
     }
@@ -860,7 +994,7 @@ int window_of(const asg_net_desc& d) {
 // Byte plan of one network's device arrays, so the footprint query and the
 // allocation agree.
 struct Plan {
-    size_t N, NA, words, S, Sf, Wmax, cE, cI, r_cap;
+    size_t N, NA, words, S, Wmax, cE, cI, r_cap;
     size_t n_exc_syn, n_inh_syn, n2a_dedup, sic_pairs;
     size_t bytes;
 };
@@ -874,7 +1008,6 @@ Plan plan_of(const asg_net_desc& d, const DerivedConn* dc) {
     const int dmax = std::max(std::max(d.exc.delay_steps, d.inh.delay_steps),
                               std::max(d.n2a.delay_steps, d.a2n.delay_steps));
     p.S = static_cast<size_t>(dmax) + p.Wmax;
-    p.Sf = static_cast<size_t>(d.a2n.delay_steps) + p.Wmax;
     p.cE = (d.n_exc + 63) / 64;
     p.cI = (d.n_inh + 63) / 64;
     p.r_cap = std::max<size_t>(size_t(1) << 20, 16 * p.N);
@@ -883,15 +1016,15 @@ Plan plan_of(const asg_net_desc& d, const DerivedConn* dc) {
     const size_t n2a_raw = d.n2a.row_start[d.n_exc];
     const size_t a2n_raw = d.a2n.row_start[d.n_astro];
     p.n2a_dedup = dc ? dc->n2a_tgt.size() : n2a_raw;
-    p.sic_pairs = dc ? dc->sic_src.size() : a2n_raw;
+    p.sic_pairs = dc ? dc->sic_src.size() : a2n_raw;  // ELL entries
     size_t b = 0;
     b += p.N * (7 * 8 + 4 + 2 * 8 + 3 * 8);        // neuron state, V_reset/b, STP
     b += p.NA * (3 * 8 + 4 * 8);                    // astrocyte state and parameters
-    b += p.S * p.N * 2 * 8 + p.S * p.NA * 8 + p.Sf * p.NA * 8;
+    b += p.S * p.N * 3 * 8 + p.S * p.NA * 8 + p.Wmax * p.NA * 8;
     b += (d.n_exc + 1 + p.n_exc_syn + d.n_inh + 1 + p.n_inh_syn) * 4;
     b += (2 * (d.n_exc + 1) + 2 * p.n2a_dedup) * 4;
-    b += (p.N + 1 + 2 * p.sic_pairs) * 4;
-    b += p.Wmax * (p.words * 8 + p.N * 4 + 8 + p.N * 16);
+    b += 2 * p.sic_pairs * 4;
+    b += p.Wmax * (p.words * 12 + p.N * 4 + 8 + p.N * 16);
     b += p.Wmax * (p.cE * p.N + p.cI * p.N + p.cE * p.NA) * 8;
     b += p.r_cap * 12;
     p.bytes = b;
@@ -970,7 +1103,7 @@ void build_slot(NetSlot& s, const asg_net_desc& d, int64_t total, hipStream_t st
     n.cE = static_cast<uint32_t>(pl.cE);
     n.cI = static_cast<uint32_t>(pl.cI);
     n.S = static_cast<int32_t>(pl.S);
-    n.Sf = static_cast<int32_t>(pl.Sf);
+    n.Wmax = static_cast<int32_t>(pl.Wmax);
     n.d_exc = d.exc.delay_steps;
     n.d_inh = d.inh.delay_steps;
     n.d_n2a = d.n2a.delay_steps;
@@ -1025,8 +1158,9 @@ void build_slot(NetSlot& s, const asg_net_desc& d, int64_t total, hipStream_t st
     n.delta_IP3 = dev_copy(s, d.delta_IP3, NA, st);
     n.ring_exc = dev_zero<double>(s, pl.S * N, st);
     n.ring_inh = dev_zero<double>(s, pl.S * N, st);
+    n.ring_sic = dev_zero<double>(s, pl.S * N, st);
     n.ring_astro = dev_zero<double>(s, pl.S * NA, st);
-    n.fac = dev_zero<double>(s, pl.Sf * NA, st);
+    n.fac = dev_zero<double>(s, pl.Wmax * NA, st);
     n.stp_x = dev_fill(s, 1.0, N, st);
     n.stp_u = dev_fill(s, d.stp.U, N, st);
     n.stp_tl = dev_fill(s, -1e12, N, st);
@@ -1038,10 +1172,12 @@ void build_slot(NetSlot& s, const asg_net_desc& d, int64_t total, hipStream_t st
     n.n2a_tgt = dev_copy(s, dc.n2a_tgt.data(), dc.n2a_tgt.size(), st);
     n.n2a_mult = dev_copy(s, dc.n2a_mult.data(), dc.n2a_mult.size(), st);
     n.n2a_raw_row = dev_copy(s, d.n2a.row_start, d.n_exc + 1, st);
-    n.sic_row = dev_copy(s, dc.sic_row.data(), dc.sic_row.size(), st);
+    n.sic_width = dc.sic_width;
+    n.sic_pairs = dc.sic_pairs;
     n.sic_src = dev_copy(s, dc.sic_src.data(), dc.sic_src.size(), st);
     n.sic_mult = dev_copy(s, dc.sic_mult.data(), dc.sic_mult.size(), st);
     n.spk_bits = dev_zero<uint64_t>(s, pl.Wmax * pl.words, st);
+    n.spk_woff = dev_zero<uint32_t>(s, pl.Wmax * pl.words, st);
     n.spk_list = dev_zero<uint32_t>(s, pl.Wmax * N, st);
     n.spk_n = dev_zero<uint32_t>(s, pl.Wmax * 2, st);
     n.spk_c = dev_zero<double>(s, pl.Wmax * N * 2, st);
@@ -1072,31 +1208,33 @@ struct Engine {
     std::vector<NetSlot> slots;
     DevNet* d_nets = nullptr;
     CellBlock* d_blocks = nullptr;
+    CellBlock* d_nblocks = nullptr;   // neuron blocks only
     int32_t* d_active = nullptr;
-    WorkItem* d_work = nullptr;
-    uint32_t* d_work_count = nullptr;
-    size_t blocks_cap = 0, active_cap = 0, work_cap = 0;
-    int n_blocks = 0, n_active = 0, Wg = 1;
+    size_t blocks_cap = 0, nblocks_cap = 0, active_cap = 0;
+    int n_blocks = 0, n_nblocks = 0, n_active = 0, Wg = 1;
     bool dirty = true;
     bool pending = false;  // async work in flight
+    // Per-kernel event timing (off unless asked for).
+    bool profiling = false;
+    std::vector<hipEvent_t> events;   // 5 per window of the last step() call
+    int64_t timed_windows = 0, pending_windows = 0;
+    double kernel_ms[4] = {0, 0, 0, 0};
 
     explicit Engine(int max_n, int64_t budget_bytes) : max_nets(max_n), budget(budget_bytes) {
         HIP_CHECK(hipGetDevice(&device));
         HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         slots.resize(static_cast<size_t>(max_n));
         HIP_CHECK(hipMalloc(&d_nets, sizeof(DevNet) * static_cast<size_t>(max_n)));
-        HIP_CHECK(hipMalloc(&d_work_count, sizeof(uint32_t)));
-        HIP_CHECK(hipMemset(d_work_count, 0, sizeof(uint32_t)));
     }
 
     ~Engine() {
         if (stream) (void)hipStreamSynchronize(stream);
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
         for (NetSlot& s : slots) release(s);
         (void)hipFree(d_nets);
         (void)hipFree(d_blocks);
+        (void)hipFree(d_nblocks);
         (void)hipFree(d_active);
-        (void)hipFree(d_work);
-        (void)hipFree(d_work_count);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -1112,6 +1250,26 @@ struct Engine {
         if (pending) {
             HIP_CHECK(hipStreamSynchronize(stream));
             pending = false;
+            if (pending_windows > 0) {
+                for (int64_t w = 0; w < pending_windows; ++w) {
+                    for (int k = 0; k < 4; ++k) {
+                        float ms = 0.0f;
+                        HIP_CHECK(hipEventElapsedTime(&ms, events[5 * w + k], events[5 * w + k + 1]));
+                        kernel_ms[k] += ms;
+                    }
+                }
+                timed_windows += pending_windows;
+                pending_windows = 0;
+            }
+        }
+    }
+
+    void ensure_events(int64_t windows) {
+        const size_t need = static_cast<size_t>(windows) * 5;
+        while (events.size() < need) {
+            hipEvent_t e;
+            HIP_CHECK(hipEventCreate(&e));
+            events.push_back(e);
         }
     }
 
@@ -1138,28 +1296,36 @@ struct Engine {
     }
 
     void rebuild_tables() {
-        std::vector<CellBlock> blocks;
+        std::vector<CellBlock> blocks, nblocks;
         std::vector<int32_t> active;
-        size_t work_items = 0;
         Wg = kWindowCap;
         for (int sidx = 0; sidx < max_nets; ++sidx) {
             const NetSlot& s = slots[sidx];
             if (!s.used) continue;
             active.push_back(sidx);
             Wg = std::min(Wg, s.W);
+            uint32_t lead = 1;
             for (uint32_t f = 0; f < s.dn.n_neur; f += kCellBlock) {
-                blocks.push_back(CellBlock{sidx, 0, f, 0});
+                blocks.push_back(CellBlock{sidx, 0, f, lead});
+                nblocks.push_back(CellBlock{sidx, 0, f, lead});
+                lead = 0;
             }
             for (uint32_t f = 0; f < s.dn.n_astro; f += kCellBlock) {
-                blocks.push_back(CellBlock{sidx, 1, f, 0});
+                blocks.push_back(CellBlock{sidx, 1, f, lead});
+                lead = 0;
             }
-            work_items += static_cast<size_t>(s.W) * s.dn.n_neur;
         }
         n_blocks = static_cast<int>(blocks.size());
+        n_nblocks = static_cast<int>(nblocks.size());
         n_active = static_cast<int>(active.size());
         if (n_blocks > 0) {
             grow(d_blocks, blocks_cap, blocks.size());
             HIP_CHECK(hipMemcpyAsync(d_blocks, blocks.data(), blocks.size() * sizeof(CellBlock),
+                                     hipMemcpyHostToDevice, stream));
+        }
+        if (n_nblocks > 0) {
+            grow(d_nblocks, nblocks_cap, nblocks.size());
+            HIP_CHECK(hipMemcpyAsync(d_nblocks, nblocks.data(), nblocks.size() * sizeof(CellBlock),
                                      hipMemcpyHostToDevice, stream));
         }
         if (n_active > 0) {
@@ -1167,7 +1333,6 @@ struct Engine {
             HIP_CHECK(hipMemcpyAsync(d_active, active.data(), active.size() * sizeof(int32_t),
                                      hipMemcpyHostToDevice, stream));
         }
-        grow(d_work, work_cap, std::max<size_t>(work_items, 1));
         HIP_CHECK(hipStreamSynchronize(stream));
         dirty = false;
     }
@@ -1224,17 +1389,30 @@ struct Engine {
             windows = std::max(windows, (rem + Wg - 1) / Wg);
             s.step = target;
         }
-        const int or_grid = 2048;
+        const int deliver_items = n_nblocks * Wg;
+        const int sum_items = n_blocks * Wg;
+        if (profiling) ensure_events(windows);
+        auto mark = [&](int64_t wdx, int k) {
+            if (profiling) HIP_CHECK(hipEventRecord(events[5 * wdx + k], stream));
+        };
         for (int64_t wdx = 0; wdx < windows; ++wdx) {
+            mark(wdx, 0);
             hipLaunchKernelGGL(k_cells, dim3(n_blocks), dim3(kCellBlock), 0, stream, d_nets,
-                               d_blocks, Wg, d_work_count);
-            hipLaunchKernelGGL(k_spikes, dim3(n_active), dim3(kSpikeBlock), 0, stream, d_nets,
-                               d_active, Wg, d_work, d_work_count);
-            hipLaunchKernelGGL(k_or, dim3(or_grid), dim3(kOrBlock), 0, stream, d_nets, d_work,
-                               d_work_count);
-            hipLaunchKernelGGL(k_sum, dim3(n_blocks), dim3(kCellBlock), 0, stream, d_nets,
-                               d_blocks);
+                               d_blocks, Wg);
+            mark(wdx, 1);
+            hipLaunchKernelGGL(k_compact, dim3(n_active, Wg), dim3(64), 0, stream, d_nets,
+                               d_active);
+            mark(wdx, 2);
+            if (deliver_items > 0) {
+                hipLaunchKernelGGL(k_deliver, dim3((deliver_items + 3) / 4), dim3(256), 0, stream,
+                                   d_nets, d_nblocks, deliver_items, Wg);
+            }
+            mark(wdx, 3);
+            hipLaunchKernelGGL(k_sum, dim3((sum_items + 3) / 4), dim3(256), 0, stream, d_nets,
+                               d_blocks, sum_items, Wg);
+            mark(wdx, 4);
         }
+        if (profiling) pending_windows = windows;
         HIP_CHECK(hipGetLastError());
         pending = true;
         if (!async) wait();
@@ -1368,13 +1546,7 @@ asg_status counters_common(Engine& e, int32_t slot, asg_counters* out) {
     if (n.c_steps > n.d_a2n) {
         pend = (n.c_steps - 1 - n.d_a2n) / n.sic_interval + 1;
     }
-    int64_t pairs = 0;
-    if (n.n_neur > 0) {
-        uint32_t last = 0;
-        HIP_CHECK(hipMemcpy(&last, n.sic_row + n.n_neur, sizeof(uint32_t), hipMemcpyDeviceToHost));
-        pairs = last;
-    }
-    out->sic_entries = pend * pairs;
+    out->sic_entries = pend * n.sic_pairs;
     return ASG_OK;
 }
 
@@ -1554,6 +1726,44 @@ asg_status asg_batch_counters(asg_batch* batch, int32_t slot, asg_counters* out)
 
 int64_t asg_batch_bytes_in_use(const asg_batch* batch) {
     return batch == nullptr ? -1 : batch->eng->bytes_in_use();
+}
+
+namespace {
+asg_status kernel_times(Engine& e, asg_kernel_times* out) {
+    if (out == nullptr) return fail(ASG_ERR_INVALID, "NULL output");
+    return guarded([&]() -> asg_status {
+        e.wait();
+        for (int k = 0; k < 4; ++k) out->ms[k] = e.kernel_ms[k];
+        out->windows = e.timed_windows;
+        return ASG_OK;
+    });
+}
+asg_status set_profiling(Engine& e, int32_t on) {
+    return guarded([&]() -> asg_status {
+        e.wait();
+        e.profiling = on != 0;
+        for (double& v : e.kernel_ms) v = 0.0;
+        e.timed_windows = 0;
+        return ASG_OK;
+    });
+}
+}  // namespace
+
+asg_status asg_net_set_profiling(asg_net* net, int32_t on) {
+    if (net == nullptr) return fail(ASG_ERR_INVALID, "NULL handle");
+    return set_profiling(*net->eng, on);
+}
+asg_status asg_net_kernel_times(asg_net* net, asg_kernel_times* out) {
+    if (net == nullptr) return fail(ASG_ERR_INVALID, "NULL handle");
+    return kernel_times(*net->eng, out);
+}
+asg_status asg_batch_set_profiling(asg_batch* batch, int32_t on) {
+    if (batch == nullptr) return fail(ASG_ERR_INVALID, "NULL handle");
+    return set_profiling(*batch->eng, on);
+}
+asg_status asg_batch_kernel_times(asg_batch* batch, asg_kernel_times* out) {
+    if (batch == nullptr) return fail(ASG_ERR_INVALID, "NULL handle");
+    return kernel_times(*batch->eng, out);
 }
 
 asg_status asg_device_math(int32_t fn, const double* x, double* y, int64_t n) {

@@ -149,6 +149,16 @@ class DeviceNetwork:
         check(lib().asg_net_counters(self._h, C.byref(c)))
         return {name: getattr(c, name) for name, _ in capi.Counters._fields_}
 
+    def set_profiling(self, on: bool) -> None:
+        check(lib().asg_net_set_profiling(self._h, 1 if on else 0))
+
+    def kernel_times(self) -> dict:
+        """Device milliseconds per kernel since profiling was switched on."""
+        t = capi.KernelTimes()
+        check(lib().asg_net_kernel_times(self._h, C.byref(t)))
+        return {"cells": t.ms[0], "compact": t.ms[1], "deliver": t.ms[2], "sum": t.ms[3],
+                "windows": t.windows}
+
     def close(self):
         if getattr(self, "_h", None):
             lib().asg_net_destroy(self._h)
@@ -221,6 +231,15 @@ class Batch:
         c = capi.Counters()
         check(lib().asg_batch_counters(self._h, int(slot), C.byref(c)))
         return {name: getattr(c, name) for name, _ in capi.Counters._fields_}
+
+    def set_profiling(self, on: bool) -> None:
+        check(lib().asg_batch_set_profiling(self._h, 1 if on else 0))
+
+    def kernel_times(self) -> dict:
+        t = capi.KernelTimes()
+        check(lib().asg_batch_kernel_times(self._h, C.byref(t)))
+        return {"cells": t.ms[0], "compact": t.ms[1], "deliver": t.ms[2], "sum": t.ms[3],
+                "windows": t.windows}
 
     def bytes_in_use(self) -> int:
         return int(lib().asg_batch_bytes_in_use(self._h))

@@ -301,6 +301,19 @@ int64_t asg_batch_bytes_in_use(const asg_batch* batch);
 /* Diagnostics.                                                              */
 /* ------------------------------------------------------------------------ */
 
+/* Device time per kernel of the window pipeline, measured with HIP events on
+ * the handle's own stream while profiling is on: [0] cell update, [1] spike
+ * compaction, [2] SIC sums + plasticity + synapse marking, [3] ordered delivery sums. */
+typedef struct asg_kernel_times {
+    double ms[4];
+    int64_t windows;   /* windows timed */
+} asg_kernel_times;
+
+asg_status asg_net_set_profiling(asg_net* net, int32_t on);
+asg_status asg_net_kernel_times(asg_net* net, asg_kernel_times* out);
+asg_status asg_batch_set_profiling(asg_batch* batch, int32_t on);
+asg_status asg_batch_kernel_times(asg_batch* batch, asg_kernel_times* out);
+
 enum { ASG_MATH_EXP = 0, ASG_MATH_LOG = 1, ASG_MATH_SIN = 2, ASG_MATH_COS = 3, ASG_MATH_SQRT = 4 };
 
 /* Evaluates one of the step's FP64 functions on the device for n host
