@@ -90,6 +90,13 @@ class KernelTimes(C.Structure):
     _fields_ = [("ms", C.c_double * 4), ("windows", C.c_int64)]
 
 
+class CalciumSummary(C.Structure):
+    _fields_ = [("cells", C.c_int32), ("active", C.c_int32), ("transients", C.c_int64),
+                ("pairs", C.c_int64), ("mean_duration", C.c_double), ("sd_duration", C.c_double),
+                ("mean_correlation", C.c_double), ("threshold", C.c_double),
+                ("width", C.c_double), ("shift", C.c_double)]
+
+
 class ModelOverrides(C.Structure):
     _fields_ = [("seed", C.c_int64), ("sim_time", C.c_double), ("pre_time", C.c_double),
                 ("n_astro", C.c_int64), ("assignments", C.c_char_p)]
@@ -136,10 +143,17 @@ PROTOTYPES = {
     "asg_net_create": (C.c_int, [C.POINTER(NetDesc), PP]),
     "asg_net_destroy": (C.c_int, [P]),
     "asg_net_step": (C.c_int, [P, C.c_int64]),
+    "asg_net_step_async": (C.c_int, [P, C.c_int64]),
+    "asg_net_sync": (C.c_int, [P]),
     "asg_net_current_step": (C.c_int64, [P]),
     "asg_net_spikes": (C.c_int, [P, I64P, U32P, C.c_int64, I64P]),
     "asg_net_read_state": (C.c_int, [P, C.c_int32, C.c_int64, C.c_int64, DBLP]),
     "asg_net_counters": (C.c_int, [P, C.POINTER(Counters)]),
+    "asg_net_set_recording": (C.c_int, [P, C.c_int64, C.c_int32, C.c_uint32, C.c_int64]),
+    "asg_net_read_trace": (C.c_int, [P, DBLP, I64P, C.c_int64, I64P]),
+    "asg_batch_set_recording": (C.c_int, [P, C.c_int32, C.c_int64, C.c_int32, C.c_uint32,
+                                           C.c_int64]),
+    "asg_batch_read_trace": (C.c_int, [P, C.c_int32, DBLP, I64P, C.c_int64, I64P]),
     "asg_query_footprint": (C.c_int64, [C.POINTER(NetDesc)]),
     "asg_batch_create": (C.c_int, [C.c_int32, C.c_int64, PP]),
     "asg_batch_destroy": (C.c_int, [P]),
@@ -153,6 +167,14 @@ PROTOTYPES = {
     "asg_batch_counters": (C.c_int, [P, C.c_int32, C.POINTER(Counters)]),
     "asg_batch_bytes_in_use": (C.c_int64, [P]),
     "asg_device_math": (C.c_int, [C.c_int32, DBLP, DBLP, C.c_int64]),
+    "asg_analyze_calcium": (C.c_int, [DBLP, DBLP, C.c_int32, C.c_int64, C.c_double, C.c_double,
+                                      C.c_double, C.c_double, C.c_double, C.c_double, C.c_int32,
+                                      C.POINTER(CalciumSummary)]),
+    "asg_detect_events": (C.c_int, [DBLP, DBLP, C.c_int64, C.c_double, C.c_int32, C.c_double,
+                                    DBLP, DBLP, C.c_int64, I64P]),
+    "asg_pearson": (C.c_double, [DBLP, DBLP, C.c_int64]),
+    "asg_sliding_window_rate": (C.c_int, [DBLP, C.c_int64, C.c_double, C.c_double, C.c_double,
+                                          C.c_double, DBLP, C.c_int64, I64P]),
     "asg_net_set_profiling": (C.c_int, [P, C.c_int32]),
     "asg_net_kernel_times": (C.c_int, [P, C.POINTER(KernelTimes)]),
     "asg_batch_set_profiling": (C.c_int, [P, C.c_int32]),

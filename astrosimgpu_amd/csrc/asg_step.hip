@@ -44,6 +44,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstddef>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -52,6 +53,7 @@
 #include "asg.h"
 #include "amath.h"
 #include "cell_math.h"
+#include "cell_math_f32.h"
 #include "rng.h"
 
 namespace asg {
@@ -143,6 +145,9 @@ struct DevNet {
     NeuronK pk[2];
     InputK in[2];
     AstroPopK ap;
+    int32_t precision;     // ASG_FP64 or ASG_FP32
+    NeuronKf pkf[2];
+    AstroKf akf;
     int32_t stp_on, fac_on;
     double U, one_minus_U, tau_rec, tau_fac;
     double w_exc, w_inh, w_n2a, w_a2n;
@@ -156,6 +161,10 @@ struct DevNet {
     // rings [S][targets]: exc, inh and SIC per neuron, n2a per astrocyte
     double *ring_exc, *ring_inh, *ring_sic, *ring_astro;
     double* fac;           // [Wmax][n_astro]: SIC factor of each step of the window
+    // FP32 networks keep their state, parameters and rings here instead
+    float *fV, *fw, *fgex, *fdgex, *fgin, *fdgin, *fIsic;
+    float *fCa, *fIP3, *fhh, *fCa_tot, *fIP3_0, *finv_tau, *fdelta;
+    float *fring_exc, *fring_inh, *fring_sic, *fring_astro, *ffac;
     double *stp_x, *stp_u, *stp_tl;
     // connectivity
     const uint32_t *exc_row, *exc_tgt, *inh_row, *inh_tgt;
@@ -177,9 +186,24 @@ struct DevNet {
     int64_t* r_step;
     uint32_t* r_src;
     int64_t r_cap, r_head;
+    // astrocyte recording: record r = (step - rec_start) / rec_every lands in
+    // ring slot r % rec_cap as [Ca x rec_n, IP3 x rec_n]
+    int64_t rec_start, rec_cap;
+    int32_t rec_every;
+    uint32_t rec_n;
+    double* rec_buf;
     // counters
     int64_t c_steps, c_spikes, c_syn, c_sic;
 };
+
+__device__ inline void record_astro(const DevNet& n, uint32_t a, int64_t T, double Ca, double IP3) {
+    if (a < n.rec_n && T >= n.rec_start && (T - n.rec_start) % n.rec_every == 0) {
+        const int64_t r = ((T - n.rec_start) / n.rec_every) % n.rec_cap;
+        double* rec = n.rec_buf + static_cast<size_t>(r) * 2 * n.rec_n;
+        rec[a] = Ca;
+        rec[n.rec_n + a] = IP3;
+    }
+}
 
 struct CellBlock {
     int32_t net;
@@ -388,6 +412,7 @@ __device__ void astro_window(const DevNet& n, uint32_t a, int64_t T0, int W) {
         }
         astro_step(ap.k, Ca_tot, IP3_0, tau, delta, ip3_in, noise, hs, half_h, h_sixth, substeps,
                    Ca, IP3, hh);
+        record_astro(n, a, T, Ca, IP3);
         if (T % sic_interval == 0) {
             fac[static_cast<size_t>(k) * NA + a] = sic_factor(Ca, ap.SIC_th, ap.SIC_scale);
         }
@@ -395,6 +420,230 @@ __device__ void astro_window(const DevNet& n, uint32_t a, int64_t T0, int W) {
     n.Ca[a] = Ca;
     n.IP3[a] = IP3;
     n.hh[a] = hh;
+}
+
+// ----------------------------------------------------- FP32 cell windows ---
+
+__device__ void neuron_window_f32(const DevNet& n, uint32_t i, int64_t T0, int W) {
+    const uint32_t N = n.n_neur;
+    const bool valid = i < N;
+    const int pi = (i < n.n_exc) ? 0 : 1;
+    const NeuronKf pf = n.pkf[pi];
+    const NeuronK pd = n.pk[pi];
+    const InputK in = n.in[pi];
+    const int S = n.S;
+    const int d_a2n = n.d_a2n;
+    const int sic_interval = n.sic_interval;
+    const int substeps = n.substeps;
+    const uint64_t seed = n.seed;
+    const float hs = static_cast<float>(n.h), half_h = static_cast<float>(n.half_h),
+                h_sixth = static_cast<float>(n.h_sixth);
+    const uint32_t words = n.words;
+    float* __restrict__ ring_exc = n.fring_exc;
+    float* __restrict__ ring_inh = n.fring_inh;
+    const float* __restrict__ ring_sic = n.fring_sic;
+    uint64_t* __restrict__ spk_bits = n.spk_bits;
+    const uint64_t salt_shared = pi == 0 ? kSaltNoiseSharedExc : kSaltNoiseSharedInh;
+
+    float V = 0, w = 0, gex = 0, dgex = 0, gin = 0, dgin = 0, Isic = 0, Vr = 0, b = 0;
+    int refr = 0;
+    float ex_n = 0, ih_n = 0, sic_n = 0;
+    if (valid) {
+        V = n.fV[i];
+        w = n.fw[i];
+        gex = n.fgex[i];
+        dgex = n.fdgex[i];
+        gin = n.fgin[i];
+        dgin = n.fdgin[i];
+        Isic = n.fIsic[i];
+        refr = n.refr[i];
+        Vr = static_cast<float>(n.V_reset[i]);
+        b = static_cast<float>(n.b[i]);
+        const size_t s0 = static_cast<size_t>(T0 % S) * N + i;
+        ex_n = ring_exc[s0];
+        ih_n = ring_inh[s0];
+        sic_n = ring_sic[s0];
+    }
+    int64_t noise_idx = -1;
+    float noise_val = 0.0f;
+    const uint32_t lane = threadIdx.x & 63;
+    for (int k = 0; k < W; ++k) {
+        const int64_t T = T0 + k;
+        bool spiked = false;
+        if (valid) {
+            const size_t slot = static_cast<size_t>(T % S) * N + i;
+            const float ex = ex_n, ih = ih_n, sic = sic_n;
+            if (k + 1 < W) {
+                const size_t nx = static_cast<size_t>((T + 1) % S) * N + i;
+                ex_n = ring_exc[nx];
+                ih_n = ring_inh[nx];
+                sic_n = ring_sic[nx];
+            }
+            float exc_in = 0.0f, inh_in = 0.0f;
+            if (ex != 0.0f) {
+                if (ex > 0.0f) {
+                    exc_in += ex;
+                } else {
+                    inh_in -= ex;
+                }
+                ring_exc[slot] = 0.0f;
+            }
+            if (ih != 0.0f) {
+                if (ih < 0.0f) {
+                    exc_in -= ih;
+                } else {
+                    inh_in += ih;
+                }
+                ring_inh[slot] = 0.0f;
+            }
+            if (T >= d_a2n && (T - d_a2n) % sic_interval == 0) {
+                Isic = sic;
+            }
+            // Same counter streams as the FP64 step: identical Poisson events.
+            if (in.poiss_rate > 0.0) {
+                const int ev = rng_poisson_fresh(seed ^ kSaltNeuronPoisson,
+                                                 static_cast<uint64_t>(T) * kStreamStride + i,
+                                                 in.lambda, in.limit);
+                if (ev > 0) {
+                    exc_in += static_cast<float>(in.poiss_weight * static_cast<double>(ev));
+                }
+            }
+            float I_ext = Isic;
+            if (in.noise_std > 0.0) {
+                const int64_t idx = T / in.steps_per_noise;
+                if (idx != noise_idx) {
+                    noise_idx = idx;
+                    uint64_t ctr = 0;
+                    const double z = in.independent
+                                         ? rng_normal_fresh(seed ^ kSaltNeuronNoise,
+                                                            static_cast<uint64_t>(idx) * kNoiseStride + i,
+                                                            ctr)
+                                         : rng_normal_fresh(seed ^ salt_shared,
+                                                            static_cast<uint64_t>(idx), ctr);
+                    noise_val = static_cast<float>(in.noise_std * z);
+                }
+                I_ext += noise_val;
+            }
+            if (exc_in != 0.0f) {
+                dgex += exc_in * pf.psc_ex;
+            }
+            if (inh_in != 0.0f) {
+                dgin += inh_in * pf.psc_in;
+            }
+            for (int sidx = 0; sidx < substeps; ++sidx) {
+                const bool sp = adex_substep_mixed(pf, pd, Vr, b, I_ext, hs, half_h, h_sixth, n.h,
+                                                   n.half_h, n.h_sixth, V, w, gex, dgex, gin,
+                                                   dgin, refr);
+                spiked = spiked || sp;
+            }
+        }
+        const uint64_t mask = __ballot(spiked);
+        if (lane == 0 && (i & ~63u) < N) {
+            spk_bits[static_cast<size_t>(k) * words + (i >> 6)] = mask;
+        }
+    }
+    if (valid) {
+        n.fV[i] = V;
+        n.fw[i] = w;
+        n.fgex[i] = gex;
+        n.fdgex[i] = dgex;
+        n.fgin[i] = gin;
+        n.fdgin[i] = dgin;
+        n.fIsic[i] = Isic;
+        n.refr[i] = refr;
+    }
+}
+
+__device__ void astro_window_f32(const DevNet& n, uint32_t a, int64_t T0, int W) {
+    const uint32_t NA = n.n_astro;
+    if (a >= NA) {
+        return;
+    }
+    const AstroKf kf = n.akf;
+    const InputK ain = n.ap.in;
+    const double SIC_th = n.ap.SIC_th, SIC_scale = n.ap.SIC_scale;
+    const int S = n.S;
+    const int sic_interval = n.sic_interval;
+    const int substeps = n.substeps;
+    const uint64_t seed = n.seed;
+    const float hs = static_cast<float>(n.h), half_h = static_cast<float>(n.half_h),
+                h_sixth = static_cast<float>(n.h_sixth);
+    float* __restrict__ ring_astro = n.fring_astro;
+    float* __restrict__ fac = n.ffac;
+    float Ca = n.fCa[a], IP3 = n.fIP3[a], hh = n.fhh[a];
+    const float Ca_tot = n.fCa_tot[a], IP3_0 = n.fIP3_0[a], inv_tau = n.finv_tau[a],
+                delta = n.fdelta[a];
+    float v_n = ring_astro[static_cast<size_t>(T0 % S) * NA + a];
+    int64_t noise_idx = -1;
+    float noise = 0.0f;
+    for (int k = 0; k < W; ++k) {
+        const int64_t T = T0 + k;
+        const size_t slot = static_cast<size_t>(T % S) * NA + a;
+        const float v = v_n;
+        if (k + 1 < W) {
+            v_n = ring_astro[static_cast<size_t>((T + 1) % S) * NA + a];
+        }
+        float ip3_in = 0.0f;
+        if (v != 0.0f) {
+            ip3_in += v;
+            ring_astro[slot] = 0.0f;
+        }
+        if (ain.poiss_rate > 0.0) {
+            const int ev = rng_poisson_fresh(seed ^ kSaltAstroPoisson,
+                                             static_cast<uint64_t>(T) * kStreamStride + a,
+                                             ain.lambda, ain.limit);
+            if (ev > 0) {
+                ip3_in += static_cast<float>(ain.poiss_weight * static_cast<double>(ev));
+            }
+        }
+        if (ain.noise_std > 0.0) {
+            const int64_t idx = T / ain.steps_per_noise;
+            if (idx != noise_idx) {
+                noise_idx = idx;
+                const uint64_t stream = ain.independent
+                                            ? static_cast<uint64_t>(idx) * kStreamStride + a
+                                            : static_cast<uint64_t>(idx);
+                noise = static_cast<float>(ain.noise_std *
+                                           rng_normal_clamped(seed ^ kSaltAstroNoise, stream));
+            }
+        }
+        astro_step_f32(kf, Ca_tot, IP3_0, inv_tau, delta, ip3_in, noise, hs, half_h, h_sixth,
+                       substeps, Ca, IP3, hh);
+        record_astro(n, a, T, Ca, IP3);
+        if (T % sic_interval == 0) {
+            fac[static_cast<size_t>(k) * NA + a] = sic_factor_mixed(Ca, SIC_th, SIC_scale);
+        }
+    }
+    n.fCa[a] = Ca;
+    n.fIP3[a] = IP3;
+    n.fhh[a] = hh;
+}
+
+// FP32 SIC of one (neuron, window step): sum over (astrocyte, multiplicity)
+// of m * w_a2n * factor, compensated.
+__device__ void deliver_sic_f32(const DevNet& n, uint32_t t, int k) {
+    const uint32_t N = n.n_neur;
+    const int64_t e = n.win_T0 + k;
+    if (e % n.sic_interval != 0) {
+        return;
+    }
+    const float* __restrict__ f = n.ffac + static_cast<size_t>(k) * n.n_astro;
+    const uint32_t* __restrict__ src = n.sic_src;
+    const uint32_t* __restrict__ mult = n.sic_mult;
+    const uint32_t width = n.sic_width;
+    const float w_a2n = static_cast<float>(n.w_a2n);
+    float acc = 0.0f, comp = 0.0f;
+    for (uint32_t q = 0; q < width; ++q) {
+        const uint32_t m = mult[static_cast<size_t>(q) * N + t];
+        if (m == 0) {
+            break;
+        }
+        const float fa = f[src[static_cast<size_t>(q) * N + t]];
+        if (fa != 0.0f) {
+            comp_add(acc, comp, static_cast<float>(m) * (w_a2n * fa));
+        }
+    }
+    n.fring_sic[static_cast<size_t>((e + n.d_a2n) % n.S) * N + t] = acc + comp;
 }
 
 // Window pass 1: every cell of every network advances W steps.
@@ -412,7 +661,13 @@ __global__ __launch_bounds__(kCellBlock) void k_cells(DevNet* nets, const CellBl
         return;
     }
     const uint32_t i = cb.first + threadIdx.x;
-    if (cb.kind == 0) {
+    if (n.precision == ASG_FP32) {
+        if (cb.kind == 0) {
+            neuron_window_f32(n, i, T0, W);
+        } else {
+            astro_window_f32(n, i, T0, W);
+        }
+    } else if (cb.kind == 0) {
         neuron_window(n, i, T0, W);
     } else {
         astro_window(n, i, T0, W);
@@ -660,7 +915,11 @@ __global__ __launch_bounds__(256) void k_deliver(DevNet* nets, const CellBlock* 
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t t = cb.first + lane;
     if (t < n.n_neur) {
-        deliver_sic(n, t, k);
+        if (n.precision == ASG_FP32) {
+            deliver_sic_f32(n, t, k);
+        } else {
+            deliver_sic(n, t, k);
+        }
     }
     mark_step(n, cb.first, lane, k);
     if (k == 0) {
@@ -680,6 +939,87 @@ __device__ inline uint32_t n2a_multiplicity(const DevNet& n, uint32_t s, uint32_
         }
     }
     return (lo < n.n2a_row[s + 1] && n.n2a_tgt[lo] == a) ? n.n2a_mult[lo] : 0u;
+}
+
+// FP32 variant of the ordered sums: the same spike order, accumulated with
+// Neumaier compensation into FP32 rings.
+__device__ void sum_f32(const DevNet& n, int kind, uint32_t i, int k, int64_t T0, uint32_t tot,
+                        uint32_t nE, uint32_t nI, const double* cvals) {
+    const uint32_t N = n.n_neur;
+    if (kind == 0) {
+        if (i >= N) {
+            return;
+        }
+        if (nE > 0) {
+            const size_t slot = static_cast<size_t>((T0 + k + n.d_exc) % n.S) * N + i;
+            float acc = n.fring_exc[slot], comp = 0.0f;
+            bool any = false;
+            for (uint32_t c = 0; c < (nE + 63) / 64; ++c) {
+                uint64_t* mp = n.mask_exc + (static_cast<size_t>(k) * n.cE + c) * N + i;
+                uint64_t m = *mp;
+                if (m) {
+                    *mp = 0;
+                    any = true;
+                    while (m) {
+                        const uint32_t j = c * 64 + (__ffsll(static_cast<long long>(m)) - 1);
+                        comp_add(acc, comp, static_cast<float>(cvals[2 * j]));
+                        m &= m - 1;
+                    }
+                }
+            }
+            if (any) {
+                n.fring_exc[slot] = acc + comp;
+            }
+        }
+        if (nI > 0) {
+            const size_t slot = static_cast<size_t>((T0 + k + n.d_inh) % n.S) * N + i;
+            float acc = n.fring_inh[slot], comp = 0.0f;
+            bool any = false;
+            for (uint32_t c = 0; c < (nI + 63) / 64; ++c) {
+                uint64_t* mp = n.mask_inh + (static_cast<size_t>(k) * n.cI + c) * N + i;
+                uint64_t m = *mp;
+                if (m) {
+                    *mp = 0;
+                    any = true;
+                    while (m) {
+                        const uint32_t j = nE + c * 64 + (__ffsll(static_cast<long long>(m)) - 1);
+                        comp_add(acc, comp, -static_cast<float>(cvals[2 * j]));
+                        m &= m - 1;
+                    }
+                }
+            }
+            if (any) {
+                n.fring_inh[slot] = acc + comp;
+            }
+        }
+        return;
+    }
+    const uint32_t NA = n.n_astro;
+    if (i >= NA || nE == 0) {
+        return;
+    }
+    (void)tot;
+    const uint32_t* list = n.spk_list + static_cast<size_t>(k) * N;
+    const size_t slot = static_cast<size_t>((T0 + k + n.d_n2a) % n.S) * NA + i;
+    float acc = n.fring_astro[slot], comp = 0.0f;
+    bool any = false;
+    for (uint32_t c = 0; c < (nE + 63) / 64; ++c) {
+        uint64_t* mp = n.mask_astro + (static_cast<size_t>(k) * n.cE + c) * NA + i;
+        uint64_t m = *mp;
+        if (m) {
+            *mp = 0;
+            any = true;
+            while (m) {
+                const uint32_t j = c * 64 + (__ffsll(static_cast<long long>(m)) - 1);
+                const uint32_t mult = n2a_multiplicity(n, list[j], i);
+                comp_add(acc, comp, static_cast<float>(mult) * static_cast<float>(cvals[2 * j + 1]));
+                m &= m - 1;
+            }
+        }
+    }
+    if (any) {
+        n.fring_astro[slot] = acc + comp;
+    }
 }
 
 // Window pass 4, one lane per (target, step), four wavefronts per block: add
@@ -716,6 +1056,10 @@ __global__ __launch_bounds__(256) void k_sum(DevNet* nets, const CellBlock* bloc
     const uint32_t N = n.n_neur;
     const uint32_t tot = n.spk_n[2 * k], nE = n.spk_n[2 * k + 1], nI = tot - nE;
     const double* cvals = n.spk_c + static_cast<size_t>(k) * N * 2;
+    if (n.precision == ASG_FP32) {
+        sum_f32(n, cb.kind, i, k, T0, tot, nE, nI, cvals);
+        return;
+    }
     if (cb.kind == 0) {
         if (i >= N) {
             return;
@@ -851,6 +1195,44 @@ InputK make_input_k(const asg_input_params& in, double dt) {
     k.noise_std = in.gauss_noise_std;
     k.independent = in.independent_noise != 0;
     k.steps_per_noise = steps_per_noise(in.noise_dt, dt);
+    return k;
+}
+
+NeuronKf make_neuron_kf(const asg_neuron_params& s, double h) {
+    NeuronKf p{};
+    p.inv_C_m = static_cast<float>(1.0 / s.C_m);
+    p.neg_g_L = static_cast<float>(-s.g_L);
+    p.E_L = static_cast<float>(s.E_L);
+    p.V_th = static_cast<float>(s.V_th);
+    p.inv_Delta_T = static_cast<float>(1.0 / s.Delta_T);
+    p.a = static_cast<float>(s.a);
+    p.inv_tau_w = static_cast<float>(1.0 / s.tau_w);
+    p.V_peak = static_cast<float>(s.V_peak);
+    p.E_ex = static_cast<float>(s.E_ex);
+    p.E_in = static_cast<float>(s.E_in);
+    p.I_e = static_cast<float>(s.I_e);
+    p.gL_DeltaT = static_cast<float>(s.g_L * s.Delta_T);
+    constexpr double kE = 2.7182818284590452353602874713527;
+    p.psc_ex = static_cast<float>(kE / s.tau_syn_ex);
+    p.psc_in = static_cast<float>(kE / s.tau_syn_in);
+    p.ex_decay = static_cast<float>(asg_exp(-h / s.tau_syn_ex));
+    p.in_decay = static_cast<float>(asg_exp(-h / s.tau_syn_in));
+    p.near_V = static_cast<float>(s.V_th - kNearThresholdSlopes * s.Delta_T);
+    return p;
+}
+
+AstroKf make_astro_kf(const asg_astro_constants& c) {
+    AstroKf k{};
+    k.inv_ratio_ER_cyt = static_cast<float>(1.0 / c.ratio_ER_cyt);
+    k.Kd_IP3_1 = static_cast<float>(c.Kd_IP3_1);
+    k.Kd_IP3_2 = static_cast<float>(c.Kd_IP3_2);
+    k.Kd_act = static_cast<float>(c.Kd_act);
+    k.ratio_rate_IP3R = static_cast<float>(c.ratio_ER_cyt * c.rate_IP3R);
+    k.ratio_rate_L = static_cast<float>(c.ratio_ER_cyt * c.rate_L);
+    k.rate_SERCA = static_cast<float>(c.rate_SERCA);
+    k.Km_SERCA_sq = static_cast<float>(c.Km_SERCA * c.Km_SERCA);
+    k.k_Kd_inh = static_cast<float>(c.k_IP3R * c.Kd_inh);
+    k.k_IP3R = static_cast<float>(c.k_IP3R);
     return k;
 }
 
@@ -1017,10 +1399,11 @@ Plan plan_of(const asg_net_desc& d, const DerivedConn* dc) {
     const size_t a2n_raw = d.a2n.row_start[d.n_astro];
     p.n2a_dedup = dc ? dc->n2a_tgt.size() : n2a_raw;
     p.sic_pairs = dc ? dc->sic_src.size() : a2n_raw;  // ELL entries
+    const size_t fb = d.precision == ASG_FP32 ? 4 : 8;  // bytes of a state value
     size_t b = 0;
-    b += p.N * (7 * 8 + 4 + 2 * 8 + 3 * 8);        // neuron state, V_reset/b, STP
-    b += p.NA * (3 * 8 + 4 * 8);                    // astrocyte state and parameters
-    b += p.S * p.N * 3 * 8 + p.S * p.NA * 8 + p.Wmax * p.NA * 8;
+    b += p.N * (7 * fb + 4 + 2 * 8 + 3 * 8);       // neuron state, V_reset/b, STP
+    b += p.NA * (3 * fb + 4 * fb);                  // astrocyte state and parameters
+    b += p.S * p.N * 3 * fb + p.S * p.NA * fb + p.Wmax * p.NA * fb;
     b += (d.n_exc + 1 + p.n_exc_syn + d.n_inh + 1 + p.n_inh_syn) * 4;
     b += (2 * (d.n_exc + 1) + 2 * p.n2a_dedup) * 4;
     b += 2 * p.sic_pairs * 4;
@@ -1040,6 +1423,8 @@ struct NetSlot {
     int W = 1;
     int64_t step = 0;
     int64_t total = 0;            // run length; INT64_MAX for asg_net
+    void* rec_alloc = nullptr;    // recording ring (owned separately: it can be replaced)
+    int64_t rec_read = 0;         // records handed to the host so far
     std::vector<int64_t> q_step;  // drained raster not yet handed out
     std::vector<uint32_t> q_src;
     size_t q_pos = 0;
@@ -1084,6 +1469,11 @@ void release(NetSlot& s) {
         (void)hipFree(p);
     }
     s.allocs.clear();
+    if (s.rec_alloc != nullptr) {
+        (void)hipFree(s.rec_alloc);
+        s.rec_alloc = nullptr;
+    }
+    s.rec_read = 0;
     s.used = false;
     s.q_step.clear();
     s.q_src.clear();
@@ -1134,33 +1524,70 @@ void build_slot(NetSlot& s, const asg_net_desc& d, int64_t total, hipStream_t st
     n.w_n2a = d.n2a.weight;
     n.w_a2n = d.a2n.weight;
 
+    n.precision = d.precision;
+    n.pkf[0] = make_neuron_kf(d.neuron[0], n.h);
+    n.pkf[1] = make_neuron_kf(d.neuron[1], n.h);
+    n.akf = make_astro_kf(d.astro.c);
     const size_t N = pl.N, NA = pl.NA;
     std::vector<double> V0(N);
     for (uint32_t i = 0; i < N; ++i) {
         V0[i] = d.neuron[i < d.n_exc ? 0 : 1].V_m_init;
     }
-    n.V = dev_copy(s, V0.data(), N, st);
-    n.w = dev_zero<double>(s, N, st);
-    n.gex = dev_zero<double>(s, N, st);
-    n.dgex = dev_zero<double>(s, N, st);
-    n.gin = dev_zero<double>(s, N, st);
-    n.dgin = dev_zero<double>(s, N, st);
-    n.Isic = dev_zero<double>(s, N, st);
     n.refr = dev_zero<int32_t>(s, N, st);
     n.V_reset = dev_copy(s, d.V_reset, N, st);
     n.b = dev_copy(s, d.b, N, st);
-    n.Ca = dev_fill(s, d.astro.Ca_init, NA, st);
-    n.IP3 = dev_fill(s, d.astro.IP3_init, NA, st);
-    n.hh = dev_fill(s, d.astro.h_init, NA, st);
-    n.Ca_tot = dev_copy(s, d.Ca_tot, NA, st);
-    n.IP3_0 = dev_copy(s, d.IP3_0, NA, st);
-    n.tau_IP3 = dev_copy(s, d.tau_IP3, NA, st);
-    n.delta_IP3 = dev_copy(s, d.delta_IP3, NA, st);
-    n.ring_exc = dev_zero<double>(s, pl.S * N, st);
-    n.ring_inh = dev_zero<double>(s, pl.S * N, st);
-    n.ring_sic = dev_zero<double>(s, pl.S * N, st);
-    n.ring_astro = dev_zero<double>(s, pl.S * NA, st);
-    n.fac = dev_zero<double>(s, pl.Wmax * NA, st);
+    if (d.precision == ASG_FP32) {
+        auto to_f = [](const double* src, size_t cnt) {
+            std::vector<float> v(cnt);
+            for (size_t q = 0; q < cnt; ++q) v[q] = static_cast<float>(src[q]);
+            return v;
+        };
+        const std::vector<float> V0f = to_f(V0.data(), N);
+        n.fV = dev_copy(s, V0f.data(), N, st);
+        n.fw = dev_zero<float>(s, N, st);
+        n.fgex = dev_zero<float>(s, N, st);
+        n.fdgex = dev_zero<float>(s, N, st);
+        n.fgin = dev_zero<float>(s, N, st);
+        n.fdgin = dev_zero<float>(s, N, st);
+        n.fIsic = dev_zero<float>(s, N, st);
+        n.fCa = dev_fill(s, static_cast<float>(d.astro.Ca_init), NA, st);
+        n.fIP3 = dev_fill(s, static_cast<float>(d.astro.IP3_init), NA, st);
+        n.fhh = dev_fill(s, static_cast<float>(d.astro.h_init), NA, st);
+        const std::vector<float> Ca_tot = to_f(d.Ca_tot, NA), IP3_0 = to_f(d.IP3_0, NA),
+                                 delta = to_f(d.delta_IP3, NA);
+        std::vector<float> inv_tau(NA);
+        for (size_t q = 0; q < NA; ++q) inv_tau[q] = static_cast<float>(1.0 / d.tau_IP3[q]);
+        n.fCa_tot = dev_copy(s, Ca_tot.data(), NA, st);
+        n.fIP3_0 = dev_copy(s, IP3_0.data(), NA, st);
+        n.finv_tau = dev_copy(s, inv_tau.data(), NA, st);
+        n.fdelta = dev_copy(s, delta.data(), NA, st);
+        n.fring_exc = dev_zero<float>(s, pl.S * N, st);
+        n.fring_inh = dev_zero<float>(s, pl.S * N, st);
+        n.fring_sic = dev_zero<float>(s, pl.S * N, st);
+        n.fring_astro = dev_zero<float>(s, pl.S * NA, st);
+        n.ffac = dev_zero<float>(s, pl.Wmax * NA, st);
+        HIP_CHECK(hipStreamSynchronize(st));
+    } else {
+        n.V = dev_copy(s, V0.data(), N, st);
+        n.w = dev_zero<double>(s, N, st);
+        n.gex = dev_zero<double>(s, N, st);
+        n.dgex = dev_zero<double>(s, N, st);
+        n.gin = dev_zero<double>(s, N, st);
+        n.dgin = dev_zero<double>(s, N, st);
+        n.Isic = dev_zero<double>(s, N, st);
+        n.Ca = dev_fill(s, d.astro.Ca_init, NA, st);
+        n.IP3 = dev_fill(s, d.astro.IP3_init, NA, st);
+        n.hh = dev_fill(s, d.astro.h_init, NA, st);
+        n.Ca_tot = dev_copy(s, d.Ca_tot, NA, st);
+        n.IP3_0 = dev_copy(s, d.IP3_0, NA, st);
+        n.tau_IP3 = dev_copy(s, d.tau_IP3, NA, st);
+        n.delta_IP3 = dev_copy(s, d.delta_IP3, NA, st);
+        n.ring_exc = dev_zero<double>(s, pl.S * N, st);
+        n.ring_inh = dev_zero<double>(s, pl.S * N, st);
+        n.ring_sic = dev_zero<double>(s, pl.S * N, st);
+        n.ring_astro = dev_zero<double>(s, pl.S * NA, st);
+        n.fac = dev_zero<double>(s, pl.Wmax * NA, st);
+    }
     n.stp_x = dev_fill(s, 1.0, N, st);
     n.stp_u = dev_fill(s, d.stp.U, N, st);
     n.stp_tl = dev_fill(s, -1e12, N, st);
@@ -1466,31 +1893,95 @@ struct Engine {
         return m;
     }
 
+    void set_recording(int slot, int64_t start, int32_t every, uint32_t na, int64_t cap) {
+        wait();
+        NetSlot& s = slots[slot];
+        if (s.rec_alloc != nullptr) {
+            HIP_CHECK(hipFree(s.rec_alloc));
+            s.rec_alloc = nullptr;
+        }
+        DevNet& n = s.dn;
+        n.rec_start = start;
+        n.rec_every = every;
+        n.rec_n = na;
+        n.rec_cap = cap;
+        n.rec_buf = nullptr;
+        if (na > 0 && cap > 0) {
+            HIP_CHECK(hipMalloc(&s.rec_alloc, static_cast<size_t>(cap) * 2 * na * sizeof(double)));
+            n.rec_buf = static_cast<double*>(s.rec_alloc);
+        } else {
+            n.rec_n = 0;
+        }
+        s.rec_read = 0;
+        // Only the recording fields change; write them alone so the device's
+        // step counters are not overwritten with the host's copy.
+        HIP_CHECK(hipMemcpyAsync(&d_nets[slot].rec_start, &n.rec_start,
+                                 offsetof(DevNet, rec_buf) + sizeof(double*) -
+                                     offsetof(DevNet, rec_start),
+                                 hipMemcpyHostToDevice, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+    }
+
+    // Records available = recording steps in [rec_start, step).
+    int64_t read_trace(int slot, double* out, int64_t* steps, int64_t max_rec, bool& overflow) {
+        wait();
+        NetSlot& s = slots[slot];
+        const DevNet& n = s.dn;
+        overflow = false;
+        if (n.rec_n == 0 || s.step <= n.rec_start) return 0;
+        const int64_t avail = (s.step - 1 - n.rec_start) / n.rec_every + 1;
+        if (avail - s.rec_read > n.rec_cap) {
+            overflow = true;
+            return 0;
+        }
+        const int64_t m = std::min(avail - s.rec_read, max_rec);
+        const size_t rec_doubles = 2 * static_cast<size_t>(n.rec_n);
+        for (int64_t q = 0; q < m; ++q) {
+            const int64_t r = s.rec_read + q;
+            HIP_CHECK(hipMemcpyAsync(out + q * rec_doubles,
+                                     n.rec_buf + static_cast<size_t>(r % n.rec_cap) * rec_doubles,
+                                     rec_doubles * sizeof(double), hipMemcpyDeviceToHost, stream));
+            if (steps != nullptr) steps[q] = n.rec_start + r * n.rec_every;
+        }
+        HIP_CHECK(hipStreamSynchronize(stream));
+        s.rec_read += m;
+        return m;
+    }
+
     void read_state(int slot, int field, int64_t off, int64_t cnt, double* out) {
         wait();
         const DevNet& n = slots[slot].dn;
+        if (cnt <= 0) return;
         const double* src = nullptr;
+        const float* fsrc = nullptr;
+        const bool f32 = n.precision == ASG_FP32;
         switch (field) {
-            case ASG_FIELD_V: src = n.V; break;
-            case ASG_FIELD_W: src = n.w; break;
-            case ASG_FIELD_G_EX: src = n.gex; break;
-            case ASG_FIELD_DG_EX: src = n.dgex; break;
-            case ASG_FIELD_G_IN: src = n.gin; break;
-            case ASG_FIELD_DG_IN: src = n.dgin; break;
-            case ASG_FIELD_I_SIC: src = n.Isic; break;
-            case ASG_FIELD_CA: src = n.Ca; break;
-            case ASG_FIELD_IP3: src = n.IP3; break;
-            case ASG_FIELD_H: src = n.hh; break;
-            case ASG_FIELD_STP_X: src = n.stp_x; break;
-            case ASG_FIELD_STP_U: src = n.stp_u; break;
-            case ASG_FIELD_STP_TLAST: src = n.stp_tl; break;
+            case ASG_FIELD_V: src = n.V; fsrc = n.fV; break;
+            case ASG_FIELD_W: src = n.w; fsrc = n.fw; break;
+            case ASG_FIELD_G_EX: src = n.gex; fsrc = n.fgex; break;
+            case ASG_FIELD_DG_EX: src = n.dgex; fsrc = n.fdgex; break;
+            case ASG_FIELD_G_IN: src = n.gin; fsrc = n.fgin; break;
+            case ASG_FIELD_DG_IN: src = n.dgin; fsrc = n.fdgin; break;
+            case ASG_FIELD_I_SIC: src = n.Isic; fsrc = n.fIsic; break;
+            case ASG_FIELD_CA: src = n.Ca; fsrc = n.fCa; break;
+            case ASG_FIELD_IP3: src = n.IP3; fsrc = n.fIP3; break;
+            case ASG_FIELD_H: src = n.hh; fsrc = n.fhh; break;
+            case ASG_FIELD_STP_X: src = n.stp_x; fsrc = nullptr; break;
+            case ASG_FIELD_STP_U: src = n.stp_u; fsrc = nullptr; break;
+            case ASG_FIELD_STP_TLAST: src = n.stp_tl; fsrc = nullptr; break;
             default: break;
         }
-        if (cnt > 0) {
-            HIP_CHECK(hipMemcpyAsync(out, src + off, cnt * sizeof(double), hipMemcpyDeviceToHost,
-                                     stream));
+        if (f32 && fsrc != nullptr) {
+            std::vector<float> tmp(static_cast<size_t>(cnt));
+            HIP_CHECK(hipMemcpyAsync(tmp.data(), fsrc + off, cnt * sizeof(float),
+                                     hipMemcpyDeviceToHost, stream));
             HIP_CHECK(hipStreamSynchronize(stream));
+            for (int64_t q = 0; q < cnt; ++q) out[q] = tmp[static_cast<size_t>(q)];
+            return;
         }
+        HIP_CHECK(hipMemcpyAsync(out, src + off, cnt * sizeof(double), hipMemcpyDeviceToHost,
+                                 stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
     }
 };
 
@@ -1569,9 +2060,6 @@ asg_status asg_net_create(const asg_net_desc* desc, asg_net** out) {
     if (desc == nullptr || out == nullptr) return fail(ASG_ERR_INVALID, "NULL argument");
     const std::string err = asg::validate(*desc);
     if (!err.empty()) return fail(ASG_ERR_INVALID, err);
-    if (desc->precision != ASG_FP64) {
-        return fail(ASG_ERR_INVALID, "asg_net_create: only ASG_FP64 is available here");
-    }
     return guarded([&]() -> asg_status {
         auto net = std::make_unique<asg_net>();
         net->eng = std::make_unique<Engine>(1, 0);
@@ -1592,6 +2080,25 @@ asg_status asg_net_step(asg_net* net, int64_t n_steps) {
     if (net == nullptr || n_steps < 0) return fail(ASG_ERR_INVALID, "bad argument");
     return guarded([&]() -> asg_status {
         net->eng->step(n_steps, false);
+        if (net->eng->overflowed(0)) {
+            return fail(ASG_ERR_OVERFLOW, "raster buffer overflowed; drain spikes more often");
+        }
+        return ASG_OK;
+    });
+}
+
+asg_status asg_net_step_async(asg_net* net, int64_t n_steps) {
+    if (net == nullptr || n_steps < 0) return fail(ASG_ERR_INVALID, "bad argument");
+    return guarded([&]() -> asg_status {
+        net->eng->step(n_steps, true);
+        return ASG_OK;
+    });
+}
+
+asg_status asg_net_sync(asg_net* net) {
+    if (net == nullptr) return fail(ASG_ERR_INVALID, "NULL handle");
+    return guarded([&]() -> asg_status {
+        net->eng->wait();
         if (net->eng->overflowed(0)) {
             return fail(ASG_ERR_OVERFLOW, "raster buffer overflowed; drain spikes more often");
         }
@@ -1626,6 +2133,59 @@ asg_status asg_net_counters(asg_net* net, asg_counters* out) {
     return guarded([&] { return counters_common(*net->eng, 0, out); });
 }
 
+namespace {
+asg_status set_recording_common(Engine& e, int32_t slot, int64_t start, int32_t every,
+                                uint32_t na, int64_t cap) {
+    if (slot < 0 || slot >= e.max_nets || !e.slots[slot].used) {
+        return fail(ASG_ERR_INVALID, "no network in this slot");
+    }
+    if (start < 0 || every < 1 || cap < 0 || na > e.slots[slot].dn.n_astro) {
+        return fail(ASG_ERR_INVALID, "bad recording setup");
+    }
+    return guarded([&]() -> asg_status {
+        e.set_recording(slot, start, every, na, cap);
+        return ASG_OK;
+    });
+}
+asg_status read_trace_common(Engine& e, int32_t slot, double* out, int64_t* steps,
+                             int64_t max_rec, int64_t* n_out) {
+    if (slot < 0 || slot >= e.max_nets || !e.slots[slot].used || n_out == nullptr ||
+        max_rec < 0 || (max_rec > 0 && out == nullptr)) {
+        return fail(ASG_ERR_INVALID, "bad argument");
+    }
+    return guarded([&]() -> asg_status {
+        bool overflow = false;
+        *n_out = e.read_trace(slot, out, steps, max_rec, overflow);
+        if (overflow) return fail(ASG_ERR_OVERFLOW, "recording ring wrapped before it was read");
+        return ASG_OK;
+    });
+}
+}  // namespace
+
+asg_status asg_net_set_recording(asg_net* net, int64_t start_step, int32_t every,
+                                 uint32_t n_astro, int64_t capacity) {
+    if (net == nullptr) return fail(ASG_ERR_INVALID, "NULL handle");
+    return set_recording_common(*net->eng, 0, start_step, every, n_astro, capacity);
+}
+
+asg_status asg_net_read_trace(asg_net* net, double* out, int64_t* steps, int64_t max_records,
+                              int64_t* n_out) {
+    if (net == nullptr) return fail(ASG_ERR_INVALID, "NULL handle");
+    return read_trace_common(*net->eng, 0, out, steps, max_records, n_out);
+}
+
+asg_status asg_batch_set_recording(asg_batch* batch, int32_t slot, int64_t start_step,
+                                   int32_t every, uint32_t n_astro, int64_t capacity) {
+    if (batch == nullptr) return fail(ASG_ERR_INVALID, "NULL handle");
+    return set_recording_common(*batch->eng, slot, start_step, every, n_astro, capacity);
+}
+
+asg_status asg_batch_read_trace(asg_batch* batch, int32_t slot, double* out, int64_t* steps,
+                                int64_t max_records, int64_t* n_out) {
+    if (batch == nullptr) return fail(ASG_ERR_INVALID, "NULL handle");
+    return read_trace_common(*batch->eng, slot, out, steps, max_records, n_out);
+}
+
 asg_status asg_batch_create(int32_t max_nets, int64_t budget_bytes, asg_batch** out) {
     if (max_nets < 1 || out == nullptr || budget_bytes < 0) {
         return fail(ASG_ERR_INVALID, "bad argument");
@@ -1650,9 +2210,6 @@ asg_status asg_batch_add(asg_batch* batch, const asg_net_desc* desc, int64_t tot
     }
     const std::string err = asg::validate(*desc);
     if (!err.empty()) return fail(ASG_ERR_INVALID, err);
-    if (desc->precision != ASG_FP64) {
-        return fail(ASG_ERR_INVALID, "asg_batch_add: only ASG_FP64 is available here");
-    }
     return guarded([&]() -> asg_status {
         const int s = batch->eng->add(*desc, total_steps);
         if (s == -1) return fail(ASG_ERR_CAPACITY, "no free slot");

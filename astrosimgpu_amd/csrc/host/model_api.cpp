@@ -107,3 +107,72 @@ asg_status asg_model_get_info(const asg_model* model, asg_model_info* out) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------- analysis ---
+
+#include "analysis.h"
+
+extern "C" {
+
+asg_status asg_analyze_calcium(const double* times, const double* ca, int32_t n_cells,
+                               int64_t n_samples, double threshold, double merge_gap,
+                               double pre_sim_time, double sim_time, double hist_window,
+                               double hist_shift, int32_t round6, asg_calcium_summary* out) {
+    if (out == nullptr || n_cells < 0 || n_samples < 0 ||
+        (n_cells > 0 && n_samples > 0 && (times == nullptr || ca == nullptr))) {
+        return asg_host_fail(ASG_ERR_INVALID, "bad argument");
+    }
+    const asg_host::CalciumSummary s = asg_host::analyze_calcium(
+        times, ca, n_cells, n_samples, threshold, merge_gap, pre_sim_time, sim_time, hist_window,
+        hist_shift, round6 != 0);
+    out->cells = s.cells;
+    out->active = s.active;
+    out->transients = s.transients;
+    out->pairs = s.pairs;
+    out->mean_duration = s.mean_duration;
+    out->sd_duration = s.sd_duration;
+    out->mean_correlation = s.mean_correlation;
+    out->threshold = s.threshold;
+    out->width = s.width;
+    out->shift = s.shift;
+    return ASG_OK;
+}
+
+asg_status asg_detect_events(const double* times, const double* values, int64_t n,
+                             double threshold, int32_t merge, double max_gap, double* onset,
+                             double* offset, int64_t capacity, int64_t* n_events) {
+    if (n < 0 || n_events == nullptr || (n > 0 && (times == nullptr || values == nullptr))) {
+        return asg_host_fail(ASG_ERR_INVALID, "bad argument");
+    }
+    const std::vector<double> t(times, times + n), v(values, values + n);
+    const asg_host::Events ev = asg_host::detect_events(t, v, threshold, merge != 0, max_gap);
+    const int64_t m = static_cast<int64_t>(std::max(ev.onset.size(), ev.offset.size()));
+    *n_events = m;
+    if (m > capacity) return asg_host_fail(ASG_ERR_CAPACITY, "output arrays too small");
+    for (int64_t i = 0; i < m; ++i) {
+        onset[i] = i < static_cast<int64_t>(ev.onset.size()) ? ev.onset[i] : -1.0;
+        offset[i] = i < static_cast<int64_t>(ev.offset.size()) ? ev.offset[i] : -1.0;
+    }
+    return ASG_OK;
+}
+
+double asg_pearson(const double* a, const double* b, int64_t n) {
+    if (n <= 0 || a == nullptr || b == nullptr) return 0.0;
+    return asg_host::pearson(std::vector<double>(a, a + n), std::vector<double>(b, b + n));
+}
+
+asg_status asg_sliding_window_rate(const double* events, int64_t n, double start, double end,
+                                   double width, double shift, double* out, int64_t capacity,
+                                   int64_t* n_out) {
+    if (n < 0 || n_out == nullptr || (n > 0 && events == nullptr)) {
+        return asg_host_fail(ASG_ERR_INVALID, "bad argument");
+    }
+    const std::vector<double> h =
+        asg_host::sliding_window_rate(std::vector<double>(events, events + n), start, end, width, shift);
+    *n_out = static_cast<int64_t>(h.size());
+    if (*n_out > capacity) return asg_host_fail(ASG_ERR_CAPACITY, "output array too small");
+    for (size_t i = 0; i < h.size(); ++i) out[i] = h[i];
+    return ASG_OK;
+}
+
+}  // extern "C"

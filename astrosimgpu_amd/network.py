@@ -113,8 +113,15 @@ class DeviceNetwork:
         self._h = C.c_void_p()
         check(lib().asg_net_create(model.desc, C.byref(self._h)))
 
-    def step(self, n_steps: int) -> None:
-        check(lib().asg_net_step(self._h, int(n_steps)))
+    def step(self, n_steps: int, wait: bool = True) -> None:
+        """Advance n_steps; with wait=False only queue the work (see sync)."""
+        if wait:
+            check(lib().asg_net_step(self._h, int(n_steps)))
+        else:
+            check(lib().asg_net_step_async(self._h, int(n_steps)))
+
+    def sync(self) -> None:
+        check(lib().asg_net_sync(self._h))
 
     @property
     def current_step(self) -> int:
@@ -152,6 +159,18 @@ class DeviceNetwork:
     def set_profiling(self, on: bool) -> None:
         check(lib().asg_net_set_profiling(self._h, 1 if on else 0))
 
+    def set_recording(self, start_step: int, every: int, n_astro: int, capacity: int) -> None:
+        """Record Ca and IP3 of astrocytes [0, n_astro) every `every` steps from
+        start_step on, into a device ring of `capacity` records."""
+        check(lib().asg_net_set_recording(self._h, int(start_step), int(every), int(n_astro),
+                                          int(capacity)))
+        self._rec_n = int(n_astro)
+
+    def read_trace(self, max_records: int = 1 << 16):
+        """Drain recorded samples: (steps int64[r], Ca[r, n], IP3[r, n])."""
+        return _read_trace(lambda out, st, cap, n: lib().asg_net_read_trace(
+            self._h, out, st, cap, n), self._rec_n, max_records)
+
     def kernel_times(self) -> dict:
         """Device milliseconds per kernel since profiling was switched on."""
         t = capi.KernelTimes()
@@ -175,6 +194,7 @@ class Batch:
         self._h = C.c_void_p()
         check(lib().asg_batch_create(int(max_nets), int(budget_bytes), C.byref(self._h)))
         self.models = {}
+        self._rec_n = {}
 
     def add(self, model: Model, total_steps: int) -> int:
         slot = C.c_int32()
@@ -235,6 +255,16 @@ class Batch:
     def set_profiling(self, on: bool) -> None:
         check(lib().asg_batch_set_profiling(self._h, 1 if on else 0))
 
+    def set_recording(self, slot: int, start_step: int, every: int, n_astro: int,
+                      capacity: int) -> None:
+        check(lib().asg_batch_set_recording(self._h, int(slot), int(start_step), int(every),
+                                            int(n_astro), int(capacity)))
+        self._rec_n[slot] = int(n_astro)
+
+    def read_trace(self, slot: int, max_records: int = 1 << 16):
+        return _read_trace(lambda out, st, cap, n: lib().asg_batch_read_trace(
+            self._h, int(slot), out, st, cap, n), self._rec_n.get(slot, 0), max_records)
+
     def kernel_times(self) -> dict:
         t = capi.KernelTimes()
         check(lib().asg_batch_kernel_times(self._h, C.byref(t)))
@@ -251,6 +281,25 @@ class Batch:
 
     def __del__(self):
         self.close()
+
+
+def _read_trace(call, n_rec: int, max_records: int):
+    steps_out, ca_out, ip3_out = [], [], []
+    if n_rec == 0:
+        return np.zeros(0, np.int64), np.zeros((0, 0)), np.zeros((0, 0))
+    buf = np.empty((max_records, 2, n_rec), np.float64)
+    st = np.empty(max_records, np.int64)
+    n = C.c_int64()
+    while True:
+        check(call(buf.ctypes.data_as(capi.DBLP), st.ctypes.data_as(capi.I64P), max_records,
+                   C.byref(n)))
+        k = n.value
+        steps_out.append(st[:k].copy())
+        ca_out.append(buf[:k, 0, :].copy())
+        ip3_out.append(buf[:k, 1, :].copy())
+        if k < max_records:
+            break
+    return np.concatenate(steps_out), np.concatenate(ca_out), np.concatenate(ip3_out)
 
 
 def device_math(fn: str, x: np.ndarray) -> np.ndarray:

@@ -194,6 +194,37 @@ void asg_model_free(asg_model* model);
 const asg_net_desc* asg_model_desc(const asg_model* model);
 asg_status asg_model_get_info(const asg_model* model, asg_model_info* out);
 
+/* Post-run calcium analysis (analysis.cpp; its use in main.cpp:266-317). */
+typedef struct asg_calcium_summary {
+    int32_t cells, active;        /* astrocytes analysed / with >= 1 transient */
+    int64_t transients, pairs;    /* events after merging / correlated pairs */
+    double mean_duration, sd_duration, mean_correlation;
+    double threshold, width, shift;  /* the threshold and window actually used */
+} asg_calcium_summary;
+
+/* ca is [n_samples][n_cells] sampled at times[n_samples] (ms) over the
+ * recorded window [pre_sim_time, pre_sim_time + sim_time]. round6 != 0
+ * rounds times and values to 6 significant digits first, as the
+ * reference's CSV round trip does. */
+asg_status asg_analyze_calcium(const double* times, const double* ca, int32_t n_cells,
+                               int64_t n_samples, double threshold, double merge_gap,
+                               double pre_sim_time, double sim_time, double hist_window,
+                               double hist_shift, int32_t round6, asg_calcium_summary* out);
+
+/* detect_events, analysis.cpp:10-64: onsets/offsets of threshold crossings,
+ * merged when closer than max_gap (if merge != 0). */
+asg_status asg_detect_events(const double* times, const double* values, int64_t n,
+                             double threshold, int32_t merge, double max_gap, double* onset,
+                             double* offset, int64_t capacity, int64_t* n_events);
+
+/* pearson, analysis.cpp:98-123 (0 when either series has no variance). */
+double asg_pearson(const double* a, const double* b, int64_t n);
+
+/* sliding_window_rate, analysis.cpp:66-96: events per ms per window. */
+asg_status asg_sliding_window_rate(const double* events, int64_t n, double start, double end,
+                                   double width, double shift, double* out, int64_t capacity,
+                                   int64_t* n_out);
+
 /* ------------------------------------------------------------------------ */
 /* Layer 1: astrocyte backend (drop-in for astrocyte_cuda.hpp:13-27).        */
 /* FP64, default stream semantics: every call completes before returning.    */
@@ -239,6 +270,12 @@ asg_status asg_net_destroy(asg_net* net);
 /* Advance n_steps >= 0 steps. Returns after the device has finished. */
 asg_status asg_net_step(asg_net* net, int64_t n_steps);
 
+/* Same as asg_net_step but returns once the work is queued on the handle's
+ * stream, so several networks can run on the device at once; asg_net_sync
+ * (or any other call on the handle) waits for it. */
+asg_status asg_net_step_async(asg_net* net, int64_t n_steps);
+asg_status asg_net_sync(asg_net* net);
+
 /* Number of steps taken so far. */
 int64_t asg_net_current_step(const asg_net* net);
 
@@ -253,6 +290,21 @@ asg_status asg_net_read_state(asg_net* net, int32_t field, int64_t offset, int64
                               double* out);
 
 asg_status asg_net_counters(asg_net* net, asg_counters* out);
+
+/* Astrocyte recording (Network::run, network.cpp:449-456): from step
+ * start_step on, every `every` steps, the Ca and IP3 of astrocytes
+ * [0, n_astro) after that step's update are written on the device into a
+ * ring of `capacity` records. A record is 2 * n_astro doubles: Ca then IP3.
+ * Replaces any previous recording setup. */
+asg_status asg_net_set_recording(asg_net* net, int64_t start_step, int32_t every,
+                                 uint32_t n_astro, int64_t capacity);
+
+/* Move up to `max_records` recorded samples (oldest first) to `out`
+ * (max_records * 2 * n_astro doubles) and their step numbers to `steps`;
+ * *n_out is the count. ASG_ERR_OVERFLOW if the ring wrapped before it was
+ * drained. */
+asg_status asg_net_read_trace(asg_net* net, double* out, int64_t* steps, int64_t max_records,
+                              int64_t* n_out);
 
 /* Bytes of device memory asg_net_create would allocate for `desc`. */
 int64_t asg_query_footprint(const asg_net_desc* desc);
@@ -293,6 +345,10 @@ asg_status asg_batch_spikes(asg_batch* batch, int32_t slot, int64_t* step, uint3
 asg_status asg_batch_read_state(asg_batch* batch, int32_t slot, int32_t field, int64_t offset,
                                 int64_t count, double* out);
 asg_status asg_batch_counters(asg_batch* batch, int32_t slot, asg_counters* out);
+asg_status asg_batch_set_recording(asg_batch* batch, int32_t slot, int64_t start_step,
+                                   int32_t every, uint32_t n_astro, int64_t capacity);
+asg_status asg_batch_read_trace(asg_batch* batch, int32_t slot, double* out, int64_t* steps,
+                                int64_t max_records, int64_t* n_out);
 
 /* Device bytes currently held by the batch's networks. */
 int64_t asg_batch_bytes_in_use(const asg_batch* batch);
