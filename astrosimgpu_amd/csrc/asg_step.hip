@@ -111,6 +111,8 @@ asg_status guarded(F&& f) {
 
 constexpr int kCellBlock = 64;     // one wavefront per block of the per-cell kernels
 constexpr int kWindowCap = 16;     // longest window (bounds scratch memory)
+constexpr size_t kMaskBudget = size_t(256) << 20;  // delivery masks per network
+constexpr size_t kMinChunks = 4;
 
 struct InputK {
     double poiss_rate, poiss_weight, lambda, limit;
@@ -136,7 +138,8 @@ struct DevNet {
     int32_t d_exc, d_inh, d_n2a, d_a2n;
     int32_t sic_interval, substeps;
     int32_t win_W;         // length of the current window
-    int32_t overflow;      // raster buffer overflowed
+    int32_t overflow;      // bit 0: raster buffer full; bit 1: a step had more
+                           // spikes than the delivery masks hold (cE|cI * 64)
     int64_t step;          // first step of the next window
     int64_t target;        // step this call stops at
     int64_t win_T0;        // first step of the current window
@@ -730,6 +733,9 @@ __global__ __launch_bounds__(64) void k_compact(DevNet* nets, const int32_t* act
     if (lane == 0) {
         n.spk_n[2 * k] = total;
         n.spk_n[2 * k + 1] = totalE;
+        if (totalE > 64 * n.cE || total - totalE > 64 * n.cI) {
+            atomicOr(&n.overflow, 2);
+        }
         if (total > 0) {
             add_i64(&n.c_spikes, total);
         }
@@ -816,6 +822,9 @@ __device__ void mark_step(const DevNet& n, uint32_t first, uint32_t lane, int k)
         const uint32_t js = off + static_cast<uint32_t>(__popcll(word & ((1ULL << L) - 1ULL)));
         if (s < n.n_exc) {
             const uint64_t bit = 1ULL << (js & 63);
+            if ((js >> 6) >= n.cE) {
+                continue;  // beyond the delivery capacity; k_compact flagged it
+            }
             const size_t chunk = static_cast<size_t>(k) * n.cE + (js >> 6);
             mark_row(n.exc_tgt, n.exc_row[s], n.exc_row[s + 1], n.mask_exc + chunk * N, bit, lane);
             mark_row(n.n2a_tgt, n.n2a_row[s], n.n2a_row[s + 1], n.mask_astro + chunk * n.n_astro,
@@ -823,6 +832,9 @@ __device__ void mark_step(const DevNet& n, uint32_t first, uint32_t lane, int k)
         } else {
             const uint32_t jj = js - nE;
             const uint64_t bit = 1ULL << (jj & 63);
+            if ((jj >> 6) >= n.cI) {
+                continue;
+            }
             const size_t chunk = static_cast<size_t>(k) * n.cI + (jj >> 6);
             const uint32_t r = s - n.n_exc;
             mark_row(n.inh_tgt, n.inh_row[r], n.inh_row[r + 1], n.mask_inh + chunk * N, bit, lane);
@@ -895,7 +907,7 @@ __device__ void deliver_sources(DevNet& n, uint32_t first, uint32_t lane) {
         add_i64(&n.c_syn, syn);
     }
     if (!room && window_total > 0 && t == 0) {
-        n.overflow = 1;
+        atomicOr(&n.overflow, 1);
     }
 }
 
@@ -954,7 +966,7 @@ __device__ void sum_f32(const DevNet& n, int kind, uint32_t i, int k, int64_t T0
             const size_t slot = static_cast<size_t>((T0 + k + n.d_exc) % n.S) * N + i;
             float acc = n.fring_exc[slot], comp = 0.0f;
             bool any = false;
-            for (uint32_t c = 0; c < (nE + 63) / 64; ++c) {
+            for (uint32_t c = 0; c < min((nE + 63) / 64, n.cE); ++c) {
                 uint64_t* mp = n.mask_exc + (static_cast<size_t>(k) * n.cE + c) * N + i;
                 uint64_t m = *mp;
                 if (m) {
@@ -975,7 +987,7 @@ __device__ void sum_f32(const DevNet& n, int kind, uint32_t i, int k, int64_t T0
             const size_t slot = static_cast<size_t>((T0 + k + n.d_inh) % n.S) * N + i;
             float acc = n.fring_inh[slot], comp = 0.0f;
             bool any = false;
-            for (uint32_t c = 0; c < (nI + 63) / 64; ++c) {
+            for (uint32_t c = 0; c < min((nI + 63) / 64, n.cI); ++c) {
                 uint64_t* mp = n.mask_inh + (static_cast<size_t>(k) * n.cI + c) * N + i;
                 uint64_t m = *mp;
                 if (m) {
@@ -1003,7 +1015,7 @@ __device__ void sum_f32(const DevNet& n, int kind, uint32_t i, int k, int64_t T0
     const size_t slot = static_cast<size_t>((T0 + k + n.d_n2a) % n.S) * NA + i;
     float acc = n.fring_astro[slot], comp = 0.0f;
     bool any = false;
-    for (uint32_t c = 0; c < (nE + 63) / 64; ++c) {
+    for (uint32_t c = 0; c < min((nE + 63) / 64, n.cE); ++c) {
         uint64_t* mp = n.mask_astro + (static_cast<size_t>(k) * n.cE + c) * NA + i;
         uint64_t m = *mp;
         if (m) {
@@ -1068,7 +1080,7 @@ __global__ __launch_bounds__(256) void k_sum(DevNet* nets, const CellBlock* bloc
             const size_t slot = static_cast<size_t>((T0 + k + n.d_exc) % n.S) * N + i;
             double acc = n.ring_exc[slot];
             bool any = false;
-            for (uint32_t c = 0; c < (nE + 63) / 64; ++c) {
+            for (uint32_t c = 0; c < min((nE + 63) / 64, n.cE); ++c) {
                 uint64_t* mp = n.mask_exc + (static_cast<size_t>(k) * n.cE + c) * N + i;
                 uint64_t m = *mp;
                 if (m) {
@@ -1089,7 +1101,7 @@ __global__ __launch_bounds__(256) void k_sum(DevNet* nets, const CellBlock* bloc
             const size_t slot = static_cast<size_t>((T0 + k + n.d_inh) % n.S) * N + i;
             double acc = n.ring_inh[slot];
             bool any = false;
-            for (uint32_t c = 0; c < (nI + 63) / 64; ++c) {
+            for (uint32_t c = 0; c < min((nI + 63) / 64, n.cI); ++c) {
                 uint64_t* mp = n.mask_inh + (static_cast<size_t>(k) * n.cI + c) * N + i;
                 uint64_t m = *mp;
                 if (m) {
@@ -1115,7 +1127,7 @@ __global__ __launch_bounds__(256) void k_sum(DevNet* nets, const CellBlock* bloc
         const size_t slot = static_cast<size_t>((T0 + k + n.d_n2a) % n.S) * NA + i;
         double acc = n.ring_astro[slot];
         bool any = false;
-        for (uint32_t c = 0; c < (nE + 63) / 64; ++c) {
+        for (uint32_t c = 0; c < min((nE + 63) / 64, n.cE); ++c) {
             uint64_t* mp = n.mask_astro + (static_cast<size_t>(k) * n.cE + c) * NA + i;
             uint64_t m = *mp;
             if (m) {
@@ -1390,8 +1402,22 @@ Plan plan_of(const asg_net_desc& d, const DerivedConn* dc) {
     const int dmax = std::max(std::max(d.exc.delay_steps, d.inh.delay_steps),
                               std::max(d.n2a.delay_steps, d.a2n.delay_steps));
     p.S = static_cast<size_t>(dmax) + p.Wmax;
+    // Delivery masks hold 64 spikes per chunk. Every source spiking in one
+    // step needs ceil(n / 64) chunks; that is kept while the masks fit in
+    // kMaskBudget, otherwise the chunk count is cut to what fits (at least
+    // kMinChunks, i.e. 256 spikes per step and type), and a step that exceeds
+    // it is reported as ASG_ERR_OVERFLOW rather than delivered in part.
     p.cE = (d.n_exc + 63) / 64;
     p.cI = (d.n_inh + 63) / 64;
+    {
+        const size_t per_chunk = p.Wmax * (2 * p.N + p.NA) * 8;
+        const size_t full = p.Wmax * (p.cE * p.N + p.cI * p.N + p.cE * p.NA) * 8;
+        if (full > kMaskBudget && per_chunk > 0) {
+            const size_t cap = std::max<size_t>(kMinChunks, kMaskBudget / per_chunk);
+            p.cE = std::min(p.cE, cap);
+            p.cI = std::min(p.cI, cap);
+        }
+    }
     p.r_cap = std::max<size_t>(size_t(1) << 20, 16 * p.N);
     p.n_exc_syn = d.exc.row_start[d.n_exc];
     p.n_inh_syn = d.inh.row_start[d.n_inh];
@@ -1867,6 +1893,15 @@ struct Engine {
         }
     }
 
+    // 0, or the overflow bits of a slot (see DevNet::overflow).
+    int overflow_bits(int slot) {
+        int32_t flag = 0;
+        HIP_CHECK(hipMemcpyAsync(&flag, &d_nets[slot].overflow, sizeof(int32_t),
+                                 hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+        return flag;
+    }
+
     bool overflowed(int slot) {
         int32_t flag = 0;
         HIP_CHECK(hipMemcpyAsync(&flag, &d_nets[slot].overflow, sizeof(int32_t),
@@ -2003,6 +2038,14 @@ struct asg_batch {
 
 namespace {
 
+const char* overflow_message(int bits) {
+    if (bits & 2) {
+        return "a step had more spikes than the delivery capacity of this network "
+               "(its deliveries were incomplete; results from that step on are invalid)";
+    }
+    return "raster buffer overflowed; drain spikes more often";
+}
+
 int64_t field_size(const asg::NetSlot& s, int32_t field) {
     if (field >= ASG_FIELD_CA && field <= ASG_FIELD_H) return s.dn.n_astro;
     return s.dn.n_neur;
@@ -2080,8 +2123,8 @@ asg_status asg_net_step(asg_net* net, int64_t n_steps) {
     if (net == nullptr || n_steps < 0) return fail(ASG_ERR_INVALID, "bad argument");
     return guarded([&]() -> asg_status {
         net->eng->step(n_steps, false);
-        if (net->eng->overflowed(0)) {
-            return fail(ASG_ERR_OVERFLOW, "raster buffer overflowed; drain spikes more often");
+        if (const int bits = net->eng->overflow_bits(0)) {
+            return fail(ASG_ERR_OVERFLOW, overflow_message(bits));
         }
         return ASG_OK;
     });
@@ -2099,8 +2142,8 @@ asg_status asg_net_sync(asg_net* net) {
     if (net == nullptr) return fail(ASG_ERR_INVALID, "NULL handle");
     return guarded([&]() -> asg_status {
         net->eng->wait();
-        if (net->eng->overflowed(0)) {
-            return fail(ASG_ERR_OVERFLOW, "raster buffer overflowed; drain spikes more often");
+        if (const int bits = net->eng->overflow_bits(0)) {
+            return fail(ASG_ERR_OVERFLOW, overflow_message(bits));
         }
         return ASG_OK;
     });
@@ -2262,8 +2305,8 @@ asg_status asg_batch_spikes(asg_batch* batch, int32_t slot, int64_t* step, uint3
         return fail(ASG_ERR_INVALID, "bad argument");
     }
     return guarded([&]() -> asg_status {
-        if (batch->eng->overflowed(slot)) {
-            return fail(ASG_ERR_OVERFLOW, "raster buffer overflowed; drain spikes more often");
+        if (const int bits = batch->eng->overflow_bits(slot)) {
+            return fail(ASG_ERR_OVERFLOW, overflow_message(bits));
         }
         *n_out = batch->eng->spikes_out(slot, step, source, capacity);
         return ASG_OK;

@@ -18,7 +18,7 @@
 
 namespace asg {
 
-constexpr float kNearThresholdSlopes = 5.0f;
+constexpr float kNearThresholdSlopes = 2.0f;
 
 struct NeuronKf {
     float inv_C_m, neg_g_L, E_L, V_th, inv_Delta_T, a, inv_tau_w, V_peak, E_ex, E_in, I_e;

@@ -198,6 +198,14 @@ ModelConfig map_config(const JsonValue& root, const std::string& name, const Ove
         c.read("pool_size", cfg.conn.pool_size);
         c.read("allow_autapses", cfg.conn.allow_autapses);
         c.read("unique_third_out", cfg.conn.unique_third_out);
+        std::string generator = "sequential";
+        c.read("generator", generator);
+        if (generator == "per_row") {
+            cfg.conn.per_row = true;
+        } else if (generator != "sequential") {
+            throw std::runtime_error("unknown connectivity.generator '" + generator +
+                                     "' (expected \"sequential\" or \"per_row\")");
+        }
         std::string pool_type = "block";
         c.read("pool_type", pool_type);
         if (pool_type == "block") {

@@ -26,6 +26,12 @@ struct Connectivity {  // parameters.hpp:104-122
     PoolType pool_type = PoolType::Block;
     bool allow_autapses = false;
     bool unique_third_out = false;
+    // Connection generator: false = the reference's single sequential stream
+    // (bit-identical networks, O(N_pre * N_post) draws); true = one counter
+    // stream per row with geometric gap sampling (the same Bernoulli
+    // distribution of connections, O(connections) draws, rows in parallel).
+    // Key "generator": "sequential" | "per_row"; not a reference key.
+    bool per_row = false;
 };
 
 struct Synapses {  // parameters.hpp:69-99

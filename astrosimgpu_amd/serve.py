@@ -1,0 +1,203 @@
+"""Serving loop: simulation jobs arrive over time and share the device.
+
+Continuous batching: all running jobs are networks of one asg_batch and
+advance together, one chunk of time steps per device call. Between chunks the
+loop
+  * hands newly arrived jobs to host builder threads (network construction
+    with the per-row generator, which scales to 1e5-neuron jobs),
+  * admits built jobs in arrival order while a slot is free and their device
+    footprint fits the memory budget (asg_batch_add refuses otherwise),
+  * retires finished jobs: their raster and summary are collected and their
+    device memory is freed at once, which is what lets waiting jobs in.
+A job's result does not depend on which other jobs shared its launches: each
+network's arithmetic is its own, so a job run alone and in the mix gives the
+same bits.
+
+Reported: neuron-steps/s over the whole run (every job's neurons x steps /
+makespan) and p50 / p99 of job latency (completion - arrival).
+"""
+
+from __future__ import annotations
+
+import concurrent.futures as cf
+import math
+import time
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from . import _capi as capi
+from .network import Batch, Model
+
+
+@dataclass
+class Job:
+    id: int
+    arrival_s: float
+    n: int
+    n_astro: int
+    duration_ms: float
+    w_n2a: float
+    seed: int
+    precision: int = capi.ASG_FP32
+    # filled by the server
+    model: Optional[Model] = None
+    slot: Optional[int] = None
+    total_steps: int = 0
+    admitted_s: float = math.nan
+    done_s: float = math.nan
+    spike_steps: list = field(default_factory=list)
+    spike_src: list = field(default_factory=list)
+
+    @property
+    def latency_s(self) -> float:
+        return self.done_s - self.arrival_s
+
+    def raster(self):
+        if not self.spike_steps:
+            return np.zeros(0, np.int64), np.zeros(0, np.uint32)
+        return np.concatenate(self.spike_steps), np.concatenate(self.spike_src)
+
+
+def job_stream(count: int, rate_per_s: float, seed: int = 7, n_min: int = 500,
+               n_max: int = 100000, astro_ratio=(0.2, 1.0), duration_ms=(1000.0, 10000.0),
+               w_range=(0.10, 0.40), precision: int = capi.ASG_FP32):
+    """Poisson arrivals at `rate_per_s`; sizes log-uniform, astrocyte ratio,
+    duration and IP3 weight uniform, independent seeds."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    jobs = []
+    for i in range(count):
+        t += rng.exponential(1.0 / rate_per_s)
+        n = int(round(math.exp(rng.uniform(math.log(n_min), math.log(n_max)))))
+        na = max(1, int(round(rng.uniform(*astro_ratio) * n)))
+        jobs.append(Job(i, t, n, na, float(rng.uniform(*duration_ms)), float(rng.uniform(*w_range)),
+                        int(rng.integers(1, 2**31)), precision))
+    return jobs
+
+
+def build_job(base_config: str, job: Job, in_degree: float = 100.0) -> Model:
+    n_exc = int(round(0.8 * job.n))
+    m = Model(base_config, seed=job.seed, sim_time=job.duration_ms, pre_time=0.0,
+              assignments={"N.N_E": n_exc, "N.N_I": job.n - n_exc, "N.N_A": job.n_astro,
+                           "synapse.w_n2a": job.w_n2a,
+                           "connectivity.p_primary": min(1.0, in_degree / job.n),
+                           "connectivity.generator": "per_row",
+                           "simulation.record_astro": False})
+    m.desc.contents.precision = job.precision
+    return m
+
+
+@dataclass
+class ServeReport:
+    jobs: int
+    makespan_s: float
+    neuron_steps: int
+    neuron_steps_per_s: float
+    latency_p50_s: float
+    latency_p99_s: float
+    refused_admissions: int    # jobs that waited for a slot or memory
+    peak_running: int
+    peak_bytes: int
+
+
+class Server:
+    """`batch` is the device batch (an asg_batch with `max_slots` slots and a
+    memory budget of `budget_bytes`, 0 = free device memory, by default);
+    `builder(job) -> Model` constructs a job's network on the host."""
+
+    def __init__(self, base_config: str, max_slots: int = 512, budget_bytes: int = 0,
+                 chunk_steps: int = 2000, builders: int = 2, batch=None, builder=None):
+        self.base_config = base_config
+        self.batch = batch if batch is not None else Batch(max_slots, budget_bytes)
+        self.builder = builder if builder is not None else (lambda j: build_job(base_config, j))
+        self.chunk_steps = chunk_steps
+        self.pool = cf.ThreadPoolExecutor(max_workers=builders)
+        self.refused: set = set()   # ids of jobs that had to wait for room
+
+    def run(self, jobs, clock=None, sleep=None, progress=None) -> ServeReport:
+        """Serve `jobs` (arrival times in seconds from the start). `clock()`
+        returns seconds since the start and `sleep(s)` waits; both default to
+        the wall clock."""
+        if clock is None:
+            t0 = time.perf_counter()
+            clock = lambda: time.perf_counter() - t0  # noqa: E731
+        if sleep is None:
+            sleep = time.sleep
+        pending = sorted(jobs, key=lambda j: j.arrival_s)
+        building: list = []      # (job, future)
+        ready: list = []         # built, waiting for room
+        running: dict = {}       # slot -> job
+        done: list = []
+        peak_running = 0
+        peak_bytes = 0
+        i = 0
+        while len(done) < len(jobs):
+            now = clock()
+            # arrivals -> builders
+            while i < len(pending) and pending[i].arrival_s <= now:
+                j = pending[i]
+                building.append((j, self.pool.submit(self.builder, j)))
+                i += 1
+            # finished builds -> ready
+            still = []
+            for j, fut in building:
+                if fut.done():
+                    j.model = fut.result()
+                    j.total_steps = j.model.info["sim_steps"]
+                    ready.append(j)
+                else:
+                    still.append((j, fut))
+            building = still
+            ready.sort(key=lambda j: j.arrival_s)
+            # admission by slot and footprint, first come first served: the
+            # oldest waiting job blocks later ones, so none is starved
+            while ready and not (building and building[0][0].arrival_s < ready[0].arrival_s):
+                j = ready[0]
+                slot = self.batch.try_add(j.model, j.total_steps)
+                if slot is None:
+                    self.refused.add(j.id)
+                    break
+                ready.pop(0)
+                j.slot = slot
+                j.admitted_s = clock()
+                running[slot] = j
+            peak_running = max(peak_running, len(running))
+            peak_bytes = max(peak_bytes, self.batch.bytes_in_use())
+            if not running:
+                if building:
+                    cf.wait([f for _, f in building], return_when=cf.FIRST_COMPLETED)
+                elif i < len(pending):
+                    sleep(max(0.0, pending[i].arrival_s - clock()))
+                elif ready:
+                    raise RuntimeError(f"job {ready[0].id} does not fit an empty batch")
+                continue
+            # one chunk for every running job
+            self.batch.step(self.chunk_steps)
+            finished = []
+            for slot, j in running.items():
+                st, src = self.batch.spikes(slot, 1 << 16)
+                j.spike_steps.append(st)
+                j.spike_src.append(src)
+                if self.batch.net_step(slot) >= j.total_steps:
+                    finished.append(slot)
+            for slot in finished:
+                j = running.pop(slot)
+                self.batch.remove(slot)
+                j.done_s = clock()
+                j.model = None
+                done.append(j)
+                if progress is not None:
+                    progress(len(done), len(jobs))
+        makespan = clock()
+        ns = sum(j.n * j.total_steps for j in done)
+        lat = np.array([j.latency_s for j in done])
+        return ServeReport(len(done), makespan, ns, ns / makespan if makespan > 0 else 0.0,
+                           float(np.percentile(lat, 50)) if len(lat) else 0.0,
+                           float(np.percentile(lat, 99)) if len(lat) else 0.0,
+                           len(self.refused), peak_running, peak_bytes)
+
+    def close(self):
+        self.pool.shutdown(wait=True)
+        self.batch.close()
