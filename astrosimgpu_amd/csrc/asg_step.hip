@@ -1150,6 +1150,47 @@ __global__ __launch_bounds__(256) void k_sum(DevNet* nets, const CellBlock* bloc
 }
 
 // Element-wise evaluation of the step's FP64 functions (diagnostics).
+// Per-call bookkeeping of a batch, one launch each instead of one copy per
+// network: the new run targets of the active networks, and the raster heads
+// and overflow bits read back together.
+__global__ __launch_bounds__(256) void k_set_targets(DevNet* nets, const int32_t* active,
+                                                     int32_t n_active, const int64_t* targets) {
+    const int j = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (j < n_active) {
+        nets[active[j]].target = targets[j];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_raster_heads(const DevNet* nets, const int32_t* active,
+                                                      int32_t n_active, int64_t* heads,
+                                                      int32_t* flags) {
+    const int j = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (j < n_active) {
+        heads[j] = nets[active[j]].r_head;
+        flags[j] = nets[active[j]].overflow;
+    }
+}
+
+// Packs the raster entries of active network blockIdx.y (heads[y] of them)
+// at offsets[y] of one staging buffer and empties that network's raster.
+__global__ __launch_bounds__(256) void k_raster_gather(DevNet* nets, const int32_t* active,
+                                                       const int64_t* heads,
+                                                       const int64_t* offsets, int64_t* out_step,
+                                                       uint32_t* out_src) {
+    DevNet& n = nets[active[blockIdx.y]];
+    const int64_t cnt = heads[blockIdx.y];
+    const int64_t off = offsets[blockIdx.y];
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < cnt;
+         e += stride) {
+        out_step[off + e] = n.r_step[e];
+        out_src[off + e] = n.r_src[e];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0 && cnt > 0) {
+        n.r_head = 0;  // every kernel that appends has finished (same stream)
+    }
+}
+
 __global__ void k_math(int32_t fn, const double* x, double* y, int64_t n) {
     for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
          i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
@@ -1454,6 +1495,7 @@ struct NetSlot {
     std::vector<int64_t> q_step;  // drained raster not yet handed out
     std::vector<uint32_t> q_src;
     size_t q_pos = 0;
+    int32_t overflow = 0;         // overflow bits read at the last drain
 };
 
 template <typename T>
@@ -1504,6 +1546,7 @@ void release(NetSlot& s) {
     s.q_step.clear();
     s.q_src.clear();
     s.q_pos = 0;
+    s.overflow = 0;
 }
 
 void build_slot(NetSlot& s, const asg_net_desc& d, int64_t total, hipStream_t st) {
@@ -1665,7 +1708,25 @@ struct Engine {
     int32_t* d_active = nullptr;
     size_t blocks_cap = 0, nblocks_cap = 0, active_cap = 0;
     int n_blocks = 0, n_nblocks = 0, n_active = 0, Wg = 1;
+    std::vector<int32_t> h_active;     // host copy of d_active
     bool dirty = true;
+    // Batch-wide bookkeeping buffers: targets in, raster heads and overflow
+    // bits out (pinned host + device, max_nets each), and the raster staging
+    // buffer that one drain packs every network's new entries into.
+    int64_t* h_targets = nullptr;
+    int64_t* d_targets = nullptr;
+    int64_t* h_heads = nullptr;
+    int64_t* d_heads = nullptr;
+    int32_t* h_flags = nullptr;
+    int32_t* d_flags = nullptr;
+    int64_t* h_offsets = nullptr;
+    int64_t* d_offsets = nullptr;
+    int64_t* d_stage_step = nullptr;
+    uint32_t* d_stage_src = nullptr;
+    int64_t* h_stage_step = nullptr;
+    uint32_t* h_stage_src = nullptr;
+    size_t stage_cap = 0;
+    bool raster_dirty = false;         // steps ran since the last drain
     bool pending = false;  // async work in flight
     // Per-kernel event timing (off unless asked for).
     bool profiling = false;
@@ -1677,7 +1738,16 @@ struct Engine {
         HIP_CHECK(hipGetDevice(&device));
         HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         slots.resize(static_cast<size_t>(max_n));
-        HIP_CHECK(hipMalloc(&d_nets, sizeof(DevNet) * static_cast<size_t>(max_n)));
+        const size_t m = static_cast<size_t>(max_n);
+        HIP_CHECK(hipMalloc(&d_nets, sizeof(DevNet) * m));
+        HIP_CHECK(hipHostMalloc(&h_targets, sizeof(int64_t) * m, hipHostMallocDefault));
+        HIP_CHECK(hipHostMalloc(&h_heads, sizeof(int64_t) * m, hipHostMallocDefault));
+        HIP_CHECK(hipHostMalloc(&h_flags, sizeof(int32_t) * m, hipHostMallocDefault));
+        HIP_CHECK(hipHostMalloc(&h_offsets, sizeof(int64_t) * m, hipHostMallocDefault));
+        HIP_CHECK(hipMalloc(&d_targets, sizeof(int64_t) * m));
+        HIP_CHECK(hipMalloc(&d_heads, sizeof(int64_t) * m));
+        HIP_CHECK(hipMalloc(&d_flags, sizeof(int32_t) * m));
+        HIP_CHECK(hipMalloc(&d_offsets, sizeof(int64_t) * m));
     }
 
     ~Engine() {
@@ -1688,6 +1758,18 @@ struct Engine {
         (void)hipFree(d_blocks);
         (void)hipFree(d_nblocks);
         (void)hipFree(d_active);
+        (void)hipFree(d_targets);
+        (void)hipFree(d_heads);
+        (void)hipFree(d_flags);
+        (void)hipFree(d_offsets);
+        (void)hipFree(d_stage_step);
+        (void)hipFree(d_stage_src);
+        (void)hipHostFree(h_targets);
+        (void)hipHostFree(h_heads);
+        (void)hipHostFree(h_flags);
+        (void)hipHostFree(h_offsets);
+        (void)hipHostFree(h_stage_step);
+        (void)hipHostFree(h_stage_src);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -1771,6 +1853,7 @@ struct Engine {
         n_blocks = static_cast<int>(blocks.size());
         n_nblocks = static_cast<int>(nblocks.size());
         n_active = static_cast<int>(active.size());
+        h_active = active;
         if (n_blocks > 0) {
             grow(d_blocks, blocks_cap, blocks.size());
             HIP_CHECK(hipMemcpyAsync(d_blocks, blocks.data(), blocks.size() * sizeof(CellBlock),
@@ -1816,7 +1899,9 @@ struct Engine {
     }
 
     void remove(int slot) {
-        wait();
+        // Drain first: the active list the drain walks still holds this slot
+        // until the tables are rebuilt, and its buffers are about to go.
+        drain_all();
         release(slots[slot]);
         dirty = true;
     }
@@ -1828,20 +1913,22 @@ struct Engine {
         if (dirty) rebuild_tables();
         if (n_active == 0) return;
         int64_t windows = 0;
-        for (int sidx = 0; sidx < max_nets; ++sidx) {
-            NetSlot& s = slots[sidx];
-            if (!s.used) continue;
+        // The run target is the only descriptor field the host changes
+        // between calls: all of them go down in one copy and one launch.
+        for (int j = 0; j < n_active; ++j) {
+            NetSlot& s = slots[h_active[j]];
             const int64_t target = std::min(s.total, s.step + n);
-            if (target != s.dn.target) {
-                s.dn.target = target;
-                // Only the target field changes on the host side.
-                HIP_CHECK(hipMemcpyAsync(&d_nets[sidx].target, &s.dn.target, sizeof(int64_t),
-                                         hipMemcpyHostToDevice, stream));
-            }
+            s.dn.target = target;
+            h_targets[j] = target;
             const int64_t rem = target - s.step;
             windows = std::max(windows, (rem + Wg - 1) / Wg);
             s.step = target;
         }
+        HIP_CHECK(hipMemcpyAsync(d_targets, h_targets, sizeof(int64_t) * n_active,
+                                 hipMemcpyHostToDevice, stream));
+        hipLaunchKernelGGL(k_set_targets, dim3((n_active + 255) / 256), dim3(256), 0, stream,
+                           d_nets, d_active, n_active, d_targets);
+        raster_dirty = true;
         const int deliver_items = n_nblocks * Wg;
         const int sum_items = n_blocks * Wg;
         if (profiling) ensure_events(windows);
@@ -1871,47 +1958,76 @@ struct Engine {
         if (!async) wait();
     }
 
-    // Move the device raster of a slot into its host queue.
-    void drain(int slot) {
+    // Move the device rasters of all networks into their host queues: one
+    // launch + copy for the heads and overflow bits, one launch packing every
+    // network's new entries into the staging buffer, one copy back.
+    void drain_all() {
         wait();
-        NetSlot& s = slots[slot];
-        download(slot);
-        const int64_t nr = s.dn.r_head;
-        if (nr > 0) {
-            const size_t old = s.q_step.size();
-            s.q_step.resize(old + static_cast<size_t>(nr));
-            s.q_src.resize(old + static_cast<size_t>(nr));
-            HIP_CHECK(hipMemcpyAsync(s.q_step.data() + old, s.dn.r_step, nr * sizeof(int64_t),
-                                     hipMemcpyDeviceToHost, stream));
-            HIP_CHECK(hipMemcpyAsync(s.q_src.data() + old, s.dn.r_src, nr * sizeof(uint32_t),
-                                     hipMemcpyDeviceToHost, stream));
-            const int64_t zero = 0;
-            HIP_CHECK(hipMemcpyAsync(&d_nets[slot].r_head, &zero, sizeof(int64_t),
-                                     hipMemcpyHostToDevice, stream));
-            HIP_CHECK(hipStreamSynchronize(stream));
-            s.dn.r_head = 0;
+        if (!raster_dirty || n_active == 0) return;
+        hipLaunchKernelGGL(k_raster_heads, dim3((n_active + 255) / 256), dim3(256), 0, stream,
+                           d_nets, d_active, n_active, d_heads, d_flags);
+        HIP_CHECK(hipMemcpyAsync(h_heads, d_heads, sizeof(int64_t) * n_active,
+                                 hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipMemcpyAsync(h_flags, d_flags, sizeof(int32_t) * n_active,
+                                 hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+        int64_t total = 0, most = 0;
+        for (int j = 0; j < n_active; ++j) {
+            h_offsets[j] = total;
+            total += h_heads[j];
+            most = std::max(most, h_heads[j]);
+            slots[h_active[j]].overflow = h_flags[j];
+        }
+        raster_dirty = false;
+        if (total == 0) return;
+        if (static_cast<size_t>(total) > stage_cap) {
+            const size_t cap = std::max(static_cast<size_t>(total), 2 * stage_cap);
+            (void)hipFree(d_stage_step);
+            (void)hipFree(d_stage_src);
+            (void)hipHostFree(h_stage_step);
+            (void)hipHostFree(h_stage_src);
+            d_stage_step = nullptr;
+            d_stage_src = nullptr;
+            h_stage_step = nullptr;
+            h_stage_src = nullptr;
+            stage_cap = 0;
+            HIP_CHECK(hipMalloc(&d_stage_step, cap * sizeof(int64_t)));
+            HIP_CHECK(hipMalloc(&d_stage_src, cap * sizeof(uint32_t)));
+            HIP_CHECK(hipHostMalloc(&h_stage_step, cap * sizeof(int64_t), hipHostMallocDefault));
+            HIP_CHECK(hipHostMalloc(&h_stage_src, cap * sizeof(uint32_t), hipHostMallocDefault));
+            stage_cap = cap;
+        }
+        HIP_CHECK(hipMemcpyAsync(d_offsets, h_offsets, sizeof(int64_t) * n_active,
+                                 hipMemcpyHostToDevice, stream));
+        const int64_t gx = std::min<int64_t>(64, (most + 255) / 256);
+        hipLaunchKernelGGL(k_raster_gather, dim3(static_cast<uint32_t>(gx), n_active), dim3(256),
+                           0, stream, d_nets, d_active, d_heads, d_offsets, d_stage_step,
+                           d_stage_src);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipMemcpyAsync(h_stage_step, d_stage_step, sizeof(int64_t) * total,
+                                 hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipMemcpyAsync(h_stage_src, d_stage_src, sizeof(uint32_t) * total,
+                                 hipMemcpyDeviceToHost, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
+        for (int j = 0; j < n_active; ++j) {
+            const int64_t nr = h_heads[j];
+            if (nr == 0) continue;
+            NetSlot& s = slots[h_active[j]];
+            const int64_t* ps = h_stage_step + h_offsets[j];
+            const uint32_t* pr = h_stage_src + h_offsets[j];
+            s.q_step.insert(s.q_step.end(), ps, ps + nr);
+            s.q_src.insert(s.q_src.end(), pr, pr + nr);
         }
     }
 
     // 0, or the overflow bits of a slot (see DevNet::overflow).
     int overflow_bits(int slot) {
-        int32_t flag = 0;
-        HIP_CHECK(hipMemcpyAsync(&flag, &d_nets[slot].overflow, sizeof(int32_t),
-                                 hipMemcpyDeviceToHost, stream));
-        HIP_CHECK(hipStreamSynchronize(stream));
-        return flag;
-    }
-
-    bool overflowed(int slot) {
-        int32_t flag = 0;
-        HIP_CHECK(hipMemcpyAsync(&flag, &d_nets[slot].overflow, sizeof(int32_t),
-                                 hipMemcpyDeviceToHost, stream));
-        HIP_CHECK(hipStreamSynchronize(stream));
-        return flag != 0;
+        drain_all();
+        return slots[slot].overflow;
     }
 
     int64_t spikes_out(int slot, int64_t* step, uint32_t* src, int64_t cap) {
-        drain(slot);
+        drain_all();
         NetSlot& s = slots[slot];
         const int64_t avail = static_cast<int64_t>(s.q_step.size() - s.q_pos);
         const int64_t m = std::min(avail, cap);

@@ -1,22 +1,39 @@
 #!/usr/bin/env python3
-"""Throughput of the simulation step on one MI355X.
+"""Throughput of the simulation step on one MI355X: the 256-network sweep,
+batched FP32 against the plain FP64 path.
 
-Workload (default): the reference's "Sparse" network -- 8,000 excitatory +
-2,000 inhibitory AdEx neurons and 10,000 Li-Rinzel astrocytes built from
-paper_sparse.json -- whole step on the device in FP64 (the plain drop-in
-path), seeds 1, 2 and 3 resident side by side, each on its own HIP stream so
-their launches overlap. One bench step advances each of the three networks by
-1,000 time steps (0.1 s simulated) and drains its spike raster to the host. The default
---warmup 10 --steps 100 is the reference's run of those seeds: a 1 s transient
-(untimed) and 10 s recorded (timed).
+Workload: 256 independent tripartite networks of uneven size -- N drawn
+log-uniformly from [200, 5000] neurons (80/20 excitatory/inhibitory),
+round(0.2 N) astrocytes, IP3 weight w_n2a uniform in [0.10, 0.40], seed
+1000 + index, every other parameter from use_case.json, fixed mean in-degree
+(p_primary = 100 / N) -- built on the host once, resident on the device.
 
-Reported on one JSON line (rank 0):
-  value      neuron-steps/s over the timed region = networks x 10,000 neurons
-             x time steps / wall seconds (inputs already resident on the device)
-  roofline   the dominant kernel (the cell-update kernel): algorithmic bytes
-             per launch / mean launch time from HIP events on its stream
-  cpu_baseline  the reference simulator built from its own sources
-             (oracle/_ref) on this host's cores, first 2 s of seed 1
+  value        batched FP32 (state FP32, FP64 near threshold and for the Ca
+               threshold test): all 256 networks in one asg_batch, every
+               window one launch per kernel for all of them. Neuron-steps/s =
+               sum over networks of N x time steps / wall seconds of the timed
+               bench steps. One bench step = 1,000 time steps (0.1 s
+               simulated) of every network plus draining their spike rasters;
+               --warmup bench steps run first as the untimed transient, then
+               --steps are timed (default 5 + 50: 0.5 s transient, 5 s timed).
+  plain_fp64   the drop-in path: each network alone on the device in FP64,
+               one after the other (every 8th network of the sweep, timed
+               over the same time steps).
+  batched_fp64 the same batch in FP64; it is bit-identical to the plain runs
+               and serves as the FP64 reference of the accuracy figures.
+  accuracy     per network, FP32 against FP64: firing rate over the timed
+               window, active-astrocyte fraction, mean pairwise Ca
+               correlation (the reference's analysis of Ca recorded every
+               record_every steps).
+  roofline     the kernel with the largest share of device time: algorithmic
+               bytes per launch / mean launch time from HIP events recorded on
+               the batch's stream.
+  cpu_baseline the reference simulator built from its own sources
+               (oracle/_ref) on this host's cores, on a sample of the sweep's
+               networks.
+  sparse_fp64  the reference's 'Sparse' network (10,000 neurons, 10,000
+               astrocytes, paper_sparse.json), FP64, seeds 1-3 side by side,
+               1 s transient + --sparse-seconds timed.
 """
 
 from __future__ import annotations
@@ -36,73 +53,204 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 METRIC = "neuron-steps/s batched FP32 vs plain FP64 on 1 GPU; HBM GB/s; p99 job latency"
-CONFIG = os.path.join(HERE, "tests", "golden", "configs", "paper_sparse.json")
+SWEEP_CONFIG = os.path.join(HERE, "tests", "golden", "configs", "use_case.json")
+SPARSE_CONFIG = os.path.join(HERE, "tests", "golden", "configs", "paper_sparse.json")
 HBM_PEAK_GBS = 8000.0
-SIM_STEPS_PER_BENCH_STEP = 1000
-SEEDS = (1, 2, 3)
-# Published mean rates of these seeds (reference docs/validation.md:83-89).
-REFERENCE_RATE_HZ = {1: 4.7390, 2: 4.7309, 3: 4.7636}
+STEPS_PER_BENCH_STEP = 1000
+SPARSE_SEEDS = (1, 2, 3)
+# Published mean rates of the Sparse seeds over 1 s transient + 10 s
+# (reference docs/validation.md:83-89).
+SPARSE_REFERENCE_RATE_HZ = {1: 4.7390, 2: 4.7309, 3: 4.7636}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cells_bytes_per_launch(model, window: int) -> float:
-    """Algorithmic HBM bytes of one cell-update launch (one window).
+# Algorithmic HBM bytes of the cell-update kernel. It keeps each cell's
+# state in registers for a whole window of W steps, so state moves once per
+# window and ring slots every step.
+#   neuron, per window: V, w, g_ex, dg_ex, g_in, dg_in, I_sic read + written
+#     (7 x 2 x s), refractory counter read + written (8), V_reset and b read
+#     (16); per step: exc, inh and SIC ring slots read (3 s) and the exc and
+#     inh slots cleared (2 s).
+#   astrocyte, per window: Ca, IP3, h read + written (6 s), Ca_tot, IP3_0,
+#     1/tau_IP3 and delta_IP3 read (4 s); per step: the IP3 ring slot read
+#     and cleared (2 s) and the SIC factor written (s).
+# s = 8 bytes in FP64, 4 in FP32.
+def cell_bytes(n_neurons: int, n_astro: int, window: int, state_bytes: int) -> float:
+    s = state_bytes
+    neuron = 14 * s + 8 + 16 + window * 5 * s
+    astro = 10 * s + window * 3 * s
+    return n_neurons * neuron + n_astro * astro
 
-    The kernel keeps each cell's state in registers for the whole window, so
-    state moves once per window; ring slots move every step. Per neuron:
-    state V, w, g_ex, dg_ex, g_in, dg_in, I_sic (8 B each) read + written,
-    refractory counter (4 B) read + written, V_reset and b read (136 B); per
-    neuron-step: the exc and inh ring slots read and cleared (32 B) and the
-    SIC slot read (8 B). Per astrocyte: Ca, IP3, h read + written and Ca_tot,
-    IP3_0, tau_IP3, delta_IP3 read (80 B); per astrocyte-step: its ring slot
-    read and cleared (16 B) and its SIC factor written (8 B).
-    """
-    n = model.n_neurons
-    na = model.n_astro
-    per_window = n * 136.0 + na * 80.0
-    per_step = n * 40.0 + na * 24.0
-    return per_window + window * per_step
+
+def build_sweep(count: int, transient_ms: float, timed_ms: float):
+    from astrosimgpu_amd import sweep
+    specs = sweep.sweep_specs(count)
+    return specs, [sweep.build(SWEEP_CONFIG, s, timed_ms, transient_ms) for s in specs]
 
 
-def reference_cpu_baseline(sample_ms: float = 2000.0):
-    """Time the reference build (oracle/_ref) on the Sparse network, seed 1,
-    first `sample_ms` of simulated time, on min(16, cpus) OpenMP threads.
-    Falls back to the repo's CPU oracle if the reference build is absent."""
+def run_batch(models, precision, warmup: int, steps: int, profile: bool):
+    """One asg_batch of all models; returns elapsed seconds of the timed
+    bench steps, per-network spike steps, Ca traces and kernel times."""
+    import astrosimgpu_amd as A
+    from astrosimgpu_amd import sweep
+    batch = A.Batch(len(models))
+    slots = []
+    for m in models:
+        m.desc.contents.precision = precision
+        start, every, n_rec, cap, total = sweep._record_plan(m)
+        s = batch.add(m, total)
+        batch.set_recording(s, start, every, n_rec, cap)
+        slots.append(s)
+    spikes = [[] for _ in models]
+
+    def drain():
+        for k, s in enumerate(slots):
+            st, _ = batch.spikes(s, 1 << 16)
+            spikes[k].append(st)
+
+    for _ in range(warmup):
+        batch.step(STEPS_PER_BENCH_STEP)
+        drain()
+    if profile:
+        batch.set_profiling(True)
+    # asg_batch_step returns when the device has finished the steps
+    # (hipStreamSynchronize on the batch's stream), so the clock brackets
+    # exactly the timed work.
+    t0 = time.perf_counter()
+    for i in range(steps):
+        batch.step(STEPS_PER_BENCH_STEP)
+        drain()
+        if (i + 1) % 10 == 0:
+            log(f"    bench step {i + 1}/{steps}")
+    elapsed = time.perf_counter() - t0
+    kt = batch.kernel_times() if profile else None
+    traces = []
+    for s in slots:
+        st, ca, _ = batch.read_trace(s)
+        traces.append((st, ca))
+    batch.close()
+    spk = [np.concatenate(x) if x else np.zeros(0, np.int64) for x in spikes]
+    return elapsed, spk, traces, kt
+
+
+def run_plain(models, total_steps: int):
+    """Each model alone in FP64 through the plain per-network path."""
+    import astrosimgpu_amd as A
+    from astrosimgpu_amd import _capi as capi
+    elapsed = 0.0
+    ns = 0
+    for m in models:
+        m.desc.contents.precision = capi.ASG_FP64
+        net = A.DeviceNetwork(m)
+        t0 = time.perf_counter()
+        done = 0
+        while done < total_steps:
+            k = min(10 * STEPS_PER_BENCH_STEP, total_steps - done)
+            net.step(k)
+            net.spikes(1 << 16)
+            done += k
+        elapsed += time.perf_counter() - t0
+        ns += m.n_neurons * total_steps
+        net.close()
+    return ns, elapsed
+
+
+def sparse_fp64(transient_bench_steps: int, timed_bench_steps: int):
+    """The Sparse network, FP64, seeds 1-3 resident together, each on its
+    own stream."""
+    import astrosimgpu_amd as A
+    models = [A.Model(SPARSE_CONFIG, seed=s) for s in SPARSE_SEEDS]
+    nets = [A.DeviceNetwork(m) for m in models]
+    spikes = {s: [] for s in SPARSE_SEEDS}
+
+    def bench_step():
+        for net in nets:
+            net.step(STEPS_PER_BENCH_STEP, wait=False)
+        for s, net in zip(SPARSE_SEEDS, nets):
+            net.sync()
+            st, _ = net.spikes()
+            spikes[s].append(st)
+
+    for _ in range(transient_bench_steps):
+        bench_step()
+    t0 = time.perf_counter()
+    for _ in range(timed_bench_steps):
+        bench_step()
+    elapsed = time.perf_counter() - t0
+    first = transient_bench_steps * STEPS_PER_BENCH_STEP
+    steps = timed_bench_steps * STEPS_PER_BENCH_STEP
+    rates = {}
+    for s in SPARSE_SEEDS:
+        st = np.concatenate(spikes[s])
+        rates[str(s)] = round(int((st >= first).sum()) * 1000.0 / (steps * 0.1 * 10000), 4)
+    for net in nets:
+        net.close()
+    out = {"value": round(3 * 10000 * steps / elapsed, 1), "unit": "neuron-steps/s",
+           "seconds_timed": steps / 10000.0, "mean_rate_hz": rates}
+    if transient_bench_steps == 10 and timed_bench_steps == 100:
+        out["reference_rate_hz"] = {str(s): r for s, r in SPARSE_REFERENCE_RATE_HZ.items()}
+    return out
+
+
+def reference_cpu_baseline(specs, models, sample_ms: float, every: int, transient_ms: float):
+    """Time the reference build (oracle/_ref) on every `every`-th sweep
+    network: `transient_ms` untimed, then `sample_ms` timed by the reference's
+    own phase profile (it times the recorded window only), on min(16, cpus)
+    OpenMP threads. Falls back to the CPU oracle when the reference build is
+    absent."""
     threads = min(16, os.cpu_count() or 1)
     ref = os.path.join(HERE, "oracle", "_ref", "astrosimgpu_ref")
+    picked = list(range(0, len(specs), every))
     steps = int(sample_ms / 0.1 + 0.5)
-    sample = (f"paper_sparse.json seed 1, steps 0-{steps} ({sample_ms / 1000:g} s simulated, "
-              "the start of the run), phase-profile total of the step loop")
+    sample = (f"sweep networks {', '.join(str(i) for i in picked)} "
+              f"({sum(specs[i].n for i in picked)} neurons), {sample_ms:g} ms each after a "
+              f"{transient_ms:g} ms transient, step-loop time of the recorded window")
+    ns = 0
+    loop_s = 0.0
     if os.path.exists(ref):
-        with tempfile.TemporaryDirectory() as out:
-            env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+        with open(SWEEP_CONFIG) as f:
+            base = json.load(f)
+        with tempfile.TemporaryDirectory() as tmp:
+            for i in picked:
+                sp = specs[i]
+                cfg = json.loads(json.dumps(base))
+                cfg["N"].update({"N_E": sp.n_exc, "N_I": sp.n_inh, "N_A": sp.n_astro})
+                cfg["synapse"]["w_n2a"] = sp.w_n2a
+                cfg["connectivity"]["p_primary"] = min(1.0, 100.0 / sp.n)
+                cfg["simulation"]["record_astro"] = False
+                path = os.path.join(tmp, f"net{i}.json")
+                with open(path, "w") as f:
+                    json.dump(cfg, f)
+                out = os.path.join(tmp, f"out{i}")
+                env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+                subprocess.run([ref, "-c", path, "-s", str(sp.seed), "-t", str(sample_ms), "-p", str(transient_ms),
+                                "-o", out, "--no-analysis"], check=True, env=env,
+                               stdout=subprocess.DEVNULL)
+                with open(os.path.join(out, "run.txt")) as f:
+                    m = re.search(r"^total\s+([0-9.]+)", f.read(), flags=re.M)
+                loop_s += float(m.group(1))
+                ns += sp.n * steps
+        kind = "reference"
+    else:
+        import oracle
+        os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+        for i in picked:
+            o = oracle.OracleRun(models[i].desc, oracle.LIBM_GLIBC)
+            o.step(int(transient_ms / 0.1 + 0.5))
             t0 = time.time()
-            subprocess.run([ref, "-c", CONFIG, "-s", "1", "-t", str(sample_ms), "-p", "0",
-                            "-o", out, "--no-analysis"], check=True, env=env,
-                           stdout=subprocess.DEVNULL)
-            wall = time.time() - t0
-            text = open(os.path.join(out, "run.txt")).read()
-        m = re.search(r"^total\s+([0-9.]+)", text, flags=re.M)
-        loop_s = float(m.group(1)) if m else wall
-        return {"value": 10000.0 * steps / loop_s, "unit": "neuron-steps/s", "cores": threads,
-                "kind": "reference", "sample": sample, "seconds": round(loop_s, 3)}
-    import oracle
-    import astrosimgpu_amd as A
-    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
-    mdl = A.Model(CONFIG, seed=1)
-    o = oracle.OracleRun(mdl.desc, oracle.LIBM_GLIBC)
-    t0 = time.time()
-    o.step(steps)
-    loop_s = time.time() - t0
-    return {"value": 10000.0 * steps / loop_s, "unit": "neuron-steps/s", "cores": threads,
-            "kind": "port", "sample": sample, "seconds": round(loop_s, 3)}
+            o.step(steps)
+            loop_s += time.time() - t0
+            ns += specs[i].n * steps
+        kind = "port"
+    return {"value": round(ns / loop_s, 1), "unit": "neuron-steps/s", "cores": threads,
+            "kind": kind, "sample": sample, "seconds": round(loop_s, 3)}
 
 
-def load_traffic():
-    path = os.path.join(HERE, "profiles", "k_cells_sparse_fp64_traffic.json")
+def load_traffic(kernel: str):
+    path = os.path.join(HERE, "profiles", f"{kernel}_sweep_fp32_traffic.json")
     if os.path.exists(path):
         with open(path) as f:
             return json.load(f).get("hbm_bytes_per_launch")
@@ -112,72 +260,97 @@ def load_traffic():
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawTextHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100, help="timed bench steps (1,000 time steps each)")
-    ap.add_argument("--warmup", type=int, default=10, help="untimed bench steps before timing")
+    ap.add_argument("--steps", type=int, default=50, help="timed bench steps (1,000 time steps each)")
+    ap.add_argument("--warmup", type=int, default=5, help="untimed bench steps before timing")
+    ap.add_argument("--networks", type=int, default=256)
+    ap.add_argument("--plain-every", type=int, default=8,
+                    help="time the plain FP64 path on every n-th network")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-ms", type=float, default=2000.0)
+    ap.add_argument("--cpu-sample-ms", type=float, default=1500.0)
+    ap.add_argument("--cpu-every", type=int, default=16)
+    ap.add_argument("--no-sparse", action="store_true")
+    ap.add_argument("--fp32-only", action="store_true",
+                    help="run only the batched FP32 sweep (for kernel profiles)")
+    ap.add_argument("--sparse-seconds", type=int, default=2,
+                    help="timed simulated seconds of the Sparse FP64 runs (10 = the full run)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (got --gpus {args.gpus}); the path "
-              "is single-device, see DESIGN.md for how it would shard", file=sys.stderr)
+        print(f"bench.py: this build runs on one GPU only (got --gpus {args.gpus}); the "
+              "networks of a sweep would be bin-packed over GPUs by footprint, one batch per "
+              "GPU, which is not built here", file=sys.stderr)
         sys.exit(2)
 
-    import astrosimgpu_amd as A
+    from astrosimgpu_amd import _capi as capi
+    from astrosimgpu_amd import sweep
 
     t_build = time.time()
-    models = [A.Model(CONFIG, seed=s) for s in SEEDS]
-    nets = [A.DeviceNetwork(m) for m in models]
-    log(f"built {len(models)} Sparse networks in {time.time() - t_build:.1f} s "
-        f"(synapses {models[0].stats()})")
-
-    spikes = {s: [] for s in SEEDS}
-
-    def bench_step():
-        # Each network has its own HIP stream: queue all three, then wait,
-        # so their kernels overlap on the device.
-        for net in nets:
-            net.step(SIM_STEPS_PER_BENCH_STEP, wait=False)
-        for s, net in zip(SEEDS, nets):
-            net.sync()
-            st, _ = net.spikes()
-            spikes[s].append(st)
-
-    for i in range(args.warmup):
-        bench_step()
-    for net in nets:
-        net.set_profiling(True)
-    # asg_net_step returns after the device has finished (hipStreamSynchronize),
-    # so the wall clock below brackets exactly the timed steps.
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        bench_step()
-        if (i + 1) % 20 == 0:
-            log(f"  timed bench step {i + 1}/{args.steps}")
-    elapsed = time.perf_counter() - t0
-
-    kt = [net.kernel_times() for net in nets]
+    transient_ms = args.warmup * STEPS_PER_BENCH_STEP * 0.1
+    timed_ms = args.steps * STEPS_PER_BENCH_STEP * 0.1
+    specs, models = build_sweep(args.networks, transient_ms, timed_ms)
+    total_steps = (args.warmup + args.steps) * STEPS_PER_BENCH_STEP
     n_total = sum(m.n_neurons for m in models)
-    sim_steps = args.steps * SIM_STEPS_PER_BENCH_STEP
-    value = n_total * sim_steps / elapsed
-    windows = sum(k["windows"] for k in kt)
-    cells_ms = sum(k["cells"] for k in kt)
-    window_len = 10  # min(d_e, d_i, d_a2n) = 10 steps for this configuration
-    bytes_launch = cells_bytes_per_launch(models[0], window_len)
-    cells_avg_s = cells_ms / 1000.0 / max(windows, 1)
-    achieved = bytes_launch / cells_avg_s / 1e9
-    first_rec = args.warmup * SIM_STEPS_PER_BENCH_STEP
-    rates = {}
-    for s in SEEDS:
-        st = np.concatenate(spikes[s]) if spikes[s] else np.zeros(0)
-        rec = st[st >= first_rec]
-        rates[s] = round(len(rec) * 1000.0 / (sim_steps * 0.1 * 10000), 4)
+    na_total = sum(m.n_astro for m in models)
+    log(f"built {len(models)} networks ({n_total} neurons, {na_total} astrocytes) "
+        f"in {time.time() - t_build:.1f} s")
+
+    log("batched FP32 ...")
+    e32, spk32, tr32, kt32 = run_batch(models, capi.ASG_FP32, args.warmup, args.steps, True)
+    timed_ns = n_total * args.steps * STEPS_PER_BENCH_STEP
+    value = timed_ns / e32
+    log(f"  {value:.3e} neuron-steps/s  kernels {kt32}")
+
+    if args.fp32_only:
+        args.no_sparse = args.no_cpu_baseline = True
+        e64 = kt64 = plain64 = cmp = sub = None
+    else:
+        log("batched FP64 ...")
+        e64, spk64, tr64, kt64 = run_batch(models, capi.ASG_FP64, args.warmup, args.steps, True)
+        log(f"  {timed_ns / e64:.3e} neuron-steps/s")
+
+        log("plain FP64 (one network at a time) ...")
+        sub = models[::args.plain_every]
+        p_ns, p_s = run_plain(sub, total_steps)
+        plain64 = p_ns / p_s
+        log(f"  {plain64:.3e} neuron-steps/s over {len(sub)} networks")
+
+        r64 = sweep.RunResult(e64, timed_ns, spk64, tr64)
+        r32 = sweep.RunResult(e32, timed_ns, spk32, tr32)
+        cmp = sweep.compare(models, r64, r32)
+        cmp.pop("rows")
+
+    # Roofline of the kernel with the largest share of device time.
+    windows = kt32["windows"]
+    share = {k: kt32[k] for k in ("cells", "compact", "deliver", "sum")}
+    dominant = max(share, key=share.get)
+    window_len = 10  # min(d_e, d_i, d_a2n) of use_case.json = 1 ms = 10 steps
+    if dominant == "cells":
+        bytes_launch = cell_bytes(n_total, na_total, window_len, 4)
+        avg_s = kt32["cells"] / 1000.0 / max(windows, 1)
+        achieved = bytes_launch / avg_s / 1e9
+        roof = {"kernel": "k_cells (every neuron and astrocyte of the batch, one window of "
+                          "10 steps, FP32)",
+                "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": load_traffic("k_cells"),
+                "algorithmic_bytes_per_launch": round(bytes_launch),
+                "mean_launch_us": round(avg_s * 1e6, 3), "launches": windows}
+    else:
+        roof = {"kernel": dominant, "bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": None, "traffic": None,
+                "mean_launch_us": round(share[dominant] * 1000.0 / max(windows, 1), 3),
+                "launches": windows}
+
+    sparse = None
+    if not args.no_sparse:
+        log("Sparse FP64 ...")
+        sparse = sparse_fp64(10, 10 * args.sparse_seconds)
 
     cpu = None
     if not args.no_cpu_baseline:
         log("timing the reference build on the host cores ...")
-        cpu = reference_cpu_baseline(args.cpu_sample_ms)
+        cpu = reference_cpu_baseline(specs, models, args.cpu_sample_ms, args.cpu_every,
+                                     transient_ms)
 
-    counters = [net.counters() for net in nets]
     line = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -185,38 +358,33 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1000.0 * elapsed / args.steps, 3),
+        "ms_per_step": round(1000.0 * e32 / args.steps, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic: Sparse network built from paper_sparse.json, Poisson drive",
+        "dtype": "f32",
+        "data": "synthetic: networks built from use_case.json with drawn sizes, IP3 weights "
+                "and seeds; Poisson and Gaussian drive",
         "config": {
-            "workload": "Sparse network (8000 E + 2000 I AdEx neurons, 10000 Li-Rinzel "
-                        "astrocytes), plain FP64 whole step, seeds 1-3, 1 s transient untimed "
-                        "+ 10 s timed, 1000 time steps (dt 0.1 ms) per bench step",
-            "networks": len(models),
-            "neurons_per_network": models[0].n_neurons,
-            "astrocytes_per_network": models[0].n_astro,
-            "time_steps_per_bench_step": SIM_STEPS_PER_BENCH_STEP,
-            "precision": "fp64",
+            "workload": f"{len(models)}-network sweep (N log-uniform in [200, 5000], 80/20 E/I, "
+                        "0.2 N astrocytes, w_n2a U[0.10, 0.40], in-degree 100), batched FP32 "
+                        f"with FP64 islands, {transient_ms / 1000:g} s transient untimed + "
+                        f"{timed_ms / 1000:g} s timed, 1000 time steps (dt 0.1 ms) per bench step",
+            "networks": len(models), "neurons": n_total, "astrocytes": na_total,
+            "time_steps_per_bench_step": STEPS_PER_BENCH_STEP,
         },
-        "roofline": {
-            "kernel": "k_cells (neuron + astrocyte update, one window of 10 steps)",
-            "bound": "hbm",
-            "achieved": round(achieved, 2),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": load_traffic(),
-            "algorithmic_bytes_per_launch": round(bytes_launch),
-            "mean_launch_us": round(cells_avg_s * 1e6, 3),
-            "launches": windows,
-        },
-        "kernel_ms": {k: round(sum(x[k] for x in kt), 3) for k in ("cells", "compact", "deliver", "sum")},
-        "mean_rate_hz": {str(s): rates[s] for s in SEEDS},
-        "reference_rate_hz": {str(s): REFERENCE_RATE_HZ[s] for s in SEEDS},
-        "synapse_events": sum(c["synapse_events"] for c in counters),
+        "plain_fp64": None if plain64 is None else
+        {"value": round(plain64, 1), "unit": "neuron-steps/s", "networks_timed": len(sub)},
+        "batched_fp64": None if e64 is None else
+        {"value": round(timed_ns / e64, 1), "unit": "neuron-steps/s"},
+        "speedup_batched_fp32_vs_plain_fp64": None if plain64 is None else round(value / plain64, 2),
+        "accuracy_fp32_vs_fp64": None if cmp is None else
+        {k: (round(v, 5) if isinstance(v, float) else v) for k, v in cmp.items()},
+        "roofline": roof,
+        "kernel_ms_fp32": {k: round(v, 3) for k, v in kt32.items() if k != "windows"},
+        "kernel_ms_fp64": None if kt64 is None else
+        {k: round(v, 3) for k, v in kt64.items() if k != "windows"},
+        "sparse_fp64": sparse,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

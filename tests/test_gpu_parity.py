@@ -210,3 +210,24 @@ def _poisson_counts(seed, step, n, lam):
             alive = alive & (prod > limit)
             c += 1
     return counts
+
+
+def test_capped_delivery_masks_equal_oracle():
+    """20,000 neurons: full delivery masks would take about 700 MB, so the
+    plan caps the chunk count (64 chunks, 4096 spikes per step and type);
+    the device still equals the oracle bit for bit."""
+    _announce("capped masks 20k")
+    m = A.Model(config_path("paper_sparse"), seed=21, sim_time=200.0, pre_time=0.0,
+                assignments={"N.N_E": 16000, "N.N_I": 4000, "N.N_A": 10000,
+                             "connectivity.p_primary": 0.005,
+                             "connectivity.generator": "per_row"})
+    assert m.footprint() < 400 << 20
+    steps = m.info["sim_steps"]
+    net, s, n = _run_device(m, steps, chunk=700)
+    o = oracle.OracleRun(m.desc, oracle.LIBM_AMATH)
+    o.step(steps)
+    os_, on = o.spikes()
+    assert len(s) > 0
+    assert np.array_equal(s, os_) and np.array_equal(n, on)
+    for f in ("V", "w", "I_sic", "Ca", "IP3"):
+        assert np.array_equal(net.state(f).view(np.uint64), o.state(f).view(np.uint64)), f
