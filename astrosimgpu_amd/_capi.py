@@ -87,7 +87,7 @@ class Counters(C.Structure):
 
 
 class KernelTimes(C.Structure):
-    _fields_ = [("ms", C.c_double * 4), ("windows", C.c_int64)]
+    _fields_ = [("ms", C.c_double * 5), ("windows", C.c_int64)]
 
 
 class CalciumSummary(C.Structure):
@@ -170,6 +170,9 @@ PROTOTYPES = {
     "asg_analyze_calcium": (C.c_int, [DBLP, DBLP, C.c_int32, C.c_int64, C.c_double, C.c_double,
                                       C.c_double, C.c_double, C.c_double, C.c_double, C.c_int32,
                                       C.POINTER(CalciumSummary)]),
+    "asg_analyze_calcium_cells": (C.c_int, [DBLP, DBLP, I64P, C.c_int32, C.c_double,
+                                            C.c_double, C.c_double, C.c_double, C.c_double,
+                                            C.c_double, C.c_int32, C.POINTER(CalciumSummary)]),
     "asg_detect_events": (C.c_int, [DBLP, DBLP, C.c_int64, C.c_double, C.c_int32, C.c_double,
                                     DBLP, DBLP, C.c_int64, I64P]),
     "asg_pearson": (C.c_double, [DBLP, DBLP, C.c_int64]),

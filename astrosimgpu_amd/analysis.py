@@ -29,6 +29,24 @@ def analyze_calcium(times_ms, ca, threshold, merge_gap=2000.0, pre_sim_time=0.0,
     return {name: getattr(out, name) for name, _ in capi.CalciumSummary._fields_}
 
 
+def analyze_calcium_cells(times, ca, threshold, merge_gap=2000.0, pre_sim_time=0.0,
+                          sim_time=0.0, hist_window=2000.0, hist_shift=400.0,
+                          round6=False) -> dict:
+    """The same over per-cell traces of their own lengths: times[c], ca[c]
+    are cell c's sample times and values."""
+    offsets = np.zeros(len(ca) + 1, np.int64)
+    offsets[1:] = np.cumsum([len(x) for x in ca])
+    t = np.ascontiguousarray(np.concatenate(times) if len(times) else np.zeros(0), np.float64)
+    c = np.ascontiguousarray(np.concatenate(ca) if len(ca) else np.zeros(0), np.float64)
+    out = capi.CalciumSummary()
+    check(lib().asg_analyze_calcium_cells(t.ctypes.data_as(capi.DBLP), c.ctypes.data_as(capi.DBLP),
+                                          offsets.ctypes.data_as(capi.I64P), len(ca),
+                                          float(threshold), float(merge_gap), float(pre_sim_time),
+                                          float(sim_time), float(hist_window), float(hist_shift),
+                                          1 if round6 else 0, C.byref(out)))
+    return {name: getattr(out, name) for name, _ in capi.CalciumSummary._fields_}
+
+
 def detect_events(times, values, threshold, merge=False, max_gap=0.0):
     t = np.ascontiguousarray(times, np.float64)
     v = np.ascontiguousarray(values, np.float64)

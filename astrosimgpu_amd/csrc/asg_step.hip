@@ -5,17 +5,21 @@
 // smallest synaptic delay in steps (10 in every shipped configuration). No
 // signal emitted inside a window can arrive inside it, so within a window
 // every cell is independent of every other cell and the per-cell state stays
-// in registers for W steps. A window is four launches:
+// in registers for W steps. A window is five launches (six when FP64 and
+// FP32 networks share a batch):
 //
-//   k_cells    one lane per neuron / astrocyte, W steps each: arrivals from
-//              the ring slots, Poisson drive, noise, RK4, spike test (a
-//              wavefront ballot writes one 64-bit spike word per 64 neurons
-//              and step), astrocyte update and its SIC factor.
+//   k_drive    one lane per neuron, W steps: the Poisson event counts and
+//              noise current of the external drive, from counter streams.
+//   k_cells    one lane per neuron / astrocyte, W steps each, one kernel per
+//              precision: arrivals from the ring slots, drive, RK4, spike
+//              test (a wavefront ballot writes one 64-bit spike word per 64
+//              neurons and step), astrocyte update and its SIC factor.
 //   k_compact  one workgroup per (network, step): prefix scan of the spike
 //              words into a source-ordered spike list, one work item per
 //              spike.
 //   k_deliver  lanes per (neuron, step): the SIC each neuron receives from
-//              that step's exchange, and each spiking source's
+//              that step's exchange (skipped when no astrocyte of the network
+//              has a nonzero factor), and each spiking source's
 //              Tsodyks-Markram update in time order and raster entries;
 //              workgroups per spike: set the spike's bit in the per-target
 //              64-bit mask of every synapse of its rows.
@@ -111,6 +115,7 @@ asg_status guarded(F&& f) {
 
 constexpr int kCellBlock = 64;     // one wavefront per block of the per-cell kernels
 constexpr int kWindowCap = 16;     // longest window (bounds scratch memory)
+constexpr int64_t kSicWaves = 4096; // SIC wavefronts that fill the device (256 CUs x 16)
 constexpr size_t kMaskBudget = size_t(256) << 20;  // delivery masks per network
 constexpr size_t kMinChunks = 4;
 
@@ -164,6 +169,12 @@ struct DevNet {
     // rings [S][targets]: exc, inh and SIC per neuron, n2a per astrocyte
     double *ring_exc, *ring_inh, *ring_sic, *ring_astro;
     double* fac;           // [Wmax][n_astro]: SIC factor of each step of the window
+    // External drive of the window's steps, written by k_drive before the
+    // cell pass: Poisson event counts [Wmax][n_neur] (when a population has
+    // a rate) and the noise current [Wmax][n_neur] (when one has noise).
+    int32_t* drive_n;
+    double* drive_noise;
+    float* fdrive_noise;
     // FP32 networks keep their state, parameters and rings here instead
     float *fV, *fw, *fgex, *fdgex, *fgin, *fdgin, *fIsic;
     float *fCa, *fIP3, *fhh, *fCa_tot, *fIP3_0, *finv_tau, *fdelta;
@@ -197,6 +208,8 @@ struct DevNet {
     double* rec_buf;
     // counters
     int64_t c_steps, c_spikes, c_syn, c_sic;
+    uint32_t sic_live;     // bit k: some astrocyte has a nonzero SIC factor at
+                           // window step k (set by k_cells, cleared by k_sum)
 };
 
 __device__ inline void record_astro(const DevNet& n, uint32_t a, int64_t T, double Ca, double IP3) {
@@ -205,6 +218,22 @@ __device__ inline void record_astro(const DevNet& n, uint32_t a, int64_t T, doub
         double* rec = n.rec_buf + static_cast<size_t>(r) * 2 * n.rec_n;
         rec[a] = Ca;
         rec[n.rec_n + a] = IP3;
+    }
+}
+
+// Bit k of `bits`: this lane's SIC factor at window step k is nonzero. One
+// atomic per wavefront publishes the OR over its active lanes.
+__device__ inline void mark_sic_live(const DevNet& n, uint32_t bits, int W) {
+    uint32_t any = 0;
+    for (int k = 0; k < W; ++k) {
+        if (__ballot((bits >> k) & 1u) != 0) {
+            any |= 1u << k;
+        }
+    }
+    const uint64_t active = __ballot(true);
+    if (any != 0 &&
+        (threadIdx.x & 63) == static_cast<uint32_t>(__ffsll(static_cast<long long>(active)) - 1)) {
+        atomicOr(const_cast<uint32_t*>(&n.sic_live), any);
     }
 }
 
@@ -238,14 +267,14 @@ __device__ void neuron_window(const DevNet& n, uint32_t i, int64_t T0, int W) {
     const int d_a2n = n.d_a2n;
     const int sic_interval = n.sic_interval;
     const int substeps = n.substeps;
-    const uint64_t seed = n.seed;
     const double hs = n.h, half_h = n.half_h, h_sixth = n.h_sixth;
     const uint32_t words = n.words;
     double* __restrict__ ring_exc = n.ring_exc;
     double* __restrict__ ring_inh = n.ring_inh;
     const double* __restrict__ ring_sic = n.ring_sic;
     uint64_t* __restrict__ spk_bits = n.spk_bits;
-    const uint64_t salt_shared = pi == 0 ? kSaltNoiseSharedExc : kSaltNoiseSharedInh;
+    const int32_t* __restrict__ drive_n = n.drive_n;
+    const double* __restrict__ drive_noise = n.drive_noise;
 
     double V = 0, w = 0, gex = 0, dgex = 0, gin = 0, dgin = 0, Isic = 0, Vr = 0, b = 0;
     int refr = 0;
@@ -266,15 +295,13 @@ __device__ void neuron_window(const DevNet& n, uint32_t i, int64_t T0, int W) {
         ih_n = ring_inh[s0];
         sic_n = ring_sic[s0];
     }
-    // The independent noise of one noise interval is one draw; keep it.
-    int64_t noise_idx = -1;
-    double noise_val = 0.0;
     const uint32_t lane = threadIdx.x & 63;
     for (int k = 0; k < W; ++k) {
         const int64_t T = T0 + k;
         bool spiked = false;
         if (valid) {
             const size_t slot = static_cast<size_t>(T % S) * N + i;
+            const size_t dk = static_cast<size_t>(k) * N + i;
             const double ex = ex_n, ih = ih_n, sic = sic_n;
             if (k + 1 < W) {
                 const size_t nx = static_cast<size_t>((T + 1) % S) * N + i;
@@ -306,34 +333,16 @@ __device__ void neuron_window(const DevNet& n, uint32_t i, int64_t T0, int W) {
             if (T >= d_a2n && (T - d_a2n) % sic_interval == 0) {
                 Isic = sic;
             }
-            // NeuronPopulation::update, neuron.cpp:143-170
+            // NeuronPopulation::update, neuron.cpp:143-170 (draws from k_drive)
             if (in.poiss_rate > 0.0) {
-                const int ev = rng_poisson_fresh(seed ^ kSaltNeuronPoisson,
-                                                 static_cast<uint64_t>(T) * kStreamStride + i,
-                                                 in.lambda, in.limit);
+                const int ev = drive_n[dk];
                 if (ev > 0) {
                     exc_in += in.poiss_weight * static_cast<double>(ev);
                 }
             }
             double I_ext = Isic;
             if (in.noise_std > 0.0) {
-                const int64_t idx = T / in.steps_per_noise;
-                if (idx != noise_idx) {
-                    noise_idx = idx;
-                    uint64_t ctr = 0;
-                    if (in.independent) {
-                        noise_val = 0.0 + in.noise_std *
-                                              rng_normal_fresh(seed ^ kSaltNeuronNoise,
-                                                               static_cast<uint64_t>(idx) *
-                                                                       kNoiseStride + i,
-                                                               ctr);
-                    } else {
-                        noise_val = 0.0 + in.noise_std *
-                                              rng_normal_fresh(seed ^ salt_shared,
-                                                               static_cast<uint64_t>(idx), ctr);
-                    }
-                }
-                I_ext += noise_val;
+                I_ext += drive_noise[dk];
             }
             if (exc_in != 0.0) {
                 dgex += exc_in * p.psc_ex;
@@ -380,6 +389,7 @@ __device__ void astro_window(const DevNet& n, uint32_t a, int64_t T0, int W) {
     double v_n = ring_astro[static_cast<size_t>(T0 % S) * NA + a];
     int64_t noise_idx = -1;
     double noise = 0.0;
+    uint32_t live_bits = 0;
     for (int k = 0; k < W; ++k) {
         const int64_t T = T0 + k;
         const size_t slot = static_cast<size_t>(T % S) * NA + a;
@@ -417,12 +427,15 @@ __device__ void astro_window(const DevNet& n, uint32_t a, int64_t T0, int W) {
                    Ca, IP3, hh);
         record_astro(n, a, T, Ca, IP3);
         if (T % sic_interval == 0) {
-            fac[static_cast<size_t>(k) * NA + a] = sic_factor(Ca, ap.SIC_th, ap.SIC_scale);
+            const double f = sic_factor(Ca, ap.SIC_th, ap.SIC_scale);
+            fac[static_cast<size_t>(k) * NA + a] = f;
+            live_bits |= static_cast<uint32_t>(f != 0.0) << k;
         }
     }
     n.Ca[a] = Ca;
     n.IP3[a] = IP3;
     n.hh[a] = hh;
+    mark_sic_live(n, live_bits, W);
 }
 
 // ----------------------------------------------------- FP32 cell windows ---
@@ -438,7 +451,6 @@ __device__ void neuron_window_f32(const DevNet& n, uint32_t i, int64_t T0, int W
     const int d_a2n = n.d_a2n;
     const int sic_interval = n.sic_interval;
     const int substeps = n.substeps;
-    const uint64_t seed = n.seed;
     const float hs = static_cast<float>(n.h), half_h = static_cast<float>(n.half_h),
                 h_sixth = static_cast<float>(n.h_sixth);
     const uint32_t words = n.words;
@@ -446,7 +458,8 @@ __device__ void neuron_window_f32(const DevNet& n, uint32_t i, int64_t T0, int W
     float* __restrict__ ring_inh = n.fring_inh;
     const float* __restrict__ ring_sic = n.fring_sic;
     uint64_t* __restrict__ spk_bits = n.spk_bits;
-    const uint64_t salt_shared = pi == 0 ? kSaltNoiseSharedExc : kSaltNoiseSharedInh;
+    const int32_t* __restrict__ drive_n = n.drive_n;
+    const float* __restrict__ drive_noise = n.fdrive_noise;
 
     float V = 0, w = 0, gex = 0, dgex = 0, gin = 0, dgin = 0, Isic = 0, Vr = 0, b = 0;
     int refr = 0;
@@ -467,14 +480,13 @@ __device__ void neuron_window_f32(const DevNet& n, uint32_t i, int64_t T0, int W
         ih_n = ring_inh[s0];
         sic_n = ring_sic[s0];
     }
-    int64_t noise_idx = -1;
-    float noise_val = 0.0f;
     const uint32_t lane = threadIdx.x & 63;
     for (int k = 0; k < W; ++k) {
         const int64_t T = T0 + k;
         bool spiked = false;
         if (valid) {
             const size_t slot = static_cast<size_t>(T % S) * N + i;
+            const size_t dk = static_cast<size_t>(k) * N + i;
             const float ex = ex_n, ih = ih_n, sic = sic_n;
             if (k + 1 < W) {
                 const size_t nx = static_cast<size_t>((T + 1) % S) * N + i;
@@ -502,30 +514,17 @@ __device__ void neuron_window_f32(const DevNet& n, uint32_t i, int64_t T0, int W
             if (T >= d_a2n && (T - d_a2n) % sic_interval == 0) {
                 Isic = sic;
             }
-            // Same counter streams as the FP64 step: identical Poisson events.
+            // Same counter streams as the FP64 step (k_drive): identical
+            // Poisson events, the noise current rounded to FP32.
             if (in.poiss_rate > 0.0) {
-                const int ev = rng_poisson_fresh(seed ^ kSaltNeuronPoisson,
-                                                 static_cast<uint64_t>(T) * kStreamStride + i,
-                                                 in.lambda, in.limit);
+                const int ev = drive_n[dk];
                 if (ev > 0) {
                     exc_in += static_cast<float>(in.poiss_weight * static_cast<double>(ev));
                 }
             }
             float I_ext = Isic;
             if (in.noise_std > 0.0) {
-                const int64_t idx = T / in.steps_per_noise;
-                if (idx != noise_idx) {
-                    noise_idx = idx;
-                    uint64_t ctr = 0;
-                    const double z = in.independent
-                                         ? rng_normal_fresh(seed ^ kSaltNeuronNoise,
-                                                            static_cast<uint64_t>(idx) * kNoiseStride + i,
-                                                            ctr)
-                                         : rng_normal_fresh(seed ^ salt_shared,
-                                                            static_cast<uint64_t>(idx), ctr);
-                    noise_val = static_cast<float>(in.noise_std * z);
-                }
-                I_ext += noise_val;
+                I_ext += drive_noise[dk];
             }
             if (exc_in != 0.0f) {
                 dgex += exc_in * pf.psc_ex;
@@ -579,6 +578,7 @@ __device__ void astro_window_f32(const DevNet& n, uint32_t a, int64_t T0, int W)
     float v_n = ring_astro[static_cast<size_t>(T0 % S) * NA + a];
     int64_t noise_idx = -1;
     float noise = 0.0f;
+    uint32_t live_bits = 0;
     for (int k = 0; k < W; ++k) {
         const int64_t T = T0 + k;
         const size_t slot = static_cast<size_t>(T % S) * NA + a;
@@ -614,42 +614,135 @@ __device__ void astro_window_f32(const DevNet& n, uint32_t a, int64_t T0, int W)
                        substeps, Ca, IP3, hh);
         record_astro(n, a, T, Ca, IP3);
         if (T % sic_interval == 0) {
-            fac[static_cast<size_t>(k) * NA + a] = sic_factor_mixed(Ca, SIC_th, SIC_scale);
+            const float f = sic_factor_mixed(Ca, SIC_th, SIC_scale);
+            fac[static_cast<size_t>(k) * NA + a] = f;
+            live_bits |= static_cast<uint32_t>(f != 0.0f) << k;
         }
     }
     n.fCa[a] = Ca;
     n.fIP3[a] = IP3;
     n.fhh[a] = hh;
+    mark_sic_live(n, live_bits, W);
 }
 
 // FP32 SIC of one (neuron, window step): sum over (astrocyte, multiplicity)
 // of m * w_a2n * factor, compensated.
-__device__ void deliver_sic_f32(const DevNet& n, uint32_t t, int k) {
+template <int GM>
+__device__ void deliver_sic_f32(const DevNet& n, uint32_t t, int g0, int G) {
     const uint32_t N = n.n_neur;
-    const int64_t e = n.win_T0 + k;
-    if (e % n.sic_interval != 0) {
-        return;
-    }
-    const float* __restrict__ f = n.ffac + static_cast<size_t>(k) * n.n_astro;
-    const uint32_t* __restrict__ src = n.sic_src;
-    const uint32_t* __restrict__ mult = n.sic_mult;
-    const uint32_t width = n.sic_width;
-    const float w_a2n = static_cast<float>(n.w_a2n);
-    float acc = 0.0f, comp = 0.0f;
-    for (uint32_t q = 0; q < width; ++q) {
-        const uint32_t m = mult[static_cast<size_t>(q) * N + t];
-        if (m == 0) {
-            break;
-        }
-        const float fa = f[src[static_cast<size_t>(q) * N + t]];
-        if (fa != 0.0f) {
-            comp_add(acc, comp, static_cast<float>(m) * (w_a2n * fa));
+    const int64_t T0 = n.win_T0;
+    uint32_t send = 0;
+    for (int j = 0; j < G; ++j) {
+        if ((T0 + g0 + j) % n.sic_interval == 0) {
+            send |= 1u << j;
         }
     }
-    n.fring_sic[static_cast<size_t>((e + n.d_a2n) % n.S) * N + t] = acc + comp;
+    const uint32_t live = send & (n.sic_live >> g0);
+    float acc[GM], comp[GM];
+#pragma unroll
+    for (int j = 0; j < GM; ++j) {
+        acc[j] = 0.0f;
+        comp[j] = 0.0f;
+    }
+    if (live != 0) {
+        const uint32_t* __restrict__ src = n.sic_src;
+        const uint32_t* __restrict__ mult = n.sic_mult;
+        const float* __restrict__ fac = n.ffac + static_cast<size_t>(g0) * n.n_astro;
+        const size_t NA = n.n_astro;
+        const float w_a2n = static_cast<float>(n.w_a2n);
+        for (uint32_t q = 0; q < n.sic_width; ++q) {
+            const uint32_t m = mult[static_cast<size_t>(q) * N + t];
+            if (m == 0) {
+                break;
+            }
+            const uint32_t a = src[static_cast<size_t>(q) * N + t];
+            const float fm = static_cast<float>(m);
+#pragma unroll
+            for (int j = 0; j < GM; ++j) {
+                if ((live >> j) & 1u) {
+                    const float fa = fac[j * NA + a];
+                    if (fa != 0.0f) {
+                        comp_add(acc[j], comp[j], fm * (w_a2n * fa));
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < GM; ++j) {
+        if ((send >> j) & 1u) {
+            n.fring_sic[static_cast<size_t>((T0 + g0 + j + n.d_a2n) % n.S) * N + t] =
+                acc[j] + comp[j];
+        }
+    }
 }
 
-// Window pass 1: every cell of every network advances W steps.
+// Window pass 0: the external drive of every neuron for the window's steps,
+// from the reference's counter streams -- Poisson event counts of the
+// background input (neuron.cpp:143-151) and the Gaussian noise current held
+// over noise_dt (neuron.cpp:152-166) -- written per (step, neuron) for the
+// cell pass. The draws are chains of 64-bit multiplies with data-dependent
+// trip counts: cheap in this small kernel at high occupancy, costly inside
+// the register-heavy RK4 pass. One wavefront per neuron block.
+__global__ __launch_bounds__(256) void k_drive(DevNet* nets, const CellBlock* nblocks,
+                                               int32_t n_items, int32_t Wg) {
+    const int item = static_cast<int>(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (item >= n_items) {
+        return;
+    }
+    const CellBlock cb = nblocks[item];
+    const DevNet& n = nets[cb.net];
+    const int W = window_len(n, Wg);
+    const uint32_t N = n.n_neur;
+    const uint32_t i = cb.first + (threadIdx.x & 63);
+    if (W == 0 || i >= N) {
+        return;
+    }
+    const int pi = (i < n.n_exc) ? 0 : 1;
+    const InputK in = n.in[pi];
+    const int64_t T0 = n.step;
+    const uint64_t seed = n.seed;
+    if (in.poiss_rate > 0.0) {
+        int32_t* __restrict__ out = n.drive_n;
+        for (int k = 0; k < W; ++k) {
+            const int64_t T = T0 + k;
+            out[static_cast<size_t>(k) * N + i] =
+                rng_poisson_fresh(seed ^ kSaltNeuronPoisson,
+                                  static_cast<uint64_t>(T) * kStreamStride + i, in.lambda,
+                                  in.limit);
+        }
+    }
+    if (in.noise_std > 0.0) {
+        // One draw per noise interval (neuron.cpp:152-166), held across it.
+        const uint64_t salt_shared = pi == 0 ? kSaltNoiseSharedExc : kSaltNoiseSharedInh;
+        const bool f32 = n.precision == ASG_FP32;
+        int64_t noise_idx = -1;
+        double z = 0.0;
+        for (int k = 0; k < W; ++k) {
+            const int64_t idx = (T0 + k) / in.steps_per_noise;
+            if (idx != noise_idx) {
+                noise_idx = idx;
+                uint64_t ctr = 0;
+                z = in.independent
+                        ? rng_normal_fresh(seed ^ kSaltNeuronNoise,
+                                           static_cast<uint64_t>(idx) * kNoiseStride + i, ctr)
+                        : rng_normal_fresh(seed ^ salt_shared, static_cast<uint64_t>(idx), ctr);
+            }
+            const size_t dk = static_cast<size_t>(k) * N + i;
+            if (f32) {
+                n.fdrive_noise[dk] = static_cast<float>(in.noise_std * z);
+            } else {
+                n.drive_noise[dk] = 0.0 + in.noise_std * z;
+            }
+        }
+    }
+}
+
+// Window pass 1: every cell of every network advances W steps. One kernel per
+// precision, so neither carries the other's registers; neurons and
+// astrocytes share the launch, so the short astrocyte blocks overlap the
+// neuron blocks.
+template <int P>
 __global__ __launch_bounds__(kCellBlock) void k_cells(DevNet* nets, const CellBlock* blocks,
                                                       int32_t Wg) {
     const CellBlock cb = blocks[blockIdx.x];
@@ -664,16 +757,18 @@ __global__ __launch_bounds__(kCellBlock) void k_cells(DevNet* nets, const CellBl
         return;
     }
     const uint32_t i = cb.first + threadIdx.x;
-    if (n.precision == ASG_FP32) {
+    if constexpr (P == ASG_FP32) {
         if (cb.kind == 0) {
             neuron_window_f32(n, i, T0, W);
         } else {
             astro_window_f32(n, i, T0, W);
         }
-    } else if (cb.kind == 0) {
-        neuron_window(n, i, T0, W);
     } else {
-        astro_window(n, i, T0, W);
+        if (cb.kind == 0) {
+            neuron_window(n, i, T0, W);
+        } else {
+            astro_window(n, i, T0, W);
+        }
     }
 }
 
@@ -745,39 +840,65 @@ __global__ __launch_bounds__(64) void k_compact(DevNet* nets, const int32_t* act
 // The SIC neuron t receives from the exchange sent at window step k
 // (network.cpp:281-299): w_a2n * factor(a) added once per (a, t) synapse,
 // astrocytes ascending, written to ring_sic at the arrival slot.
-__device__ void deliver_sic(const DevNet& n, uint32_t t, int k) {
+template <int GM>
+__device__ void deliver_sic(const DevNet& n, uint32_t t, int g0, int G) {
     const uint32_t N = n.n_neur;
-    const int64_t e = n.win_T0 + k;
-    if (e % n.sic_interval != 0) {
-        return;
-    }
-    const double* __restrict__ f = n.fac + static_cast<size_t>(k) * n.n_astro;
-    const uint32_t* __restrict__ src = n.sic_src;
-    const uint32_t* __restrict__ mult = n.sic_mult;
-    const uint32_t width = n.sic_width;
-    const double w_a2n = n.w_a2n;
-    double acc = 0.0;
-    for (uint32_t q = 0; q < width; ++q) {
-        const uint32_t m = mult[static_cast<size_t>(q) * N + t];
-        if (m == 0) {
-            break;
-        }
-        const double fa = f[src[static_cast<size_t>(q) * N + t]];
-        if (fa != 0.0) {
-            const double v = w_a2n * fa;
-            uint32_t r = m;
-            for (; r >= 4; r -= 4) {
-                acc += v;
-                acc += v;
-                acc += v;
-                acc += v;
-            }
-            for (; r > 0; --r) {
-                acc += v;
-            }
+    const int64_t T0 = n.win_T0;
+    uint32_t send = 0;  // bit j: window step g0 + j sends an exchange
+    for (int j = 0; j < G; ++j) {
+        if ((T0 + g0 + j) % n.sic_interval == 0) {
+            send |= 1u << j;
         }
     }
-    n.ring_sic[static_cast<size_t>((e + n.d_a2n) % n.S) * N + t] = acc;
+    // Steps at which no astrocyte of the network has a nonzero factor have
+    // an empty sum.
+    const uint32_t live = send & (n.sic_live >> g0);
+    double acc[GM];
+#pragma unroll
+    for (int j = 0; j < GM; ++j) {
+        acc[j] = 0.0;
+    }
+    if (live != 0) {
+        const uint32_t* __restrict__ src = n.sic_src;
+        const uint32_t* __restrict__ mult = n.sic_mult;
+        const double* __restrict__ fac = n.fac + static_cast<size_t>(g0) * n.n_astro;
+        const size_t NA = n.n_astro;
+        const double w_a2n = n.w_a2n;
+        for (uint32_t q = 0; q < n.sic_width; ++q) {
+            const uint32_t m = mult[static_cast<size_t>(q) * N + t];
+            if (m == 0) {
+                break;
+            }
+            const uint32_t a = src[static_cast<size_t>(q) * N + t];
+            double v[GM];
+            uint32_t nz = 0;
+#pragma unroll
+            for (int j = 0; j < GM; ++j) {
+                v[j] = 0.0;
+                if ((live >> j) & 1u) {
+                    const double fa = fac[j * NA + a];
+                    v[j] = w_a2n * fa;
+                    nz |= static_cast<uint32_t>(fa != 0.0) << j;
+                }
+            }
+            // the m additions of one (astrocyte, target) pair; the chains of
+            // the group's steps are independent and interleave
+            if (nz != 0) {
+                for (uint32_t r = 0; r < m; ++r) {
+#pragma unroll
+                    for (int j = 0; j < GM; ++j) {
+                        acc[j] = ((nz >> j) & 1u) ? acc[j] + v[j] : acc[j];
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < GM; ++j) {
+        if ((send >> j) & 1u) {
+            n.ring_sic[static_cast<size_t>((T0 + g0 + j + n.d_a2n) % n.S) * N + t] = acc[j];
+        }
+    }
 }
 
 // Set bit (j mod 64) of chunk j / 64 in the mask of every target of one row,
@@ -912,26 +1033,39 @@ __device__ void deliver_sources(DevNet& n, uint32_t first, uint32_t lane) {
 }
 
 // Window pass 3: one wavefront per (64 neurons, window step); four per block.
+// Window pass 3. Per neuron block, Wg + ceil(Wg / G) wavefronts: one per
+// window step k for the marking of that step's spikes (and, at k = 0, the
+// sources' plasticity and raster entries), then one per group of G steps for
+// the SIC sums of the block's neurons. G trades re-reading the astrocyte
+// lists (G = Wg reads them once per window) against parallelism for small
+// networks (G = 1).
+template <int G>
 __global__ __launch_bounds__(256) void k_deliver(DevNet* nets, const CellBlock* nblocks,
                                                  int32_t n_items, int32_t Wg) {
     const int e = static_cast<int>(blockIdx.x) * 4 + static_cast<int>(threadIdx.x >> 6);
     if (e >= n_items) {
         return;
     }
-    const CellBlock cb = nblocks[e / Wg];
-    const int k = e % Wg;
+    const int per_block = Wg + (Wg + G - 1) / G;
+    const CellBlock cb = nblocks[e / per_block];
+    const int k = e % per_block;
     DevNet& n = nets[cb.net];
-    if (k >= n.win_W) {
+    const uint32_t lane = threadIdx.x & 63;
+    if (k >= Wg) {
+        const int g0 = (k - Wg) * G;
+        const int len = min(G, n.win_W - g0);
+        const uint32_t t = cb.first + lane;
+        if (len > 0 && t < n.n_neur) {
+            if (n.precision == ASG_FP32) {
+                deliver_sic_f32<G>(n, t, g0, len);
+            } else {
+                deliver_sic<G>(n, t, g0, len);
+            }
+        }
         return;
     }
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t t = cb.first + lane;
-    if (t < n.n_neur) {
-        if (n.precision == ASG_FP32) {
-            deliver_sic_f32(n, t, k);
-        } else {
-            deliver_sic(n, t, k);
-        }
+    if (k >= n.win_W) {
+        return;
     }
     mark_step(n, cb.first, lane, k);
     if (k == 0) {
@@ -1060,6 +1194,7 @@ __global__ __launch_bounds__(256) void k_sum(DevNet* nets, const CellBlock* bloc
         }
         n.c_steps += W;
         n.step = T0 + W;
+        n.sic_live = 0;
     }
     if (k >= W) {
         return;
@@ -1271,6 +1406,9 @@ NeuronKf make_neuron_kf(const asg_neuron_params& s, double h) {
     p.ex_decay = static_cast<float>(asg_exp(-h / s.tau_syn_ex));
     p.in_decay = static_cast<float>(asg_exp(-h / s.tau_syn_in));
     p.near_V = static_cast<float>(s.V_th - kNearThresholdSlopes * s.Delta_T);
+    p.inv_C_m_d = 1.0 / s.C_m;
+    p.inv_Delta_T_d = 1.0 / s.Delta_T;
+    p.inv_tau_w_d = 1.0 / s.tau_w;
     return p;
 }
 
@@ -1477,6 +1615,9 @@ Plan plan_of(const asg_net_desc& d, const DerivedConn* dc) {
     b += p.Wmax * (p.words * 12 + p.N * 4 + 8 + p.N * 16);
     b += p.Wmax * (p.cE * p.N + p.cI * p.N + p.cE * p.NA) * 8;
     b += p.r_cap * 12;
+    const bool poiss = d.input_exc.poiss_rate > 0.0 || d.input_inh.poiss_rate > 0.0;
+    const bool noise = d.input_exc.gauss_noise_std > 0.0 || d.input_inh.gauss_noise_std > 0.0;
+    b += (poiss ? p.Wmax * p.N * 4 : 0) + (noise ? p.Wmax * p.N * fb : 0);
     p.bytes = b;
     return p;
 }
@@ -1680,6 +1821,16 @@ void build_slot(NetSlot& s, const asg_net_desc& d, int64_t total, hipStream_t st
     n.mask_exc = dev_zero<uint64_t>(s, pl.Wmax * pl.cE * N, st);
     n.mask_inh = dev_zero<uint64_t>(s, pl.Wmax * pl.cI * N, st);
     n.mask_astro = dev_zero<uint64_t>(s, pl.Wmax * pl.cE * NA, st);
+    if (n.in[0].poiss_rate > 0.0 || n.in[1].poiss_rate > 0.0) {
+        n.drive_n = dev_zero<int32_t>(s, pl.Wmax * N, st);
+    }
+    if (n.in[0].noise_std > 0.0 || n.in[1].noise_std > 0.0) {
+        if (d.precision == ASG_FP32) {
+            n.fdrive_noise = dev_zero<float>(s, pl.Wmax * N, st);
+        } else {
+            n.drive_noise = dev_zero<double>(s, pl.Wmax * N, st);
+        }
+    }
     n.r_cap = static_cast<int64_t>(pl.r_cap);
     n.r_step = dev_alloc<int64_t>(s, pl.r_cap);
     n.r_src = dev_alloc<uint32_t>(s, pl.r_cap);
@@ -1732,7 +1883,11 @@ struct Engine {
     bool profiling = false;
     std::vector<hipEvent_t> events;   // 5 per window of the last step() call
     int64_t timed_windows = 0, pending_windows = 0;
-    double kernel_ms[4] = {0, 0, 0, 0};
+    // [0] cells, [1] compaction, [2] delivery pass, [3] ordered sums, [4] drive
+    double kernel_ms[5] = {0, 0, 0, 0, 0};
+    // d_blocks holds two ranges, [cell_range[r], cell_range[r + 1]): the
+    // FP64 networks' blocks, then the FP32 networks'.
+    int cell_range[3] = {0, 0, 0};
 
     explicit Engine(int max_n, int64_t budget_bytes) : max_nets(max_n), budget(budget_bytes) {
         HIP_CHECK(hipGetDevice(&device));
@@ -1786,11 +1941,13 @@ struct Engine {
             HIP_CHECK(hipStreamSynchronize(stream));
             pending = false;
             if (pending_windows > 0) {
+                // event intervals: drive, cells, compaction, delivery, sums
+                constexpr int kSlot[5] = {4, 0, 1, 2, 3};
                 for (int64_t w = 0; w < pending_windows; ++w) {
-                    for (int k = 0; k < 4; ++k) {
+                    for (int k = 0; k < 5; ++k) {
                         float ms = 0.0f;
-                        HIP_CHECK(hipEventElapsedTime(&ms, events[5 * w + k], events[5 * w + k + 1]));
-                        kernel_ms[k] += ms;
+                        HIP_CHECK(hipEventElapsedTime(&ms, events[6 * w + k], events[6 * w + k + 1]));
+                        kernel_ms[kSlot[k]] += ms;
                     }
                 }
                 timed_windows += pending_windows;
@@ -1800,7 +1957,7 @@ struct Engine {
     }
 
     void ensure_events(int64_t windows) {
-        const size_t need = static_cast<size_t>(windows) * 5;
+        const size_t need = static_cast<size_t>(windows) * 6;
         while (events.size() < need) {
             hipEvent_t e;
             HIP_CHECK(hipEventCreate(&e));
@@ -1834,21 +1991,27 @@ struct Engine {
         std::vector<CellBlock> blocks, nblocks;
         std::vector<int32_t> active;
         Wg = kWindowCap;
-        for (int sidx = 0; sidx < max_nets; ++sidx) {
-            const NetSlot& s = slots[sidx];
-            if (!s.used) continue;
-            active.push_back(sidx);
-            Wg = std::min(Wg, s.W);
-            uint32_t lead = 1;
-            for (uint32_t f = 0; f < s.dn.n_neur; f += kCellBlock) {
-                blocks.push_back(CellBlock{sidx, 0, f, lead});
-                nblocks.push_back(CellBlock{sidx, 0, f, lead});
-                lead = 0;
+        // Blocks in two contiguous ranges, one per cell kernel: the FP64
+        // networks' blocks, then the FP32 networks'. Within a network the
+        // neuron blocks come first; its first block is its lead block.
+        for (int range = 0; range < 2; ++range) {
+            for (int sidx = 0; sidx < max_nets; ++sidx) {
+                const NetSlot& s = slots[sidx];
+                if (!s.used || (s.dn.precision == ASG_FP32) != (range == 1)) continue;
+                active.push_back(sidx);
+                Wg = std::min(Wg, s.W);
+                uint32_t lead = 1;
+                for (uint32_t f = 0; f < s.dn.n_neur; f += kCellBlock) {
+                    blocks.push_back(CellBlock{sidx, 0, f, lead});
+                    nblocks.push_back(CellBlock{sidx, 0, f, lead});
+                    lead = 0;
+                }
+                for (uint32_t f = 0; f < s.dn.n_astro; f += kCellBlock) {
+                    blocks.push_back(CellBlock{sidx, 1, f, lead});
+                    lead = 0;
+                }
             }
-            for (uint32_t f = 0; f < s.dn.n_astro; f += kCellBlock) {
-                blocks.push_back(CellBlock{sidx, 1, f, lead});
-                lead = 0;
-            }
+            cell_range[range + 1] = static_cast<int>(blocks.size());
         }
         n_blocks = static_cast<int>(blocks.size());
         n_nblocks = static_cast<int>(nblocks.size());
@@ -1906,6 +2069,37 @@ struct Engine {
         dirty = true;
     }
 
+    // k_deliver for SIC group size G (its accumulators are registers).
+    void launch_deliver(int G, int items, int32_t Wg_) {
+        const dim3 grid((items + 3) / 4), block(256);
+        switch (G) {
+            case 1:
+                hipLaunchKernelGGL(k_deliver<1>, grid, block, 0, stream, d_nets, d_nblocks, items, Wg_);
+                break;
+            case 2:
+                hipLaunchKernelGGL(k_deliver<2>, grid, block, 0, stream, d_nets, d_nblocks, items, Wg_);
+                break;
+            case 4:
+                hipLaunchKernelGGL(k_deliver<4>, grid, block, 0, stream, d_nets, d_nblocks, items, Wg_);
+                break;
+            case 8:
+                hipLaunchKernelGGL(k_deliver<8>, grid, block, 0, stream, d_nets, d_nblocks, items, Wg_);
+                break;
+            default:
+                hipLaunchKernelGGL(k_deliver<16>, grid, block, 0, stream, d_nets, d_nblocks, items, Wg_);
+                break;
+        }
+    }
+
+    template <int P>
+    void launch_cells(int range, int32_t Wg_) {
+        const int count = cell_range[range + 1] - cell_range[range];
+        if (count > 0) {
+            hipLaunchKernelGGL(k_cells<P>, dim3(count), dim3(kCellBlock), 0, stream, d_nets,
+                               d_blocks + cell_range[range], Wg_);
+        }
+    }
+
     // Advance every used slot by up to n steps (to its total).
     void step(int64_t n, bool async) {
         wait();
@@ -1929,28 +2123,41 @@ struct Engine {
         hipLaunchKernelGGL(k_set_targets, dim3((n_active + 255) / 256), dim3(256), 0, stream,
                            d_nets, d_active, n_active, d_targets);
         raster_dirty = true;
-        const int deliver_items = n_nblocks * Wg;
+        // SIC group (1, 2, 4, 8 or 16 steps): the whole window when the
+        // batch has wavefronts enough to fill the device, otherwise fewer
+        // steps per wavefront.
+        int sic_group = 1;
+        while (sic_group < Wg) sic_group *= 2;
+        while (sic_group > 1 &&
+               static_cast<int64_t>(n_nblocks) * ((Wg + sic_group - 1) / sic_group) < kSicWaves) {
+            sic_group /= 2;
+        }
+        const int deliver_items = n_nblocks * (Wg + (Wg + sic_group - 1) / sic_group);
         const int sum_items = n_blocks * Wg;
         if (profiling) ensure_events(windows);
         auto mark = [&](int64_t wdx, int k) {
-            if (profiling) HIP_CHECK(hipEventRecord(events[5 * wdx + k], stream));
+            if (profiling) HIP_CHECK(hipEventRecord(events[6 * wdx + k], stream));
         };
         for (int64_t wdx = 0; wdx < windows; ++wdx) {
             mark(wdx, 0);
-            hipLaunchKernelGGL(k_cells, dim3(n_blocks), dim3(kCellBlock), 0, stream, d_nets,
-                               d_blocks, Wg);
+            if (n_nblocks > 0) {
+                hipLaunchKernelGGL(k_drive, dim3((n_nblocks + 3) / 4), dim3(256), 0, stream,
+                                   d_nets, d_nblocks, n_nblocks, Wg);
+            }
             mark(wdx, 1);
+            launch_cells<ASG_FP64>(0, Wg);
+            launch_cells<ASG_FP32>(1, Wg);
+            mark(wdx, 2);
             hipLaunchKernelGGL(k_compact, dim3(n_active, Wg), dim3(64), 0, stream, d_nets,
                                d_active);
-            mark(wdx, 2);
-            if (deliver_items > 0) {
-                hipLaunchKernelGGL(k_deliver, dim3((deliver_items + 3) / 4), dim3(256), 0, stream,
-                                   d_nets, d_nblocks, deliver_items, Wg);
-            }
             mark(wdx, 3);
+            if (deliver_items > 0) {
+                launch_deliver(sic_group, deliver_items, Wg);
+            }
+            mark(wdx, 4);
             hipLaunchKernelGGL(k_sum, dim3((sum_items + 3) / 4), dim3(256), 0, stream, d_nets,
                                d_blocks, sum_items, Wg);
-            mark(wdx, 4);
+            mark(wdx, 5);
         }
         if (profiling) pending_windows = windows;
         HIP_CHECK(hipGetLastError());
@@ -2449,7 +2656,7 @@ asg_status kernel_times(Engine& e, asg_kernel_times* out) {
     if (out == nullptr) return fail(ASG_ERR_INVALID, "NULL output");
     return guarded([&]() -> asg_status {
         e.wait();
-        for (int k = 0; k < 4; ++k) out->ms[k] = e.kernel_ms[k];
+        for (int k = 0; k < 5; ++k) out->ms[k] = e.kernel_ms[k];
         out->windows = e.timed_windows;
         return ASG_OK;
     });

@@ -6,7 +6,9 @@
 //  * an AdEx substep whose membrane potential is within kNearThresholdSlopes
 //    slope factors of V_th (or above) runs entirely in FP64 -- near
 //    threshold the exponential term grows like e^((V - V_th)/Delta_T) and
-//    decides the spike time;
+//    decides the spike time. This island multiplies by FP64 reciprocals
+//    instead of dividing and uses a table-free FP64 exp accurate to ~3e-13:
+//    its result is rounded to FP32 at the end of the substep;
 //  * the astrocyte's Ca-threshold test of the SIC, (Ca - SIC_th) * 1000 > 1,
 //    is evaluated in FP64 (Ca is a small difference above a threshold).
 // Synaptic input is summed with Neumaier's compensated summation.
@@ -24,6 +26,7 @@ struct NeuronKf {
     float inv_C_m, neg_g_L, E_L, V_th, inv_Delta_T, a, inv_tau_w, V_peak, E_ex, E_in, I_e;
     float gL_DeltaT, psc_ex, psc_in, ex_decay, in_decay;
     float near_V;  // V_th - kNearThresholdSlopes * Delta_T
+    double inv_C_m_d, inv_Delta_T_d, inv_tau_w_d;  // FP64 reciprocals of the island
 };
 
 struct AstroKf {
@@ -51,28 +54,81 @@ __device__ inline void adex_rhs_f32(const NeuronKf& p, float V, float w, float g
     dw = (p.a * (Vc - p.E_L) - w) * p.inv_tau_w;
 }
 
-// One substep of the AdEx + alpha-conductance update in FP32, or in FP64 via
-// the exact cell math when the cell is near threshold. Returns true on a
-// spike (the caller keeps at most one per step).
+// e^x in FP64 for the near-threshold island (|x| below ~40 there):
+// x = n ln2 + r with n = rint(x / ln2) and a two-part ln2 (Cody-Waite), so
+// |r| <= ln2 / 2, then e^r by its degree-10 Taylor polynomial (truncation
+// below r^11/11! < 3e-13 relative) and the scale 2^n.
+__device__ inline double island_exp(double x) {
+    constexpr double kLog2e = 1.4426950408889634074;
+    constexpr double kLn2Hi = 6.93147180369123816490e-01;
+    constexpr double kLn2Lo = 1.90821492927058770002e-10;
+    const double n = rint(x * kLog2e);
+    const double r = fma(-n, kLn2Lo, fma(-n, kLn2Hi, x));
+    double p = 1.0 / 3628800.0;
+    p = fma(p, r, 1.0 / 362880.0);
+    p = fma(p, r, 1.0 / 40320.0);
+    p = fma(p, r, 1.0 / 5040.0);
+    p = fma(p, r, 1.0 / 720.0);
+    p = fma(p, r, 1.0 / 120.0);
+    p = fma(p, r, 1.0 / 24.0);
+    p = fma(p, r, 1.0 / 6.0);
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    return ldexp(p, static_cast<int>(n));
+}
+
+// The AdEx right-hand side in FP64 for the island (reference operation
+// order, neuron.cpp:85-99, with the divisions by C_m, Delta_T and tau_w
+// taken as products with their reciprocals).
+__device__ inline void island_rhs(const NeuronK& p, const NeuronKf& pf, double V, double w,
+                                  double g_ex, double g_in, double I_ext, double& dV,
+                                  double& dw) {
+    const double Vc = (p.V_peak < V) ? p.V_peak : V;
+    const double I_spike = p.gL_DeltaT * island_exp((Vc - p.V_th) * pf.inv_Delta_T_d);
+    const double I_syn_ex = g_ex * (Vc - p.E_ex);
+    const double I_syn_in = g_in * (Vc - p.E_in);
+    dV = (p.neg_g_L * (Vc - p.E_L) + I_spike - I_syn_ex - I_syn_in - w + p.I_e + I_ext) *
+         pf.inv_C_m_d;
+    dw = (p.a * (Vc - p.E_L) - w) * pf.inv_tau_w_d;
+}
+
+// One substep of the AdEx + alpha-conductance update in FP32, or in FP64 when
+// the cell is near threshold and not refractory. Returns true on a spike (the
+// caller keeps at most one per step).
 __device__ inline bool adex_substep_mixed(const NeuronKf& pf, const NeuronK& pd, float V_reset,
                                           float b, float I_ext, float hs, float half_h,
                                           float h_sixth, double hs_d, double half_h_d,
                                           double h_sixth_d, float& V, float& w, float& g_ex,
                                           float& dg_ex, float& g_in, float& dg_in, int& refr) {
     if (V > pf.near_V && refr == 0) {
-        double Vd = V, wd = w, ge = g_ex, dge = dg_ex, gi = g_in, dgi = dg_in;
-        int r = refr;
-        const bool spiked = adex_step(pd, static_cast<double>(V_reset), static_cast<double>(b),
-                                      static_cast<double>(I_ext), hs_d, half_h_d, h_sixth_d, 1,
-                                      Vd, wd, ge, dge, gi, dgi, r);
+        // adex_step's non-refractory substep in FP64
+        const double ge = g_ex, dge = dg_ex, gi = g_in, dgi = dg_in, I = I_ext;
+        const double ge_next = pd.ex_decay * (ge + hs_d * dge);
+        const double gi_next = pd.in_decay * (gi + hs_d * dgi);
+        const double ge_mid = 0.5 * (ge + ge_next);
+        const double gi_mid = 0.5 * (gi + gi_next);
+        double Vd = V, wd = w;
+        double k1V, k1w, k2V, k2w, k3V, k3w, k4V, k4w;
+        island_rhs(pd, pf, Vd, wd, ge, gi, I, k1V, k1w);
+        island_rhs(pd, pf, Vd + half_h_d * k1V, wd + half_h_d * k1w, ge_mid, gi_mid, I, k2V, k2w);
+        island_rhs(pd, pf, Vd + half_h_d * k2V, wd + half_h_d * k2w, ge_mid, gi_mid, I, k3V, k3w);
+        island_rhs(pd, pf, Vd + hs_d * k3V, wd + hs_d * k3w, ge_next, gi_next, I, k4V, k4w);
+        Vd += h_sixth_d * (k1V + 2.0 * k2V + 2.0 * k3V + k4V);
+        wd += h_sixth_d * (k1w + 2.0 * k2w + 2.0 * k3w + k4w);
+        g_ex = static_cast<float>(ge_next);
+        dg_ex = static_cast<float>(pd.ex_decay * dge);
+        g_in = static_cast<float>(gi_next);
+        dg_in = static_cast<float>(pd.in_decay * dgi);
+        if (Vd >= pd.V_peak) {
+            V = V_reset;
+            w = static_cast<float>(wd + static_cast<double>(b));
+            refr = pd.refr_steps;
+            return true;
+        }
         V = static_cast<float>(Vd);
         w = static_cast<float>(wd);
-        g_ex = static_cast<float>(ge);
-        dg_ex = static_cast<float>(dge);
-        g_in = static_cast<float>(gi);
-        dg_in = static_cast<float>(dgi);
-        refr = r;
-        return spiked;
+        return false;
     }
     const float g_ex_next = pf.ex_decay * (g_ex + hs * dg_ex);
     const float dg_ex_next = pf.ex_decay * dg_ex;

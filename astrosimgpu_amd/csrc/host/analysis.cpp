@@ -150,26 +150,22 @@ double round_6_digits(double x) {
     return std::strtod(buf, nullptr);
 }
 
-CalciumSummary analyze_calcium(const double* times, const double* ca, int32_t n_cells,
-                               int64_t n_samples, double threshold, double merge_gap,
-                               double pre_sim_time, double sim_time, double hist_window,
-                               double hist_shift, bool round6) {
+namespace {
+
+// The calcium block of main.cpp:264-313 over per-cell traces: transients per
+// cell (merged when closer than merge_gap), durations, then the pairwise
+// correlation of the cells' onset histograms over the recorded window.
+CalciumSummary analyze_traces(const std::vector<std::vector<double>>& times,
+                              const std::vector<std::vector<double>>& ca, double threshold,
+                              double merge_gap, double pre_sim_time, double sim_time,
+                              double hist_window, double hist_shift) {
     CalciumSummary out;
-    out.cells = n_cells;
+    out.cells = static_cast<int32_t>(ca.size());
     out.threshold = threshold;
-    std::vector<double> t(static_cast<size_t>(n_samples));
-    for (int64_t k = 0; k < n_samples; ++k) {
-        t[static_cast<size_t>(k)] = round6 ? round_6_digits(times[k]) : times[k];
-    }
     std::vector<std::vector<double>> onsets;
     std::vector<double> durations;
-    std::vector<double> trace(static_cast<size_t>(n_samples));
-    for (int32_t c = 0; c < n_cells; ++c) {
-        for (int64_t k = 0; k < n_samples; ++k) {
-            const double v = ca[k * n_cells + c];
-            trace[static_cast<size_t>(k)] = round6 ? round_6_digits(v) : v;
-        }
-        const Events ev = detect_events(t, trace, threshold, true, merge_gap);
+    for (size_t c = 0; c < ca.size(); ++c) {
+        const Events ev = detect_events(times[c], ca[c], threshold, true, merge_gap);
         onsets.push_back(ev.onset);
         durations.insert(durations.end(), ev.duration.begin(), ev.duration.end());
         if (!ev.onset.empty()) ++out.active;
@@ -196,6 +192,44 @@ CalciumSummary analyze_calcium(const double* times, const double* ca, int32_t n_
     out.mean_duration = dur.mean;
     out.sd_duration = dur.stddev;
     return out;
+}
+
+}  // namespace
+
+CalciumSummary analyze_calcium(const double* times, const double* ca, int32_t n_cells,
+                               int64_t n_samples, double threshold, double merge_gap,
+                               double pre_sim_time, double sim_time, double hist_window,
+                               double hist_shift, bool round6) {
+    std::vector<double> t(static_cast<size_t>(n_samples));
+    for (int64_t k = 0; k < n_samples; ++k) {
+        t[static_cast<size_t>(k)] = round6 ? round_6_digits(times[k]) : times[k];
+    }
+    std::vector<std::vector<double>> tc(static_cast<size_t>(n_cells), t), vc(static_cast<size_t>(n_cells));
+    for (int32_t c = 0; c < n_cells; ++c) {
+        std::vector<double>& trace = vc[static_cast<size_t>(c)];
+        trace.resize(static_cast<size_t>(n_samples));
+        for (int64_t k = 0; k < n_samples; ++k) {
+            const double v = ca[k * n_cells + c];
+            trace[static_cast<size_t>(k)] = round6 ? round_6_digits(v) : v;
+        }
+    }
+    return analyze_traces(tc, vc, threshold, merge_gap, pre_sim_time, sim_time, hist_window,
+                          hist_shift);
+}
+
+CalciumSummary analyze_calcium_cells(const double* times, const double* ca, const int64_t* offsets,
+                                     int32_t n_cells, double threshold, double merge_gap,
+                                     double pre_sim_time, double sim_time, double hist_window,
+                                     double hist_shift, bool round6) {
+    std::vector<std::vector<double>> tc(static_cast<size_t>(n_cells)), vc(static_cast<size_t>(n_cells));
+    for (int32_t c = 0; c < n_cells; ++c) {
+        for (int64_t k = offsets[c]; k < offsets[c + 1]; ++k) {
+            tc[static_cast<size_t>(c)].push_back(round6 ? round_6_digits(times[k]) : times[k]);
+            vc[static_cast<size_t>(c)].push_back(round6 ? round_6_digits(ca[k]) : ca[k]);
+        }
+    }
+    return analyze_traces(tc, vc, threshold, merge_gap, pre_sim_time, sim_time, hist_window,
+                          hist_shift);
 }
 
 }  // namespace asg_host

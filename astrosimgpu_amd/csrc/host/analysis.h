@@ -53,6 +53,14 @@ CalciumSummary analyze_calcium(const double* times, const double* ca, int32_t n_
                                double pre_sim_time, double sim_time, double hist_window,
                                double hist_shift, bool round6);
 
+// The same over per-cell traces of their own lengths: cell c's samples are
+// times/ca[offsets[c] .. offsets[c + 1]) (main.cpp:46-74 reads each cell's
+// rows into its own series).
+CalciumSummary analyze_calcium_cells(const double* times, const double* ca, const int64_t* offsets,
+                                     int32_t n_cells, double threshold, double merge_gap,
+                                     double pre_sim_time, double sim_time, double hist_window,
+                                     double hist_shift, bool round6);
+
 double round_6_digits(double x);
 
 }  // namespace asg_host

@@ -112,6 +112,21 @@ asg_status asg_model_get_info(const asg_model* model, asg_model_info* out) {
 
 #include "analysis.h"
 
+namespace {
+void copy_summary(const asg_host::CalciumSummary& s, asg_calcium_summary* out) {
+    out->cells = s.cells;
+    out->active = s.active;
+    out->transients = s.transients;
+    out->pairs = s.pairs;
+    out->mean_duration = s.mean_duration;
+    out->sd_duration = s.sd_duration;
+    out->mean_correlation = s.mean_correlation;
+    out->threshold = s.threshold;
+    out->width = s.width;
+    out->shift = s.shift;
+}
+}  // namespace
+
 extern "C" {
 
 asg_status asg_analyze_calcium(const double* times, const double* ca, int32_t n_cells,
@@ -125,16 +140,30 @@ asg_status asg_analyze_calcium(const double* times, const double* ca, int32_t n_
     const asg_host::CalciumSummary s = asg_host::analyze_calcium(
         times, ca, n_cells, n_samples, threshold, merge_gap, pre_sim_time, sim_time, hist_window,
         hist_shift, round6 != 0);
-    out->cells = s.cells;
-    out->active = s.active;
-    out->transients = s.transients;
-    out->pairs = s.pairs;
-    out->mean_duration = s.mean_duration;
-    out->sd_duration = s.sd_duration;
-    out->mean_correlation = s.mean_correlation;
-    out->threshold = s.threshold;
-    out->width = s.width;
-    out->shift = s.shift;
+    copy_summary(s, out);
+    return ASG_OK;
+}
+
+asg_status asg_analyze_calcium_cells(const double* times, const double* ca,
+                                     const int64_t* offsets, int32_t n_cells, double threshold,
+                                     double merge_gap, double pre_sim_time, double sim_time,
+                                     double hist_window, double hist_shift, int32_t round6,
+                                     asg_calcium_summary* out) {
+    if (out == nullptr || n_cells < 0 || (n_cells > 0 && offsets == nullptr)) {
+        return asg_host_fail(ASG_ERR_INVALID, "bad argument");
+    }
+    for (int32_t c = 0; c < n_cells; ++c) {
+        if (offsets[c] < 0 || offsets[c + 1] < offsets[c]) {
+            return asg_host_fail(ASG_ERR_INVALID, "offsets must be non-decreasing from 0");
+        }
+    }
+    if (n_cells > 0 && offsets[n_cells] > 0 && (times == nullptr || ca == nullptr)) {
+        return asg_host_fail(ASG_ERR_INVALID, "bad argument");
+    }
+    copy_summary(asg_host::analyze_calcium_cells(times, ca, offsets, n_cells, threshold,
+                                                 merge_gap, pre_sim_time, sim_time, hist_window,
+                                                 hist_shift, round6 != 0),
+                 out);
     return ASG_OK;
 }
 

@@ -176,7 +176,7 @@ class DeviceNetwork:
         t = capi.KernelTimes()
         check(lib().asg_net_kernel_times(self._h, C.byref(t)))
         return {"cells": t.ms[0], "compact": t.ms[1], "deliver": t.ms[2], "sum": t.ms[3],
-                "windows": t.windows}
+                "drive": t.ms[4], "windows": t.windows}
 
     def close(self):
         if getattr(self, "_h", None):
@@ -269,7 +269,7 @@ class Batch:
         t = capi.KernelTimes()
         check(lib().asg_batch_kernel_times(self._h, C.byref(t)))
         return {"cells": t.ms[0], "compact": t.ms[1], "deliver": t.ms[2], "sum": t.ms[3],
-                "windows": t.windows}
+                "drive": t.ms[4], "windows": t.windows}
 
     def bytes_in_use(self) -> int:
         return int(lib().asg_batch_bytes_in_use(self._h))

@@ -105,15 +105,17 @@ class ServeReport:
 class Server:
     """`batch` is the device batch (an asg_batch with `max_slots` slots and a
     memory budget of `budget_bytes`, 0 = free device memory, by default);
-    `builder(job) -> Model` constructs a job's network on the host."""
+    `builder(job) -> Model` constructs a job's network on the host, on
+    `executor` (a thread pool of `builders` threads by default)."""
 
     def __init__(self, base_config: str, max_slots: int = 512, budget_bytes: int = 0,
-                 chunk_steps: int = 2000, builders: int = 2, batch=None, builder=None):
+                 chunk_steps: int = 2000, builders: int = 2, batch=None, builder=None,
+                 executor=None):
         self.base_config = base_config
         self.batch = batch if batch is not None else Batch(max_slots, budget_bytes)
         self.builder = builder if builder is not None else (lambda j: build_job(base_config, j))
         self.chunk_steps = chunk_steps
-        self.pool = cf.ThreadPoolExecutor(max_workers=builders)
+        self.pool = executor if executor is not None else cf.ThreadPoolExecutor(max_workers=builders)
         self.refused: set = set()   # ids of jobs that had to wait for room
 
     def run(self, jobs, clock=None, sleep=None, progress=None) -> ServeReport:

@@ -271,6 +271,9 @@ def main():
     ap.add_argument("--no-sparse", action="store_true")
     ap.add_argument("--fp32-only", action="store_true",
                     help="run only the batched FP32 sweep (for kernel profiles)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy kernel of MIB MiB five times "
+                         "(the reference point for HBM byte counters) and exit")
     ap.add_argument("--sparse-seconds", type=int, default=2,
                     help="timed simulated seconds of the Sparse FP64 runs (10 = the full run)")
     args = ap.parse_args()
@@ -279,6 +282,18 @@ def main():
               "networks of a sweep would be bin-packed over GPUs by footprint, one batch per "
               "GPU, which is not built here", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration > 0:
+        import torch
+        n = args.copy_calibration << 20
+        src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_kernel_bytes_read": n, "copy_kernel_bytes_written": n,
+                          "launches": 5}), flush=True)
+        return
 
     from astrosimgpu_amd import _capi as capi
     from astrosimgpu_amd import sweep

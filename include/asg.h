@@ -211,6 +211,15 @@ asg_status asg_analyze_calcium(const double* times, const double* ca, int32_t n_
                                double pre_sim_time, double sim_time, double hist_window,
                                double hist_shift, int32_t round6, asg_calcium_summary* out);
 
+/* The same over per-cell traces of their own lengths, as main.cpp:46-74
+ * reads them from astrocytes.csv: cell c's samples are
+ * times/ca[offsets[c] .. offsets[c + 1]), offsets has n_cells + 1 entries. */
+asg_status asg_analyze_calcium_cells(const double* times, const double* ca,
+                                     const int64_t* offsets, int32_t n_cells, double threshold,
+                                     double merge_gap, double pre_sim_time, double sim_time,
+                                     double hist_window, double hist_shift, int32_t round6,
+                                     asg_calcium_summary* out);
+
 /* detect_events, analysis.cpp:10-64: onsets/offsets of threshold crossings,
  * merged when closer than max_gap (if merge != 0). */
 asg_status asg_detect_events(const double* times, const double* values, int64_t n,
@@ -359,9 +368,10 @@ int64_t asg_batch_bytes_in_use(const asg_batch* batch);
 
 /* Device time per kernel of the window pipeline, measured with HIP events on
  * the handle's own stream while profiling is on: [0] cell update, [1] spike
- * compaction, [2] SIC sums + plasticity + synapse marking, [3] ordered delivery sums. */
+ * compaction, [2] SIC sums + plasticity + synapse marking, [3] ordered delivery
+ * sums, [4] external drive draws (Poisson counts, noise current). */
 typedef struct asg_kernel_times {
-    double ms[4];
+    double ms[5];
     int64_t windows;   /* windows timed */
 } asg_kernel_times;
 

@@ -4,6 +4,8 @@ within the budget left) so arrival order, refusals and retirement can be
 checked without a GPU. Footprints are the library's own asg_query_footprint
 figures of the built networks."""
 
+import concurrent.futures as cf
+
 import numpy as np
 
 import astrosimgpu_amd.serve as S
@@ -20,6 +22,18 @@ class FakeClock:
 
     def sleep(self, s):
         self.t += s
+
+
+class InlineExecutor:
+    """Builds at submission, so the schedule depends on the fake clock only."""
+
+    def submit(self, fn, *args):
+        f = cf.Future()
+        f.set_result(fn(*args))
+        return f
+
+    def shutdown(self, wait=True):
+        pass
 
 
 class FakeBatch:
@@ -99,7 +113,7 @@ def test_admission_respects_budget_and_arrival_order():
     budget = int(1.5 * max(fps))
     clk = FakeClock()
     fb = FakeBatch(4, budget, clk)
-    srv = S.Server(base, chunk_steps=500, batch=fb)
+    srv = S.Server(base, chunk_steps=500, batch=fb, executor=InlineExecutor())
     rep = srv.run(jobs, clock=clk, sleep=clk.sleep)
     srv.close()
     assert rep.jobs == len(jobs)
@@ -122,7 +136,7 @@ def test_a_job_larger_than_the_budget_is_not_admitted_ahead_of_order():
     fps = [S.build_job(base, j).footprint() for j in jobs]
     clk = FakeClock()
     fb = FakeBatch(8, max(fps), clk)            # one job at a time at worst
-    srv = S.Server(base, chunk_steps=400, batch=fb)
+    srv = S.Server(base, chunk_steps=400, batch=fb, executor=InlineExecutor())
     rep = srv.run(jobs, clock=clk, sleep=clk.sleep)
     srv.close()
     adds = [slot for ev, slot, _ in fb.log if ev == "add"]
@@ -134,7 +148,7 @@ def test_a_job_larger_than_the_budget_is_not_admitted_ahead_of_order():
 def test_a_job_that_cannot_fit_an_empty_batch_is_an_error():
     base = config_path("paper_sparse")
     clk = FakeClock()
-    srv = S.Server(base, batch=FakeBatch(4, 1000, clk))
+    srv = S.Server(base, batch=FakeBatch(4, 1000, clk), executor=InlineExecutor())
     try:
         srv.run(_jobs()[:1], clock=clk, sleep=clk.sleep)
     except RuntimeError as e:
