@@ -90,6 +90,18 @@ class KernelTimes(C.Structure):
     _fields_ = [("ms", C.c_double * 5), ("windows", C.c_int64)]
 
 
+class AstroScalingParams(C.Structure):
+    _fields_ = [("Ca_tot", C.c_double), ("IP3_0", C.c_double), ("tau_IP3", C.c_double),
+                ("delta_IP3", C.c_double), ("dt", C.c_double), ("lambda_", C.c_double),
+                ("randomized", C.c_int32), ("lower", C.c_double), ("upper", C.c_double),
+                ("var", C.c_double)]
+
+
+class AstroScalingResult(C.Structure):
+    _fields_ = [("update_ms", C.c_double), ("total_ms", C.c_double),
+                ("bytes_per_step", C.c_double), ("steps", C.c_int64)]
+
+
 class CalciumSummary(C.Structure):
     _fields_ = [("cells", C.c_int32), ("active", C.c_int32), ("transients", C.c_int64),
                 ("pairs", C.c_int64), ("mean_duration", C.c_double), ("sd_duration", C.c_double),
@@ -173,6 +185,10 @@ PROTOTYPES = {
     "asg_analyze_calcium_cells": (C.c_int, [DBLP, DBLP, I64P, C.c_int32, C.c_double,
                                             C.c_double, C.c_double, C.c_double, C.c_double,
                                             C.c_double, C.c_int32, C.POINTER(CalciumSummary)]),
+    "asg_astro_scaling": (C.c_int, [C.c_uint32, C.c_int32, C.c_int32, C.POINTER(AstroConstants),
+                                    C.POINTER(AstroScalingParams),
+                                    C.c_uint64, DBLP, DBLP, DBLP, C.POINTER(AstroScalingResult)]),
+    "asg_astro_scaling_bytes": (C.c_int64, [C.c_uint32, C.c_int32, C.c_int32]),
     "asg_detect_events": (C.c_int, [DBLP, DBLP, C.c_int64, C.c_double, C.c_int32, C.c_double,
                                     DBLP, DBLP, C.c_int64, I64P]),
     "asg_pearson": (C.c_double, [DBLP, DBLP, C.c_int64]),

@@ -131,17 +131,7 @@ void asg_astro_update(asg_astro* s, const asg_astro_constants* c, double h_step,
     if (s == nullptr || s->n == 0) {
         return;
     }
-    asg::AstroK k{};
-    k.ratio_ER_cyt = c->ratio_ER_cyt;
-    k.Kd_IP3_1 = c->Kd_IP3_1;
-    k.Kd_IP3_2 = c->Kd_IP3_2;
-    k.Kd_act = c->Kd_act;
-    k.ratio_rate_IP3R = c->ratio_ER_cyt * c->rate_IP3R;
-    k.ratio_rate_L = c->ratio_ER_cyt * c->rate_L;
-    k.rate_SERCA = c->rate_SERCA;
-    k.Km_SERCA_sq = c->Km_SERCA * c->Km_SERCA;
-    k.k_Kd_inh = c->k_IP3R * c->Kd_inh;
-    k.k_IP3R = c->k_IP3R;
+    const asg::AstroK k = asg::make_astro_k(*c);
     const int grid = static_cast<int>((s->n + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_astro_update, dim3(grid), dim3(kBlock), 0, 0, s->n, k, h_step,
                        0.5 * h_step, h_step / 6.0, substeps, noise_std, independent_noise,

@@ -1412,36 +1412,6 @@ NeuronKf make_neuron_kf(const asg_neuron_params& s, double h) {
     return p;
 }
 
-AstroKf make_astro_kf(const asg_astro_constants& c) {
-    AstroKf k{};
-    k.inv_ratio_ER_cyt = static_cast<float>(1.0 / c.ratio_ER_cyt);
-    k.Kd_IP3_1 = static_cast<float>(c.Kd_IP3_1);
-    k.Kd_IP3_2 = static_cast<float>(c.Kd_IP3_2);
-    k.Kd_act = static_cast<float>(c.Kd_act);
-    k.ratio_rate_IP3R = static_cast<float>(c.ratio_ER_cyt * c.rate_IP3R);
-    k.ratio_rate_L = static_cast<float>(c.ratio_ER_cyt * c.rate_L);
-    k.rate_SERCA = static_cast<float>(c.rate_SERCA);
-    k.Km_SERCA_sq = static_cast<float>(c.Km_SERCA * c.Km_SERCA);
-    k.k_Kd_inh = static_cast<float>(c.k_IP3R * c.Kd_inh);
-    k.k_IP3R = static_cast<float>(c.k_IP3R);
-    return k;
-}
-
-AstroK make_astro_k(const asg_astro_constants& c) {
-    AstroK k{};
-    k.ratio_ER_cyt = c.ratio_ER_cyt;
-    k.Kd_IP3_1 = c.Kd_IP3_1;
-    k.Kd_IP3_2 = c.Kd_IP3_2;
-    k.Kd_act = c.Kd_act;
-    k.ratio_rate_IP3R = c.ratio_ER_cyt * c.rate_IP3R;
-    k.ratio_rate_L = c.ratio_ER_cyt * c.rate_L;
-    k.rate_SERCA = c.rate_SERCA;
-    k.Km_SERCA_sq = c.Km_SERCA * c.Km_SERCA;
-    k.k_Kd_inh = c.k_IP3R * c.Kd_inh;
-    k.k_IP3R = c.k_IP3R;
-    return k;
-}
-
 std::string check_csr(const char* name, const asg_csr& c, uint32_t rows, uint32_t n_targets) {
     if (c.n_rows != rows) {
         return std::string(name) + ": n_rows does not match the population size";

@@ -8,6 +8,7 @@
 #pragma once
 
 #include "amath.h"
+#include "asg.h"
 
 namespace asg {
 
@@ -23,6 +24,21 @@ struct AstroK {
     double k_Kd_inh;          // k_IP3R * Kd_inh
     double k_IP3R;
 };
+
+inline AstroK make_astro_k(const asg_astro_constants& c) {
+    AstroK k{};
+    k.ratio_ER_cyt = c.ratio_ER_cyt;
+    k.Kd_IP3_1 = c.Kd_IP3_1;
+    k.Kd_IP3_2 = c.Kd_IP3_2;
+    k.Kd_act = c.Kd_act;
+    k.ratio_rate_IP3R = c.ratio_ER_cyt * c.rate_IP3R;
+    k.ratio_rate_L = c.ratio_ER_cyt * c.rate_L;
+    k.rate_SERCA = c.rate_SERCA;
+    k.Km_SERCA_sq = c.Km_SERCA * c.Km_SERCA;
+    k.k_Kd_inh = c.k_IP3R * c.Kd_inh;
+    k.k_IP3R = c.k_IP3R;
+    return k;
+}
 
 // astro_derivatives, astrocyte_kernel.hpp:47-68.
 ASG_HD void astro_rhs(const AstroK& c, double Ca_tot, double IP3_0, double tau_IP3, double Ca,

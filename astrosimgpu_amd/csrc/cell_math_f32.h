@@ -34,6 +34,21 @@ struct AstroKf {
     float rate_SERCA, Km_SERCA_sq, k_Kd_inh, k_IP3R;
 };
 
+inline AstroKf make_astro_kf(const asg_astro_constants& c) {
+    AstroKf k{};
+    k.inv_ratio_ER_cyt = static_cast<float>(1.0 / c.ratio_ER_cyt);
+    k.Kd_IP3_1 = static_cast<float>(c.Kd_IP3_1);
+    k.Kd_IP3_2 = static_cast<float>(c.Kd_IP3_2);
+    k.Kd_act = static_cast<float>(c.Kd_act);
+    k.ratio_rate_IP3R = static_cast<float>(c.ratio_ER_cyt * c.rate_IP3R);
+    k.ratio_rate_L = static_cast<float>(c.ratio_ER_cyt * c.rate_L);
+    k.rate_SERCA = static_cast<float>(c.rate_SERCA);
+    k.Km_SERCA_sq = static_cast<float>(c.Km_SERCA * c.Km_SERCA);
+    k.k_Kd_inh = static_cast<float>(c.k_IP3R * c.Kd_inh);
+    k.k_IP3R = static_cast<float>(c.k_IP3R);
+    return k;
+}
+
 // Neumaier's compensated sum: s + c carries the running total.
 __device__ inline void comp_add(float& s, float& c, float x) {
     const float t = s + x;

@@ -264,6 +264,42 @@ void asg_astro_update(asg_astro* state, const asg_astro_constants* c, double h_s
                       int32_t substeps, double noise_std, int32_t independent_noise,
                       double shared_noise, uint64_t noise_seed, uint64_t noise_index);
 
+/* Isolated astrocyte update at scale (the astrocyte half of the step,
+ * astro_update_kernel of astrocyte_cuda.cu:27-58): n cells in SoA arrays in
+ * device memory, one update launch per step reading each cell's state and
+ * presynaptic spike count (Poisson(lambda), int32, drawn into device memory
+ * by a separate launch) and writing the state back. No counterpart in the
+ * reference's API; it measures the update kernel's HBM throughput. Initial
+ * state and per-cell parameters are drawn from `seed` (see
+ * csrc/asg_astro_scale.hip); both precisions start from the same state. */
+typedef struct asg_astro_scaling_params {
+    double Ca_tot, IP3_0, tau_IP3, delta_IP3;   /* nominal per-cell parameters */
+    double dt;                                  /* step (ms), one RK4 substep */
+    double lambda;                              /* mean spike count per step, <= 30 */
+    int32_t randomized;                         /* per-cell parameters drawn like */
+    double lower, upper, var;                   /* draw_scaled, astrocyte.cpp:13-23 */
+} asg_astro_scaling_params;
+
+typedef struct asg_astro_scaling_result {
+    double update_ms;        /* device time of the update launches (HIP events) */
+    double total_ms;         /* first update start to last update end */
+    double bytes_per_step;   /* algorithmic HBM bytes of one update launch */
+    int64_t steps;
+} asg_astro_scaling_result;
+
+/* Runs `steps` steps in `precision` and copies the final Ca, IP3, h (as
+ * doubles; any may be NULL) to the host. */
+asg_status asg_astro_scaling(uint32_t n, int32_t steps, int32_t precision,
+                             const asg_astro_constants* constants,
+                             const asg_astro_scaling_params* params, uint64_t seed,
+                             double* Ca_out, double* IP3_out, double* h_out,
+                             asg_astro_scaling_result* out);
+
+/* Algorithmic bytes of one update launch: per cell Ca, IP3, h read and
+ * written, the int32 count read, and the four per-cell parameters read when
+ * randomized. */
+int64_t asg_astro_scaling_bytes(uint32_t n, int32_t precision, int32_t randomized);
+
 /* ------------------------------------------------------------------------ */
 /* Layer 2: one network, whole step on the device (replaces the body of      */
 /* Network::run's step loop, network.cpp:385-468).                           */

@@ -42,6 +42,10 @@ def lib():
         h.oracle_read_state.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
         h.oracle_phase_seconds.argtypes = [C.c_void_p, C.c_void_p]
         h.oracle_math.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int64]
+        h.oracle_astro_scaling.argtypes = [C.c_uint32, C.c_int32, C.c_uint64, C.c_void_p,
+                                           C.c_void_p, C.c_double, C.c_double, C.c_int,
+                                           C.c_double, C.c_double, C.c_double, C.c_void_p,
+                                           C.c_void_p, C.c_void_p]
         _lib = h
     return _lib
 
@@ -110,3 +114,16 @@ def math(fn: str, x: np.ndarray, libm: int) -> np.ndarray:
     y = np.empty_like(x)
     lib().oracle_math(codes[fn], libm, x.ctypes.data, y.ctypes.data, x.size)
     return y
+
+
+def astro_scaling(n, steps, seed, constants, nominal, dt, lam, randomized=False, lower=1.0,
+                  upper=1.0, var=0.0):
+    """The isolated astrocyte update at scale on the CPU (FP64); `constants`
+    is a ctypes asg_astro_constants, `nominal` (Ca_tot, IP3_0, tau_IP3,
+    delta_IP3). Returns the final (Ca, IP3, h)."""
+    nom = np.ascontiguousarray(nominal, np.float64)
+    ca, ip3, h = (np.empty(n) for _ in range(3))
+    lib().oracle_astro_scaling(n, steps, seed, C.byref(constants), nom.ctypes.data, dt, lam,
+                               1 if randomized else 0, lower, upper, var, ca.ctypes.data,
+                               ip3.ctypes.data, h.ctypes.data)
+    return ca, ip3, h

@@ -617,6 +617,61 @@ void oracle_phase_seconds(const oracle_run* r, double* out6) {
 
 // Element-wise exp/log/sin/cos/sqrt through the host C library (mode 0) or
 // amath.h (mode 1), for checking amath against the C library.
+// The isolated astrocyte update at scale (asg_astro_scaling), FP64 on the
+// CPU: the same initial state, per-cell parameters and spike counts, drawn
+// from the same counter streams, then astro_advance per cell and step.
+namespace {
+double fresh_normal(u64 seed, u64 stream, u64& ctr) {
+    double u1;
+    do {
+        u1 = unif(seed, stream, ctr++);
+    } while (u1 <= 1e-300);
+    const double u2 = unif(seed, stream, ctr++);
+    return AMath::sqrt(-2.0 * AMath::log(u1)) * AMath::cos(6.283185307179586476925286766559 * u2);
+}
+
+double scaled(u64 seed, u64 stream, double mean, double lower, double upper, double var) {
+    const double lo = mean < 0.0 ? mean * upper : mean * lower;
+    const double hi = mean < 0.0 ? mean * lower : mean * upper;
+    const double sd = var * std::fabs(mean);
+    u64 ctr = 0;
+    for (int tries = 0; tries < 256; ++tries) {
+        const double v = mean + sd * fresh_normal(seed, stream, ctr);
+        if (v >= lo && v <= hi) return v;
+    }
+    return mean;
+}
+}  // namespace
+
+extern "C" void oracle_astro_scaling(uint32_t n, int32_t steps, uint64_t seed,
+                                     const asg_astro_constants* c, const double* nominal4,
+                                     double dt, double lambda, int randomized, double lower,
+                                     double upper, double var, double* Ca, double* IP3,
+                                     double* h) {
+    const u64 s_init = seed ^ 0x6A09E667F3BCC909ULL;
+    const u64 s_par = seed ^ 0xBB67AE8584CAA73BULL;
+    std::vector<double> p4(4 * static_cast<size_t>(n));
+    for (uint32_t i = 0; i < n; ++i) {
+        Ca[i] = 0.05 + 0.35 * unif(s_init, i, 0);
+        IP3[i] = 0.01 + 0.49 * unif(s_init, i, 1);
+        h[i] = 0.50 + 0.45 * unif(s_init, i, 2);
+        for (int q = 0; q < 4; ++q) {
+            p4[4 * i + q] = randomized ? scaled(s_par, 4 * static_cast<u64>(i) + q, nominal4[q],
+                                                lower, upper, var)
+                                       : nominal4[q];
+        }
+    }
+    for (int32_t s = 0; s < steps; ++s) {
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < static_cast<int64_t>(n); ++i) {
+            Rng<AMath> rng(seed, static_cast<u64>(s) * n + static_cast<u64>(i));
+            const int k = rng.poisson(lambda);
+            astro_advance<AMath>(*c, p4[4 * i], p4[4 * i + 1], p4[4 * i + 2], p4[4 * i + 3],
+                                 static_cast<double>(k), 0.0, dt, 1, Ca[i], IP3[i], h[i]);
+        }
+    }
+}
+
 extern "C" void oracle_math(int fn, int mode, const double* x, double* y, int64_t n) {
     for (int64_t i = 0; i < n; ++i) {
         const double v = x[i];

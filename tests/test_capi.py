@@ -153,3 +153,17 @@ def test_footprint_of_sparse_network_fits_one_gpu():
     m = A.Model(config_path("paper_sparse"))
     fp = m.footprint()
     assert 100e6 < fp < 2e9
+
+
+def test_astro_scaling_bytes_formula():
+    """Per cell: Ca, IP3, h read + written, int32 count, 4 parameters when
+    randomized."""
+    from astrosimgpu_amd._capi import lib
+    assert lib().asg_astro_scaling_bytes(1000, 0, 0) == 1000 * (6 * 8 + 4)
+    assert lib().asg_astro_scaling_bytes(1000, 1, 0) == 1000 * (6 * 4 + 4)
+    assert lib().asg_astro_scaling_bytes(1000, 0, 1) == 1000 * (10 * 8 + 4)
+    from astrosimgpu_amd import astro_bench as AB
+    c, p = AB.setup(randomized=True)
+    assert p.dt == 0.1 and p.lambda_ == 0.05 and p.randomized == 1
+    assert (p.lower, p.upper, p.var) == (0.9, 1.05, 0.1)
+    assert c.rate_SERCA == 0.0009
