@@ -81,7 +81,7 @@ def rel_err(a, b):
     return float(d.max()), float(d.mean())
 
 
-def scaling(sizes, steps: int, randomized: bool = False):
+def scaling(sizes, steps: int, randomized: bool = False, echo: bool = False):
     constants, params = setup(randomized=randomized)
     rows = []
     for n in sizes:
@@ -98,7 +98,8 @@ def scaling(sizes, steps: int, randomized: bool = False):
                      "bytes_per_launch": r32["bytes_per_launch"]},
             "fp32_vs_fp64_rel_err": {k: {"max": v[0], "mean": v[1]} for k, v in err.items()},
         })
-        print(json.dumps(rows[-1]), flush=True)
+        if echo:
+            print(json.dumps(rows[-1]), flush=True)
     return rows
 
 
@@ -109,7 +110,7 @@ def main(argv=None):
     ap.add_argument("--randomized", action="store_true")
     args = ap.parse_args(argv)
     sizes = [int(float(x)) for x in args.sizes.split(",")]
-    scaling(sizes, args.steps, args.randomized)
+    scaling(sizes, args.steps, args.randomized, echo=True)
 
 
 if __name__ == "__main__":

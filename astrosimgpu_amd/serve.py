@@ -21,6 +21,8 @@ from __future__ import annotations
 
 import concurrent.futures as cf
 import math
+import os
+import sys
 import time
 from dataclasses import dataclass, field
 from typing import Optional
@@ -29,6 +31,10 @@ import numpy as np
 
 from . import _capi as capi
 from .network import Batch, Model
+
+
+DEFAULT_CONFIG = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                              "tests", "golden", "configs", "paper_sparse.json")
 
 
 @dataclass
@@ -203,3 +209,95 @@ class Server:
     def close(self):
         self.pool.shutdown(wait=True)
         self.batch.close()
+
+
+def _progress_printer(t0):
+    last = [0.0]
+
+    def cb(done, total):
+        now = time.perf_counter() - t0
+        if now - last[0] >= 10.0 or done == total:
+            last[0] = now
+            print(f"  {done}/{total} jobs done at {now:.1f} s", file=sys.stderr, flush=True)
+    return cb
+
+
+def calibrate(base_config: str, count: int, precision: int, chunk_steps: int, builders: int,
+              seed: int = 99) -> float:
+    """Capacity in neuron-steps/s: `count` jobs of the serving distribution all
+    present at t = 0, run to completion with continuous batching."""
+    jobs = job_stream(count, rate_per_s=1e9, seed=seed, precision=precision)
+    srv = Server(base_config, chunk_steps=chunk_steps, builders=builders)
+    rep = srv.run(jobs)
+    srv.close()
+    return rep.neuron_steps_per_s
+
+
+def main(argv=None):
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description="Serve a Poisson stream of simulation jobs on one "
+                                             "GPU and report throughput and job latency.")
+    ap.add_argument("--jobs", type=int, default=2000)
+    ap.add_argument("--rate", type=float, default=0.0,
+                    help="arrivals per second; 0 = 0.8 x the capacity measured first")
+    ap.add_argument("--calibration-jobs", type=int, default=64)
+    ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp32")
+    ap.add_argument("--chunk-steps", type=int, default=2000)
+    ap.add_argument("--builders", type=int, default=4)
+    ap.add_argument("--max-slots", type=int, default=512)
+    ap.add_argument("--budget-gb", type=float, default=0.0, help="0 = free device memory")
+    ap.add_argument("--config", default=DEFAULT_CONFIG)
+    args = ap.parse_args(argv)
+    prec = capi.ASG_FP32 if args.precision == "fp32" else capi.ASG_FP64
+    mean_ns = _mean_job_neuron_steps()
+    capacity = None
+    rate = args.rate
+    if rate <= 0.0:
+        print(f"calibrating with {args.calibration_jobs} jobs present at once ...",
+              file=sys.stderr, flush=True)
+        capacity = calibrate(args.config, args.calibration_jobs, prec, args.chunk_steps,
+                             args.builders)
+        rate = 0.8 * capacity / mean_ns
+        print(f"  capacity {capacity:.3e} neuron-steps/s -> rate {rate:.3f} jobs/s",
+              file=sys.stderr, flush=True)
+    jobs = job_stream(args.jobs, rate, seed=args.seed, precision=prec)
+    srv = Server(args.config, max_slots=args.max_slots,
+                 budget_bytes=int(args.budget_gb * (1 << 30)), chunk_steps=args.chunk_steps,
+                 builders=args.builders)
+    t0 = time.perf_counter()
+    rep = srv.run(jobs, progress=_progress_printer(t0))
+    srv.close()
+    lat = np.array([j.latency_s for j in jobs])
+    wait = np.array([j.admitted_s - j.arrival_s for j in jobs])
+    out = {
+        "jobs": rep.jobs, "rate_per_s": round(rate, 4), "precision": args.precision,
+        "capacity_neuron_steps_per_s": capacity,
+        "makespan_s": round(rep.makespan_s, 3),
+        "neuron_steps": rep.neuron_steps,
+        "neuron_steps_per_s": round(rep.neuron_steps_per_s, 1),
+        "latency_p50_s": round(rep.latency_p50_s, 3),
+        "latency_p99_s": round(rep.latency_p99_s, 3),
+        "latency_mean_s": round(float(lat.mean()), 3),
+        "admission_wait_p99_s": round(float(np.percentile(wait, 99)), 3),
+        "jobs_that_waited_for_room": rep.refused_admissions,
+        "peak_running": rep.peak_running,
+        "peak_device_bytes": rep.peak_bytes,
+        "chunk_steps": args.chunk_steps,
+        "workload": "Poisson arrivals; N log-uniform [500, 1e5] (80/20 E/I, in-degree 100, "
+                    "per-row generator), astrocytes U[0.2, 1.0] N, duration U[1, 10] s, "
+                    "w_n2a U[0.10, 0.40], own seed; paper_sparse.json otherwise",
+    }
+    print(json.dumps(out), flush=True)
+
+
+def _mean_job_neuron_steps(n_min=500, n_max=100000, d_min=1000.0, d_max=10000.0, dt=0.1):
+    """Expected neuron-steps of one job: E[N] of the log-uniform size times
+    E[steps] of the uniform duration."""
+    mean_n = (n_max - n_min) / math.log(n_max / n_min)
+    return mean_n * 0.5 * (d_min + d_max) / dt
+
+
+if __name__ == "__main__":
+    sys.exit(main())

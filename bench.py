@@ -34,6 +34,12 @@ round(0.2 N) astrocytes, IP3 weight w_n2a uniform in [0.10, 0.40], seed
   sparse_fp64  the reference's 'Sparse' network (10,000 neurons, 10,000
                astrocytes, paper_sparse.json), FP64, seeds 1-3 side by side,
                1 s transient + --sparse-seconds timed.
+  astro_scaling the isolated astrocyte update at 1e7 cells, 1,000 steps, one
+               launch per step: HBM GB/s of the update launches, FP64 and FP32,
+               and the FP32-vs-FP64 state error.
+  serving      a bounded sample of the serving loop (--serve-jobs jobs,
+               Poisson arrivals at 0.8 x the capacity measured first): job
+               latency p50/p99 and throughput.
 """
 
 from __future__ import annotations
@@ -72,15 +78,17 @@ def log(*a):
 # window and ring slots every step.
 #   neuron, per window: V, w, g_ex, dg_ex, g_in, dg_in, I_sic read + written
 #     (7 x 2 x s), refractory counter read + written (8), V_reset and b read
-#     (16); per step: exc, inh and SIC ring slots read (3 s) and the exc and
-#     inh slots cleared (2 s).
+#     (16); per step: exc, inh and SIC ring slots read (3 s), the exc and inh
+#     slots cleared (2 s), and the external drive k_drive wrote for it: the
+#     int32 Poisson count (4) and, with noise on, the noise current (s).
 #   astrocyte, per window: Ca, IP3, h read + written (6 s), Ca_tot, IP3_0,
 #     1/tau_IP3 and delta_IP3 read (4 s); per step: the IP3 ring slot read
 #     and cleared (2 s) and the SIC factor written (s).
 # s = 8 bytes in FP64, 4 in FP32.
-def cell_bytes(n_neurons: int, n_astro: int, window: int, state_bytes: int) -> float:
+def cell_bytes(n_neurons: int, n_astro: int, window: int, state_bytes: int,
+               noise: bool = True) -> float:
     s = state_bytes
-    neuron = 14 * s + 8 + 16 + window * 5 * s
+    neuron = 14 * s + 8 + 16 + window * (5 * s + 4 + (s if noise else 0))
     astro = 10 * s + window * 3 * s
     return n_neurons * neuron + n_astro * astro
 
@@ -276,6 +284,9 @@ def main():
                          "(the reference point for HBM byte counters) and exit")
     ap.add_argument("--sparse-seconds", type=int, default=2,
                     help="timed simulated seconds of the Sparse FP64 runs (10 = the full run)")
+    ap.add_argument("--no-astro-scaling", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=120,
+                    help="jobs of the serving sample (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (got --gpus {args.gpus}); the "
@@ -315,7 +326,8 @@ def main():
     log(f"  {value:.3e} neuron-steps/s  kernels {kt32}")
 
     if args.fp32_only:
-        args.no_sparse = args.no_cpu_baseline = True
+        args.no_sparse = args.no_cpu_baseline = args.no_astro_scaling = True
+        args.serve_jobs = 0
         e64 = kt64 = plain64 = cmp = sub = None
     else:
         log("batched FP64 ...")
@@ -339,7 +351,8 @@ def main():
     dominant = max(share, key=share.get)
     window_len = 10  # min(d_e, d_i, d_a2n) of use_case.json = 1 ms = 10 steps
     if dominant == "cells":
-        bytes_launch = cell_bytes(n_total, na_total, window_len, 4)
+        # use_case.json drives neurons with Poisson input and noise
+        bytes_launch = cell_bytes(n_total, na_total, window_len, 4, noise=True)
         avg_s = kt32["cells"] / 1000.0 / max(windows, 1)
         achieved = bytes_launch / avg_s / 1e9
         roof = {"kernel": "k_cells (every neuron and astrocyte of the batch, one window of "
@@ -359,6 +372,34 @@ def main():
     if not args.no_sparse:
         log("Sparse FP64 ...")
         sparse = sparse_fp64(10, 10 * args.sparse_seconds)
+
+    astro = None
+    if not args.no_astro_scaling:
+        log("astrocyte update at 1e7 cells ...")
+        from astrosimgpu_amd import astro_bench
+        row = astro_bench.scaling([10_000_000], 1000)[0]
+        astro = {"cells": row["n"], "steps": row["steps"],
+                 "fp64_GBps": row["fp64"]["GB/s"], "fp32_GBps": row["fp32"]["GB/s"],
+                 "peak_GBps": HBM_PEAK_GBS,
+                 "fp32_rel_err_max": {k: v["max"] for k, v in row["fp32_vs_fp64_rel_err"].items()}}
+
+    serving = None
+    if args.serve_jobs > 0:
+        log(f"serving sample, {args.serve_jobs} jobs ...")
+        from astrosimgpu_amd import serve
+        cap = serve.calibrate(serve.DEFAULT_CONFIG, 48, capi.ASG_FP32, 2000, 4)
+        rate = 0.8 * cap / serve._mean_job_neuron_steps()
+        jobs = serve.job_stream(args.serve_jobs, rate, seed=7)
+        srv = serve.Server(serve.DEFAULT_CONFIG, chunk_steps=2000, builders=4)
+        rep_ = srv.run(jobs)
+        srv.close()
+        serving = {"jobs": rep_.jobs, "rate_per_s": round(rate, 3),
+                   "latency_p50_s": round(rep_.latency_p50_s, 3),
+                   "latency_p99_s": round(rep_.latency_p99_s, 3),
+                   "neuron_steps_per_s": round(rep_.neuron_steps_per_s, 1),
+                   "capacity_neuron_steps_per_s": round(cap, 1),
+                   "workload": "Poisson arrivals at 0.8 x capacity; N log-uniform [500, 1e5], "
+                               "astrocytes U[0.2, 1] N, 1-10 s each, FP32"}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -400,6 +441,8 @@ def main():
         "kernel_ms_fp64": None if kt64 is None else
         {k: round(v, 3) for k, v in kt64.items() if k != "windows"},
         "sparse_fp64": sparse,
+        "astro_scaling": astro,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
