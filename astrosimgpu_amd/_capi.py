@@ -102,6 +102,10 @@ class AstroScalingResult(C.Structure):
                 ("bytes_per_step", C.c_double), ("steps", C.c_int64)]
 
 
+class Schedule(C.Structure):
+    _fields_ = [("cell_waves", C.c_int32), ("sic_group", C.c_int32), ("ms_per_window", C.c_float)]
+
+
 class CalciumSummary(C.Structure):
     _fields_ = [("cells", C.c_int32), ("active", C.c_int32), ("transients", C.c_int64),
                 ("pairs", C.c_int64), ("mean_duration", C.c_double), ("sd_duration", C.c_double),
@@ -189,6 +193,11 @@ PROTOTYPES = {
                                     C.POINTER(AstroScalingParams),
                                     C.c_uint64, DBLP, DBLP, DBLP, C.POINTER(AstroScalingResult)]),
     "asg_astro_scaling_bytes": (C.c_int64, [C.c_uint32, C.c_int32, C.c_int32]),
+    "asg_batch_set_schedule": (C.c_int, [P, C.POINTER(Schedule)]),
+    "asg_batch_get_schedule": (C.c_int, [P, C.POINTER(Schedule)]),
+    "asg_batch_autotune": (C.c_int, [P, C.c_int32, C.c_int32, C.POINTER(Schedule)]),
+    "asg_batch_set_graphs": (C.c_int, [P, C.c_int32]),
+    "asg_net_set_graphs": (C.c_int, [P, C.c_int32]),
     "asg_detect_events": (C.c_int, [DBLP, DBLP, C.c_int64, C.c_double, C.c_int32, C.c_double,
                                     DBLP, DBLP, C.c_int64, I64P]),
     "asg_pearson": (C.c_double, [DBLP, DBLP, C.c_int64]),

@@ -742,21 +742,27 @@ __global__ __launch_bounds__(256) void k_drive(DevNet* nets, const CellBlock* nb
 // precision, so neither carries the other's registers; neurons and
 // astrocytes share the launch, so the short astrocyte blocks overlap the
 // neuron blocks.
-template <int P>
-__global__ __launch_bounds__(kCellBlock) void k_cells(DevNet* nets, const CellBlock* blocks,
-                                                      int32_t Wg) {
-    const CellBlock cb = blocks[blockIdx.x];
+// WPB wavefronts (64-cell blocks) per workgroup, a launch-schedule knob.
+template <int P, int WPB>
+__global__ __launch_bounds__(kCellBlock * WPB) void k_cells(DevNet* nets, const CellBlock* blocks,
+                                                            int32_t count, int32_t Wg) {
+    const int32_t b = static_cast<int32_t>(blockIdx.x) * WPB + static_cast<int32_t>(threadIdx.x >> 6);
+    if (b >= count) {
+        return;
+    }
+    const uint32_t lane = threadIdx.x & 63;
+    const CellBlock cb = blocks[b];
     DevNet& n = nets[cb.net];
     const int W = window_len(n, Wg);
     const int64_t T0 = n.step;
-    if (cb.lead && threadIdx.x == 0) {
+    if (cb.lead && lane == 0) {
         n.win_T0 = T0;
         n.win_W = W;
     }
     if (W == 0) {
         return;
     }
-    const uint32_t i = cb.first + threadIdx.x;
+    const uint32_t i = cb.first + lane;
     if constexpr (P == ASG_FP32) {
         if (cb.kind == 0) {
             neuron_window_f32(n, i, T0, W);
@@ -1848,6 +1854,20 @@ struct Engine {
     uint32_t* h_stage_src = nullptr;
     size_t stage_cap = 0;
     bool raster_dirty = false;         // steps ran since the last drain
+    // Launch schedule (results do not depend on it): cell-kernel wavefronts
+    // per workgroup (1, 2, 4) and SIC group (1-16 steps, 0 = by batch size).
+    int sched_cell_waves = 1;
+    int sched_sic_group = 0;
+    float tuned_ms_per_window = 0.0f;   // the autotuner's best median
+    // Window sequences replayed from captured graphs (when use_graphs and
+    // not profiling); cleared whenever the tables change.
+    bool use_graphs = false;
+    struct GraphEntry {
+        int64_t windows;
+        int Wg, G, wpb;
+        hipGraphExec_t exec;
+    };
+    std::vector<GraphEntry> graphs;
     bool pending = false;  // async work in flight
     // Per-kernel event timing (off unless asked for).
     bool profiling = false;
@@ -1877,6 +1897,7 @@ struct Engine {
 
     ~Engine() {
         if (stream) (void)hipStreamSynchronize(stream);
+        clear_graphs();
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
         for (NetSlot& s : slots) release(s);
         (void)hipFree(d_nets);
@@ -2004,6 +2025,7 @@ struct Engine {
         }
         HIP_CHECK(hipStreamSynchronize(stream));
         dirty = false;
+        clear_graphs();
     }
 
     int add(const asg_net_desc& d, int64_t total) {
@@ -2062,12 +2084,92 @@ struct Engine {
     }
 
     template <int P>
-    void launch_cells(int range, int32_t Wg_) {
+    void launch_cells(int range, int32_t Wg_, int wpb) {
         const int count = cell_range[range + 1] - cell_range[range];
-        if (count > 0) {
-            hipLaunchKernelGGL(k_cells<P>, dim3(count), dim3(kCellBlock), 0, stream, d_nets,
-                               d_blocks + cell_range[range], Wg_);
+        if (count == 0) return;
+        const CellBlock* blocks = d_blocks + cell_range[range];
+        const dim3 grid((count + wpb - 1) / wpb);
+        switch (wpb) {
+            case 4:
+                hipLaunchKernelGGL((k_cells<P, 4>), grid, dim3(4 * kCellBlock), 0, stream, d_nets,
+                                   blocks, count, Wg_);
+                break;
+            case 2:
+                hipLaunchKernelGGL((k_cells<P, 2>), grid, dim3(2 * kCellBlock), 0, stream, d_nets,
+                                   blocks, count, Wg_);
+                break;
+            default:
+                hipLaunchKernelGGL((k_cells<P, 1>), grid, dim3(kCellBlock), 0, stream, d_nets,
+                                   blocks, count, Wg_);
+                break;
         }
+    }
+
+    // The SIC group the schedule asks for, or (0) the largest power of two
+    // up to the window that still leaves kSicWaves wavefronts.
+    int sic_group_for(int requested) const {
+        int top = 1;
+        while (top < Wg) top *= 2;
+        if (requested > 0) return std::min(requested, top);
+        int g = top;
+        while (g > 1 && static_cast<int64_t>(n_nblocks) * ((Wg + g - 1) / g) < kSicWaves) {
+            g /= 2;
+        }
+        return g;
+    }
+
+    // The launches of `windows` windows on the stream (eager, or recorded
+    // into a graph when the stream is capturing).
+    void launch_windows(int64_t windows, int G, int wpb, bool marks) {
+        const int deliver_items = n_nblocks * (Wg + (Wg + G - 1) / G);
+        const int sum_items = n_blocks * Wg;
+        auto mark = [&](int64_t wdx, int k) {
+            if (marks) HIP_CHECK(hipEventRecord(events[6 * wdx + k], stream));
+        };
+        for (int64_t wdx = 0; wdx < windows; ++wdx) {
+            mark(wdx, 0);
+            if (n_nblocks > 0) {
+                hipLaunchKernelGGL(k_drive, dim3((n_nblocks + 3) / 4), dim3(256), 0, stream,
+                                   d_nets, d_nblocks, n_nblocks, Wg);
+            }
+            mark(wdx, 1);
+            launch_cells<ASG_FP64>(0, Wg, wpb);
+            launch_cells<ASG_FP32>(1, Wg, wpb);
+            mark(wdx, 2);
+            hipLaunchKernelGGL(k_compact, dim3(n_active, Wg), dim3(64), 0, stream, d_nets,
+                               d_active);
+            mark(wdx, 3);
+            if (deliver_items > 0) {
+                launch_deliver(G, deliver_items, Wg);
+            }
+            mark(wdx, 4);
+            hipLaunchKernelGGL(k_sum, dim3((sum_items + 3) / 4), dim3(256), 0, stream, d_nets,
+                               d_blocks, sum_items, Wg);
+            mark(wdx, 5);
+        }
+    }
+
+    // Graph of `windows` windows for the current tables and schedule,
+    // captured on first use.
+    hipGraphExec_t graph_for(int64_t windows, int G, int wpb) {
+        for (const GraphEntry& g : graphs) {
+            if (g.windows == windows && g.G == G && g.wpb == wpb && g.Wg == Wg) return g.exec;
+        }
+        hipGraph_t graph = nullptr;
+        HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        launch_windows(windows, G, wpb, false);
+        HIP_CHECK(hipStreamEndCapture(stream, &graph));
+        hipGraphExec_t exec = nullptr;
+        const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIP_CHECK(e);
+        graphs.push_back(GraphEntry{windows, Wg, G, wpb, exec});
+        return exec;
+    }
+
+    void clear_graphs() {
+        for (GraphEntry& g : graphs) (void)hipGraphExecDestroy(g.exec);
+        graphs.clear();
     }
 
     // Advance every used slot by up to n steps (to its total).
@@ -2093,46 +2195,84 @@ struct Engine {
         hipLaunchKernelGGL(k_set_targets, dim3((n_active + 255) / 256), dim3(256), 0, stream,
                            d_nets, d_active, n_active, d_targets);
         raster_dirty = true;
-        // SIC group (1, 2, 4, 8 or 16 steps): the whole window when the
-        // batch has wavefronts enough to fill the device, otherwise fewer
-        // steps per wavefront.
-        int sic_group = 1;
-        while (sic_group < Wg) sic_group *= 2;
-        while (sic_group > 1 &&
-               static_cast<int64_t>(n_nblocks) * ((Wg + sic_group - 1) / sic_group) < kSicWaves) {
-            sic_group /= 2;
+        const int G = sic_group_for(sched_sic_group);
+        if (profiling) {
+            // per-kernel events need eager launches
+            ensure_events(windows);
+            launch_windows(windows, G, sched_cell_waves, true);
+            pending_windows = windows;
+        } else if (use_graphs && windows > 0) {
+            HIP_CHECK(hipGraphLaunch(graph_for(windows, G, sched_cell_waves), stream));
+        } else {
+            launch_windows(windows, G, sched_cell_waves, false);
         }
-        const int deliver_items = n_nblocks * (Wg + (Wg + sic_group - 1) / sic_group);
-        const int sum_items = n_blocks * Wg;
-        if (profiling) ensure_events(windows);
-        auto mark = [&](int64_t wdx, int k) {
-            if (profiling) HIP_CHECK(hipEventRecord(events[6 * wdx + k], stream));
-        };
-        for (int64_t wdx = 0; wdx < windows; ++wdx) {
-            mark(wdx, 0);
-            if (n_nblocks > 0) {
-                hipLaunchKernelGGL(k_drive, dim3((n_nblocks + 3) / 4), dim3(256), 0, stream,
-                                   d_nets, d_nblocks, n_nblocks, Wg);
-            }
-            mark(wdx, 1);
-            launch_cells<ASG_FP64>(0, Wg);
-            launch_cells<ASG_FP32>(1, Wg);
-            mark(wdx, 2);
-            hipLaunchKernelGGL(k_compact, dim3(n_active, Wg), dim3(64), 0, stream, d_nets,
-                               d_active);
-            mark(wdx, 3);
-            if (deliver_items > 0) {
-                launch_deliver(sic_group, deliver_items, Wg);
-            }
-            mark(wdx, 4);
-            hipLaunchKernelGGL(k_sum, dim3((sum_items + 3) / 4), dim3(256), 0, stream, d_nets,
-                               d_blocks, sum_items, Wg);
-            mark(wdx, 5);
-        }
-        if (profiling) pending_windows = windows;
         HIP_CHECK(hipGetLastError());
         pending = true;
         if (!async) wait();
+    }
+
+    // Autotuner: every candidate schedule (cell wavefronts per workgroup x
+    // SIC group) runs `wpt` windows of the live batch, `rounds` times in
+    // round-robin order so slow drifts in activity hit all alike; the one
+    // with the smallest median time is kept. The schedule changes launch
+    // geometry only, never results, so tuning advances the simulation like
+    // any other call (by rounds x candidates x wpt windows).
+    void autotune(int wpt, int rounds) {
+        wait();
+        if (dirty) rebuild_tables();
+        if (n_active == 0 || wpt <= 0 || rounds <= 0) return;
+        int top = 1;
+        while (top < Wg) top *= 2;
+        std::vector<std::pair<int, int>> cand;
+        for (int wpb : {1, 2, 4}) {
+            for (int g = 1; g <= top; g *= 2) cand.emplace_back(wpb, g);
+        }
+        std::vector<std::vector<float>> times(cand.size());
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        HIP_CHECK(hipEventCreate(&e0));
+        HIP_CHECK(hipEventCreate(&e1));
+        const bool prof = profiling, graphs_on = use_graphs;
+        profiling = false;
+        use_graphs = false;
+        try {
+            for (int r = 0; r < rounds; ++r) {
+                for (size_t c = 0; c < cand.size(); ++c) {
+                    sched_cell_waves = cand[c].first;
+                    sched_sic_group = cand[c].second;
+                    HIP_CHECK(hipEventRecord(e0, stream));
+                    step(static_cast<int64_t>(wpt) * Wg, true);
+                    HIP_CHECK(hipEventRecord(e1, stream));
+                    wait();
+                    float ms = 0.0f;
+                    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                    times[c].push_back(ms);
+                }
+            }
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            profiling = prof;
+            use_graphs = graphs_on;
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        profiling = prof;
+        use_graphs = graphs_on;
+        size_t best = 0;
+        float best_t = 0.0f;
+        for (size_t c = 0; c < cand.size(); ++c) {
+            std::vector<float> v = times[c];
+            std::sort(v.begin(), v.end());
+            const float med = v[v.size() / 2];
+            if (c == 0 || med < best_t) {
+                best = c;
+                best_t = med;
+            }
+        }
+        sched_cell_waves = cand[best].first;
+        sched_sic_group = cand[best].second;
+        tuned_ms_per_window = best_t / static_cast<float>(wpt);
     }
 
     // Move the device rasters of all networks into their host queues: one
@@ -2201,6 +2341,16 @@ struct Engine {
     int overflow_bits(int slot) {
         drain_all();
         return slots[slot].overflow;
+    }
+
+    // OR of the overflow bits of every network.
+    int overflow_bits_any() {
+        drain_all();
+        int bits = 0;
+        for (const NetSlot& s : slots) {
+            if (s.used) bits |= s.overflow;
+        }
+        return bits;
     }
 
     int64_t spikes_out(int slot, int64_t* step, uint32_t* src, int64_t cap) {
@@ -2619,6 +2769,68 @@ asg_status asg_batch_counters(asg_batch* batch, int32_t slot, asg_counters* out)
 
 int64_t asg_batch_bytes_in_use(const asg_batch* batch) {
     return batch == nullptr ? -1 : batch->eng->bytes_in_use();
+}
+
+namespace {
+asg_status set_graphs(Engine& e, int32_t on) {
+    return guarded([&]() -> asg_status {
+        e.wait();
+        e.use_graphs = on != 0;
+        if (!e.use_graphs) e.clear_graphs();
+        return ASG_OK;
+    });
+}
+}  // namespace
+
+asg_status asg_net_set_graphs(asg_net* net, int32_t on) {
+    if (net == nullptr) return fail(ASG_ERR_INVALID, "NULL handle");
+    return set_graphs(*net->eng, on);
+}
+
+asg_status asg_batch_set_graphs(asg_batch* batch, int32_t on) {
+    if (batch == nullptr) return fail(ASG_ERR_INVALID, "NULL handle");
+    return set_graphs(*batch->eng, on);
+}
+
+asg_status asg_batch_set_schedule(asg_batch* batch, const asg_schedule* sched) {
+    if (batch == nullptr || sched == nullptr) return fail(ASG_ERR_INVALID, "bad argument");
+    const int w = sched->cell_waves, g = sched->sic_group;
+    if ((w != 1 && w != 2 && w != 4) || (g != 0 && g != 1 && g != 2 && g != 4 && g != 8 && g != 16)) {
+        return fail(ASG_ERR_INVALID, "cell_waves must be 1, 2 or 4; sic_group 0, 1, 2, 4, 8 or 16");
+    }
+    return guarded([&]() -> asg_status {
+        batch->eng->wait();
+        batch->eng->sched_cell_waves = w;
+        batch->eng->sched_sic_group = g;
+        return ASG_OK;
+    });
+}
+
+asg_status asg_batch_get_schedule(asg_batch* batch, asg_schedule* out) {
+    if (batch == nullptr || out == nullptr) return fail(ASG_ERR_INVALID, "bad argument");
+    out->cell_waves = batch->eng->sched_cell_waves;
+    out->sic_group = batch->eng->sched_sic_group;
+    out->ms_per_window = batch->eng->tuned_ms_per_window;
+    return ASG_OK;
+}
+
+asg_status asg_batch_autotune(asg_batch* batch, int32_t windows_per_try, int32_t rounds,
+                              asg_schedule* chosen) {
+    if (batch == nullptr || windows_per_try <= 0 || rounds <= 0) {
+        return fail(ASG_ERR_INVALID, "bad argument");
+    }
+    return guarded([&]() -> asg_status {
+        batch->eng->autotune(windows_per_try, rounds);
+        if (const int bits = batch->eng->overflow_bits_any()) {
+            return fail(ASG_ERR_OVERFLOW, overflow_message(bits));
+        }
+        if (chosen != nullptr) {
+            chosen->cell_waves = batch->eng->sched_cell_waves;
+            chosen->sic_group = batch->eng->sched_sic_group;
+            chosen->ms_per_window = batch->eng->tuned_ms_per_window;
+        }
+        return ASG_OK;
+    });
 }
 
 namespace {

@@ -159,6 +159,9 @@ class DeviceNetwork:
     def set_profiling(self, on: bool) -> None:
         check(lib().asg_net_set_profiling(self._h, 1 if on else 0))
 
+    def set_graphs(self, on: bool) -> None:
+        check(lib().asg_net_set_graphs(self._h, 1 if on else 0))
+
     def set_recording(self, start_step: int, every: int, n_astro: int, capacity: int) -> None:
         """Record Ca and IP3 of astrocytes [0, n_astro) every `every` steps from
         start_step on, into a device ring of `capacity` records."""
@@ -273,6 +276,27 @@ class Batch:
 
     def bytes_in_use(self) -> int:
         return int(lib().asg_batch_bytes_in_use(self._h))
+
+    def autotune(self, windows_per_try: int = 2, rounds: int = 3) -> dict:
+        """Pick the fastest launch schedule on the live batch (advances every
+        network; results do not depend on the schedule)."""
+        sch = capi.Schedule()
+        check(lib().asg_batch_autotune(self._h, int(windows_per_try), int(rounds), C.byref(sch)))
+        return {"cell_waves": sch.cell_waves, "sic_group": sch.sic_group,
+                "ms_per_window": sch.ms_per_window}
+
+    def schedule(self) -> dict:
+        sch = capi.Schedule()
+        check(lib().asg_batch_get_schedule(self._h, C.byref(sch)))
+        return {"cell_waves": sch.cell_waves, "sic_group": sch.sic_group,
+                "ms_per_window": sch.ms_per_window}
+
+    def set_schedule(self, cell_waves: int = 1, sic_group: int = 0) -> None:
+        sch = capi.Schedule(int(cell_waves), int(sic_group), 0.0)
+        check(lib().asg_batch_set_schedule(self._h, C.byref(sch)))
+
+    def set_graphs(self, on: bool) -> None:
+        check(lib().asg_batch_set_graphs(self._h, 1 if on else 0))
 
     def close(self):
         if getattr(self, "_h", None):

@@ -398,6 +398,34 @@ asg_status asg_batch_read_trace(asg_batch* batch, int32_t slot, double* out, int
 /* Device bytes currently held by the batch's networks. */
 int64_t asg_batch_bytes_in_use(const asg_batch* batch);
 
+/* Launch schedule of the window pipeline: cell-kernel workgroup size in
+ * wavefronts (1, 2 or 4) and SIC group (window steps per SIC wavefront: 1,
+ * 2, 4, 8 or 16; 0 = the largest that still fills the device). Results do
+ * not depend on it. ms_per_window is the autotuner's measured time (output
+ * only). */
+typedef struct asg_schedule {
+    int32_t cell_waves;
+    int32_t sic_group;
+    float ms_per_window;
+} asg_schedule;
+asg_status asg_batch_set_schedule(asg_batch* batch, const asg_schedule* sched);
+asg_status asg_batch_get_schedule(asg_batch* batch, asg_schedule* out);
+
+/* Autotuner: runs every candidate schedule for windows_per_try windows of
+ * the batch, `rounds` times round-robin, and keeps the one with the smallest
+ * median time (written to *chosen if not NULL). The networks advance by
+ * rounds x candidates x windows_per_try windows, exactly as a step call of
+ * that length would advance them. */
+asg_status asg_batch_autotune(asg_batch* batch, int32_t windows_per_try, int32_t rounds,
+                              asg_schedule* chosen);
+
+/* Replay each step call's window sequence from a hipGraph captured on first
+ * use (per window count, schedule and batch layout; recaptured after
+ * networks are added or removed). Off by default; per-kernel profiling keeps
+ * the launches eager. */
+asg_status asg_batch_set_graphs(asg_batch* batch, int32_t on);
+asg_status asg_net_set_graphs(asg_net* net, int32_t on);
+
 /* ------------------------------------------------------------------------ */
 /* Diagnostics.                                                              */
 /* ------------------------------------------------------------------------ */

@@ -1,0 +1,98 @@
+"""Launch schedules, the autotuner and graph replay change launch geometry
+and host overhead only: every variant gives the bits of the default eager
+run."""
+
+import numpy as np
+import pytest
+
+import astrosimgpu_amd as A
+from astrosimgpu_amd import _capi as capi
+
+from conftest import config_path
+
+pytestmark = pytest.mark.gpu
+
+SPECS = [("quick", 3, capi.ASG_FP64), ("use_case", 4, capi.ASG_FP32), ("random_pool", 5, capi.ASG_FP64),
+         ("quick", 6, capi.ASG_FP32)]
+
+
+def _models():
+    ms = []
+    for cfg, seed, prec in SPECS:
+        m = A.Model(config_path(cfg), seed=seed)
+        m.desc.contents.precision = prec
+        ms.append(m)
+    return ms
+
+
+def _run(configure, chunks=(700, 1000, 1000, 300), total=3000):
+    """Every chunk's spikes are drained and a state array read between the
+    calls (eager work between graph replays)."""
+    models = _models()
+    b = A.Batch(8)
+    slots = [b.add(m, total) for m in models]
+    configure(b)
+    parts = {s: ([], []) for s in slots}
+    for c in chunks:
+        b.step(c)
+        for s in slots:
+            st, src = b.spikes(s)
+            parts[s][0].append(st)
+            parts[s][1].append(src)
+        b.state(slots[1], "V")
+    out = []
+    for s in slots:
+        out.append((np.concatenate(parts[s][0]), np.concatenate(parts[s][1]), b.state(s, "V"),
+                    b.state(s, "Ca")))
+    b.close()
+    return out
+
+
+def _same(a, b):
+    for (s1, n1, v1, c1), (s2, n2, v2, c2) in zip(a, b):
+        assert np.array_equal(s1, s2) and np.array_equal(n1, n2)
+        assert np.array_equal(v1.view(np.uint64), v2.view(np.uint64))
+        assert np.array_equal(c1.view(np.uint64), c2.view(np.uint64))
+
+
+def test_every_schedule_gives_the_same_bits():
+    print("[gpu case] schedules", flush=True)
+    ref = _run(lambda b: None)
+    for waves in (1, 2, 4):
+        for group in (1, 2, 4, 8, 16):
+            _same(ref, _run(lambda b, w=waves, g=group: b.set_schedule(w, g)))
+
+
+def test_graph_replays_equal_eager_launches():
+    """The same 1000-step call replayed from one graph several times, with
+    eager work (raster drains, state reads) and syncs between the replays;
+    mixed FP64 / FP32 networks (both cell kernels in the graph)."""
+    print("[gpu case] graphs", flush=True)
+    ref = _run(lambda b: None, chunks=(1000, 1000, 1000))
+    _same(ref, _run(lambda b: b.set_graphs(True), chunks=(1000, 1000, 1000)))
+    # uneven chunks: several graphs (one per window count)
+    ref = _run(lambda b: None)
+    _same(ref, _run(lambda b: b.set_graphs(True)))
+
+
+def test_autotune_then_step_equals_plain_run():
+    print("[gpu case] autotune", flush=True)
+    models = _models()
+    b = A.Batch(8)
+    slots = [b.add(m, 4000) for m in models]
+    chosen = b.autotune(windows_per_try=1, rounds=2)
+    assert chosen["cell_waves"] in (1, 2, 4) and chosen["sic_group"] in (1, 2, 4, 8, 16)
+    assert chosen["ms_per_window"] > 0.0
+    done = b.net_step(slots[0])
+    assert 0 < done < 4000 and all(b.net_step(s) == done for s in slots)
+    b.set_graphs(True)
+    b.step(4000 - done)
+    tuned = [(b.spikes(s)[0], b.state(s, "V")) for s in slots]
+    b.close()
+    plain = A.Batch(8)
+    ps = [plain.add(m, 4000) for m in _models()]
+    plain.step(4000)
+    for (st, v), s in zip(tuned, ps):
+        assert np.array_equal(st, plain.spikes(s)[0])
+        assert np.array_equal(v.view(np.uint64), plain.state(s, "V").view(np.uint64))
+    plain.close()
