@@ -255,7 +255,7 @@ __device__ inline void add_i64(int64_t* p, int64_t v) {
 
 // --------------------------------------------------------------- kernels ---
 
-__device__ void neuron_window(const DevNet& n, uint32_t i, int64_t T0, int W) {
+__device__ __forceinline__ void neuron_window(const DevNet& n, uint32_t i, int64_t T0, int W) {
     // Everything the loop reads is copied into registers once; the rings are
     // reached through restrict pointers so their stores cannot force reloads.
     const uint32_t N = n.n_neur;
@@ -370,7 +370,7 @@ __device__ void neuron_window(const DevNet& n, uint32_t i, int64_t T0, int W) {
     }
 }
 
-__device__ void astro_window(const DevNet& n, uint32_t a, int64_t T0, int W) {
+__device__ __forceinline__ void astro_window(const DevNet& n, uint32_t a, int64_t T0, int W) {
     const uint32_t NA = n.n_astro;
     if (a >= NA) {
         return;
@@ -440,7 +440,7 @@ __device__ void astro_window(const DevNet& n, uint32_t a, int64_t T0, int W) {
 
 // ----------------------------------------------------- FP32 cell windows ---
 
-__device__ void neuron_window_f32(const DevNet& n, uint32_t i, int64_t T0, int W) {
+__device__ __forceinline__ void neuron_window_f32(const DevNet& n, uint32_t i, int64_t T0, int W) {
     const uint32_t N = n.n_neur;
     const bool valid = i < N;
     const int pi = (i < n.n_exc) ? 0 : 1;
@@ -556,7 +556,7 @@ __device__ void neuron_window_f32(const DevNet& n, uint32_t i, int64_t T0, int W
     }
 }
 
-__device__ void astro_window_f32(const DevNet& n, uint32_t a, int64_t T0, int W) {
+__device__ __forceinline__ void astro_window_f32(const DevNet& n, uint32_t a, int64_t T0, int W) {
     const uint32_t NA = n.n_astro;
     if (a >= NA) {
         return;
@@ -686,7 +686,7 @@ __device__ void deliver_sic_f32(const DevNet& n, uint32_t t, int g0, int G) {
 // the register-heavy RK4 pass. One wavefront per neuron block.
 __global__ __launch_bounds__(256) void k_drive(DevNet* nets, const CellBlock* nblocks,
                                                int32_t n_items, int32_t Wg) {
-    const int item = static_cast<int>(blockIdx.x * 4 + (threadIdx.x >> 6));
+    const int item = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x * 4 + (threadIdx.x >> 6)));
     if (item >= n_items) {
         return;
     }
@@ -746,7 +746,11 @@ __global__ __launch_bounds__(256) void k_drive(DevNet* nets, const CellBlock* nb
 template <int P, int WPB>
 __global__ __launch_bounds__(kCellBlock * WPB) void k_cells(DevNet* nets, const CellBlock* blocks,
                                                             int32_t count, int32_t Wg) {
-    const int32_t b = static_cast<int32_t>(blockIdx.x) * WPB + static_cast<int32_t>(threadIdx.x >> 6);
+    // The wavefront's block index is uniform across its lanes: said so
+    // explicitly, so the block and network descriptors stay in scalar
+    // registers.
+    const int32_t b = __builtin_amdgcn_readfirstlane(static_cast<int32_t>(blockIdx.x) * WPB +
+                                                     static_cast<int32_t>(threadIdx.x >> 6));
     if (b >= count) {
         return;
     }
@@ -1048,7 +1052,8 @@ __device__ void deliver_sources(DevNet& n, uint32_t first, uint32_t lane) {
 template <int G>
 __global__ __launch_bounds__(256) void k_deliver(DevNet* nets, const CellBlock* nblocks,
                                                  int32_t n_items, int32_t Wg) {
-    const int e = static_cast<int>(blockIdx.x) * 4 + static_cast<int>(threadIdx.x >> 6);
+    const int e = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x) * 4 +
+                                                 static_cast<int>(threadIdx.x >> 6));
     if (e >= n_items) {
         return;
     }
@@ -1180,7 +1185,8 @@ __device__ void sum_f32(const DevNet& n, int kind, uint32_t i, int k, int64_t T0
 // The lead wavefront then closes the window.
 __global__ __launch_bounds__(256) void k_sum(DevNet* nets, const CellBlock* blocks,
                                              int32_t n_items, int32_t Wg) {
-    const int e = static_cast<int>(blockIdx.x) * 4 + static_cast<int>(threadIdx.x >> 6);
+    const int e = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x) * 4 +
+                                                 static_cast<int>(threadIdx.x >> 6));
     if (e >= n_items) {
         return;
     }

@@ -328,7 +328,7 @@ def main():
     if args.fp32_only:
         args.no_sparse = args.no_cpu_baseline = args.no_astro_scaling = True
         args.serve_jobs = 0
-        e64 = kt64 = plain64 = cmp = sub = None
+        e64 = kt64 = plain64 = cmp = floor = sub = None
     else:
         log("batched FP64 ...")
         e64, spk64, tr64, kt64 = run_batch(models, capi.ASG_FP64, args.warmup, args.steps, True)
@@ -344,6 +344,25 @@ def main():
         r32 = sweep.RunResult(e32, timed_ns, spk32, tr32)
         cmp = sweep.compare(models, r64, r32)
         cmp.pop("rows")
+
+        # The networks are chaotic: any perturbation, however small, changes
+        # individual spike times and then whole trajectories. The floor of
+        # per-network agreement is what FP64 itself does when every initial
+        # membrane potential moves by 1e-6 mV, a quarter of one FP32 ulp of V
+        # (a one-ulp FP64 change is absorbed by the first step's rounding).
+        log("FP64 with the initial V 1e-6 mV higher (chaos floor) ...")
+        saved = []
+        for m in models:
+            d = m.desc.contents
+            saved.append((d.neuron[0].V_m_init, d.neuron[1].V_m_init))
+            for k in (0, 1):
+                d.neuron[k].V_m_init = d.neuron[k].V_m_init + 1e-6
+        _, spkp, trp, _ = run_batch(models, capi.ASG_FP64, args.warmup, args.steps, False)
+        for m, (v0, v1) in zip(models, saved):
+            m.desc.contents.neuron[0].V_m_init = v0
+            m.desc.contents.neuron[1].V_m_init = v1
+        floor = sweep.compare(models, r64, sweep.RunResult(0.0, timed_ns, spkp, trp))
+        floor.pop("rows")
 
     # Roofline of the kernel with the largest share of device time.
     windows = kt32["windows"]
@@ -436,6 +455,8 @@ def main():
         "speedup_batched_fp32_vs_plain_fp64": None if plain64 is None else round(value / plain64, 2),
         "accuracy_fp32_vs_fp64": None if cmp is None else
         {k: (round(v, 5) if isinstance(v, float) else v) for k, v in cmp.items()},
+        "chaos_floor_fp64_vs_fp64_dV_1e-6_mV": None if floor is None else
+        {k: (round(v, 5) if isinstance(v, float) else v) for k, v in floor.items()},
         "roofline": roof,
         "kernel_ms_fp32": {k: round(v, 3) for k, v in kt32.items() if k != "windows"},
         "kernel_ms_fp64": None if kt64 is None else
