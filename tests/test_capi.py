@@ -167,3 +167,25 @@ def test_astro_scaling_bytes_formula():
     assert p.dt == 0.1 and p.lambda_ == 0.05 and p.randomized == 1
     assert (p.lower, p.upper, p.var) == (0.9, 1.05, 0.1)
     assert c.rate_SERCA == 0.0009
+
+
+def test_cli_usage_and_errors_without_a_device(tmp_path):
+    """astrosim-amd: -h prints the reference's options; an unknown option is
+    exit 2 with the usage; a bad configuration is exit 1 with the loader's
+    message -- all before any device call."""
+    import subprocess
+    cli = os.path.join(REPO, "astrosimgpu_amd", "astrosim-amd")
+    r = subprocess.run([cli, "-h"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0
+    for opt in ("--config", "--output", "--seed", "--sim-time", "--pre-time", "--astrocytes",
+                "--threads", "--no-analysis"):
+        assert opt in r.stdout
+    r = subprocess.run([cli, "--bogus"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "unknown option: --bogus" in r.stderr
+    r = subprocess.run([cli, "-c", str(tmp_path / "missing.json")], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 1 and "cannot open configuration file" in r.stderr
+    bad = tmp_path / "bad.json"
+    bad.write_text('{"N": {"N_E": 10, "N_I": 2, "N_A": 3}, "simulation": {"nope": 1}}')
+    r = subprocess.run([cli, "-c", str(bad)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "error:" in r.stderr
