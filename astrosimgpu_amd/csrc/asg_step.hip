@@ -116,6 +116,7 @@ asg_status guarded(F&& f) {
 constexpr int kCellBlock = 64;     // one wavefront per block of the per-cell kernels
 constexpr int kWindowCap = 16;     // longest window (bounds scratch memory)
 constexpr int64_t kSicWaves = 4096; // SIC wavefronts that fill the device (256 CUs x 16)
+constexpr int64_t kGroupWaves = 16384;  // per-step wavefronts beyond which steps are grouped
 constexpr size_t kMaskBudget = size_t(256) << 20;  // delivery masks per network
 constexpr size_t kMinChunks = 4;
 
@@ -1043,27 +1044,29 @@ __device__ void deliver_sources(DevNet& n, uint32_t first, uint32_t lane) {
 }
 
 // Window pass 3: one wavefront per (64 neurons, window step); four per block.
-// Window pass 3. Per neuron block, Wg + ceil(Wg / G) wavefronts: one per
-// window step k for the marking of that step's spikes (and, at k = 0, the
-// sources' plasticity and raster entries), then one per group of G steps for
-// the SIC sums of the block's neurons. G trades re-reading the astrocyte
-// lists (G = Wg reads them once per window) against parallelism for small
-// networks (G = 1).
+// Window pass 3. Per neuron block, ceil(Wg / MG) + ceil(Wg / G) wavefronts:
+// one per group of MG window steps for the marking of those steps' spikes
+// (the first also does the sources' plasticity and raster entries), then one
+// per group of G steps for the SIC sums of the block's neurons. G trades
+// re-reading the astrocyte lists (G = Wg reads them once per window) against
+// parallelism for small networks (G = 1); MG = Wg saves the dispatch of
+// wavefronts that mostly find no spike in a large batch.
 template <int G>
 __global__ __launch_bounds__(256) void k_deliver(DevNet* nets, const CellBlock* nblocks,
-                                                 int32_t n_items, int32_t Wg) {
+                                                 int32_t n_items, int32_t Wg, int32_t MG) {
     const int e = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x) * 4 +
                                                  static_cast<int>(threadIdx.x >> 6));
     if (e >= n_items) {
         return;
     }
-    const int per_block = Wg + (Wg + G - 1) / G;
+    const int mark_waves = (Wg + MG - 1) / MG;
+    const int per_block = mark_waves + (Wg + G - 1) / G;
     const CellBlock cb = nblocks[e / per_block];
     const int k = e % per_block;
     DevNet& n = nets[cb.net];
     const uint32_t lane = threadIdx.x & 63;
-    if (k >= Wg) {
-        const int g0 = (k - Wg) * G;
+    if (k >= mark_waves) {
+        const int g0 = (k - mark_waves) * G;
         const int len = min(G, n.win_W - g0);
         const uint32_t t = cb.first + lane;
         if (len > 0 && t < n.n_neur) {
@@ -1075,11 +1078,12 @@ __global__ __launch_bounds__(256) void k_deliver(DevNet* nets, const CellBlock* 
         }
         return;
     }
-    if (k >= n.win_W) {
-        return;
+    const int k0 = k * MG;
+    const int k1 = min(k0 + MG, n.win_W);
+    for (int kk = k0; kk < k1; ++kk) {
+        mark_step(n, cb.first, lane, kk);
     }
-    mark_step(n, cb.first, lane, k);
-    if (k == 0) {
+    if (k0 == 0 && n.win_W > 0) {
         deliver_sources(n, cb.first, lane);
     }
 }
@@ -1179,47 +1183,12 @@ __device__ void sum_f32(const DevNet& n, int kind, uint32_t i, int k, int64_t T0
     }
 }
 
-// Window pass 4, one lane per (target, step), four wavefronts per block: add
-// the marked spikes' weights into the arrival slot in ascending spike order,
-// i.e. ascending source as the reference's scatter (network.cpp:249-279).
-// The lead wavefront then closes the window.
-__global__ __launch_bounds__(256) void k_sum(DevNet* nets, const CellBlock* blocks,
-                                             int32_t n_items, int32_t Wg) {
-    const int e = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x) * 4 +
-                                                 static_cast<int>(threadIdx.x >> 6));
-    if (e >= n_items) {
-        return;
-    }
-    const CellBlock cb = blocks[e / Wg];
-    const int k = e % Wg;
-    const uint32_t lane = threadIdx.x & 63;
-    DevNet& n = nets[cb.net];
-    const int W = n.win_W;
-    const int64_t T0 = n.win_T0;
-    if (cb.lead && k == 0 && lane == 0) {
-        int64_t tot = 0;
-        for (int k2 = 0; k2 < W; ++k2) {
-            tot += n.spk_n[2 * k2];
-        }
-        if (n.r_head + tot <= n.r_cap) {
-            n.r_head += tot;
-        }
-        n.c_steps += W;
-        n.step = T0 + W;
-        n.sic_live = 0;
-    }
-    if (k >= W) {
-        return;
-    }
-    const uint32_t i = cb.first + lane;
+// FP64 ordered sums of one (target, window step): the marked spikes' weights
+// added in ascending spike order into the ring slot.
+__device__ void sum_f64(const DevNet& n, int kind, uint32_t i, int k, int64_t T0, uint32_t nE,
+                        uint32_t nI, const double* cvals) {
     const uint32_t N = n.n_neur;
-    const uint32_t tot = n.spk_n[2 * k], nE = n.spk_n[2 * k + 1], nI = tot - nE;
-    const double* cvals = n.spk_c + static_cast<size_t>(k) * N * 2;
-    if (n.precision == ASG_FP32) {
-        sum_f32(n, cb.kind, i, k, T0, tot, nE, nI, cvals);
-        return;
-    }
-    if (cb.kind == 0) {
+    if (kind == 0) {
         if (i >= N) {
             return;
         }
@@ -1292,6 +1261,53 @@ __global__ __launch_bounds__(256) void k_sum(DevNet* nets, const CellBlock* bloc
         }
         if (any) {
             n.ring_astro[slot] = acc;
+        }
+    }
+}
+
+// Window pass 4, one lane per (target, step), four wavefronts per block: add
+// the marked spikes' weights into the arrival slot in ascending spike order,
+// i.e. ascending source as the reference's scatter (network.cpp:249-279).
+// The lead wavefront then closes the window.
+__global__ __launch_bounds__(256) void k_sum(DevNet* nets, const CellBlock* blocks,
+                                             int32_t n_items, int32_t Wg, int32_t SG) {
+    const int e = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x) * 4 +
+                                                 static_cast<int>(threadIdx.x >> 6));
+    if (e >= n_items) {
+        return;
+    }
+    const int per_block = (Wg + SG - 1) / SG;
+    const CellBlock cb = blocks[e / per_block];
+    const int k0 = (e % per_block) * SG;
+    const uint32_t lane = threadIdx.x & 63;
+    DevNet& n = nets[cb.net];
+    const int W = n.win_W;
+    const int64_t T0 = n.win_T0;
+    if (cb.lead && k0 == 0 && lane == 0) {
+        int64_t tot = 0;
+        for (int k2 = 0; k2 < W; ++k2) {
+            tot += n.spk_n[2 * k2];
+        }
+        if (n.r_head + tot <= n.r_cap) {
+            n.r_head += tot;
+        }
+        n.c_steps += W;
+        n.step = T0 + W;
+        n.sic_live = 0;
+    }
+    const uint32_t i = cb.first + lane;
+    const uint32_t N = n.n_neur;
+    const int k1 = min(k0 + SG, W);
+    for (int k = k0; k < k1; ++k) {
+        const uint32_t tot = n.spk_n[2 * k], nE = n.spk_n[2 * k + 1], nI = tot - nE;
+        if (tot == 0) {
+            continue;
+        }
+        const double* cvals = n.spk_c + static_cast<size_t>(k) * N * 2;
+        if (n.precision == ASG_FP32) {
+            sum_f32(n, cb.kind, i, k, T0, tot, nE, nI, cvals);
+        } else {
+            sum_f64(n, cb.kind, i, k, T0, nE, nI, cvals);
         }
     }
 }
@@ -1864,13 +1880,14 @@ struct Engine {
     // per workgroup (1, 2, 4) and SIC group (1-16 steps, 0 = by batch size).
     int sched_cell_waves = 1;
     int sched_sic_group = 0;
+    int sched_step_group = 0;
     float tuned_ms_per_window = 0.0f;   // the autotuner's best median
     // Window sequences replayed from captured graphs (when use_graphs and
     // not profiling); cleared whenever the tables change.
     bool use_graphs = false;
     struct GraphEntry {
         int64_t windows;
-        int Wg, G, wpb;
+        int Wg, G, wpb, SG;
         hipGraphExec_t exec;
     };
     std::vector<GraphEntry> graphs;
@@ -2068,25 +2085,32 @@ struct Engine {
     }
 
     // k_deliver for SIC group size G (its accumulators are registers).
-    void launch_deliver(int G, int items, int32_t Wg_) {
+    void launch_deliver(int G, int items, int32_t Wg_, int MG) {
         const dim3 grid((items + 3) / 4), block(256);
         switch (G) {
             case 1:
-                hipLaunchKernelGGL(k_deliver<1>, grid, block, 0, stream, d_nets, d_nblocks, items, Wg_);
+                hipLaunchKernelGGL(k_deliver<1>, grid, block, 0, stream, d_nets, d_nblocks, items, Wg_, MG);
                 break;
             case 2:
-                hipLaunchKernelGGL(k_deliver<2>, grid, block, 0, stream, d_nets, d_nblocks, items, Wg_);
+                hipLaunchKernelGGL(k_deliver<2>, grid, block, 0, stream, d_nets, d_nblocks, items, Wg_, MG);
                 break;
             case 4:
-                hipLaunchKernelGGL(k_deliver<4>, grid, block, 0, stream, d_nets, d_nblocks, items, Wg_);
+                hipLaunchKernelGGL(k_deliver<4>, grid, block, 0, stream, d_nets, d_nblocks, items, Wg_, MG);
                 break;
             case 8:
-                hipLaunchKernelGGL(k_deliver<8>, grid, block, 0, stream, d_nets, d_nblocks, items, Wg_);
+                hipLaunchKernelGGL(k_deliver<8>, grid, block, 0, stream, d_nets, d_nblocks, items, Wg_, MG);
                 break;
             default:
-                hipLaunchKernelGGL(k_deliver<16>, grid, block, 0, stream, d_nets, d_nblocks, items, Wg_);
+                hipLaunchKernelGGL(k_deliver<16>, grid, block, 0, stream, d_nets, d_nblocks, items, Wg_, MG);
                 break;
         }
+    }
+
+    // Window steps per marking / summing wavefront: as requested, or (0) the
+    // whole window once per-step wavefronts would number beyond kGroupWaves.
+    int step_group_for(int requested) const {
+        if (requested > 0) return std::min(requested, Wg);
+        return static_cast<int64_t>(n_blocks) * Wg > kGroupWaves ? Wg : 1;
     }
 
     template <int P>
@@ -2126,9 +2150,9 @@ struct Engine {
 
     // The launches of `windows` windows on the stream (eager, or recorded
     // into a graph when the stream is capturing).
-    void launch_windows(int64_t windows, int G, int wpb, bool marks) {
-        const int deliver_items = n_nblocks * (Wg + (Wg + G - 1) / G);
-        const int sum_items = n_blocks * Wg;
+    void launch_windows(int64_t windows, int G, int wpb, int SG, bool marks) {
+        const int deliver_items = n_nblocks * ((Wg + SG - 1) / SG + (Wg + G - 1) / G);
+        const int sum_items = n_blocks * ((Wg + SG - 1) / SG);
         auto mark = [&](int64_t wdx, int k) {
             if (marks) HIP_CHECK(hipEventRecord(events[6 * wdx + k], stream));
         };
@@ -2146,30 +2170,32 @@ struct Engine {
                                d_active);
             mark(wdx, 3);
             if (deliver_items > 0) {
-                launch_deliver(G, deliver_items, Wg);
+                launch_deliver(G, deliver_items, Wg, SG);
             }
             mark(wdx, 4);
             hipLaunchKernelGGL(k_sum, dim3((sum_items + 3) / 4), dim3(256), 0, stream, d_nets,
-                               d_blocks, sum_items, Wg);
+                               d_blocks, sum_items, Wg, SG);
             mark(wdx, 5);
         }
     }
 
     // Graph of `windows` windows for the current tables and schedule,
     // captured on first use.
-    hipGraphExec_t graph_for(int64_t windows, int G, int wpb) {
+    hipGraphExec_t graph_for(int64_t windows, int G, int wpb, int SG) {
         for (const GraphEntry& g : graphs) {
-            if (g.windows == windows && g.G == G && g.wpb == wpb && g.Wg == Wg) return g.exec;
+            if (g.windows == windows && g.G == G && g.wpb == wpb && g.SG == SG && g.Wg == Wg) {
+                return g.exec;
+            }
         }
         hipGraph_t graph = nullptr;
         HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        launch_windows(windows, G, wpb, false);
+        launch_windows(windows, G, wpb, SG, false);
         HIP_CHECK(hipStreamEndCapture(stream, &graph));
         hipGraphExec_t exec = nullptr;
         const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         HIP_CHECK(e);
-        graphs.push_back(GraphEntry{windows, Wg, G, wpb, exec});
+        graphs.push_back(GraphEntry{windows, Wg, G, wpb, SG, exec});
         return exec;
     }
 
@@ -2202,15 +2228,16 @@ struct Engine {
                            d_nets, d_active, n_active, d_targets);
         raster_dirty = true;
         const int G = sic_group_for(sched_sic_group);
+        const int SG = step_group_for(sched_step_group);
         if (profiling) {
             // per-kernel events need eager launches
             ensure_events(windows);
-            launch_windows(windows, G, sched_cell_waves, true);
+            launch_windows(windows, G, sched_cell_waves, SG, true);
             pending_windows = windows;
         } else if (use_graphs && windows > 0) {
-            HIP_CHECK(hipGraphLaunch(graph_for(windows, G, sched_cell_waves), stream));
+            HIP_CHECK(hipGraphLaunch(graph_for(windows, G, sched_cell_waves, SG), stream));
         } else {
-            launch_windows(windows, G, sched_cell_waves, false);
+            launch_windows(windows, G, sched_cell_waves, SG, false);
         }
         HIP_CHECK(hipGetLastError());
         pending = true;
@@ -2229,9 +2256,15 @@ struct Engine {
         if (n_active == 0 || wpt <= 0 || rounds <= 0) return;
         int top = 1;
         while (top < Wg) top *= 2;
-        std::vector<std::pair<int, int>> cand;
+        struct Cand {
+            int wpb, g, sg;
+        };
+        std::vector<Cand> cand;
         for (int wpb : {1, 2, 4}) {
-            for (int g = 1; g <= top; g *= 2) cand.emplace_back(wpb, g);
+            for (int g = 1; g <= top; g *= 2) {
+                cand.push_back({wpb, g, 1});
+                if (Wg > 1) cand.push_back({wpb, g, Wg});
+            }
         }
         std::vector<std::vector<float>> times(cand.size());
         hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -2243,8 +2276,9 @@ struct Engine {
         try {
             for (int r = 0; r < rounds; ++r) {
                 for (size_t c = 0; c < cand.size(); ++c) {
-                    sched_cell_waves = cand[c].first;
-                    sched_sic_group = cand[c].second;
+                    sched_cell_waves = cand[c].wpb;
+                    sched_sic_group = cand[c].g;
+                    sched_step_group = cand[c].sg;
                     HIP_CHECK(hipEventRecord(e0, stream));
                     step(static_cast<int64_t>(wpt) * Wg, true);
                     HIP_CHECK(hipEventRecord(e1, stream));
@@ -2276,8 +2310,9 @@ struct Engine {
                 best_t = med;
             }
         }
-        sched_cell_waves = cand[best].first;
-        sched_sic_group = cand[best].second;
+        sched_cell_waves = cand[best].wpb;
+        sched_sic_group = cand[best].g;
+        sched_step_group = cand[best].sg;
         tuned_ms_per_window = best_t / static_cast<float>(wpt);
     }
 
@@ -2800,14 +2835,17 @@ asg_status asg_batch_set_graphs(asg_batch* batch, int32_t on) {
 
 asg_status asg_batch_set_schedule(asg_batch* batch, const asg_schedule* sched) {
     if (batch == nullptr || sched == nullptr) return fail(ASG_ERR_INVALID, "bad argument");
-    const int w = sched->cell_waves, g = sched->sic_group;
-    if ((w != 1 && w != 2 && w != 4) || (g != 0 && g != 1 && g != 2 && g != 4 && g != 8 && g != 16)) {
-        return fail(ASG_ERR_INVALID, "cell_waves must be 1, 2 or 4; sic_group 0, 1, 2, 4, 8 or 16");
+    const int w = sched->cell_waves, g = sched->sic_group, sg = sched->step_group;
+    if ((w != 1 && w != 2 && w != 4) || (g != 0 && g != 1 && g != 2 && g != 4 && g != 8 && g != 16) ||
+        sg < 0 || sg > asg::kWindowCap) {
+        return fail(ASG_ERR_INVALID,
+                    "cell_waves must be 1, 2 or 4; sic_group 0, 1, 2, 4, 8 or 16; step_group 0-16");
     }
     return guarded([&]() -> asg_status {
         batch->eng->wait();
         batch->eng->sched_cell_waves = w;
         batch->eng->sched_sic_group = g;
+        batch->eng->sched_step_group = sg;
         return ASG_OK;
     });
 }
@@ -2816,6 +2854,7 @@ asg_status asg_batch_get_schedule(asg_batch* batch, asg_schedule* out) {
     if (batch == nullptr || out == nullptr) return fail(ASG_ERR_INVALID, "bad argument");
     out->cell_waves = batch->eng->sched_cell_waves;
     out->sic_group = batch->eng->sched_sic_group;
+    out->step_group = batch->eng->sched_step_group;
     out->ms_per_window = batch->eng->tuned_ms_per_window;
     return ASG_OK;
 }
@@ -2833,6 +2872,7 @@ asg_status asg_batch_autotune(asg_batch* batch, int32_t windows_per_try, int32_t
         if (chosen != nullptr) {
             chosen->cell_waves = batch->eng->sched_cell_waves;
             chosen->sic_group = batch->eng->sched_sic_group;
+            chosen->step_group = batch->eng->sched_step_group;
             chosen->ms_per_window = batch->eng->tuned_ms_per_window;
         }
         return ASG_OK;

@@ -283,16 +283,16 @@ class Batch:
         sch = capi.Schedule()
         check(lib().asg_batch_autotune(self._h, int(windows_per_try), int(rounds), C.byref(sch)))
         return {"cell_waves": sch.cell_waves, "sic_group": sch.sic_group,
-                "ms_per_window": sch.ms_per_window}
+                "step_group": sch.step_group, "ms_per_window": sch.ms_per_window}
 
     def schedule(self) -> dict:
         sch = capi.Schedule()
         check(lib().asg_batch_get_schedule(self._h, C.byref(sch)))
         return {"cell_waves": sch.cell_waves, "sic_group": sch.sic_group,
-                "ms_per_window": sch.ms_per_window}
+                "step_group": sch.step_group, "ms_per_window": sch.ms_per_window}
 
-    def set_schedule(self, cell_waves: int = 1, sic_group: int = 0) -> None:
-        sch = capi.Schedule(int(cell_waves), int(sic_group), 0.0)
+    def set_schedule(self, cell_waves: int = 1, sic_group: int = 0, step_group: int = 0) -> None:
+        sch = capi.Schedule(int(cell_waves), int(sic_group), int(step_group), 0.0)
         check(lib().asg_batch_set_schedule(self._h, C.byref(sch)))
 
     def set_graphs(self, on: bool) -> None:

@@ -399,13 +399,15 @@ asg_status asg_batch_read_trace(asg_batch* batch, int32_t slot, double* out, int
 int64_t asg_batch_bytes_in_use(const asg_batch* batch);
 
 /* Launch schedule of the window pipeline: cell-kernel workgroup size in
- * wavefronts (1, 2 or 4) and SIC group (window steps per SIC wavefront: 1,
- * 2, 4, 8 or 16; 0 = the largest that still fills the device). Results do
- * not depend on it. ms_per_window is the autotuner's measured time (output
- * only). */
+ * wavefronts (1, 2 or 4), SIC group (window steps per SIC wavefront: 1, 2, 4,
+ * 8 or 16; 0 = the largest that still fills the device) and step group
+ * (window steps per spike-marking and per summing wavefront, 1-16; 0 = the
+ * whole window for large batches, 1 otherwise). Results do not depend on it.
+ * ms_per_window is the autotuner's measured time (output only). */
 typedef struct asg_schedule {
     int32_t cell_waves;
     int32_t sic_group;
+    int32_t step_group;
     float ms_per_window;
 } asg_schedule;
 asg_status asg_batch_set_schedule(asg_batch* batch, const asg_schedule* sched);

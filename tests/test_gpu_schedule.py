@@ -60,7 +60,8 @@ def test_every_schedule_gives_the_same_bits():
     ref = _run(lambda b: None)
     for waves in (1, 2, 4):
         for group in (1, 2, 4, 8, 16):
-            _same(ref, _run(lambda b, w=waves, g=group: b.set_schedule(w, g)))
+            for steps in (1, 3, 16):   # 3 does not divide the window; 16 clamps to it
+                _same(ref, _run(lambda b, w=waves, g=group, sg=steps: b.set_schedule(w, g, sg)))
 
 
 def test_graph_replays_equal_eager_launches():
@@ -82,6 +83,7 @@ def test_autotune_then_step_equals_plain_run():
     slots = [b.add(m, 4000) for m in models]
     chosen = b.autotune(windows_per_try=1, rounds=2)
     assert chosen["cell_waves"] in (1, 2, 4) and chosen["sic_group"] in (1, 2, 4, 8, 16)
+    assert chosen["step_group"] >= 1
     assert chosen["ms_per_window"] > 0.0
     done = b.net_step(slots[0])
     assert 0 < done < 4000 and all(b.net_step(s) == done for s in slots)
