@@ -280,6 +280,9 @@ __device__ __forceinline__ void neuron_window(const DevNet& n, uint32_t i, int64
     double V = 0, w = 0, gex = 0, dgex = 0, gin = 0, dgin = 0, Isic = 0, Vr = 0, b = 0;
     int refr = 0;
     double ex_n = 0, ih_n = 0, sic_n = 0;  // ring values of the next step
+    int ev_n = 0;                           // drive of the next step
+    double noise_n = 0.0;
+    const bool has_poiss = in.poiss_rate > 0.0, has_noise = in.noise_std > 0.0;
     if (valid) {
         V = n.V[i];
         w = n.w[i];
@@ -295,6 +298,8 @@ __device__ __forceinline__ void neuron_window(const DevNet& n, uint32_t i, int64
         ex_n = ring_exc[s0];
         ih_n = ring_inh[s0];
         sic_n = ring_sic[s0];
+        if (has_poiss) ev_n = drive_n[i];
+        if (has_noise) noise_n = drive_noise[i];
     }
     const uint32_t lane = threadIdx.x & 63;
     for (int k = 0; k < W; ++k) {
@@ -302,13 +307,17 @@ __device__ __forceinline__ void neuron_window(const DevNet& n, uint32_t i, int64
         bool spiked = false;
         if (valid) {
             const size_t slot = static_cast<size_t>(T % S) * N + i;
-            const size_t dk = static_cast<size_t>(k) * N + i;
             const double ex = ex_n, ih = ih_n, sic = sic_n;
+            const int ev = ev_n;
+            const double noise = noise_n;
             if (k + 1 < W) {
                 const size_t nx = static_cast<size_t>((T + 1) % S) * N + i;
                 ex_n = ring_exc[nx];
                 ih_n = ring_inh[nx];
                 sic_n = ring_sic[nx];
+                const size_t dk1 = static_cast<size_t>(k + 1) * N + i;
+                if (has_poiss) ev_n = drive_n[dk1];
+                if (has_noise) noise_n = drive_noise[dk1];
             }
             double exc_in = 0.0, inh_in = 0.0;
             // apply_arrivals, network.cpp:301-327
@@ -335,15 +344,12 @@ __device__ __forceinline__ void neuron_window(const DevNet& n, uint32_t i, int64
                 Isic = sic;
             }
             // NeuronPopulation::update, neuron.cpp:143-170 (draws from k_drive)
-            if (in.poiss_rate > 0.0) {
-                const int ev = drive_n[dk];
-                if (ev > 0) {
-                    exc_in += in.poiss_weight * static_cast<double>(ev);
-                }
+            if (has_poiss && ev > 0) {
+                exc_in += in.poiss_weight * static_cast<double>(ev);
             }
             double I_ext = Isic;
-            if (in.noise_std > 0.0) {
-                I_ext += drive_noise[dk];
+            if (has_noise) {
+                I_ext += noise;
             }
             if (exc_in != 0.0) {
                 dgex += exc_in * p.psc_ex;
@@ -465,6 +471,9 @@ __device__ __forceinline__ void neuron_window_f32(const DevNet& n, uint32_t i, i
     float V = 0, w = 0, gex = 0, dgex = 0, gin = 0, dgin = 0, Isic = 0, Vr = 0, b = 0;
     int refr = 0;
     float ex_n = 0, ih_n = 0, sic_n = 0;
+    int ev_n = 0;
+    float noise_n = 0.0f;
+    const bool has_poiss = in.poiss_rate > 0.0, has_noise = in.noise_std > 0.0;
     if (valid) {
         V = n.fV[i];
         w = n.fw[i];
@@ -480,6 +489,8 @@ __device__ __forceinline__ void neuron_window_f32(const DevNet& n, uint32_t i, i
         ex_n = ring_exc[s0];
         ih_n = ring_inh[s0];
         sic_n = ring_sic[s0];
+        if (has_poiss) ev_n = drive_n[i];
+        if (has_noise) noise_n = drive_noise[i];
     }
     const uint32_t lane = threadIdx.x & 63;
     for (int k = 0; k < W; ++k) {
@@ -487,13 +498,17 @@ __device__ __forceinline__ void neuron_window_f32(const DevNet& n, uint32_t i, i
         bool spiked = false;
         if (valid) {
             const size_t slot = static_cast<size_t>(T % S) * N + i;
-            const size_t dk = static_cast<size_t>(k) * N + i;
             const float ex = ex_n, ih = ih_n, sic = sic_n;
+            const int ev = ev_n;
+            const float noise = noise_n;
             if (k + 1 < W) {
                 const size_t nx = static_cast<size_t>((T + 1) % S) * N + i;
                 ex_n = ring_exc[nx];
                 ih_n = ring_inh[nx];
                 sic_n = ring_sic[nx];
+                const size_t dk1 = static_cast<size_t>(k + 1) * N + i;
+                if (has_poiss) ev_n = drive_n[dk1];
+                if (has_noise) noise_n = drive_noise[dk1];
             }
             float exc_in = 0.0f, inh_in = 0.0f;
             if (ex != 0.0f) {
@@ -517,15 +532,12 @@ __device__ __forceinline__ void neuron_window_f32(const DevNet& n, uint32_t i, i
             }
             // Same counter streams as the FP64 step (k_drive): identical
             // Poisson events, the noise current rounded to FP32.
-            if (in.poiss_rate > 0.0) {
-                const int ev = drive_n[dk];
-                if (ev > 0) {
-                    exc_in += static_cast<float>(in.poiss_weight * static_cast<double>(ev));
-                }
+            if (has_poiss && ev > 0) {
+                exc_in += static_cast<float>(in.poiss_weight * static_cast<double>(ev));
             }
             float I_ext = Isic;
-            if (in.noise_std > 0.0) {
-                I_ext += drive_noise[dk];
+            if (has_noise) {
+                I_ext += noise;
             }
             if (exc_in != 0.0f) {
                 dgex += exc_in * pf.psc_ex;
