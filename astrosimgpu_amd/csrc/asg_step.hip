@@ -694,38 +694,113 @@ __device__ void deliver_sic_f32(const DevNet& n, uint32_t t, int g0, int G) {
 // from the reference's counter streams -- Poisson event counts of the
 // background input (neuron.cpp:143-151) and the Gaussian noise current held
 // over noise_dt (neuron.cpp:152-166) -- written per (step, neuron) for the
-// cell pass. The draws are chains of 64-bit multiplies with data-dependent
-// trip counts: cheap in this small kernel at high occupancy, costly inside
-// the register-heavy RK4 pass. One wavefront per neuron block.
-__global__ __launch_bounds__(256) void k_drive(DevNet* nets, const CellBlock* nblocks,
-                                               int32_t n_items, int32_t Wg) {
-    const int item = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x * 4 + (threadIdx.x >> 6)));
+// cell pass. One wavefront per neuron block.
+//
+// Knuth's Poisson draw (rng.hpp:107-124) multiplies uniforms until the
+// product falls to exp(-lambda); at lambda ~ 0.2 most draws end after one
+// uniform, but a wavefront looping per lane would run as long as its slowest
+// lane. Here every (step, neuron) draws its first uniform together; the ones
+// still above the limit go to a queue in LDS (product, step, lane, draws so
+// far) and the wavefront works through the queue 64 entries at a time,
+// compacting survivors in place, until it is empty. Every uniform is the
+// same (seed, stream, counter) as in the sequential loop, so the counts are
+// identical.
+__global__ __launch_bounds__(64) void k_drive(DevNet* nets, const CellBlock* nblocks,
+                                              int32_t n_items, int32_t Wg) {
+    extern __shared__ double drive_lds[];
+    double* q_prod = drive_lds;                                             // [64 * Wg]
+    uint32_t* q_key = reinterpret_cast<uint32_t*>(drive_lds + 64 * Wg);     // [64 * Wg]
+    const int item = static_cast<int>(blockIdx.x);
     if (item >= n_items) {
         return;
     }
     const CellBlock cb = nblocks[item];
     const DevNet& n = nets[cb.net];
     const int W = window_len(n, Wg);
-    const uint32_t N = n.n_neur;
-    const uint32_t i = cb.first + (threadIdx.x & 63);
-    if (W == 0 || i >= N) {
+    if (W == 0) {
         return;
     }
+    const uint32_t N = n.n_neur;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t i = cb.first + lane;
+    const bool valid = i < N;
     const int pi = (i < n.n_exc) ? 0 : 1;
     const InputK in = n.in[pi];
     const int64_t T0 = n.step;
     const uint64_t seed = n.seed;
-    if (in.poiss_rate > 0.0) {
+    if (n.in[0].poiss_rate > 0.0 || n.in[1].poiss_rate > 0.0) {
         int32_t* __restrict__ out = n.drive_n;
-        for (int k = 0; k < W; ++k) {
-            const int64_t T = T0 + k;
-            out[static_cast<size_t>(k) * N + i] =
-                rng_poisson_fresh(seed ^ kSaltNeuronPoisson,
-                                  static_cast<uint64_t>(T) * kStreamStride + i, in.lambda,
-                                  in.limit);
+        const uint64_t pseed = seed ^ kSaltNeuronPoisson;
+        if (n.in[0].lambda > 30.0 || n.in[1].lambda > 30.0) {
+            // the normal approximation above lambda = 30: one lane per draw
+            for (int k = 0; valid && k < W; ++k) {
+                out[static_cast<size_t>(k) * N + i] =
+                    rng_poisson_fresh(pseed, static_cast<uint64_t>(T0 + k) * kStreamStride + i,
+                                      in.lambda, in.limit);
+            }
+        } else {
+            const bool mine = valid && in.lambda > 0.0;
+            uint32_t qn = 0;
+            for (int k = 0; k < W; ++k) {
+                bool pend = false;
+                double p = 0.0;
+                if (mine) {
+                    p = rng_uniform(pseed, static_cast<uint64_t>(T0 + k) * kStreamStride + i, 0);
+                    pend = p > in.limit;
+                }
+                if (valid && !pend) {
+                    out[static_cast<size_t>(k) * N + i] = 0;
+                }
+                const uint64_t b = __ballot(pend);
+                if (pend) {
+                    const uint32_t pos =
+                        qn + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(b >> 32),
+                                                       __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(b), 0));
+                    q_prod[pos] = p;
+                    q_key[pos] = (static_cast<uint32_t>(k) << 6) | lane;  // draws so far: 0 extra
+                }
+                qn += static_cast<uint32_t>(__popcll(b));
+            }
+            __syncthreads();
+            while (qn > 0) {
+                uint32_t nn = 0;  // survivors, compacted to the front (nn <= off)
+                for (uint32_t off = 0; off < qn; off += 64) {
+                    const uint32_t idx = off + lane;
+                    bool pend = false;
+                    double p = 0.0;
+                    uint32_t key = 0;
+                    if (idx < qn) {
+                        p = q_prod[idx];
+                        key = q_key[idx];
+                        const uint32_t l = key & 63u, kk = (key >> 6) & 15u, cnt = (key >> 10) + 1;
+                        const uint32_t ii = cb.first + l;
+                        const double limit = n.in[ii < n.n_exc ? 0 : 1].limit;
+                        p *= rng_uniform(pseed, static_cast<uint64_t>(T0 + kk) * kStreamStride + ii,
+                                         cnt);
+                        if (p > limit) {
+                            pend = true;
+                            key = (cnt << 10) | (kk << 6) | l;
+                        } else {
+                            out[static_cast<size_t>(kk) * N + ii] = static_cast<int32_t>(cnt);
+                        }
+                    }
+                    __syncthreads();  // this chunk is read before survivors overwrite it
+                    const uint64_t b = __ballot(pend);
+                    if (pend) {
+                        const uint32_t pos =
+                            nn + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(b >> 32),
+                                                           __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(b), 0));
+                        q_prod[pos] = p;
+                        q_key[pos] = key;
+                    }
+                    nn += static_cast<uint32_t>(__popcll(b));
+                    __syncthreads();
+                }
+                qn = nn;
+            }
         }
     }
-    if (in.noise_std > 0.0) {
+    if (valid && in.noise_std > 0.0) {
         // One draw per noise interval (neuron.cpp:152-166), held across it.
         const uint64_t salt_shared = pi == 0 ? kSaltNoiseSharedExc : kSaltNoiseSharedInh;
         const bool f32 = n.precision == ASG_FP32;
@@ -2171,8 +2246,9 @@ struct Engine {
         for (int64_t wdx = 0; wdx < windows; ++wdx) {
             mark(wdx, 0);
             if (n_nblocks > 0) {
-                hipLaunchKernelGGL(k_drive, dim3((n_nblocks + 3) / 4), dim3(256), 0, stream,
-                                   d_nets, d_nblocks, n_nblocks, Wg);
+                hipLaunchKernelGGL(k_drive, dim3(n_nblocks), dim3(64),
+                                   static_cast<size_t>(64 * Wg) * (sizeof(double) + sizeof(uint32_t)),
+                                   stream, d_nets, d_nblocks, n_nblocks, Wg);
             }
             mark(wdx, 1);
             launch_cells<ASG_FP64>(0, Wg, wpb);
