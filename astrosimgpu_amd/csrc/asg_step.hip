@@ -213,10 +213,30 @@ struct DevNet {
                            // window step k (set by k_cells, cleared by k_sum)
 };
 
-__device__ inline void record_astro(const DevNet& n, uint32_t a, int64_t T, double Ca, double IP3) {
-    if (a < n.rec_n && T >= n.rec_start && (T - n.rec_start) % n.rec_every == 0) {
-        const int64_t r = ((T - n.rec_start) / n.rec_every) % n.rec_cap;
-        double* rec = n.rec_buf + static_cast<size_t>(r) * 2 * n.rec_n;
+// Recording cadence of one window: `next` is the window offset of the next
+// recorded step, `slot` its ring slot; advance() moves to the one after.
+struct RecCursor {
+    int64_t next, slot;
+    __device__ __forceinline__ RecCursor(const DevNet& n, int64_t T0) {
+        next = n.rec_n == 0 ? INT64_MAX : 0;
+        slot = 0;
+        if (n.rec_n != 0) {
+            const int64_t d = T0 - n.rec_start;
+            const int64_t q = d <= 0 ? 0 : (d + n.rec_every - 1) / n.rec_every;
+            next = n.rec_start + q * n.rec_every - T0;
+            slot = q % n.rec_cap;
+        }
+    }
+    __device__ __forceinline__ void advance(const DevNet& n) {
+        next += n.rec_every;
+        slot = slot + 1 == n.rec_cap ? 0 : slot + 1;
+    }
+};
+
+__device__ inline void record_astro(const DevNet& n, uint32_t a, int64_t slot, double Ca,
+                                    double IP3) {
+    if (a < n.rec_n) {
+        double* rec = n.rec_buf + static_cast<size_t>(slot) * 2 * n.rec_n;
         rec[a] = Ca;
         rec[n.rec_n + a] = IP3;
     }
@@ -250,6 +270,20 @@ __device__ inline int window_len(const DevNet& n, int32_t Wg) {
     return rem <= 0 ? 0 : (rem < Wg ? static_cast<int>(rem) : Wg);
 }
 
+// The windows carry every step-periodic quantity as a counter from T0 instead
+// of dividing by the period each step: the ring row T % S, and the window
+// offset of the next step T >= origin with (T - origin) % period == 0.
+__device__ __forceinline__ int ring_next(int r, int S) { return r + 1 == S ? 0 : r + 1; }
+
+__device__ __forceinline__ int64_t first_hit(int64_t T0, int64_t origin, int64_t period) {
+    const int64_t d = T0 - origin;
+    if (d <= 0) {
+        return -d;
+    }
+    const int64_t r = d % period;
+    return r == 0 ? 0 : period - r;
+}
+
 __device__ inline void add_i64(int64_t* p, int64_t v) {
     atomicAdd(reinterpret_cast<unsigned long long*>(p), static_cast<unsigned long long>(v));
 }
@@ -277,6 +311,7 @@ __device__ __forceinline__ void neuron_window(const DevNet& n, uint32_t i, int64
     const int32_t* __restrict__ drive_n = n.drive_n;
     const double* __restrict__ drive_noise = n.drive_noise;
 
+    int row = static_cast<int>(T0 % S);  // ring row of the current step
     double V = 0, w = 0, gex = 0, dgex = 0, gin = 0, dgin = 0, Isic = 0, Vr = 0, b = 0;
     int refr = 0;
     double ex_n = 0, ih_n = 0, sic_n = 0;  // ring values of the next step
@@ -294,7 +329,7 @@ __device__ __forceinline__ void neuron_window(const DevNet& n, uint32_t i, int64
         refr = n.refr[i];
         Vr = n.V_reset[i];
         b = n.b[i];
-        const size_t s0 = static_cast<size_t>(T0 % S) * N + i;
+        const size_t s0 = static_cast<size_t>(row) * N + i;
         ex_n = ring_exc[s0];
         ih_n = ring_inh[s0];
         sic_n = ring_sic[s0];
@@ -302,16 +337,21 @@ __device__ __forceinline__ void neuron_window(const DevNet& n, uint32_t i, int64
         if (has_noise) noise_n = drive_noise[i];
     }
     const uint32_t lane = threadIdx.x & 63;
+    int64_t sic_next = first_hit(T0, d_a2n, sic_interval);
     for (int k = 0; k < W; ++k) {
-        const int64_t T = T0 + k;
         bool spiked = false;
+        const int row_n = ring_next(row, S);
+        const bool sic_step = k == sic_next;
+        if (sic_step) {
+            sic_next += sic_interval;
+        }
         if (valid) {
-            const size_t slot = static_cast<size_t>(T % S) * N + i;
+            const size_t slot = static_cast<size_t>(row) * N + i;
             const double ex = ex_n, ih = ih_n, sic = sic_n;
             const int ev = ev_n;
             const double noise = noise_n;
             if (k + 1 < W) {
-                const size_t nx = static_cast<size_t>((T + 1) % S) * N + i;
+                const size_t nx = static_cast<size_t>(row_n) * N + i;
                 ex_n = ring_exc[nx];
                 ih_n = ring_inh[nx];
                 sic_n = ring_sic[nx];
@@ -340,7 +380,7 @@ __device__ __forceinline__ void neuron_window(const DevNet& n, uint32_t i, int64
             }
             // A slot carries a new SIC exactly when an exchange was sent
             // d_a2n steps earlier; otherwise the last value is held.
-            if (T >= d_a2n && (T - d_a2n) % sic_interval == 0) {
+            if (sic_step) {
                 Isic = sic;
             }
             // NeuronPopulation::update, neuron.cpp:143-170 (draws from k_drive)
@@ -364,6 +404,7 @@ __device__ __forceinline__ void neuron_window(const DevNet& n, uint32_t i, int64
         if (lane == 0 && (i & ~63u) < N) {
             spk_bits[static_cast<size_t>(k) * words + (i >> 6)] = mask;
         }
+        row = row_n;
     }
     if (valid) {
         n.V[i] = V;
@@ -393,16 +434,24 @@ __device__ __forceinline__ void astro_window(const DevNet& n, uint32_t a, int64_
     double Ca = n.Ca[a], IP3 = n.IP3[a], hh = n.hh[a];
     const double Ca_tot = n.Ca_tot[a], IP3_0 = n.IP3_0[a], tau = n.tau_IP3[a],
                  delta = n.delta_IP3[a];
-    double v_n = ring_astro[static_cast<size_t>(T0 % S) * NA + a];
-    int64_t noise_idx = -1;
+    int row = static_cast<int>(T0 % S);  // ring row of the current step
+    double v_n = ring_astro[static_cast<size_t>(row) * NA + a];
+    // noise index T / steps_per_noise, redrawn at the window start and at
+    // every multiple of steps_per_noise
+    const int64_t spn = ap.in.steps_per_noise;
+    int64_t noise_idx = T0 / spn;
+    int64_t noise_next = first_hit(T0 + 1, 0, spn) + 1;
+    int64_t sic_next = first_hit(T0, 0, sic_interval);
+    RecCursor rec(n, T0);
     double noise = 0.0;
     uint32_t live_bits = 0;
     for (int k = 0; k < W; ++k) {
         const int64_t T = T0 + k;
-        const size_t slot = static_cast<size_t>(T % S) * NA + a;
+        const int row_n = ring_next(row, S);
+        const size_t slot = static_cast<size_t>(row) * NA + a;
         const double v = v_n;
         if (k + 1 < W) {
-            v_n = ring_astro[static_cast<size_t>((T + 1) % S) * NA + a];
+            v_n = ring_astro[static_cast<size_t>(row_n) * NA + a];
         }
         double ip3_in = 0.0;
         if (v != 0.0) {
@@ -420,24 +469,33 @@ __device__ __forceinline__ void astro_window(const DevNet& n, uint32_t a, int64_
         }
         // AstrocytePopulation::update noise, astrocyte.cpp:239-248, 310-314
         if (ap.in.noise_std > 0.0) {
-            const int64_t idx = T / ap.in.steps_per_noise;
-            if (idx != noise_idx) {
-                noise_idx = idx;
+            bool redraw = k == 0;
+            if (k == noise_next) {
+                ++noise_idx;
+                noise_next += spn;
+                redraw = true;
+            }
+            if (redraw) {
                 const uint64_t nseed = seed ^ kSaltAstroNoise;
                 const uint64_t stream = ap.in.independent
-                                            ? static_cast<uint64_t>(idx) * kStreamStride + a
-                                            : static_cast<uint64_t>(idx);
+                                            ? static_cast<uint64_t>(noise_idx) * kStreamStride + a
+                                            : static_cast<uint64_t>(noise_idx);
                 noise = ap.in.noise_std * rng_normal_clamped(nseed, stream);
             }
         }
         astro_step(ap.k, Ca_tot, IP3_0, tau, delta, ip3_in, noise, hs, half_h, h_sixth, substeps,
                    Ca, IP3, hh);
-        record_astro(n, a, T, Ca, IP3);
-        if (T % sic_interval == 0) {
+        if (k == rec.next) {
+            record_astro(n, a, rec.slot, Ca, IP3);
+            rec.advance(n);
+        }
+        if (k == sic_next) {
+            sic_next += sic_interval;
             const double f = sic_factor(Ca, ap.SIC_th, ap.SIC_scale);
             fac[static_cast<size_t>(k) * NA + a] = f;
             live_bits |= static_cast<uint32_t>(f != 0.0) << k;
         }
+        row = row_n;
     }
     n.Ca[a] = Ca;
     n.IP3[a] = IP3;
@@ -468,6 +526,7 @@ __device__ __forceinline__ void neuron_window_f32(const DevNet& n, uint32_t i, i
     const int32_t* __restrict__ drive_n = n.drive_n;
     const float* __restrict__ drive_noise = n.fdrive_noise;
 
+    int row = static_cast<int>(T0 % S);  // ring row of the current step
     float V = 0, w = 0, gex = 0, dgex = 0, gin = 0, dgin = 0, Isic = 0, Vr = 0, b = 0;
     int refr = 0;
     float ex_n = 0, ih_n = 0, sic_n = 0;
@@ -485,7 +544,7 @@ __device__ __forceinline__ void neuron_window_f32(const DevNet& n, uint32_t i, i
         refr = n.refr[i];
         Vr = static_cast<float>(n.V_reset[i]);
         b = static_cast<float>(n.b[i]);
-        const size_t s0 = static_cast<size_t>(T0 % S) * N + i;
+        const size_t s0 = static_cast<size_t>(row) * N + i;
         ex_n = ring_exc[s0];
         ih_n = ring_inh[s0];
         sic_n = ring_sic[s0];
@@ -493,16 +552,21 @@ __device__ __forceinline__ void neuron_window_f32(const DevNet& n, uint32_t i, i
         if (has_noise) noise_n = drive_noise[i];
     }
     const uint32_t lane = threadIdx.x & 63;
+    int64_t sic_next = first_hit(T0, d_a2n, sic_interval);
     for (int k = 0; k < W; ++k) {
-        const int64_t T = T0 + k;
         bool spiked = false;
+        const int row_n = ring_next(row, S);
+        const bool sic_step = k == sic_next;
+        if (sic_step) {
+            sic_next += sic_interval;
+        }
         if (valid) {
-            const size_t slot = static_cast<size_t>(T % S) * N + i;
+            const size_t slot = static_cast<size_t>(row) * N + i;
             const float ex = ex_n, ih = ih_n, sic = sic_n;
             const int ev = ev_n;
             const float noise = noise_n;
             if (k + 1 < W) {
-                const size_t nx = static_cast<size_t>((T + 1) % S) * N + i;
+                const size_t nx = static_cast<size_t>(row_n) * N + i;
                 ex_n = ring_exc[nx];
                 ih_n = ring_inh[nx];
                 sic_n = ring_sic[nx];
@@ -527,7 +591,7 @@ __device__ __forceinline__ void neuron_window_f32(const DevNet& n, uint32_t i, i
                 }
                 ring_inh[slot] = 0.0f;
             }
-            if (T >= d_a2n && (T - d_a2n) % sic_interval == 0) {
+            if (sic_step) {
                 Isic = sic;
             }
             // Same counter streams as the FP64 step (k_drive): identical
@@ -556,6 +620,7 @@ __device__ __forceinline__ void neuron_window_f32(const DevNet& n, uint32_t i, i
         if (lane == 0 && (i & ~63u) < N) {
             spk_bits[static_cast<size_t>(k) * words + (i >> 6)] = mask;
         }
+        row = row_n;
     }
     if (valid) {
         n.fV[i] = V;
@@ -588,16 +653,24 @@ __device__ __forceinline__ void astro_window_f32(const DevNet& n, uint32_t a, in
     float Ca = n.fCa[a], IP3 = n.fIP3[a], hh = n.fhh[a];
     const float Ca_tot = n.fCa_tot[a], IP3_0 = n.fIP3_0[a], inv_tau = n.finv_tau[a],
                 delta = n.fdelta[a];
-    float v_n = ring_astro[static_cast<size_t>(T0 % S) * NA + a];
-    int64_t noise_idx = -1;
+    int row = static_cast<int>(T0 % S);  // ring row of the current step
+    float v_n = ring_astro[static_cast<size_t>(row) * NA + a];
+    // noise index T / steps_per_noise, redrawn at the window start and at
+    // every multiple of steps_per_noise
+    const int64_t spn = ain.steps_per_noise;
+    int64_t noise_idx = T0 / spn;
+    int64_t noise_next = first_hit(T0 + 1, 0, spn) + 1;
+    int64_t sic_next = first_hit(T0, 0, sic_interval);
+    RecCursor rec(n, T0);
     float noise = 0.0f;
     uint32_t live_bits = 0;
     for (int k = 0; k < W; ++k) {
         const int64_t T = T0 + k;
-        const size_t slot = static_cast<size_t>(T % S) * NA + a;
+        const int row_n = ring_next(row, S);
+        const size_t slot = static_cast<size_t>(row) * NA + a;
         const float v = v_n;
         if (k + 1 < W) {
-            v_n = ring_astro[static_cast<size_t>((T + 1) % S) * NA + a];
+            v_n = ring_astro[static_cast<size_t>(row_n) * NA + a];
         }
         float ip3_in = 0.0f;
         if (v != 0.0f) {
@@ -613,24 +686,33 @@ __device__ __forceinline__ void astro_window_f32(const DevNet& n, uint32_t a, in
             }
         }
         if (ain.noise_std > 0.0) {
-            const int64_t idx = T / ain.steps_per_noise;
-            if (idx != noise_idx) {
-                noise_idx = idx;
+            bool redraw = k == 0;
+            if (k == noise_next) {
+                ++noise_idx;
+                noise_next += spn;
+                redraw = true;
+            }
+            if (redraw) {
                 const uint64_t stream = ain.independent
-                                            ? static_cast<uint64_t>(idx) * kStreamStride + a
-                                            : static_cast<uint64_t>(idx);
+                                            ? static_cast<uint64_t>(noise_idx) * kStreamStride + a
+                                            : static_cast<uint64_t>(noise_idx);
                 noise = static_cast<float>(ain.noise_std *
                                            rng_normal_clamped(seed ^ kSaltAstroNoise, stream));
             }
         }
         astro_step_f32(kf, Ca_tot, IP3_0, inv_tau, delta, ip3_in, noise, hs, half_h, h_sixth,
                        substeps, Ca, IP3, hh);
-        record_astro(n, a, T, Ca, IP3);
-        if (T % sic_interval == 0) {
+        if (k == rec.next) {
+            record_astro(n, a, rec.slot, Ca, IP3);
+            rec.advance(n);
+        }
+        if (k == sic_next) {
+            sic_next += sic_interval;
             const float f = sic_factor_mixed(Ca, SIC_th, SIC_scale);
             fac[static_cast<size_t>(k) * NA + a] = f;
             live_bits |= static_cast<uint32_t>(f != 0.0f) << k;
         }
+        row = row_n;
     }
     n.fCa[a] = Ca;
     n.fIP3[a] = IP3;

@@ -180,19 +180,23 @@ __device__ inline bool adex_substep_mixed(const NeuronKf& pf, const NeuronK& pd,
     return false;
 }
 
+// a / b through the hardware reciprocal (v_rcp_f32, 1 ulp): the quotients of
+// the FP32 astrocyte RHS do not need the correctly rounded divide.
+__device__ __forceinline__ float rcp_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
 __device__ inline void astro_rhs_f32(const AstroKf& c, float Ca_tot, float IP3_0, float inv_tau,
                                      float Ca, float IP3, float h, float noise, float& dCa,
                                      float& dIP3, float& dh) {
     const float Ca_ER = (Ca_tot - Ca) * c.inv_ratio_ER_cyt;
-    const float m_inf = __fdividef(IP3, IP3 + c.Kd_IP3_1);
-    const float n_inf = __fdividef(Ca, Ca + c.Kd_act);
+    const float m_inf = rcp_div(IP3, IP3 + c.Kd_IP3_1);
+    const float n_inf = rcp_div(Ca, Ca + c.Kd_act);
     const float m3 = m_inf * m_inf * m_inf;
     const float n3 = n_inf * n_inf * n_inf;
     const float h3 = h * h * h;
     const float J_channel = c.ratio_rate_IP3R * m3 * n3 * h3 * (Ca_ER - Ca);
-    const float J_pump = __fdividef(c.rate_SERCA * Ca * Ca, c.Km_SERCA_sq + Ca * Ca);
+    const float J_pump = rcp_div(c.rate_SERCA * Ca * Ca, c.Km_SERCA_sq + Ca * Ca);
     const float J_leak = c.ratio_rate_L * (Ca_ER - Ca);
-    const float alpha_h = __fdividef(c.k_Kd_inh * (IP3 + c.Kd_IP3_1), IP3 + c.Kd_IP3_2);
+    const float alpha_h = rcp_div(c.k_Kd_inh * (IP3 + c.Kd_IP3_1), IP3 + c.Kd_IP3_2);
     const float beta_h = c.k_IP3R * Ca;
     dCa = J_channel - J_pump + J_leak + noise;
     dIP3 = (IP3_0 - IP3) * inv_tau;
