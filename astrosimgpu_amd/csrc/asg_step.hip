@@ -696,8 +696,8 @@ __device__ __forceinline__ void astro_window_f32(const DevNet& n, uint32_t a, in
                 const uint64_t stream = ain.independent
                                             ? static_cast<uint64_t>(noise_idx) * kStreamStride + a
                                             : static_cast<uint64_t>(noise_idx);
-                noise = static_cast<float>(ain.noise_std *
-                                           rng_normal_clamped(seed ^ kSaltAstroNoise, stream));
+                noise = static_cast<float>(ain.noise_std) *
+                        rng_normal_clamped_f32(seed ^ kSaltAstroNoise, stream);
             }
         }
         astro_step_f32(kf, Ca_tot, IP3_0, inv_tau, delta, ip3_in, noise, hs, half_h, h_sixth,
@@ -885,24 +885,35 @@ __global__ __launch_bounds__(64) void k_drive(DevNet* nets, const CellBlock* nbl
     if (valid && in.noise_std > 0.0) {
         // One draw per noise interval (neuron.cpp:152-166), held across it.
         const uint64_t salt_shared = pi == 0 ? kSaltNoiseSharedExc : kSaltNoiseSharedInh;
-        const bool f32 = n.precision == ASG_FP32;
-        int64_t noise_idx = -1;
-        double z = 0.0;
-        for (int k = 0; k < W; ++k) {
-            const int64_t idx = (T0 + k) / in.steps_per_noise;
-            if (idx != noise_idx) {
-                noise_idx = idx;
-                uint64_t ctr = 0;
-                z = in.independent
-                        ? rng_normal_fresh(seed ^ kSaltNeuronNoise,
-                                           static_cast<uint64_t>(idx) * kNoiseStride + i, ctr)
-                        : rng_normal_fresh(seed ^ salt_shared, static_cast<uint64_t>(idx), ctr);
+        const uint64_t nseed = in.independent ? seed ^ kSaltNeuronNoise : seed ^ salt_shared;
+        const int64_t spn = in.steps_per_noise;
+        int64_t idx = T0 / spn;
+        int64_t next = first_hit(T0 + 1, 0, spn) + 1;  // offset of the next multiple of spn
+        auto stream_of = [&](int64_t q) {
+            return in.independent ? static_cast<uint64_t>(q) * kNoiseStride + i
+                                  : static_cast<uint64_t>(q);
+        };
+        if (n.precision == ASG_FP32) {
+            // FP32 path: the same uniforms through the FP32 Box-Muller.
+            const float sd = static_cast<float>(in.noise_std);
+            float z = rng_normal_fresh_f32(nseed, stream_of(idx));
+            for (int k = 0; k < W; ++k) {
+                if (k == next) {
+                    next += spn;
+                    z = rng_normal_fresh_f32(nseed, stream_of(++idx));
+                }
+                n.fdrive_noise[static_cast<size_t>(k) * N + i] = sd * z;
             }
-            const size_t dk = static_cast<size_t>(k) * N + i;
-            if (f32) {
-                n.fdrive_noise[dk] = static_cast<float>(in.noise_std * z);
-            } else {
-                n.drive_noise[dk] = 0.0 + in.noise_std * z;
+        } else {
+            uint64_t ctr = 0;
+            double z = rng_normal_fresh(nseed, stream_of(idx), ctr);
+            for (int k = 0; k < W; ++k) {
+                if (k == next) {
+                    next += spn;
+                    ctr = 0;
+                    z = rng_normal_fresh(nseed, stream_of(++idx), ctr);
+                }
+                n.drive_noise[static_cast<size_t>(k) * N + i] = 0.0 + in.noise_std * z;
             }
         }
     }
@@ -1102,107 +1113,175 @@ __device__ inline void mark_row(const uint32_t* __restrict__ tgt, uint32_t q0, u
     }
 }
 
-// The spikes one wavefront's 64 neurons emitted at window step k: each
-// spike's bit (its index in the step's spike list) is set in the per-target
-// mask of every synapse of its rows. OR is order-independent, so
-// wavefronts may mark in any order.
-__device__ void mark_step(const DevNet& n, uint32_t first, uint32_t lane, int k) {
-    const uint32_t N = n.n_neur;
-    const uint32_t word_i = first >> 6;
-    const uint64_t word = n.spk_bits[static_cast<size_t>(k) * n.words + word_i];
-    if (word == 0) {
+// Lane k's value of v; k must be uniform across the wavefront.
+__device__ __forceinline__ uint32_t lane_u32(uint32_t v, int k) {
+    return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), k));
+}
+__device__ __forceinline__ uint64_t lane_u64(uint64_t v, int k) {
+    return static_cast<uint64_t>(lane_u32(static_cast<uint32_t>(v), k)) |
+           (static_cast<uint64_t>(lane_u32(static_cast<uint32_t>(v >> 32), k)) << 32);
+}
+
+// What one wavefront's 64 neurons spiked at window steps [k0, k0 + nk),
+// nk <= 64, loaded in one round: lane kk holds step k0 + kk's spike word of
+// the block, the spikes of the step before that word, and the step's
+// total and excitatory spike counts.
+struct StepWords {
+    uint64_t word;
+    uint32_t off, tot, nE;
+    __device__ __forceinline__ StepWords(const DevNet& n, uint32_t word_i, uint32_t lane, int k0,
+                                         int nk) {
+        word = 0;
+        off = tot = nE = 0;
+        if (lane < static_cast<uint32_t>(nk)) {
+            const size_t q = static_cast<size_t>(k0 + static_cast<int>(lane)) * n.words + word_i;
+            word = n.spk_bits[q];
+            off = n.spk_woff[q];
+            tot = n.spk_n[2 * (k0 + static_cast<int>(lane))];
+            nE = n.spk_n[2 * (k0 + static_cast<int>(lane)) + 1];
+        }
+    }
+};
+
+// The spikes one wavefront's 64 neurons emitted at window steps [k0, k1):
+// each spike's bit (its index in its step's spike list) is set in the
+// per-target mask of every synapse of its rows. OR is order-independent, so
+// wavefronts may mark in any order. The steps' words and the spiking
+// neurons' row bounds are loaded once, up front.
+__device__ void mark_steps(const DevNet& n, uint32_t first, uint32_t lane, int k0, int k1) {
+    const uint32_t N = n.n_neur, NE = n.n_exc;
+    const int nk = k1 - k0;
+    const StepWords sw(n, first >> 6, lane, k0, nk);
+    uint64_t any = 0;
+    for (int kk = 0; kk < nk; ++kk) {
+        any |= lane_u64(sw.word, kk);
+    }
+    if (any == 0) {
         return;
     }
-    const uint32_t off = n.spk_woff[static_cast<size_t>(k) * n.words + word_i];
-    const uint32_t nE = n.spk_n[2 * k + 1];
-    uint64_t m = word;
-    while (m) {
-        const uint32_t L = static_cast<uint32_t>(__ffsll(static_cast<long long>(m)) - 1);
-        m &= m - 1;
-        const uint32_t s = first + L;
-        const uint32_t js = off + static_cast<uint32_t>(__popcll(word & ((1ULL << L) - 1ULL)));
-        if (s < n.n_exc) {
-            const uint64_t bit = 1ULL << (js & 63);
-            if ((js >> 6) >= n.cE) {
-                continue;  // beyond the delivery capacity; k_compact flagged it
-            }
-            const size_t chunk = static_cast<size_t>(k) * n.cE + (js >> 6);
-            mark_row(n.exc_tgt, n.exc_row[s], n.exc_row[s + 1], n.mask_exc + chunk * N, bit, lane);
-            mark_row(n.n2a_tgt, n.n2a_row[s], n.n2a_row[s + 1], n.mask_astro + chunk * n.n_astro,
-                     bit, lane);
+    const uint32_t s_lane = first + lane;
+    uint32_t r0 = 0, r1 = 0, a0 = 0, a1 = 0;
+    if ((any >> lane) & 1ULL) {
+        if (s_lane < NE) {
+            r0 = n.exc_row[s_lane];
+            r1 = n.exc_row[s_lane + 1];
+            a0 = n.n2a_row[s_lane];
+            a1 = n.n2a_row[s_lane + 1];
         } else {
-            const uint32_t jj = js - nE;
-            const uint64_t bit = 1ULL << (jj & 63);
-            if ((jj >> 6) >= n.cI) {
-                continue;
+            r0 = n.inh_row[s_lane - NE];
+            r1 = n.inh_row[s_lane - NE + 1];
+        }
+    }
+    for (int kk = 0; kk < nk; ++kk) {
+        const uint64_t word = lane_u64(sw.word, kk);
+        if (word == 0) {
+            continue;
+        }
+        const uint32_t off = lane_u32(sw.off, kk), nE = lane_u32(sw.nE, kk);
+        const int k = k0 + kk;
+        uint64_t m = word;
+        while (m) {
+            const int L = __ffsll(static_cast<long long>(m)) - 1;
+            m &= m - 1;
+            const uint32_t s = first + static_cast<uint32_t>(L);
+            const uint32_t js = off + static_cast<uint32_t>(__popcll(word & ((1ULL << L) - 1ULL)));
+            const uint32_t q0 = lane_u32(r0, L), q1 = lane_u32(r1, L);
+            if (s < NE) {
+                if ((js >> 6) >= n.cE) {
+                    continue;  // beyond the delivery capacity; k_compact flagged it
+                }
+                const uint64_t bit = 1ULL << (js & 63);
+                const size_t chunk = static_cast<size_t>(k) * n.cE + (js >> 6);
+                mark_row(n.exc_tgt, q0, q1, n.mask_exc + chunk * N, bit, lane);
+                mark_row(n.n2a_tgt, lane_u32(a0, L), lane_u32(a1, L),
+                         n.mask_astro + chunk * n.n_astro, bit, lane);
+            } else {
+                const uint32_t jj = js - nE;
+                if ((jj >> 6) >= n.cI) {
+                    continue;
+                }
+                const uint64_t bit = 1ULL << (jj & 63);
+                const size_t chunk = static_cast<size_t>(k) * n.cI + (jj >> 6);
+                mark_row(n.inh_tgt, q0, q1, n.mask_inh + chunk * N, bit, lane);
             }
-            const size_t chunk = static_cast<size_t>(k) * n.cI + (jj >> 6);
-            const uint32_t r = s - n.n_exc;
-            mark_row(n.inh_tgt, n.inh_row[r], n.inh_row[r + 1], n.mask_inh + chunk * N, bit, lane);
         }
     }
 }
 
 // For the 64 neurons of one wavefront, over the whole window in time order:
 // each spike's Tsodyks-Markram update (network.cpp:230-247), the weights its
-// rows deliver and its raster entry.
+// rows deliver and its raster entry. The window's spike words come in one
+// round; a spiking neuron's plasticity state and row lengths are loaded
+// once and its state stored once.
 __device__ void deliver_sources(DevNet& n, uint32_t first, uint32_t lane) {
     const uint32_t N = n.n_neur;
     const uint32_t t = first + lane;
     const bool valid = t < N;
     const int W = n.win_W;
     const int64_t T0 = n.win_T0;
-    const uint32_t words = n.words;
-    const uint32_t word_i = first >> 6;
-    int64_t window_total = 0;
+    const int64_t r_head = n.r_head;
+    const StepWords sw(n, first >> 6, lane, 0, W);
+    uint32_t window_total = 0;
+    const uint32_t before_k = wave_exclusive_scan(sw.tot, window_total);  // spikes of earlier steps
+    const bool room = r_head + static_cast<int64_t>(window_total) <= n.r_cap;
+    uint32_t mine = 0;  // bit k: this lane's neuron spiked at window step k
     for (int k2 = 0; k2 < W; ++k2) {
-        window_total += n.spk_n[2 * k2];
+        mine |= static_cast<uint32_t>((lane_u64(sw.word, k2) >> lane) & 1ULL) << k2;
     }
-    const bool room = n.r_head + window_total <= n.r_cap;
+    const bool exc = t < n.n_exc;
+    const bool stp = n.stp_on != 0;
+    int64_t row_syn = 0;
+    double x = 0.0, u = 0.0, tl = 0.0;
+    if (valid && mine != 0) {
+        if (exc) {
+            row_syn = (n.exc_row[t + 1] - n.exc_row[t]) + (n.n2a_raw_row[t + 1] - n.n2a_raw_row[t]);
+        } else {
+            row_syn = n.inh_row[t - n.n_exc + 1] - n.inh_row[t - n.n_exc];
+        }
+        if (stp) {
+            x = n.stp_x[t];
+            u = n.stp_u[t];
+            tl = n.stp_tl[t];
+        }
+    }
     const uint64_t below = (1ULL << lane) - 1ULL;
-    int64_t before = 0;
     int64_t syn = 0;
     for (int k2 = 0; k2 < W; ++k2) {
-        const uint64_t word = n.spk_bits[static_cast<size_t>(k2) * words + word_i];
-        const uint32_t tot = n.spk_n[2 * k2];
-        if (word != 0) {
-            const uint32_t off = n.spk_woff[static_cast<size_t>(k2) * words + word_i];
-            if (valid && ((word >> lane) & 1ULL)) {
-                const uint32_t j = off + static_cast<uint32_t>(__popcll(word & below));
-                const bool exc = t < n.n_exc;
-                double c0 = exc ? n.w_exc : n.w_inh;
-                double c1 = n.w_n2a;
-                const double t_now = static_cast<double>(T0 + k2) * n.dt;
-                if (n.stp_on) {
-                    const double dts = t_now - n.stp_tl[t];
-                    const double x_decay = asg_exp(-dts / n.tau_rec);
-                    const double u_decay = n.fac_on ? asg_exp(-dts / n.tau_fac) : 0.0;
-                    double x = n.stp_x[t];
-                    double u = n.stp_u[t];
-                    x = 1.0 + (x - x * u - 1.0) * x_decay;
-                    u = n.U + u * n.one_minus_U * u_decay;
-                    n.stp_x[t] = x;
-                    n.stp_u[t] = u;
-                    n.stp_tl[t] = t_now;
-                    c0 = c0 * x * u;
-                    c1 = c1 * x * u;
-                }
-                double* cv = n.spk_c + (static_cast<size_t>(k2) * N + j) * 2;
-                cv[0] = c0;
-                cv[1] = c1;
-                if (exc) {
-                    syn += (n.exc_row[t + 1] - n.exc_row[t]) +
-                           (n.n2a_raw_row[t + 1] - n.n2a_raw_row[t]);
-                } else {
-                    syn += n.inh_row[t - n.n_exc + 1] - n.inh_row[t - n.n_exc];
-                }
-                if (room) {
-                    n.r_step[n.r_head + before + j] = T0 + k2;
-                    n.r_src[n.r_head + before + j] = t;
-                }
+        const uint64_t word = lane_u64(sw.word, k2);
+        if (word == 0) {
+            continue;
+        }
+        const uint32_t off = lane_u32(sw.off, k2);
+        const uint32_t before = lane_u32(before_k, k2);
+        if ((mine >> k2) & 1u) {
+            const uint32_t j = off + static_cast<uint32_t>(__popcll(word & below));
+            double c0 = exc ? n.w_exc : n.w_inh;
+            double c1 = n.w_n2a;
+            const double t_now = static_cast<double>(T0 + k2) * n.dt;
+            if (stp) {
+                const double dts = t_now - tl;
+                const double x_decay = asg_exp(-dts / n.tau_rec);
+                const double u_decay = n.fac_on ? asg_exp(-dts / n.tau_fac) : 0.0;
+                x = 1.0 + (x - x * u - 1.0) * x_decay;
+                u = n.U + u * n.one_minus_U * u_decay;
+                tl = t_now;
+                c0 = c0 * x * u;
+                c1 = c1 * x * u;
+            }
+            double* cv = n.spk_c + (static_cast<size_t>(k2) * N + j) * 2;
+            cv[0] = c0;
+            cv[1] = c1;
+            syn += row_syn;
+            if (room) {
+                n.r_step[r_head + before + j] = T0 + k2;
+                n.r_src[r_head + before + j] = t;
             }
         }
-        before += tot;
+    }
+    if (valid && mine != 0 && stp) {
+        n.stp_x[t] = x;
+        n.stp_u[t] = u;
+        n.stp_tl[t] = tl;
     }
     if (syn > 0) {
         add_i64(&n.c_syn, syn);
@@ -1249,8 +1328,8 @@ __global__ __launch_bounds__(256) void k_deliver(DevNet* nets, const CellBlock* 
     }
     const int k0 = k * MG;
     const int k1 = min(k0 + MG, n.win_W);
-    for (int kk = k0; kk < k1; ++kk) {
-        mark_step(n, cb.first, lane, kk);
+    if (k1 > k0) {
+        mark_steps(n, cb.first, lane, k0, k1);
     }
     if (k0 == 0 && n.win_W > 0) {
         deliver_sources(n, cb.first, lane);
@@ -2063,7 +2142,9 @@ struct Engine {
     bool pending = false;  // async work in flight
     // Per-kernel event timing (off unless asked for).
     bool profiling = false;
-    std::vector<hipEvent_t> events;   // 5 per window of the last step() call
+    int prof_every = 1;               // time one window in prof_every
+    int64_t prof_seq = 0;             // windows launched while profiling
+    std::vector<hipEvent_t> events;   // 6 per timed window of the last step() call
     int64_t timed_windows = 0, pending_windows = 0;
     // [0] cells, [1] compaction, [2] delivery pass, [3] ordered sums, [4] drive
     double kernel_ms[5] = {0, 0, 0, 0, 0};
@@ -2319,34 +2400,40 @@ struct Engine {
 
     // The launches of `windows` windows on the stream (eager, or recorded
     // into a graph when the stream is capturing).
-    void launch_windows(int64_t windows, int G, int wpb, int SG, bool marks) {
+    // Launches `windows` windows; with `marks`, one window in prof_every is
+    // bracketed by events (6 per timed window). Returns the windows timed.
+    int64_t launch_windows(int64_t windows, int G, int wpb, int SG, bool marks) {
         const int deliver_items = n_nblocks * ((Wg + SG - 1) / SG + (Wg + G - 1) / G);
         const int sum_items = n_blocks * ((Wg + SG - 1) / SG);
-        auto mark = [&](int64_t wdx, int k) {
-            if (marks) HIP_CHECK(hipEventRecord(events[6 * wdx + k], stream));
-        };
+        int64_t timed = 0;
         for (int64_t wdx = 0; wdx < windows; ++wdx) {
-            mark(wdx, 0);
+            const bool timed_here = marks && (prof_seq++ % prof_every) == 0;
+            auto mark = [&](int k) {
+                if (timed_here) HIP_CHECK(hipEventRecord(events[6 * timed + k], stream));
+            };
+            mark(0);
             if (n_nblocks > 0) {
                 hipLaunchKernelGGL(k_drive, dim3(n_nblocks), dim3(64),
                                    static_cast<size_t>(64 * Wg) * (sizeof(double) + sizeof(uint32_t)),
                                    stream, d_nets, d_nblocks, n_nblocks, Wg);
             }
-            mark(wdx, 1);
+            mark(1);
             launch_cells<ASG_FP64>(0, Wg, wpb);
             launch_cells<ASG_FP32>(1, Wg, wpb);
-            mark(wdx, 2);
+            mark(2);
             hipLaunchKernelGGL(k_compact, dim3(n_active, Wg), dim3(64), 0, stream, d_nets,
                                d_active);
-            mark(wdx, 3);
+            mark(3);
             if (deliver_items > 0) {
                 launch_deliver(G, deliver_items, Wg, SG);
             }
-            mark(wdx, 4);
+            mark(4);
             hipLaunchKernelGGL(k_sum, dim3((sum_items + 3) / 4), dim3(256), 0, stream, d_nets,
                                d_blocks, sum_items, Wg, SG);
-            mark(wdx, 5);
+            mark(5);
+            if (timed_here) ++timed;
         }
+        return timed;
     }
 
     // Graph of `windows` windows for the current tables and schedule,
@@ -2402,8 +2489,7 @@ struct Engine {
         if (profiling) {
             // per-kernel events need eager launches
             ensure_events(windows);
-            launch_windows(windows, G, sched_cell_waves, SG, true);
-            pending_windows = windows;
+            pending_windows = launch_windows(windows, G, sched_cell_waves, SG, true);
         } else if (use_graphs && windows > 0) {
             HIP_CHECK(hipGraphLaunch(graph_for(windows, G, sched_cell_waves, SG), stream));
         } else {
@@ -3059,10 +3145,13 @@ asg_status kernel_times(Engine& e, asg_kernel_times* out) {
         return ASG_OK;
     });
 }
-asg_status set_profiling(Engine& e, int32_t on) {
+asg_status set_profiling(Engine& e, int32_t every) {
+    if (every < 0) return fail(ASG_ERR_INVALID, "profiling period must be >= 0");
     return guarded([&]() -> asg_status {
         e.wait();
-        e.profiling = on != 0;
+        e.profiling = every != 0;
+        e.prof_every = every > 0 ? every : 1;
+        e.prof_seq = 0;
         for (double& v : e.kernel_ms) v = 0.0;
         e.timed_windows = 0;
         return ASG_OK;

@@ -11,12 +11,15 @@
 //    its result is rounded to FP32 at the end of the substep;
 //  * the astrocyte's Ca-threshold test of the SIC, (Ca - SIC_th) * 1000 > 1,
 //    is evaluated in FP64 (Ca is a small difference above a threshold).
+// Random draws use the FP64 step's counter streams: identical Poisson counts,
+// and normals from the same two uniforms through FP32 hardware functions.
 // Synaptic input is summed with Neumaier's compensated summation.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 #include "cell_math.h"
+#include "rng.h"
 
 namespace asg {
 
@@ -178,6 +181,33 @@ __device__ inline bool adex_substep_mixed(const NeuronKf& pf, const NeuronK& pd,
         return true;
     }
     return false;
+}
+
+// Box-Muller on the two uniforms the FP64 draw uses, with the hardware
+// log2, square root and cos(2 pi x) (v_log_f32, v_sqrt_f32, v_cos_f32, about
+// 1 ulp each; v_cos_f32 takes its argument in revolutions, so no range
+// reduction). u1 <= 1e-300 (probability 2^-53) takes log2(1e-300), as the
+// clamped FP64 draw does.
+__device__ __forceinline__ float box_muller_f32(double u1, double u2) {
+    constexpr float kLog2Of1em300 = -996.578428466208704f;
+    constexpr float kMinus2Ln2 = -1.386294361119890618f;
+    const float l2 = u1 <= 1e-300 ? kLog2Of1em300 : __builtin_amdgcn_logf(static_cast<float>(u1));
+    return __builtin_amdgcn_sqrtf(kMinus2Ln2 * l2) * __builtin_amdgcn_cosf(static_cast<float>(u2));
+}
+
+// rng_normal_fresh's draw (u1 redrawn while <= 1e-300) in FP32.
+__device__ __forceinline__ float rng_normal_fresh_f32(std::uint64_t seed, std::uint64_t stream) {
+    std::uint64_t counter = 0;
+    double u1;
+    do {
+        u1 = rng_uniform(seed, stream, counter++);
+    } while (u1 <= 1e-300);
+    return box_muller_f32(u1, rng_uniform(seed, stream, counter));
+}
+
+// rng_normal_clamped's draw (u1 clamped at 1e-300) in FP32.
+__device__ __forceinline__ float rng_normal_clamped_f32(std::uint64_t seed, std::uint64_t stream) {
+    return box_muller_f32(rng_uniform(seed, stream, 0), rng_uniform(seed, stream, 1));
 }
 
 // a / b through the hardware reciprocal (v_rcp_f32, 1 ulp): the quotients of

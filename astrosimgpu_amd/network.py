@@ -156,8 +156,9 @@ class DeviceNetwork:
         check(lib().asg_net_counters(self._h, C.byref(c)))
         return {name: getattr(c, name) for name, _ in capi.Counters._fields_}
 
-    def set_profiling(self, on: bool) -> None:
-        check(lib().asg_net_set_profiling(self._h, 1 if on else 0))
+    def set_profiling(self, on: bool, every: int = 1) -> None:
+        """Time the window kernels with HIP events, one window in `every`."""
+        check(lib().asg_net_set_profiling(self._h, int(every) if on else 0))
 
     def set_graphs(self, on: bool) -> None:
         check(lib().asg_net_set_graphs(self._h, 1 if on else 0))
@@ -175,7 +176,8 @@ class DeviceNetwork:
             self._h, out, st, cap, n), self._rec_n, max_records)
 
     def kernel_times(self) -> dict:
-        """Device milliseconds per kernel since profiling was switched on."""
+        """Device milliseconds per kernel, summed over the timed windows since
+        profiling was switched on."""
         t = capi.KernelTimes()
         check(lib().asg_net_kernel_times(self._h, C.byref(t)))
         return {"cells": t.ms[0], "compact": t.ms[1], "deliver": t.ms[2], "sum": t.ms[3],
@@ -255,8 +257,9 @@ class Batch:
         check(lib().asg_batch_counters(self._h, int(slot), C.byref(c)))
         return {name: getattr(c, name) for name, _ in capi.Counters._fields_}
 
-    def set_profiling(self, on: bool) -> None:
-        check(lib().asg_batch_set_profiling(self._h, 1 if on else 0))
+    def set_profiling(self, on: bool, every: int = 1) -> None:
+        """Time the window kernels with HIP events, one window in `every`."""
+        check(lib().asg_batch_set_profiling(self._h, int(every) if on else 0))
 
     def set_recording(self, slot: int, start_step: int, every: int, n_astro: int,
                       capacity: int) -> None:

@@ -435,15 +435,19 @@ asg_status asg_net_set_graphs(asg_net* net, int32_t on);
 /* Device time per kernel of the window pipeline, measured with HIP events on
  * the handle's own stream while profiling is on: [0] cell update, [1] spike
  * compaction, [2] SIC sums + plasticity + synapse marking, [3] ordered delivery
- * sums, [4] external drive draws (Poisson counts, noise current). */
+ * sums, [4] external drive draws (Poisson counts, noise current). Sums over
+ * the windows timed. */
 typedef struct asg_kernel_times {
     double ms[5];
     int64_t windows;   /* windows timed */
 } asg_kernel_times;
 
-asg_status asg_net_set_profiling(asg_net* net, int32_t on);
+/* every = 0 switches profiling off; every = n >= 1 times one window in n
+ * (the first window after the call, then every n-th), so the events' own
+ * cost stays off the other windows. Clears the accumulated times. */
+asg_status asg_net_set_profiling(asg_net* net, int32_t every);
 asg_status asg_net_kernel_times(asg_net* net, asg_kernel_times* out);
-asg_status asg_batch_set_profiling(asg_batch* batch, int32_t on);
+asg_status asg_batch_set_profiling(asg_batch* batch, int32_t every);
 asg_status asg_batch_kernel_times(asg_batch* batch, asg_kernel_times* out);
 
 enum { ASG_MATH_EXP = 0, ASG_MATH_LOG = 1, ASG_MATH_SIN = 2, ASG_MATH_COS = 3, ASG_MATH_SQRT = 4 };
