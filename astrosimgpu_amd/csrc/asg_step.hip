@@ -118,7 +118,7 @@ constexpr int kWindowCap = 16;     // longest window (bounds scratch memory)
 constexpr int64_t kSicWaves = 4096; // SIC wavefronts that fill the device (256 CUs x 16)
 constexpr int64_t kGroupWaves = 16384;  // per-step wavefronts beyond which steps are grouped
 constexpr size_t kMaskBudget = size_t(256) << 20;  // delivery masks per network
-constexpr size_t kMinChunks = 4;
+constexpr size_t kMinChunks = 64;
 
 struct InputK {
     double poiss_rate, poiss_weight, lambda, limit;
@@ -138,14 +138,14 @@ struct AstroPopK {
 struct DevNet {
     uint32_t n_exc, n_inh, n_neur, n_astro;
     uint32_t words;        // ceil(n_neur / 64)
-    uint32_t cE, cI;       // 64-spike chunks needed for all exc / inh sources
+    uint32_t cw;           // 64-spike chunks of window spike indices the masks hold
     int32_t S;             // ring slots: longest delay + longest window
     int32_t Wmax;          // longest window this network's scratch holds
     int32_t d_exc, d_inh, d_n2a, d_a2n;
     int32_t sic_interval, substeps;
     int32_t win_W;         // length of the current window
-    int32_t overflow;      // bit 0: raster buffer full; bit 1: a step had more
-                           // spikes than the delivery masks hold (cE|cI * 64)
+    int32_t overflow;      // bit 0: raster buffer full; bit 1: a window had more
+                           // spikes than the delivery masks hold (cw * 64)
     int64_t step;          // first step of the next window
     int64_t target;        // step this call stops at
     int64_t win_T0;        // first step of the current window
@@ -196,7 +196,8 @@ struct DevNet {
     uint32_t* spk_list;    // [Wmax][n_neur]
     uint32_t* spk_n;       // [Wmax][2]: total, excitatory
     double* spk_c;         // [Wmax][n_neur][2]: exc-or-inh weight, n2a weight
-    uint64_t *mask_exc, *mask_inh, *mask_astro;  // [Wmax][cE|cI][targets]
+    // delivery masks, one bit per (window spike, target): [cw][n_neur], [cw][n_astro]
+    uint64_t *mask_n, *mask_a;
     // raster
     int64_t* r_step;
     uint32_t* r_src;
@@ -211,7 +212,28 @@ struct DevNet {
     int64_t c_steps, c_spikes, c_syn, c_sic;
     uint32_t sic_live;     // bit k: some astrocyte has a nonzero SIC factor at
                            // window step k (set by k_cells, cleared by k_sum)
+    // [S]: 1 if the exchange arriving at ring row r had a nonzero factor,
+    // i.e. its ring_sic row was written; 0 means every neuron's SIC is 0
+    uint32_t* sic_hist;
 };
+
+// Bit k: the SIC exchange arriving at window step k (a step first + j *
+// period, j >= 0) had a nonzero factor, so its ring row holds the sums. One
+// round of loads for the whole window.
+__device__ __forceinline__ uint32_t live_arrivals(const uint32_t* __restrict__ hist, int row0, int S,
+                                                  int64_t first, int period, int W) {
+    uint32_t live = 0;
+    int64_t next = first;
+#pragma unroll
+    for (int k = 0; k < kWindowCap; ++k) {
+        if (k < W && k == next) {
+            next += period;
+            const int row = row0 + k < S ? row0 + k : row0 + k - S;
+            live |= static_cast<uint32_t>(hist[row] != 0) << k;
+        }
+    }
+    return live;
+}
 
 // Recording cadence of one window: `next` is the window offset of the next
 // recorded step, `slot` its ring slot; advance() moves to the one after.
@@ -307,11 +329,15 @@ __device__ __forceinline__ void neuron_window(const DevNet& n, uint32_t i, int64
     double* __restrict__ ring_exc = n.ring_exc;
     double* __restrict__ ring_inh = n.ring_inh;
     const double* __restrict__ ring_sic = n.ring_sic;
+    const uint32_t* __restrict__ sic_hist = n.sic_hist;
     uint64_t* __restrict__ spk_bits = n.spk_bits;
     const int32_t* __restrict__ drive_n = n.drive_n;
     const double* __restrict__ drive_noise = n.drive_noise;
 
     int row = static_cast<int>(T0 % S);  // ring row of the current step
+    int64_t sic_next = first_hit(T0, d_a2n, sic_interval);
+    const uint32_t live_in = live_arrivals(sic_hist, row, S, sic_next, sic_interval, W);
+    const bool sic_live_n = live_in & 1u;
     double V = 0, w = 0, gex = 0, dgex = 0, gin = 0, dgin = 0, Isic = 0, Vr = 0, b = 0;
     int refr = 0;
     double ex_n = 0, ih_n = 0, sic_n = 0;  // ring values of the next step
@@ -332,12 +358,11 @@ __device__ __forceinline__ void neuron_window(const DevNet& n, uint32_t i, int64
         const size_t s0 = static_cast<size_t>(row) * N + i;
         ex_n = ring_exc[s0];
         ih_n = ring_inh[s0];
-        sic_n = ring_sic[s0];
+        if (sic_live_n) sic_n = ring_sic[s0];
         if (has_poiss) ev_n = drive_n[i];
         if (has_noise) noise_n = drive_noise[i];
     }
     const uint32_t lane = threadIdx.x & 63;
-    int64_t sic_next = first_hit(T0, d_a2n, sic_interval);
     for (int k = 0; k < W; ++k) {
         bool spiked = false;
         const int row_n = ring_next(row, S);
@@ -345,6 +370,7 @@ __device__ __forceinline__ void neuron_window(const DevNet& n, uint32_t i, int64
         if (sic_step) {
             sic_next += sic_interval;
         }
+        const bool next_live = (live_in >> (k + 1)) & 1u;
         if (valid) {
             const size_t slot = static_cast<size_t>(row) * N + i;
             const double ex = ex_n, ih = ih_n, sic = sic_n;
@@ -354,7 +380,7 @@ __device__ __forceinline__ void neuron_window(const DevNet& n, uint32_t i, int64
                 const size_t nx = static_cast<size_t>(row_n) * N + i;
                 ex_n = ring_exc[nx];
                 ih_n = ring_inh[nx];
-                sic_n = ring_sic[nx];
+                sic_n = next_live ? ring_sic[nx] : 0.0;
                 const size_t dk1 = static_cast<size_t>(k + 1) * N + i;
                 if (has_poiss) ev_n = drive_n[dk1];
                 if (has_noise) noise_n = drive_noise[dk1];
@@ -378,8 +404,10 @@ __device__ __forceinline__ void neuron_window(const DevNet& n, uint32_t i, int64
                 }
                 ring_inh[slot] = 0.0;
             }
-            // A slot carries a new SIC exactly when an exchange was sent
-            // d_a2n steps earlier; otherwise the last value is held.
+            // A new SIC arrives exactly when an exchange was sent d_a2n
+            // steps earlier; otherwise the last value is held. An exchange
+            // without a nonzero factor (sic_hist 0) is 0 for every neuron
+            // and its ring row is not read.
             if (sic_step) {
                 Isic = sic;
             }
@@ -522,11 +550,15 @@ __device__ __forceinline__ void neuron_window_f32(const DevNet& n, uint32_t i, i
     float* __restrict__ ring_exc = n.fring_exc;
     float* __restrict__ ring_inh = n.fring_inh;
     const float* __restrict__ ring_sic = n.fring_sic;
+    const uint32_t* __restrict__ sic_hist = n.sic_hist;
     uint64_t* __restrict__ spk_bits = n.spk_bits;
     const int32_t* __restrict__ drive_n = n.drive_n;
     const float* __restrict__ drive_noise = n.fdrive_noise;
 
     int row = static_cast<int>(T0 % S);  // ring row of the current step
+    int64_t sic_next = first_hit(T0, d_a2n, sic_interval);
+    const uint32_t live_in = live_arrivals(sic_hist, row, S, sic_next, sic_interval, W);
+    const bool sic_live_n = live_in & 1u;
     float V = 0, w = 0, gex = 0, dgex = 0, gin = 0, dgin = 0, Isic = 0, Vr = 0, b = 0;
     int refr = 0;
     float ex_n = 0, ih_n = 0, sic_n = 0;
@@ -547,12 +579,11 @@ __device__ __forceinline__ void neuron_window_f32(const DevNet& n, uint32_t i, i
         const size_t s0 = static_cast<size_t>(row) * N + i;
         ex_n = ring_exc[s0];
         ih_n = ring_inh[s0];
-        sic_n = ring_sic[s0];
+        if (sic_live_n) sic_n = ring_sic[s0];
         if (has_poiss) ev_n = drive_n[i];
         if (has_noise) noise_n = drive_noise[i];
     }
     const uint32_t lane = threadIdx.x & 63;
-    int64_t sic_next = first_hit(T0, d_a2n, sic_interval);
     for (int k = 0; k < W; ++k) {
         bool spiked = false;
         const int row_n = ring_next(row, S);
@@ -560,6 +591,7 @@ __device__ __forceinline__ void neuron_window_f32(const DevNet& n, uint32_t i, i
         if (sic_step) {
             sic_next += sic_interval;
         }
+        const bool next_live = (live_in >> (k + 1)) & 1u;
         if (valid) {
             const size_t slot = static_cast<size_t>(row) * N + i;
             const float ex = ex_n, ih = ih_n, sic = sic_n;
@@ -569,7 +601,7 @@ __device__ __forceinline__ void neuron_window_f32(const DevNet& n, uint32_t i, i
                 const size_t nx = static_cast<size_t>(row_n) * N + i;
                 ex_n = ring_exc[nx];
                 ih_n = ring_inh[nx];
-                sic_n = ring_sic[nx];
+                sic_n = next_live ? ring_sic[nx] : 0.0f;
                 const size_t dk1 = static_cast<size_t>(k + 1) * N + i;
                 if (has_poiss) ev_n = drive_n[dk1];
                 if (has_noise) noise_n = drive_noise[dk1];
@@ -726,11 +758,9 @@ template <int GM>
 __device__ void deliver_sic_f32(const DevNet& n, uint32_t t, int g0, int G) {
     const uint32_t N = n.n_neur;
     const int64_t T0 = n.win_T0;
-    uint32_t send = 0;
-    for (int j = 0; j < G; ++j) {
-        if ((T0 + g0 + j) % n.sic_interval == 0) {
-            send |= 1u << j;
-        }
+    uint32_t send = 0;  // bit j: window step g0 + j sends an exchange
+    for (int64_t j = first_hit(T0 + g0, 0, n.sic_interval); j < G; j += n.sic_interval) {
+        send |= 1u << j;
     }
     const uint32_t live = send & (n.sic_live >> g0);
     float acc[GM], comp[GM];
@@ -763,11 +793,16 @@ __device__ void deliver_sic_f32(const DevNet& n, uint32_t t, int g0, int G) {
             }
         }
     }
+    // A live exchange writes every neuron's slot; the others are neither
+    // written nor read (sic_hist).
+    const int S = n.S;
+    const int row0 = static_cast<int>((T0 + g0 + n.d_a2n) % S);
 #pragma unroll
     for (int j = 0; j < GM; ++j) {
-        if ((send >> j) & 1u) {
-            n.fring_sic[static_cast<size_t>((T0 + g0 + j + n.d_a2n) % n.S) * N + t] =
-                acc[j] + comp[j];
+        const float v = acc[j] + comp[j];
+        if ((live >> j) & 1u) {
+            const int row = row0 + j < S ? row0 + j : row0 + j - S;
+            n.fring_sic[static_cast<size_t>(row) * N + t] = v;
         }
     }
 }
@@ -1019,9 +1054,6 @@ __global__ __launch_bounds__(64) void k_compact(DevNet* nets, const int32_t* act
     if (lane == 0) {
         n.spk_n[2 * k] = total;
         n.spk_n[2 * k + 1] = totalE;
-        if (totalE > 64 * n.cE || total - totalE > 64 * n.cI) {
-            atomicOr(&n.overflow, 2);
-        }
         if (total > 0) {
             add_i64(&n.c_spikes, total);
         }
@@ -1036,10 +1068,8 @@ __device__ void deliver_sic(const DevNet& n, uint32_t t, int g0, int G) {
     const uint32_t N = n.n_neur;
     const int64_t T0 = n.win_T0;
     uint32_t send = 0;  // bit j: window step g0 + j sends an exchange
-    for (int j = 0; j < G; ++j) {
-        if ((T0 + g0 + j) % n.sic_interval == 0) {
-            send |= 1u << j;
-        }
+    for (int64_t j = first_hit(T0 + g0, 0, n.sic_interval); j < G; j += n.sic_interval) {
+        send |= 1u << j;
     }
     // Steps at which no astrocyte of the network has a nonzero factor have
     // an empty sum.
@@ -1084,10 +1114,15 @@ __device__ void deliver_sic(const DevNet& n, uint32_t t, int g0, int G) {
             }
         }
     }
+    // A live exchange writes every neuron's slot; the others are neither
+    // written nor read (sic_hist).
+    const int S = n.S;
+    const int row0 = static_cast<int>((T0 + g0 + n.d_a2n) % S);
 #pragma unroll
     for (int j = 0; j < GM; ++j) {
-        if ((send >> j) & 1u) {
-            n.ring_sic[static_cast<size_t>((T0 + g0 + j + n.d_a2n) % n.S) * N + t] = acc[j];
+        if ((live >> j) & 1u) {
+            const int row = row0 + j < S ? row0 + j : row0 + j - S;
+            n.ring_sic[static_cast<size_t>(row) * N + t] = acc[j];
         }
     }
 }
@@ -1144,16 +1179,17 @@ struct StepWords {
 };
 
 // The spikes one wavefront's 64 neurons emitted at window steps [k0, k1):
-// each spike's bit (its index in its step's spike list) is set in the
-// per-target mask of every synapse of its rows. OR is order-independent, so
-// wavefronts may mark in any order. The steps' words and the spiking
-// neurons' row bounds are loaded once, up front.
+// each spike's bit -- its window index, the spikes of the earlier steps plus
+// its index in its step's list -- is set in the window mask of every target
+// of its rows. OR is order-independent, so wavefronts may mark in any order.
+// The window's words and the spiking neurons' row bounds are loaded once.
 __device__ void mark_steps(const DevNet& n, uint32_t first, uint32_t lane, int k0, int k1) {
     const uint32_t N = n.n_neur, NE = n.n_exc;
-    const int nk = k1 - k0;
-    const StepWords sw(n, first >> 6, lane, k0, nk);
+    const StepWords sw(n, first >> 6, lane, 0, n.win_W);
+    uint32_t window_total = 0;
+    const uint32_t before_k = wave_exclusive_scan(sw.tot, window_total);
     uint64_t any = 0;
-    for (int kk = 0; kk < nk; ++kk) {
+    for (int kk = k0; kk < k1; ++kk) {
         any |= lane_u64(sw.word, kk);
     }
     if (any == 0) {
@@ -1172,37 +1208,29 @@ __device__ void mark_steps(const DevNet& n, uint32_t first, uint32_t lane, int k
             r1 = n.inh_row[s_lane - NE + 1];
         }
     }
-    for (int kk = 0; kk < nk; ++kk) {
-        const uint64_t word = lane_u64(sw.word, kk);
+    const uint32_t cap = 64 * n.cw;
+    for (int k = k0; k < k1; ++k) {
+        const uint64_t word = lane_u64(sw.word, k);
         if (word == 0) {
             continue;
         }
-        const uint32_t off = lane_u32(sw.off, kk), nE = lane_u32(sw.nE, kk);
-        const int k = k0 + kk;
+        const uint32_t base = lane_u32(before_k, k) + lane_u32(sw.off, k);
         uint64_t m = word;
         while (m) {
             const int L = __ffsll(static_cast<long long>(m)) - 1;
             m &= m - 1;
-            const uint32_t s = first + static_cast<uint32_t>(L);
-            const uint32_t js = off + static_cast<uint32_t>(__popcll(word & ((1ULL << L) - 1ULL)));
+            const uint32_t jw = base + static_cast<uint32_t>(__popcll(word & ((1ULL << L) - 1ULL)));
+            if (jw >= cap) {
+                break;  // beyond the delivery capacity; deliver_sources flags it
+            }
+            const uint64_t bit = 1ULL << (jw & 63);
+            const size_t chunk = jw >> 6;
             const uint32_t q0 = lane_u32(r0, L), q1 = lane_u32(r1, L);
-            if (s < NE) {
-                if ((js >> 6) >= n.cE) {
-                    continue;  // beyond the delivery capacity; k_compact flagged it
-                }
-                const uint64_t bit = 1ULL << (js & 63);
-                const size_t chunk = static_cast<size_t>(k) * n.cE + (js >> 6);
-                mark_row(n.exc_tgt, q0, q1, n.mask_exc + chunk * N, bit, lane);
-                mark_row(n.n2a_tgt, lane_u32(a0, L), lane_u32(a1, L),
-                         n.mask_astro + chunk * n.n_astro, bit, lane);
-            } else {
-                const uint32_t jj = js - nE;
-                if ((jj >> 6) >= n.cI) {
-                    continue;
-                }
-                const uint64_t bit = 1ULL << (jj & 63);
-                const size_t chunk = static_cast<size_t>(k) * n.cI + (jj >> 6);
-                mark_row(n.inh_tgt, q0, q1, n.mask_inh + chunk * N, bit, lane);
+            mark_row(first + static_cast<uint32_t>(L) < NE ? n.exc_tgt : n.inh_tgt, q0, q1,
+                     n.mask_n + chunk * N, bit, lane);
+            if (first + static_cast<uint32_t>(L) < NE) {
+                mark_row(n.n2a_tgt, lane_u32(a0, L), lane_u32(a1, L), n.mask_a + chunk * n.n_astro,
+                         bit, lane);
             }
         }
     }
@@ -1289,6 +1317,9 @@ __device__ void deliver_sources(DevNet& n, uint32_t first, uint32_t lane) {
     if (!room && window_total > 0 && t == 0) {
         atomicOr(&n.overflow, 1);
     }
+    if (window_total > 64 * n.cw && t == 0) {
+        atomicOr(&n.overflow, 2);  // spikes beyond the delivery masks' capacity
+    }
 }
 
 // Window pass 3: one wavefront per (64 neurons, window step); four per block.
@@ -1350,173 +1381,185 @@ __device__ inline uint32_t n2a_multiplicity(const DevNet& n, uint32_t s, uint32_
     return (lo < n.n2a_row[s + 1] && n.n2a_tgt[lo] == a) ? n.n2a_mult[lo] : 0u;
 }
 
-// FP32 variant of the ordered sums: the same spike order, accumulated with
-// Neumaier compensation into FP32 rings.
-__device__ void sum_f32(const DevNet& n, int kind, uint32_t i, int k, int64_t T0, uint32_t tot,
-                        uint32_t nE, uint32_t nI, const double* cvals) {
-    const uint32_t N = n.n_neur;
-    if (kind == 0) {
-        if (i >= N) {
-            return;
+// One ordered sum of the window pass 4 in the precision of the network: FP64
+// adds in order; FP32 adds in order with Neumaier compensation and rounds
+// once at the end.
+template <int P>
+struct OrderedSum {
+    double acc;
+    __device__ __forceinline__ void start(double v) { acc = v; }
+    __device__ __forceinline__ void add(double c) { acc += c; }
+    __device__ __forceinline__ double value() const { return acc; }
+};
+template <>
+struct OrderedSum<ASG_FP32> {
+    float acc, comp;
+    __device__ __forceinline__ void start(float v) {
+        acc = v;
+        comp = 0.0f;
+    }
+    __device__ __forceinline__ void add(double c) { comp_add(acc, comp, static_cast<float>(c)); }
+    __device__ __forceinline__ void add_f32(float c) { comp_add(acc, comp, c); }
+    __device__ __forceinline__ float value() const { return acc + comp; }
+};
+
+template <int P>
+struct Rings {
+    using T = double;
+    double *exc, *inh, *astro;
+    __device__ __forceinline__ explicit Rings(const DevNet& n)
+        : exc(n.ring_exc), inh(n.ring_inh), astro(n.ring_astro) {}
+};
+template <>
+struct Rings<ASG_FP32> {
+    using T = float;
+    float *exc, *inh, *astro;
+    __device__ __forceinline__ explicit Rings(const DevNet& n)
+        : exc(n.fring_exc), inh(n.fring_inh), astro(n.fring_astro) {}
+};
+
+// The window's per-step spike counts as uniform arrays: before[k] spikes
+// precede step k in window order, nE[k] of step k's are excitatory. Loaded
+// by lanes 0..W-1 and broadcast, so any lane can look up any step.
+struct WindowCounts {
+    uint32_t before[kWindowCap + 1], nE[kWindowCap];
+    __device__ __forceinline__ WindowCounts(uint32_t l_before, uint32_t l_nE, uint32_t total) {
+#pragma unroll
+        for (int k = 0; k < kWindowCap; ++k) {
+            before[k] = lane_u32(l_before, k);
+            nE[k] = lane_u32(l_nE, k);
         }
-        if (nE > 0) {
-            const size_t slot = static_cast<size_t>((T0 + k + n.d_exc) % n.S) * N + i;
-            float acc = n.fring_exc[slot], comp = 0.0f;
-            bool any = false;
-            for (uint32_t c = 0; c < min((nE + 63) / 64, n.cE); ++c) {
-                uint64_t* mp = n.mask_exc + (static_cast<size_t>(k) * n.cE + c) * N + i;
-                uint64_t m = *mp;
-                if (m) {
-                    *mp = 0;
-                    any = true;
-                    while (m) {
-                        const uint32_t j = c * 64 + (__ffsll(static_cast<long long>(m)) - 1);
-                        comp_add(acc, comp, static_cast<float>(cvals[2 * j]));
-                        m &= m - 1;
+        before[kWindowCap] = total;
+    }
+    // The window step of window spike index jw (< total): the largest k < W
+    // with before[k] <= jw.
+    __device__ __forceinline__ int step_of(uint32_t jw, int W) const {
+        int k = 0;
+#pragma unroll
+        for (int kk = 1; kk < kWindowCap; ++kk) {
+            k += (kk < W && jw >= before[kk]) ? 1 : 0;
+        }
+        return k;
+    }
+    __device__ __forceinline__ uint32_t before_of(int k) const {
+        uint32_t v = 0;
+#pragma unroll
+        for (int kk = 0; kk < kWindowCap; ++kk) {
+            v = k == kk ? before[kk] : v;
+        }
+        return v;
+    }
+    __device__ __forceinline__ uint32_t nE_of(int k) const {
+        uint32_t v = 0;
+#pragma unroll
+        for (int kk = 0; kk < kWindowCap; ++kk) {
+            v = k == kk ? nE[kk] : v;
+        }
+        return v;
+    }
+};
+
+// The ordered sums of one target over window steps [k0, k1). Its window
+// mask has a bit per spike that reached it, at the spike's window index; the
+// group's spikes are the indices [before[k0], before[k1]). In ascending bit
+// order the spikes of one step come together, excitatory before inhibitory,
+// each run in ascending spike (source) order. Each run is added into the
+// arrival slot of its step and kind, starting from the slot's value, and
+// stored when the run ends. The bits read are cleared: a mask word that
+// lies wholly in this group's range with a store, a word shared with the
+// neighbouring group's range with an atomic AND.
+template <int P>
+__device__ __forceinline__ void sum_window(const DevNet& n, int kind, uint32_t i, int64_t T0, int W,
+                                           int k0, int k1, const WindowCounts& wc) {
+    using R = typename Rings<P>::T;
+    const Rings<P> rings(n);
+    const size_t N = n.n_neur;
+    const int S = n.S;
+    const uint32_t lo = wc.before_of(k0);
+    const uint32_t hi = min(k1 == W ? wc.before[kWindowCap] : wc.before_of(k1), 64 * n.cw);
+    const bool neuron = kind == 0;
+    const size_t targets = neuron ? N : static_cast<size_t>(n.n_astro);
+    if (i >= targets) {
+        return;
+    }
+    uint64_t* __restrict__ mask = (neuron ? n.mask_n : n.mask_a) + i;
+    const double* __restrict__ spk_c = n.spk_c;
+    const int rowE0 = static_cast<int>((T0 + n.d_exc) % S);
+    const int rowI0 = static_cast<int>((T0 + n.d_inh) % S);
+    const int rowA0 = static_cast<int>((T0 + n.d_n2a) % S);
+    OrderedSum<P> acc;
+    R* run_ring = nullptr;  // ring and slot of the open run
+    size_t run_slot = 0;
+    int run_key = -1;       // 2 * step + (inhibitory ? 1 : 0)
+    for (uint32_t c = lo >> 6; lo < hi && c <= (hi - 1) >> 6; ++c) {
+        uint64_t m = mask[c * targets];
+        const uint32_t b0 = max(lo, c * 64) - c * 64, b1 = min(hi, c * 64 + 64) - c * 64;
+        m &= (b1 == 64 ? ~0ULL : (1ULL << b1) - 1ULL) & ~((1ULL << b0) - 1ULL);
+        if (m == 0) {
+            continue;
+        }
+        const bool owned = (k0 == 0 || lo <= c * 64) && (k1 == W || hi >= c * 64 + 64);
+        if (owned) {
+            mask[c * targets] = 0;
+        } else {
+            atomicAnd(reinterpret_cast<unsigned long long*>(mask + c * targets), ~m);
+        }
+        while (m) {
+            const uint32_t jw = c * 64 + static_cast<uint32_t>(__ffsll(static_cast<long long>(m)) - 1);
+            m &= m - 1;
+            const int k = wc.step_of(jw, W);
+            const uint32_t j = jw - wc.before_of(k);
+            const size_t cv = (static_cast<size_t>(k) * N + j) * 2;
+            if (neuron) {
+                const bool inh = j >= wc.nE_of(k);
+                const int key = 2 * k + (inh ? 1 : 0);
+                if (key != run_key) {
+                    if (run_ring != nullptr) {
+                        run_ring[run_slot] = acc.value();
+                    }
+                    run_key = key;
+                    const int row0 = inh ? rowI0 : rowE0;
+                    const int row = row0 + k < S ? row0 + k : row0 + k - S;
+                    run_ring = inh ? rings.inh : rings.exc;
+                    run_slot = static_cast<size_t>(row) * N + i;
+                    acc.start(run_ring[run_slot]);
+                }
+                acc.add(inh ? -spk_c[cv] : spk_c[cv]);
+            } else {
+                if (k != run_key) {
+                    if (run_ring != nullptr) {
+                        run_ring[run_slot] = acc.value();
+                    }
+                    run_key = k;
+                    const int row = rowA0 + k < S ? rowA0 + k : rowA0 + k - S;
+                    run_ring = rings.astro;
+                    run_slot = static_cast<size_t>(row) * targets + i;
+                    acc.start(run_ring[run_slot]);
+                }
+                // spike j's source counts its n2a multiplicity m times: m
+                // repeated FP64 additions (network.cpp:249-279 adds once
+                // per synapse), or one FP32 product m * weight
+                const uint32_t mult = n2a_multiplicity(n, n.spk_list[static_cast<size_t>(k) * N + j], i);
+                const double w = spk_c[cv + 1];
+                if constexpr (P == ASG_FP32) {
+                    acc.add_f32(static_cast<float>(mult) * static_cast<float>(w));
+                } else {
+                    for (uint32_t r = mult; r > 0; --r) {
+                        acc.add(w);
                     }
                 }
             }
-            if (any) {
-                n.fring_exc[slot] = acc + comp;
-            }
-        }
-        if (nI > 0) {
-            const size_t slot = static_cast<size_t>((T0 + k + n.d_inh) % n.S) * N + i;
-            float acc = n.fring_inh[slot], comp = 0.0f;
-            bool any = false;
-            for (uint32_t c = 0; c < min((nI + 63) / 64, n.cI); ++c) {
-                uint64_t* mp = n.mask_inh + (static_cast<size_t>(k) * n.cI + c) * N + i;
-                uint64_t m = *mp;
-                if (m) {
-                    *mp = 0;
-                    any = true;
-                    while (m) {
-                        const uint32_t j = nE + c * 64 + (__ffsll(static_cast<long long>(m)) - 1);
-                        comp_add(acc, comp, -static_cast<float>(cvals[2 * j]));
-                        m &= m - 1;
-                    }
-                }
-            }
-            if (any) {
-                n.fring_inh[slot] = acc + comp;
-            }
-        }
-        return;
-    }
-    const uint32_t NA = n.n_astro;
-    if (i >= NA || nE == 0) {
-        return;
-    }
-    (void)tot;
-    const uint32_t* list = n.spk_list + static_cast<size_t>(k) * N;
-    const size_t slot = static_cast<size_t>((T0 + k + n.d_n2a) % n.S) * NA + i;
-    float acc = n.fring_astro[slot], comp = 0.0f;
-    bool any = false;
-    for (uint32_t c = 0; c < min((nE + 63) / 64, n.cE); ++c) {
-        uint64_t* mp = n.mask_astro + (static_cast<size_t>(k) * n.cE + c) * NA + i;
-        uint64_t m = *mp;
-        if (m) {
-            *mp = 0;
-            any = true;
-            while (m) {
-                const uint32_t j = c * 64 + (__ffsll(static_cast<long long>(m)) - 1);
-                const uint32_t mult = n2a_multiplicity(n, list[j], i);
-                comp_add(acc, comp, static_cast<float>(mult) * static_cast<float>(cvals[2 * j + 1]));
-                m &= m - 1;
-            }
         }
     }
-    if (any) {
-        n.fring_astro[slot] = acc + comp;
+    if (run_ring != nullptr) {
+        run_ring[run_slot] = acc.value();
     }
 }
 
-// FP64 ordered sums of one (target, window step): the marked spikes' weights
-// added in ascending spike order into the ring slot.
-__device__ void sum_f64(const DevNet& n, int kind, uint32_t i, int k, int64_t T0, uint32_t nE,
-                        uint32_t nI, const double* cvals) {
-    const uint32_t N = n.n_neur;
-    if (kind == 0) {
-        if (i >= N) {
-            return;
-        }
-        if (nE > 0) {
-            const size_t slot = static_cast<size_t>((T0 + k + n.d_exc) % n.S) * N + i;
-            double acc = n.ring_exc[slot];
-            bool any = false;
-            for (uint32_t c = 0; c < min((nE + 63) / 64, n.cE); ++c) {
-                uint64_t* mp = n.mask_exc + (static_cast<size_t>(k) * n.cE + c) * N + i;
-                uint64_t m = *mp;
-                if (m) {
-                    *mp = 0;
-                    any = true;
-                    while (m) {
-                        const uint32_t j = c * 64 + (__ffsll(static_cast<long long>(m)) - 1);
-                        acc += cvals[2 * j];
-                        m &= m - 1;
-                    }
-                }
-            }
-            if (any) {
-                n.ring_exc[slot] = acc;
-            }
-        }
-        if (nI > 0) {
-            const size_t slot = static_cast<size_t>((T0 + k + n.d_inh) % n.S) * N + i;
-            double acc = n.ring_inh[slot];
-            bool any = false;
-            for (uint32_t c = 0; c < min((nI + 63) / 64, n.cI); ++c) {
-                uint64_t* mp = n.mask_inh + (static_cast<size_t>(k) * n.cI + c) * N + i;
-                uint64_t m = *mp;
-                if (m) {
-                    *mp = 0;
-                    any = true;
-                    while (m) {
-                        const uint32_t j = nE + c * 64 + (__ffsll(static_cast<long long>(m)) - 1);
-                        acc -= cvals[2 * j];
-                        m &= m - 1;
-                    }
-                }
-            }
-            if (any) {
-                n.ring_inh[slot] = acc;
-            }
-        }
-    } else {
-        const uint32_t NA = n.n_astro;
-        if (i >= NA || nE == 0) {
-            return;
-        }
-        const uint32_t* list = n.spk_list + static_cast<size_t>(k) * N;
-        const size_t slot = static_cast<size_t>((T0 + k + n.d_n2a) % n.S) * NA + i;
-        double acc = n.ring_astro[slot];
-        bool any = false;
-        for (uint32_t c = 0; c < min((nE + 63) / 64, n.cE); ++c) {
-            uint64_t* mp = n.mask_astro + (static_cast<size_t>(k) * n.cE + c) * NA + i;
-            uint64_t m = *mp;
-            if (m) {
-                *mp = 0;
-                any = true;
-                while (m) {
-                    const uint32_t j = c * 64 + (__ffsll(static_cast<long long>(m)) - 1);
-                    const double cv = cvals[2 * j + 1];
-                    for (uint32_t r = n2a_multiplicity(n, list[j], i); r > 0; --r) {
-                        acc += cv;
-                    }
-                    m &= m - 1;
-                }
-            }
-        }
-        if (any) {
-            n.ring_astro[slot] = acc;
-        }
-    }
-}
-
-// Window pass 4, one lane per (target, step), four wavefronts per block: add
-// the marked spikes' weights into the arrival slot in ascending spike order,
-// i.e. ascending source as the reference's scatter (network.cpp:249-279).
-// The lead wavefront then closes the window.
+// Window pass 4, one lane per target and a wavefront per group of SG
+// window steps, four wavefronts per workgroup: the window's ordered sums into
+// the rings (network.cpp:249-279 scatters in ascending source order per
+// step). The lead wavefront then closes the window.
 __global__ __launch_bounds__(256) void k_sum(DevNet* nets, const CellBlock* blocks,
                                              int32_t n_items, int32_t Wg, int32_t SG) {
     const int e = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x) * 4 +
@@ -1531,32 +1574,39 @@ __global__ __launch_bounds__(256) void k_sum(DevNet* nets, const CellBlock* bloc
     DevNet& n = nets[cb.net];
     const int W = n.win_W;
     const int64_t T0 = n.win_T0;
+    uint32_t l_tot = 0, l_nE = 0;  // lane k < W: spike counts of window step k
+    if (static_cast<int>(lane) < W) {
+        l_tot = n.spk_n[2 * lane];
+        l_nE = n.spk_n[2 * lane + 1];
+    }
+    uint32_t total = 0;
+    const uint32_t l_before = wave_exclusive_scan(l_tot, total);
     if (cb.lead && k0 == 0 && lane == 0) {
-        int64_t tot = 0;
-        for (int k2 = 0; k2 < W; ++k2) {
-            tot += n.spk_n[2 * k2];
+        if (n.r_head + total <= n.r_cap) {
+            n.r_head += total;
         }
-        if (n.r_head + tot <= n.r_cap) {
-            n.r_head += tot;
+        // liveness of the window's exchanges at their arrival rows (a
+        // step that sends no exchange has sic_live 0)
+        const int S = n.S;
+        int row = static_cast<int>((T0 + n.d_a2n) % S);
+        for (int k2 = 0; k2 < W; ++k2) {
+            n.sic_hist[row] = (n.sic_live >> k2) & 1u;
+            row = ring_next(row, S);
         }
         n.c_steps += W;
         n.step = T0 + W;
         n.sic_live = 0;
     }
-    const uint32_t i = cb.first + lane;
-    const uint32_t N = n.n_neur;
     const int k1 = min(k0 + SG, W);
-    for (int k = k0; k < k1; ++k) {
-        const uint32_t tot = n.spk_n[2 * k], nE = n.spk_n[2 * k + 1], nI = tot - nE;
-        if (tot == 0) {
-            continue;
-        }
-        const double* cvals = n.spk_c + static_cast<size_t>(k) * N * 2;
-        if (n.precision == ASG_FP32) {
-            sum_f32(n, cb.kind, i, k, T0, tot, nE, nI, cvals);
-        } else {
-            sum_f64(n, cb.kind, i, k, T0, nE, nI, cvals);
-        }
+    if (total == 0 || k1 <= k0) {
+        return;
+    }
+    const WindowCounts wc(l_before, l_nE, total);
+    const uint32_t i = cb.first + lane;
+    if (n.precision == ASG_FP32) {
+        sum_window<ASG_FP32>(n, cb.kind, i, T0, W, k0, k1, wc);
+    } else {
+        sum_window<ASG_FP64>(n, cb.kind, i, T0, W, k0, k1, wc);
     }
 }
 
@@ -1813,7 +1863,7 @@ int window_of(const asg_net_desc& d) {
 // Byte plan of one network's device arrays, so the footprint query and the
 // allocation agree.
 struct Plan {
-    size_t N, NA, words, S, Wmax, cE, cI, r_cap;
+    size_t N, NA, words, S, Wmax, cw, r_cap;
     size_t n_exc_syn, n_inh_syn, n2a_dedup, sic_pairs;
     size_t bytes;
 };
@@ -1827,20 +1877,17 @@ Plan plan_of(const asg_net_desc& d, const DerivedConn* dc) {
     const int dmax = std::max(std::max(d.exc.delay_steps, d.inh.delay_steps),
                               std::max(d.n2a.delay_steps, d.a2n.delay_steps));
     p.S = static_cast<size_t>(dmax) + p.Wmax;
-    // Delivery masks hold 64 spikes per chunk. Every source spiking in one
-    // step needs ceil(n / 64) chunks; that is kept while the masks fit in
-    // kMaskBudget, otherwise the chunk count is cut to what fits (at least
-    // kMinChunks, i.e. 256 spikes per step and type), and a step that exceeds
-    // it is reported as ASG_ERR_OVERFLOW rather than delivered in part.
-    p.cE = (d.n_exc + 63) / 64;
-    p.cI = (d.n_inh + 63) / 64;
+    // Delivery masks hold 64 window spikes per chunk. Every neuron spiking
+    // at every step of a window needs ceil(Wmax * N / 64) chunks; that is
+    // kept while the masks fit in kMaskBudget, otherwise the chunk count is
+    // cut to what fits (at least kMinChunks, i.e. 4096 spikes per window), and
+    // a window that exceeds it is reported as ASG_ERR_OVERFLOW rather than
+    // delivered in part.
+    p.cw = (p.Wmax * p.N + 63) / 64;
     {
-        const size_t per_chunk = p.Wmax * (2 * p.N + p.NA) * 8;
-        const size_t full = p.Wmax * (p.cE * p.N + p.cI * p.N + p.cE * p.NA) * 8;
-        if (full > kMaskBudget && per_chunk > 0) {
-            const size_t cap = std::max<size_t>(kMinChunks, kMaskBudget / per_chunk);
-            p.cE = std::min(p.cE, cap);
-            p.cI = std::min(p.cI, cap);
+        const size_t per_chunk = (p.N + p.NA) * 8;
+        if (p.cw * per_chunk > kMaskBudget && per_chunk > 0) {
+            p.cw = std::min(p.cw, std::max<size_t>(kMinChunks, kMaskBudget / per_chunk));
         }
     }
     p.r_cap = std::max<size_t>(size_t(1) << 20, 16 * p.N);
@@ -1854,12 +1901,12 @@ Plan plan_of(const asg_net_desc& d, const DerivedConn* dc) {
     size_t b = 0;
     b += p.N * (7 * fb + 4 + 2 * 8 + 3 * 8);       // neuron state, V_reset/b, STP
     b += p.NA * (3 * fb + 4 * fb);                  // astrocyte state and parameters
-    b += p.S * p.N * 3 * fb + p.S * p.NA * fb + p.Wmax * p.NA * fb;
+    b += p.S * p.N * 3 * fb + p.S * p.NA * fb + p.Wmax * p.NA * fb + p.S * 4;
     b += (d.n_exc + 1 + p.n_exc_syn + d.n_inh + 1 + p.n_inh_syn) * 4;
     b += (2 * (d.n_exc + 1) + 2 * p.n2a_dedup) * 4;
     b += 2 * p.sic_pairs * 4;
     b += p.Wmax * (p.words * 12 + p.N * 4 + 8 + p.N * 16);
-    b += p.Wmax * (p.cE * p.N + p.cI * p.N + p.cE * p.NA) * 8;
+    b += p.cw * (p.N + p.NA) * 8;
     b += p.r_cap * 12;
     const bool poiss = d.input_exc.poiss_rate > 0.0 || d.input_inh.poiss_rate > 0.0;
     const bool noise = d.input_exc.gauss_noise_std > 0.0 || d.input_inh.gauss_noise_std > 0.0;
@@ -1946,8 +1993,7 @@ void build_slot(NetSlot& s, const asg_net_desc& d, int64_t total, hipStream_t st
     n.n_neur = static_cast<uint32_t>(pl.N);
     n.n_astro = d.n_astro;
     n.words = static_cast<uint32_t>(pl.words);
-    n.cE = static_cast<uint32_t>(pl.cE);
-    n.cI = static_cast<uint32_t>(pl.cI);
+    n.cw = static_cast<uint32_t>(pl.cw);
     n.S = static_cast<int32_t>(pl.S);
     n.Wmax = static_cast<int32_t>(pl.Wmax);
     n.d_exc = d.exc.delay_steps;
@@ -2047,6 +2093,7 @@ void build_slot(NetSlot& s, const asg_net_desc& d, int64_t total, hipStream_t st
     n.stp_x = dev_fill(s, 1.0, N, st);
     n.stp_u = dev_fill(s, d.stp.U, N, st);
     n.stp_tl = dev_fill(s, -1e12, N, st);
+    n.sic_hist = dev_zero<uint32_t>(s, pl.S, st);
     n.exc_row = dev_copy(s, d.exc.row_start, d.n_exc + 1, st);
     n.exc_tgt = dev_copy(s, d.exc.target, pl.n_exc_syn, st);
     n.inh_row = dev_copy(s, d.inh.row_start, d.n_inh + 1, st);
@@ -2064,9 +2111,8 @@ void build_slot(NetSlot& s, const asg_net_desc& d, int64_t total, hipStream_t st
     n.spk_list = dev_zero<uint32_t>(s, pl.Wmax * N, st);
     n.spk_n = dev_zero<uint32_t>(s, pl.Wmax * 2, st);
     n.spk_c = dev_zero<double>(s, pl.Wmax * N * 2, st);
-    n.mask_exc = dev_zero<uint64_t>(s, pl.Wmax * pl.cE * N, st);
-    n.mask_inh = dev_zero<uint64_t>(s, pl.Wmax * pl.cI * N, st);
-    n.mask_astro = dev_zero<uint64_t>(s, pl.Wmax * pl.cE * NA, st);
+    n.mask_n = dev_zero<uint64_t>(s, pl.cw * N, st);
+    n.mask_a = dev_zero<uint64_t>(s, pl.cw * NA, st);
     if (n.in[0].poiss_rate > 0.0 || n.in[1].poiss_rate > 0.0) {
         n.drive_n = dev_zero<int32_t>(s, pl.Wmax * N, st);
     }
@@ -2404,7 +2450,7 @@ struct Engine {
     // bracketed by events (6 per timed window). Returns the windows timed.
     int64_t launch_windows(int64_t windows, int G, int wpb, int SG, bool marks) {
         const int deliver_items = n_nblocks * ((Wg + SG - 1) / SG + (Wg + G - 1) / G);
-        const int sum_items = n_blocks * ((Wg + SG - 1) / SG);
+
         int64_t timed = 0;
         for (int64_t wdx = 0; wdx < windows; ++wdx) {
             const bool timed_here = marks && (prof_seq++ % prof_every) == 0;
@@ -2428,6 +2474,7 @@ struct Engine {
                 launch_deliver(G, deliver_items, Wg, SG);
             }
             mark(4);
+            const int sum_items = n_blocks * ((Wg + SG - 1) / SG);
             hipLaunchKernelGGL(k_sum, dim3((sum_items + 3) / 4), dim3(256), 0, stream, d_nets,
                                d_blocks, sum_items, Wg, SG);
             mark(5);
@@ -2780,8 +2827,8 @@ namespace {
 
 const char* overflow_message(int bits) {
     if (bits & 2) {
-        return "a step had more spikes than the delivery capacity of this network "
-               "(its deliveries were incomplete; results from that step on are invalid)";
+        return "a window had more spikes than the delivery capacity of this network "
+               "(its deliveries were incomplete; results from that window on are invalid)";
     }
     return "raster buffer overflowed; drain spikes more often";
 }
