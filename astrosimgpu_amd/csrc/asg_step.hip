@@ -212,27 +212,20 @@ struct DevNet {
     int64_t c_steps, c_spikes, c_syn, c_sic;
     uint32_t sic_live;     // bit k: some astrocyte has a nonzero SIC factor at
                            // window step k (set by k_cells, cleared by k_sum)
-    // [S]: 1 if the exchange arriving at ring row r had a nonzero factor,
-    // i.e. its ring_sic row was written; 0 means every neuron's SIC is 0
+    // bit T mod (hist_mask + 1): the SIC exchange arriving at step T had a
+    // nonzero factor, i.e. its ring_sic row was written; 0 means every
+    // neuron's SIC is 0 (a power-of-two ring of at least S + 32 bits)
     uint32_t* sic_hist;
+    uint32_t hist_mask;
 };
 
-// Bit k: the SIC exchange arriving at window step k (a step first + j *
-// period, j >= 0) had a nonzero factor, so its ring row holds the sums. One
-// round of loads for the whole window.
-__device__ __forceinline__ uint32_t live_arrivals(const uint32_t* __restrict__ hist, int row0, int S,
-                                                  int64_t first, int period, int W) {
-    uint32_t live = 0;
-    int64_t next = first;
-#pragma unroll
-    for (int k = 0; k < kWindowCap; ++k) {
-        if (k < W && k == next) {
-            next += period;
-            const int row = row0 + k < S ? row0 + k : row0 + k - S;
-            live |= static_cast<uint32_t>(hist[row] != 0) << k;
-        }
-    }
-    return live;
+// Bit k: the SIC exchange arriving at step T0 + k had a nonzero factor, so
+// its ring row holds the sums (bits of the next 32 steps; two loads).
+__device__ __forceinline__ uint32_t live_arrivals(const DevNet& n, int64_t T0) {
+    const uint32_t b = static_cast<uint32_t>(T0) & n.hist_mask;
+    const uint32_t w = b >> 5, words = (n.hist_mask + 1) >> 5;
+    const uint64_t lo = n.sic_hist[w], hi = n.sic_hist[(w + 1) & (words - 1)];
+    return static_cast<uint32_t>(((hi << 32) | lo) >> (b & 31));
 }
 
 // Recording cadence of one window: `next` is the window offset of the next
@@ -329,14 +322,13 @@ __device__ __forceinline__ void neuron_window(const DevNet& n, uint32_t i, int64
     double* __restrict__ ring_exc = n.ring_exc;
     double* __restrict__ ring_inh = n.ring_inh;
     const double* __restrict__ ring_sic = n.ring_sic;
-    const uint32_t* __restrict__ sic_hist = n.sic_hist;
     uint64_t* __restrict__ spk_bits = n.spk_bits;
     const int32_t* __restrict__ drive_n = n.drive_n;
     const double* __restrict__ drive_noise = n.drive_noise;
 
     int row = static_cast<int>(T0 % S);  // ring row of the current step
     int64_t sic_next = first_hit(T0, d_a2n, sic_interval);
-    const uint32_t live_in = live_arrivals(sic_hist, row, S, sic_next, sic_interval, W);
+    const uint32_t live_in = live_arrivals(n, T0);
     const bool sic_live_n = live_in & 1u;
     double V = 0, w = 0, gex = 0, dgex = 0, gin = 0, dgin = 0, Isic = 0, Vr = 0, b = 0;
     int refr = 0;
@@ -550,14 +542,13 @@ __device__ __forceinline__ void neuron_window_f32(const DevNet& n, uint32_t i, i
     float* __restrict__ ring_exc = n.fring_exc;
     float* __restrict__ ring_inh = n.fring_inh;
     const float* __restrict__ ring_sic = n.fring_sic;
-    const uint32_t* __restrict__ sic_hist = n.sic_hist;
     uint64_t* __restrict__ spk_bits = n.spk_bits;
     const int32_t* __restrict__ drive_n = n.drive_n;
     const float* __restrict__ drive_noise = n.fdrive_noise;
 
     int row = static_cast<int>(T0 % S);  // ring row of the current step
     int64_t sic_next = first_hit(T0, d_a2n, sic_interval);
-    const uint32_t live_in = live_arrivals(sic_hist, row, S, sic_next, sic_interval, W);
+    const uint32_t live_in = live_arrivals(n, T0);
     const bool sic_live_n = live_in & 1u;
     float V = 0, w = 0, gex = 0, dgex = 0, gin = 0, dgin = 0, Isic = 0, Vr = 0, b = 0;
     int refr = 0;
@@ -1585,13 +1576,13 @@ __global__ __launch_bounds__(256) void k_sum(DevNet* nets, const CellBlock* bloc
         if (n.r_head + total <= n.r_cap) {
             n.r_head += total;
         }
-        // liveness of the window's exchanges at their arrival rows (a
+        // liveness of the window's exchanges at their arrival steps (a
         // step that sends no exchange has sic_live 0)
-        const int S = n.S;
-        int row = static_cast<int>((T0 + n.d_a2n) % S);
         for (int k2 = 0; k2 < W; ++k2) {
-            n.sic_hist[row] = (n.sic_live >> k2) & 1u;
-            row = ring_next(row, S);
+            const uint32_t b = static_cast<uint32_t>(T0 + n.d_a2n + k2) & n.hist_mask;
+            const uint32_t bit = 1u << (b & 31);
+            uint32_t& word = n.sic_hist[b >> 5];
+            word = ((n.sic_live >> k2) & 1u) ? (word | bit) : (word & ~bit);
         }
         n.c_steps += W;
         n.step = T0 + W;
@@ -1863,7 +1854,7 @@ int window_of(const asg_net_desc& d) {
 // Byte plan of one network's device arrays, so the footprint query and the
 // allocation agree.
 struct Plan {
-    size_t N, NA, words, S, Wmax, cw, r_cap;
+    size_t N, NA, words, S, Wmax, cw, r_cap, hist_bits;
     size_t n_exc_syn, n_inh_syn, n2a_dedup, sic_pairs;
     size_t bytes;
 };
@@ -1877,6 +1868,8 @@ Plan plan_of(const asg_net_desc& d, const DerivedConn* dc) {
     const int dmax = std::max(std::max(d.exc.delay_steps, d.inh.delay_steps),
                               std::max(d.n2a.delay_steps, d.a2n.delay_steps));
     p.S = static_cast<size_t>(dmax) + p.Wmax;
+    p.hist_bits = 64;
+    while (p.hist_bits < p.S + 32) p.hist_bits *= 2;
     // Delivery masks hold 64 window spikes per chunk. Every neuron spiking
     // at every step of a window needs ceil(Wmax * N / 64) chunks; that is
     // kept while the masks fit in kMaskBudget, otherwise the chunk count is
@@ -1901,7 +1894,7 @@ Plan plan_of(const asg_net_desc& d, const DerivedConn* dc) {
     size_t b = 0;
     b += p.N * (7 * fb + 4 + 2 * 8 + 3 * 8);       // neuron state, V_reset/b, STP
     b += p.NA * (3 * fb + 4 * fb);                  // astrocyte state and parameters
-    b += p.S * p.N * 3 * fb + p.S * p.NA * fb + p.Wmax * p.NA * fb + p.S * 4;
+    b += p.S * p.N * 3 * fb + p.S * p.NA * fb + p.Wmax * p.NA * fb + p.hist_bits / 8;
     b += (d.n_exc + 1 + p.n_exc_syn + d.n_inh + 1 + p.n_inh_syn) * 4;
     b += (2 * (d.n_exc + 1) + 2 * p.n2a_dedup) * 4;
     b += 2 * p.sic_pairs * 4;
@@ -2093,7 +2086,8 @@ void build_slot(NetSlot& s, const asg_net_desc& d, int64_t total, hipStream_t st
     n.stp_x = dev_fill(s, 1.0, N, st);
     n.stp_u = dev_fill(s, d.stp.U, N, st);
     n.stp_tl = dev_fill(s, -1e12, N, st);
-    n.sic_hist = dev_zero<uint32_t>(s, pl.S, st);
+    n.sic_hist = dev_zero<uint32_t>(s, pl.hist_bits / 32, st);
+    n.hist_mask = static_cast<uint32_t>(pl.hist_bits - 1);
     n.exc_row = dev_copy(s, d.exc.row_start, d.n_exc + 1, st);
     n.exc_tgt = dev_copy(s, d.exc.target, pl.n_exc_syn, st);
     n.inh_row = dev_copy(s, d.inh.row_start, d.n_inh + 1, st);
