@@ -465,8 +465,9 @@ __device__ __forceinline__ void astro_window(const DevNet& n, uint32_t a, int64_
     RecCursor rec(n, T0);
     double noise = 0.0;
     uint32_t live_bits = 0;
-    for (int k = 0; k < W; ++k) {
-        const int64_t T = T0 + k;
+    // Poisson stream T * kStreamStride + a: its key advances by a constant
+    uint64_t pkey = rng_key(seed ^ kSaltAstroPoisson, static_cast<uint64_t>(T0) * kStreamStride + a);
+    for (int k = 0; k < W; ++k, pkey += kStreamStride * kStreamMul) {
         const int row_n = ring_next(row, S);
         const size_t slot = static_cast<size_t>(row) * NA + a;
         const double v = v_n;
@@ -480,9 +481,7 @@ __device__ __forceinline__ void astro_window(const DevNet& n, uint32_t a, int64_
         }
         // drive_astrocytes, network.cpp:340-355
         if (ap.in.poiss_rate > 0.0) {
-            const int ev = rng_poisson_fresh(seed ^ kSaltAstroPoisson,
-                                             static_cast<uint64_t>(T) * kStreamStride + a,
-                                             ap.in.lambda, ap.in.limit);
+            const int ev = rng_poisson_fresh_keyed(pkey, ap.in.lambda, ap.in.limit);
             if (ev > 0) {
                 ip3_in += ap.in.poiss_weight * static_cast<double>(ev);
             }
@@ -687,8 +686,9 @@ __device__ __forceinline__ void astro_window_f32(const DevNet& n, uint32_t a, in
     RecCursor rec(n, T0);
     float noise = 0.0f;
     uint32_t live_bits = 0;
-    for (int k = 0; k < W; ++k) {
-        const int64_t T = T0 + k;
+    // Poisson stream T * kStreamStride + a: its key advances by a constant
+    uint64_t pkey = rng_key(seed ^ kSaltAstroPoisson, static_cast<uint64_t>(T0) * kStreamStride + a);
+    for (int k = 0; k < W; ++k, pkey += kStreamStride * kStreamMul) {
         const int row_n = ring_next(row, S);
         const size_t slot = static_cast<size_t>(row) * NA + a;
         const float v = v_n;
@@ -701,9 +701,7 @@ __device__ __forceinline__ void astro_window_f32(const DevNet& n, uint32_t a, in
             ring_astro[slot] = 0.0f;
         }
         if (ain.poiss_rate > 0.0) {
-            const int ev = rng_poisson_fresh(seed ^ kSaltAstroPoisson,
-                                             static_cast<uint64_t>(T) * kStreamStride + a,
-                                             ain.lambda, ain.limit);
+            const int ev = rng_poisson_fresh_keyed(pkey, ain.lambda, ain.limit);
             if (ev > 0) {
                 ip3_in += static_cast<float>(ain.poiss_weight * static_cast<double>(ev));
             }
@@ -720,7 +718,7 @@ __device__ __forceinline__ void astro_window_f32(const DevNet& n, uint32_t a, in
                                             ? static_cast<uint64_t>(noise_idx) * kStreamStride + a
                                             : static_cast<uint64_t>(noise_idx);
                 noise = static_cast<float>(ain.noise_std) *
-                        rng_normal_clamped_f32(seed ^ kSaltAstroNoise, stream);
+                        rng_normal_clamped_f32(rng_key(seed ^ kSaltAstroNoise, stream));
             }
         }
         astro_step_f32(kf, Ca_tot, IP3_0, inv_tau, delta, ip3_in, noise, hs, half_h, h_sixth,
@@ -849,11 +847,15 @@ __global__ __launch_bounds__(64) void k_drive(DevNet* nets, const CellBlock* nbl
         } else {
             const bool mine = valid && in.lambda > 0.0;
             uint32_t qn = 0;
-            for (int k = 0; k < W; ++k) {
+            // stream (T0 + k) * kStreamStride + i: its key advances by a
+            // constant per step
+            constexpr uint64_t kStepKey = kStreamStride * kStreamMul;
+            uint64_t key = rng_key(pseed, static_cast<uint64_t>(T0) * kStreamStride + i);
+            for (int k = 0; k < W; ++k, key += kStepKey) {
                 bool pend = false;
                 double p = 0.0;
                 if (mine) {
-                    p = rng_uniform(pseed, static_cast<uint64_t>(T0 + k) * kStreamStride + i, 0);
+                    p = rng_uniform_keyed(key, 0);
                     pend = p > in.limit;
                 }
                 if (valid && !pend) {
@@ -913,31 +915,34 @@ __global__ __launch_bounds__(64) void k_drive(DevNet* nets, const CellBlock* nbl
         const uint64_t salt_shared = pi == 0 ? kSaltNoiseSharedExc : kSaltNoiseSharedInh;
         const uint64_t nseed = in.independent ? seed ^ kSaltNeuronNoise : seed ^ salt_shared;
         const int64_t spn = in.steps_per_noise;
-        int64_t idx = T0 / spn;
+        const int64_t idx = T0 / spn;
         int64_t next = first_hit(T0 + 1, 0, spn) + 1;  // offset of the next multiple of spn
-        auto stream_of = [&](int64_t q) {
-            return in.independent ? static_cast<uint64_t>(q) * kNoiseStride + i
-                                  : static_cast<uint64_t>(q);
-        };
+        // stream idx * kNoiseStride + i (independent) or idx (shared): the
+        // key advances by a constant per noise index
+        const uint64_t idx_key = (in.independent ? kNoiseStride : 1ULL) * kStreamMul;
+        uint64_t key = rng_key(nseed, in.independent ? static_cast<uint64_t>(idx) * kNoiseStride + i
+                                                     : static_cast<uint64_t>(idx));
         if (n.precision == ASG_FP32) {
             // FP32 path: the same uniforms through the FP32 Box-Muller.
             const float sd = static_cast<float>(in.noise_std);
-            float z = rng_normal_fresh_f32(nseed, stream_of(idx));
+            float z = rng_normal_fresh_f32(key);
             for (int k = 0; k < W; ++k) {
                 if (k == next) {
                     next += spn;
-                    z = rng_normal_fresh_f32(nseed, stream_of(++idx));
+                    key += idx_key;
+                    z = rng_normal_fresh_f32(key);
                 }
                 n.fdrive_noise[static_cast<size_t>(k) * N + i] = sd * z;
             }
         } else {
             uint64_t ctr = 0;
-            double z = rng_normal_fresh(nseed, stream_of(idx), ctr);
+            double z = rng_normal_fresh_keyed(key, ctr);
             for (int k = 0; k < W; ++k) {
                 if (k == next) {
                     next += spn;
+                    key += idx_key;
                     ctr = 0;
-                    z = rng_normal_fresh(nseed, stream_of(++idx), ctr);
+                    z = rng_normal_fresh_keyed(key, ctr);
                 }
                 n.drive_noise[static_cast<size_t>(k) * N + i] = 0.0 + in.noise_std * z;
             }

@@ -195,19 +195,21 @@ __device__ __forceinline__ float box_muller_f32(double u1, double u2) {
     return __builtin_amdgcn_sqrtf(kMinus2Ln2 * l2) * __builtin_amdgcn_cosf(static_cast<float>(u2));
 }
 
-// rng_normal_fresh's draw (u1 redrawn while <= 1e-300) in FP32.
-__device__ __forceinline__ float rng_normal_fresh_f32(std::uint64_t seed, std::uint64_t stream) {
+// rng_normal_fresh's draw (u1 redrawn while <= 1e-300) in FP32, from the
+// stream's key (rng_key).
+__device__ __forceinline__ float rng_normal_fresh_f32(std::uint64_t key) {
     std::uint64_t counter = 0;
     double u1;
     do {
-        u1 = rng_uniform(seed, stream, counter++);
+        u1 = rng_uniform_keyed(key, counter++);
     } while (u1 <= 1e-300);
-    return box_muller_f32(u1, rng_uniform(seed, stream, counter));
+    return box_muller_f32(u1, rng_uniform_keyed(key, counter));
 }
 
-// rng_normal_clamped's draw (u1 clamped at 1e-300) in FP32.
-__device__ __forceinline__ float rng_normal_clamped_f32(std::uint64_t seed, std::uint64_t stream) {
-    return box_muller_f32(rng_uniform(seed, stream, 0), rng_uniform(seed, stream, 1));
+// rng_normal_clamped's draw (u1 clamped at 1e-300) in FP32, from the
+// stream's key.
+__device__ __forceinline__ float rng_normal_clamped_f32(std::uint64_t key) {
+    return box_muller_f32(rng_uniform_keyed(key, 0), rng_uniform_keyed(key, 1));
 }
 
 // a / b through the hardware reciprocal (v_rcp_f32, 1 ulp): the quotients of
