@@ -3,12 +3,13 @@
 // State, conductances and rings are FP32; divisions by constants become
 // multiplications by FP32 reciprocals and exp/log are the hardware's fast
 // FP32 approximations. FP64 is kept where the dynamics are sensitive:
-//  * an AdEx substep whose membrane potential is within kNearThresholdSlopes
-//    slope factors of V_th (or above) runs entirely in FP64 -- near
-//    threshold the exponential term grows like e^((V - V_th)/Delta_T) and
-//    decides the spike time. This island multiplies by FP64 reciprocals
-//    instead of dividing and uses a table-free FP64 exp accurate to ~3e-13:
-//    its result is rounded to FP32 at the end of the substep;
+//  * an AdEx substep runs entirely in FP64 for a whole wavefront when any of
+//    its cells has a membrane potential within kNearThresholdSlopes slope
+//    factors of V_th (or above) -- near threshold the exponential term
+//    grows like e^((V - V_th)/Delta_T) and decides the spike time. This
+//    island multiplies by FP64 reciprocals instead of dividing and uses a
+//    table-free FP64 exp accurate to ~3e-13: its result is rounded to FP32
+//    at the end of the substep;
 //  * the astrocyte's Ca-threshold test of the SIC, (Ca - SIC_th) * 1000 > 1,
 //    is evaluated in FP64 (Ca is a small difference above a threshold).
 // Random draws use the FP64 step's counter streams: identical Poisson counts,
@@ -111,22 +112,24 @@ __device__ inline void island_rhs(const NeuronK& p, const NeuronKf& pf, double V
     dw = (p.a * (Vc - p.E_L) - w) * pf.inv_tau_w_d;
 }
 
-// One substep of the AdEx + alpha-conductance update in FP32, or in FP64 when
-// the cell is near threshold and not refractory. Returns true on a spike (the
-// caller keeps at most one per step).
-__device__ inline bool adex_substep_mixed(const NeuronKf& pf, const NeuronK& pd, float V_reset,
-                                          float b, float I_ext, float hs, float half_h,
-                                          float h_sixth, double hs_d, double half_h_d,
-                                          double h_sixth_d, float& V, float& w, float& g_ex,
-                                          float& dg_ex, float& g_in, float& dg_in, int& refr) {
-    if (V > pf.near_V && refr == 0) {
-        // adex_step's non-refractory substep in FP64
-        const double ge = g_ex, dge = dg_ex, gi = g_in, dgi = dg_in, I = I_ext;
-        const double ge_next = pd.ex_decay * (ge + hs_d * dge);
-        const double gi_next = pd.in_decay * (gi + hs_d * dgi);
+// One substep of the AdEx + alpha-conductance update (neuron.cpp:101-141)
+// in FP64 from the FP32 state, rounded back to FP32: RK4 for a
+// non-refractory cell, the clamped Euler update of w for a refractory one.
+__device__ inline bool adex_substep_f64(const NeuronKf& pf, const NeuronK& pd, float V_reset,
+                                        float b, float I_ext, double hs_d, double half_h_d,
+                                        double h_sixth_d, float& V, float& w, float& g_ex,
+                                        float& dg_ex, float& g_in, float& dg_in, int& refr) {
+    const double ge = g_ex, dge = dg_ex, gi = g_in, dgi = dg_in, I = I_ext;
+    const double ge_next = pd.ex_decay * (ge + hs_d * dge);
+    const double gi_next = pd.in_decay * (gi + hs_d * dgi);
+    double Vd = V, wd = w;
+    if (refr > 0) {
+        Vd = V_reset;
+        const double Vc = (pd.V_peak < Vd) ? pd.V_peak : Vd;
+        wd += hs_d * ((pd.a * (Vc - pd.E_L) - wd) * pf.inv_tau_w_d);
+    } else {
         const double ge_mid = 0.5 * (ge + ge_next);
         const double gi_mid = 0.5 * (gi + gi_next);
-        double Vd = V, wd = w;
         double k1V, k1w, k2V, k2w, k3V, k3w, k4V, k4w;
         island_rhs(pd, pf, Vd, wd, ge, gi, I, k1V, k1w);
         island_rhs(pd, pf, Vd + half_h_d * k1V, wd + half_h_d * k1w, ge_mid, gi_mid, I, k2V, k2w);
@@ -134,19 +137,43 @@ __device__ inline bool adex_substep_mixed(const NeuronKf& pf, const NeuronK& pd,
         island_rhs(pd, pf, Vd + hs_d * k3V, wd + hs_d * k3w, ge_next, gi_next, I, k4V, k4w);
         Vd += h_sixth_d * (k1V + 2.0 * k2V + 2.0 * k3V + k4V);
         wd += h_sixth_d * (k1w + 2.0 * k2w + 2.0 * k3w + k4w);
-        g_ex = static_cast<float>(ge_next);
-        dg_ex = static_cast<float>(pd.ex_decay * dge);
-        g_in = static_cast<float>(gi_next);
-        dg_in = static_cast<float>(pd.in_decay * dgi);
-        if (Vd >= pd.V_peak) {
-            V = V_reset;
-            w = static_cast<float>(wd + static_cast<double>(b));
-            refr = pd.refr_steps;
-            return true;
-        }
-        V = static_cast<float>(Vd);
+    }
+    g_ex = static_cast<float>(ge_next);
+    dg_ex = static_cast<float>(pd.ex_decay * dge);
+    g_in = static_cast<float>(gi_next);
+    dg_in = static_cast<float>(pd.in_decay * dgi);
+    if (refr > 0) {
+        --refr;
+        V = V_reset;
         w = static_cast<float>(wd);
         return false;
+    }
+    if (Vd >= pd.V_peak) {
+        V = V_reset;
+        w = static_cast<float>(wd + static_cast<double>(b));
+        refr = pd.refr_steps;
+        return true;
+    }
+    V = static_cast<float>(Vd);
+    w = static_cast<float>(wd);
+    return false;
+}
+
+// One substep of the AdEx + alpha-conductance update for the wavefront's
+// active lanes: in FP64 (adex_substep_f64) when any of them is near
+// threshold and not refractory, otherwise in FP32. The choice is uniform
+// across the wavefront: FP64 vector instructions issue at the FP32 rate on
+// this GPU, so one FP64 pass for all lanes costs less than an FP32 pass
+// plus a divergent FP64 pass for some. Returns true on a spike (the caller
+// keeps at most one per step).
+__device__ inline bool adex_substep_mixed(const NeuronKf& pf, const NeuronK& pd, float V_reset,
+                                          float b, float I_ext, float hs, float half_h,
+                                          float h_sixth, double hs_d, double half_h_d,
+                                          double h_sixth_d, float& V, float& w, float& g_ex,
+                                          float& dg_ex, float& g_in, float& dg_in, int& refr) {
+    if (__ballot(V > pf.near_V && refr == 0) != 0) {
+        return adex_substep_f64(pf, pd, V_reset, b, I_ext, hs_d, half_h_d, h_sixth_d, V, w, g_ex,
+                                dg_ex, g_in, dg_in, refr);
     }
     const float g_ex_next = pf.ex_decay * (g_ex + hs * dg_ex);
     const float dg_ex_next = pf.ex_decay * dg_ex;
