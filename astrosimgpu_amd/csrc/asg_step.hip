@@ -535,8 +535,7 @@ __device__ __forceinline__ void neuron_window_f32(const DevNet& n, uint32_t i, i
     const int d_a2n = n.d_a2n;
     const int sic_interval = n.sic_interval;
     const int substeps = n.substeps;
-    const float hs = static_cast<float>(n.h), half_h = static_cast<float>(n.half_h),
-                h_sixth = static_cast<float>(n.h_sixth);
+    const double hs = n.h, half_h = n.half_h, h_sixth = n.h_sixth;
     const uint32_t words = n.words;
     float* __restrict__ ring_exc = n.fring_exc;
     float* __restrict__ ring_inh = n.fring_inh;
@@ -549,7 +548,8 @@ __device__ __forceinline__ void neuron_window_f32(const DevNet& n, uint32_t i, i
     int64_t sic_next = first_hit(T0, d_a2n, sic_interval);
     const uint32_t live_in = live_arrivals(n, T0);
     const bool sic_live_n = live_in & 1u;
-    float V = 0, w = 0, gex = 0, dgex = 0, gin = 0, dgin = 0, Isic = 0, Vr = 0, b = 0;
+    // FP64 registers for the window; FP32 in memory
+    double V = 0, w = 0, gex = 0, dgex = 0, gin = 0, dgin = 0, Isic = 0, Vr = 0, b = 0;
     int refr = 0;
     float ex_n = 0, ih_n = 0, sic_n = 0;
     int ev_n = 0;
@@ -564,8 +564,8 @@ __device__ __forceinline__ void neuron_window_f32(const DevNet& n, uint32_t i, i
         dgin = n.fdgin[i];
         Isic = n.fIsic[i];
         refr = n.refr[i];
-        Vr = static_cast<float>(n.V_reset[i]);
-        b = static_cast<float>(n.b[i]);
+        Vr = n.V_reset[i];
+        b = n.b[i];
         const size_t s0 = static_cast<size_t>(row) * N + i;
         ex_n = ring_exc[s0];
         ih_n = ring_inh[s0];
@@ -596,7 +596,7 @@ __device__ __forceinline__ void neuron_window_f32(const DevNet& n, uint32_t i, i
                 if (has_poiss) ev_n = drive_n[dk1];
                 if (has_noise) noise_n = drive_noise[dk1];
             }
-            float exc_in = 0.0f, inh_in = 0.0f;
+            double exc_in = 0.0, inh_in = 0.0;
             if (ex != 0.0f) {
                 if (ex > 0.0f) {
                     exc_in += ex;
@@ -617,24 +617,23 @@ __device__ __forceinline__ void neuron_window_f32(const DevNet& n, uint32_t i, i
                 Isic = sic;
             }
             // Same counter streams as the FP64 step (k_drive): identical
-            // Poisson events, the noise current rounded to FP32.
+            // Poisson events, the noise current from FP32 draws.
             if (has_poiss && ev > 0) {
-                exc_in += static_cast<float>(in.poiss_weight * static_cast<double>(ev));
+                exc_in += in.poiss_weight * static_cast<double>(ev);
             }
-            float I_ext = Isic;
+            double I_ext = Isic;
             if (has_noise) {
                 I_ext += noise;
             }
-            if (exc_in != 0.0f) {
-                dgex += exc_in * pf.psc_ex;
+            if (exc_in != 0.0) {
+                dgex += exc_in * pd.psc_ex;
             }
-            if (inh_in != 0.0f) {
-                dgin += inh_in * pf.psc_in;
+            if (inh_in != 0.0) {
+                dgin += inh_in * pd.psc_in;
             }
             for (int sidx = 0; sidx < substeps; ++sidx) {
-                const bool sp = adex_substep_mixed(pf, pd, Vr, b, I_ext, hs, half_h, h_sixth, n.h,
-                                                   n.half_h, n.h_sixth, V, w, gex, dgex, gin,
-                                                   dgin, refr);
+                const bool sp = adex_substep_reg(pd, pf, Vr, b, I_ext, hs, half_h, h_sixth, V, w,
+                                                 gex, dgex, gin, dgin, refr);
                 spiked = spiked || sp;
             }
         }
@@ -645,13 +644,13 @@ __device__ __forceinline__ void neuron_window_f32(const DevNet& n, uint32_t i, i
         row = row_n;
     }
     if (valid) {
-        n.fV[i] = V;
-        n.fw[i] = w;
-        n.fgex[i] = gex;
-        n.fdgex[i] = dgex;
-        n.fgin[i] = gin;
-        n.fdgin[i] = dgin;
-        n.fIsic[i] = Isic;
+        n.fV[i] = static_cast<float>(V);
+        n.fw[i] = static_cast<float>(w);
+        n.fgex[i] = static_cast<float>(gex);
+        n.fdgex[i] = static_cast<float>(dgex);
+        n.fgin[i] = static_cast<float>(gin);
+        n.fdgin[i] = static_cast<float>(dgin);
+        n.fIsic[i] = static_cast<float>(Isic);
         n.refr[i] = refr;
     }
 }
@@ -1708,29 +1707,11 @@ InputK make_input_k(const asg_input_params& in, double dt) {
     return k;
 }
 
-NeuronKf make_neuron_kf(const asg_neuron_params& s, double h) {
+NeuronKf make_neuron_kf(const asg_neuron_params& s) {
     NeuronKf p{};
-    p.inv_C_m = static_cast<float>(1.0 / s.C_m);
-    p.neg_g_L = static_cast<float>(-s.g_L);
-    p.E_L = static_cast<float>(s.E_L);
-    p.V_th = static_cast<float>(s.V_th);
-    p.inv_Delta_T = static_cast<float>(1.0 / s.Delta_T);
-    p.a = static_cast<float>(s.a);
-    p.inv_tau_w = static_cast<float>(1.0 / s.tau_w);
-    p.V_peak = static_cast<float>(s.V_peak);
-    p.E_ex = static_cast<float>(s.E_ex);
-    p.E_in = static_cast<float>(s.E_in);
-    p.I_e = static_cast<float>(s.I_e);
-    p.gL_DeltaT = static_cast<float>(s.g_L * s.Delta_T);
-    constexpr double kE = 2.7182818284590452353602874713527;
-    p.psc_ex = static_cast<float>(kE / s.tau_syn_ex);
-    p.psc_in = static_cast<float>(kE / s.tau_syn_in);
-    p.ex_decay = static_cast<float>(asg_exp(-h / s.tau_syn_ex));
-    p.in_decay = static_cast<float>(asg_exp(-h / s.tau_syn_in));
-    p.near_V = static_cast<float>(s.V_th - kNearThresholdSlopes * s.Delta_T);
-    p.inv_C_m_d = 1.0 / s.C_m;
-    p.inv_Delta_T_d = 1.0 / s.Delta_T;
-    p.inv_tau_w_d = 1.0 / s.tau_w;
+    p.inv_C_m = 1.0 / s.C_m;
+    p.inv_Delta_T = 1.0 / s.Delta_T;
+    p.inv_tau_w = 1.0 / s.tau_w;
     return p;
 }
 
@@ -2025,8 +2006,8 @@ void build_slot(NetSlot& s, const asg_net_desc& d, int64_t total, hipStream_t st
     n.w_a2n = d.a2n.weight;
 
     n.precision = d.precision;
-    n.pkf[0] = make_neuron_kf(d.neuron[0], n.h);
-    n.pkf[1] = make_neuron_kf(d.neuron[1], n.h);
+    n.pkf[0] = make_neuron_kf(d.neuron[0]);
+    n.pkf[1] = make_neuron_kf(d.neuron[1]);
     n.akf = make_astro_kf(d.astro.c);
     const size_t N = pl.N, NA = pl.NA;
     std::vector<double> V0(N);

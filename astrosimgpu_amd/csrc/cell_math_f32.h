@@ -1,20 +1,18 @@
 // Per-cell math of the FP32 step (device only).
 //
-// State, conductances and rings are FP32; divisions by constants become
-// multiplications by FP32 reciprocals and exp/log are the hardware's fast
-// FP32 approximations. FP64 is kept where the dynamics are sensitive:
-//  * an AdEx substep runs entirely in FP64 for a whole wavefront when any of
-//    its cells has a membrane potential within kNearThresholdSlopes slope
-//    factors of V_th (or above) -- near threshold the exponential term
-//    grows like e^((V - V_th)/Delta_T) and decides the spike time. This
-//    island multiplies by FP64 reciprocals instead of dividing and uses a
-//    table-free FP64 exp accurate to ~3e-13: its result is rounded to FP32
-//    at the end of the substep;
-//  * the astrocyte's Ca-threshold test of the SIC, (Ca - SIC_th) * 1000 > 1,
-//    is evaluated in FP64 (Ca is a small difference above a threshold).
-// Random draws use the FP64 step's counter streams: identical Poisson counts,
-// and normals from the same two uniforms through FP32 hardware functions.
-// Synaptic input is summed with Neumaier's compensated summation.
+// State, conductances, rings and the drive are stored in FP32. A neuron's
+// state is loaded into FP64 registers for a window and integrated in FP64 --
+// FP64 vector instructions issue at the FP32 rate on this GPU, and near
+// threshold the exponential term e^((V - V_th)/Delta_T) decides the spike
+// time -- with the divisions by C_m, Delta_T and tau_w taken as products
+// with FP64 reciprocals and a table-free FP64 exp accurate to ~3e-13; it is
+// rounded to FP32 when stored at the end of the window. Astrocytes integrate
+// in FP32 with hardware reciprocals; their Ca-threshold test of the SIC,
+// (Ca - SIC_th) * 1000 > 1, is evaluated in FP64 (Ca is a small difference
+// above a threshold). Random draws use the FP64 step's counter streams:
+// identical Poisson counts, and normals from the same two uniforms through
+// FP32 hardware functions. Synaptic input is summed with Neumaier's
+// compensated summation.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -24,13 +22,9 @@
 
 namespace asg {
 
-constexpr float kNearThresholdSlopes = 2.0f;
-
+// FP64 reciprocals of the neuron parameters the FP32 step divides by.
 struct NeuronKf {
-    float inv_C_m, neg_g_L, E_L, V_th, inv_Delta_T, a, inv_tau_w, V_peak, E_ex, E_in, I_e;
-    float gL_DeltaT, psc_ex, psc_in, ex_decay, in_decay;
-    float near_V;  // V_th - kNearThresholdSlopes * Delta_T
-    double inv_C_m_d, inv_Delta_T_d, inv_tau_w_d;  // FP64 reciprocals of the island
+    double inv_C_m, inv_Delta_T, inv_tau_w;
 };
 
 struct AstroKf {
@@ -64,16 +58,7 @@ __device__ inline void comp_add(float& s, float& c, float x) {
     s = t;
 }
 
-__device__ inline void adex_rhs_f32(const NeuronKf& p, float V, float w, float g_ex, float g_in,
-                                    float I_ext, float& dV, float& dw) {
-    const float Vc = fminf(V, p.V_peak);
-    const float I_spike = p.gL_DeltaT * __expf((Vc - p.V_th) * p.inv_Delta_T);
-    dV = (p.neg_g_L * (Vc - p.E_L) + I_spike - g_ex * (Vc - p.E_ex) - g_in * (Vc - p.E_in) - w +
-          p.I_e + I_ext) * p.inv_C_m;
-    dw = (p.a * (Vc - p.E_L) - w) * p.inv_tau_w;
-}
-
-// e^x in FP64 for the near-threshold island (|x| below ~40 there):
+// e^x in FP64 for the neuron integration (|x| below ~40 there):
 // x = n ln2 + r with n = rint(x / ln2) and a two-part ln2 (Cody-Waite), so
 // |r| <= ln2 / 2, then e^r by its degree-10 Taylor polynomial (truncation
 // below r^11/11! < 3e-13 relative) and the scale 2^n.
@@ -97,111 +82,55 @@ __device__ inline double island_exp(double x) {
     return ldexp(p, static_cast<int>(n));
 }
 
-// The AdEx right-hand side in FP64 for the island (reference operation
-// order, neuron.cpp:85-99, with the divisions by C_m, Delta_T and tau_w
-// taken as products with their reciprocals).
+// The AdEx right-hand side in FP64 (reference operation order,
+// neuron.cpp:85-99, with the divisions by C_m, Delta_T and tau_w taken as
+// products with their reciprocals).
 __device__ inline void island_rhs(const NeuronK& p, const NeuronKf& pf, double V, double w,
                                   double g_ex, double g_in, double I_ext, double& dV,
                                   double& dw) {
     const double Vc = (p.V_peak < V) ? p.V_peak : V;
-    const double I_spike = p.gL_DeltaT * island_exp((Vc - p.V_th) * pf.inv_Delta_T_d);
+    const double I_spike = p.gL_DeltaT * island_exp((Vc - p.V_th) * pf.inv_Delta_T);
     const double I_syn_ex = g_ex * (Vc - p.E_ex);
     const double I_syn_in = g_in * (Vc - p.E_in);
     dV = (p.neg_g_L * (Vc - p.E_L) + I_spike - I_syn_ex - I_syn_in - w + p.I_e + I_ext) *
-         pf.inv_C_m_d;
-    dw = (p.a * (Vc - p.E_L) - w) * pf.inv_tau_w_d;
+         pf.inv_C_m;
+    dw = (p.a * (Vc - p.E_L) - w) * pf.inv_tau_w;
 }
 
 // One substep of the AdEx + alpha-conductance update (neuron.cpp:101-141)
-// in FP64 from the FP32 state, rounded back to FP32: RK4 for a
-// non-refractory cell, the clamped Euler update of w for a refractory one.
-__device__ inline bool adex_substep_f64(const NeuronKf& pf, const NeuronK& pd, float V_reset,
-                                        float b, float I_ext, double hs_d, double half_h_d,
-                                        double h_sixth_d, float& V, float& w, float& g_ex,
-                                        float& dg_ex, float& g_in, float& dg_in, int& refr) {
-    const double ge = g_ex, dge = dg_ex, gi = g_in, dgi = dg_in, I = I_ext;
-    const double ge_next = pd.ex_decay * (ge + hs_d * dge);
-    const double gi_next = pd.in_decay * (gi + hs_d * dgi);
-    double Vd = V, wd = w;
+// on FP64 registers: RK4 for a non-refractory cell, the clamped Euler update
+// of w for a refractory one. Returns true on a spike (the caller keeps at
+// most one per step).
+__device__ inline bool adex_substep_reg(const NeuronK& pd, const NeuronKf& pf, double V_reset,
+                                        double b, double I_ext, double hs, double half_h,
+                                        double h_sixth, double& V, double& w, double& g_ex,
+                                        double& dg_ex, double& g_in, double& dg_in, int& refr) {
+    const double ge_next = pd.ex_decay * (g_ex + hs * dg_ex);
+    const double gi_next = pd.in_decay * (g_in + hs * dg_in);
     if (refr > 0) {
-        Vd = V_reset;
-        const double Vc = (pd.V_peak < Vd) ? pd.V_peak : Vd;
-        wd += hs_d * ((pd.a * (Vc - pd.E_L) - wd) * pf.inv_tau_w_d);
+        V = V_reset;
+        const double Vc = (pd.V_peak < V) ? pd.V_peak : V;
+        w += hs * ((pd.a * (Vc - pd.E_L) - w) * pf.inv_tau_w);
     } else {
-        const double ge_mid = 0.5 * (ge + ge_next);
-        const double gi_mid = 0.5 * (gi + gi_next);
+        const double ge_mid = 0.5 * (g_ex + ge_next);
+        const double gi_mid = 0.5 * (g_in + gi_next);
         double k1V, k1w, k2V, k2w, k3V, k3w, k4V, k4w;
-        island_rhs(pd, pf, Vd, wd, ge, gi, I, k1V, k1w);
-        island_rhs(pd, pf, Vd + half_h_d * k1V, wd + half_h_d * k1w, ge_mid, gi_mid, I, k2V, k2w);
-        island_rhs(pd, pf, Vd + half_h_d * k2V, wd + half_h_d * k2w, ge_mid, gi_mid, I, k3V, k3w);
-        island_rhs(pd, pf, Vd + hs_d * k3V, wd + hs_d * k3w, ge_next, gi_next, I, k4V, k4w);
-        Vd += h_sixth_d * (k1V + 2.0 * k2V + 2.0 * k3V + k4V);
-        wd += h_sixth_d * (k1w + 2.0 * k2w + 2.0 * k3w + k4w);
+        island_rhs(pd, pf, V, w, g_ex, g_in, I_ext, k1V, k1w);
+        island_rhs(pd, pf, V + half_h * k1V, w + half_h * k1w, ge_mid, gi_mid, I_ext, k2V, k2w);
+        island_rhs(pd, pf, V + half_h * k2V, w + half_h * k2w, ge_mid, gi_mid, I_ext, k3V, k3w);
+        island_rhs(pd, pf, V + hs * k3V, w + hs * k3w, ge_next, gi_next, I_ext, k4V, k4w);
+        V += h_sixth * (k1V + 2.0 * k2V + 2.0 * k3V + k4V);
+        w += h_sixth * (k1w + 2.0 * k2w + 2.0 * k3w + k4w);
     }
-    g_ex = static_cast<float>(ge_next);
-    dg_ex = static_cast<float>(pd.ex_decay * dge);
-    g_in = static_cast<float>(gi_next);
-    dg_in = static_cast<float>(pd.in_decay * dgi);
+    g_ex = ge_next;
+    dg_ex = pd.ex_decay * dg_ex;
+    g_in = gi_next;
+    dg_in = pd.in_decay * dg_in;
     if (refr > 0) {
         --refr;
-        V = V_reset;
-        w = static_cast<float>(wd);
         return false;
     }
-    if (Vd >= pd.V_peak) {
-        V = V_reset;
-        w = static_cast<float>(wd + static_cast<double>(b));
-        refr = pd.refr_steps;
-        return true;
-    }
-    V = static_cast<float>(Vd);
-    w = static_cast<float>(wd);
-    return false;
-}
-
-// One substep of the AdEx + alpha-conductance update for the wavefront's
-// active lanes: in FP64 (adex_substep_f64) when any of them is near
-// threshold and not refractory, otherwise in FP32. The choice is uniform
-// across the wavefront: FP64 vector instructions issue at the FP32 rate on
-// this GPU, so one FP64 pass for all lanes costs less than an FP32 pass
-// plus a divergent FP64 pass for some. Returns true on a spike (the caller
-// keeps at most one per step).
-__device__ inline bool adex_substep_mixed(const NeuronKf& pf, const NeuronK& pd, float V_reset,
-                                          float b, float I_ext, float hs, float half_h,
-                                          float h_sixth, double hs_d, double half_h_d,
-                                          double h_sixth_d, float& V, float& w, float& g_ex,
-                                          float& dg_ex, float& g_in, float& dg_in, int& refr) {
-    if (__ballot(V > pf.near_V && refr == 0) != 0) {
-        return adex_substep_f64(pf, pd, V_reset, b, I_ext, hs_d, half_h_d, h_sixth_d, V, w, g_ex,
-                                dg_ex, g_in, dg_in, refr);
-    }
-    const float g_ex_next = pf.ex_decay * (g_ex + hs * dg_ex);
-    const float dg_ex_next = pf.ex_decay * dg_ex;
-    const float g_in_next = pf.in_decay * (g_in + hs * dg_in);
-    const float dg_in_next = pf.in_decay * dg_in;
-    if (refr > 0) {
-        V = V_reset;
-        float dV, dw;
-        adex_rhs_f32(pf, V, w, g_ex, g_in, I_ext, dV, dw);
-        w += hs * dw;
-    } else {
-        const float g_ex_mid = 0.5f * (g_ex + g_ex_next);
-        const float g_in_mid = 0.5f * (g_in + g_in_next);
-        float k1V, k1w, k2V, k2w, k3V, k3w, k4V, k4w;
-        adex_rhs_f32(pf, V, w, g_ex, g_in, I_ext, k1V, k1w);
-        adex_rhs_f32(pf, V + half_h * k1V, w + half_h * k1w, g_ex_mid, g_in_mid, I_ext, k2V, k2w);
-        adex_rhs_f32(pf, V + half_h * k2V, w + half_h * k2w, g_ex_mid, g_in_mid, I_ext, k3V, k3w);
-        adex_rhs_f32(pf, V + hs * k3V, w + hs * k3w, g_ex_next, g_in_next, I_ext, k4V, k4w);
-        V += h_sixth * (k1V + 2.0f * k2V + 2.0f * k3V + k4V);
-        w += h_sixth * (k1w + 2.0f * k2w + 2.0f * k3w + k4w);
-    }
-    g_ex = g_ex_next;
-    dg_ex = dg_ex_next;
-    g_in = g_in_next;
-    dg_in = dg_in_next;
-    if (refr > 0) {
-        --refr;
-    } else if (V >= pf.V_peak) {
+    if (V >= pd.V_peak) {
         V = V_reset;
         w += b;
         refr = pd.refr_steps;
