@@ -91,9 +91,17 @@ def log(*a):
 # s = 8 bytes in FP64, 4 in FP32.
 def cell_bytes(n_neurons: int, n_astro: int, window: int, state_bytes: int,
                noise: bool = True) -> float:
+    """Bytes one cell-kernel launch must move for a window: per neuron the 7
+    state values read and written (14 s), the refractory counter (8) and
+    V_reset, b (16), and per step the exc and inh ring slots (2 s) and the
+    drive (4, + s with noise); per astrocyte 3 state values read and written
+    and 4 parameters (10 s), and per step the n2a slot and the SIC factor
+    (2 s). Reads of SIC slots of live exchanges and clears of nonzero ring
+    slots depend on the activity and are not counted, so the bandwidth
+    derived from these bytes is a lower bound."""
     s = state_bytes
-    neuron = 14 * s + 8 + 16 + window * (5 * s + 4 + (s if noise else 0))
-    astro = 10 * s + window * 3 * s
+    neuron = 14 * s + 8 + 16 + window * (2 * s + 4 + (s if noise else 0))
+    astro = 10 * s + window * 2 * s
     return n_neurons * neuron + n_astro * astro
 
 
