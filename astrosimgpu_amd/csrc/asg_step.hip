@@ -139,6 +139,11 @@ struct DevNet {
     uint32_t n_exc, n_inh, n_neur, n_astro;
     uint32_t words;        // ceil(n_neur / 64)
     uint32_t cw;           // 64-spike chunks of window spike indices the masks hold
+    // Row strides (elements) of the [rows][neurons] and [rows][astrocytes]
+    // arrays -- rings, drive, SIC factors, delivery masks, SIC gather lists:
+    // n_neur and n_astro rounded up to 64, so that a wavefront's 64 lanes
+    // read whole aligned lines.
+    uint32_t ldn, lda;
     int32_t S;             // ring slots: longest delay + longest window
     int32_t Wmax;          // longest window this network's scratch holds
     int32_t d_exc, d_inh, d_n2a, d_a2n;
@@ -309,6 +314,7 @@ __device__ __forceinline__ void neuron_window(const DevNet& n, uint32_t i, int64
     // Everything the loop reads is copied into registers once; the rings are
     // reached through restrict pointers so their stores cannot force reloads.
     const uint32_t N = n.n_neur;
+    const size_t LN = n.ldn;  // row stride of the [rows][neurons] arrays
     const bool valid = i < N;
     const int pi = (i < n.n_exc) ? 0 : 1;
     const NeuronK p = n.pk[pi];
@@ -347,7 +353,7 @@ __device__ __forceinline__ void neuron_window(const DevNet& n, uint32_t i, int64
         refr = n.refr[i];
         Vr = n.V_reset[i];
         b = n.b[i];
-        const size_t s0 = static_cast<size_t>(row) * N + i;
+        const size_t s0 = static_cast<size_t>(row) * LN + i;
         ex_n = ring_exc[s0];
         ih_n = ring_inh[s0];
         if (sic_live_n) sic_n = ring_sic[s0];
@@ -364,16 +370,16 @@ __device__ __forceinline__ void neuron_window(const DevNet& n, uint32_t i, int64
         }
         const bool next_live = (live_in >> (k + 1)) & 1u;
         if (valid) {
-            const size_t slot = static_cast<size_t>(row) * N + i;
+            const size_t slot = static_cast<size_t>(row) * LN + i;
             const double ex = ex_n, ih = ih_n, sic = sic_n;
             const int ev = ev_n;
             const double noise = noise_n;
             if (k + 1 < W) {
-                const size_t nx = static_cast<size_t>(row_n) * N + i;
+                const size_t nx = static_cast<size_t>(row_n) * LN + i;
                 ex_n = ring_exc[nx];
                 ih_n = ring_inh[nx];
                 sic_n = next_live ? ring_sic[nx] : 0.0;
-                const size_t dk1 = static_cast<size_t>(k + 1) * N + i;
+                const size_t dk1 = static_cast<size_t>(k + 1) * LN + i;
                 if (has_poiss) ev_n = drive_n[dk1];
                 if (has_noise) noise_n = drive_noise[dk1];
             }
@@ -440,6 +446,7 @@ __device__ __forceinline__ void neuron_window(const DevNet& n, uint32_t i, int64
 
 __device__ __forceinline__ void astro_window(const DevNet& n, uint32_t a, int64_t T0, int W) {
     const uint32_t NA = n.n_astro;
+    const size_t LA = n.lda;  // row stride of the [rows][astrocytes] arrays
     if (a >= NA) {
         return;
     }
@@ -455,7 +462,7 @@ __device__ __forceinline__ void astro_window(const DevNet& n, uint32_t a, int64_
     const double Ca_tot = n.Ca_tot[a], IP3_0 = n.IP3_0[a], tau = n.tau_IP3[a],
                  delta = n.delta_IP3[a];
     int row = static_cast<int>(T0 % S);  // ring row of the current step
-    double v_n = ring_astro[static_cast<size_t>(row) * NA + a];
+    double v_n = ring_astro[static_cast<size_t>(row) * LA + a];
     // noise index T / steps_per_noise, redrawn at the window start and at
     // every multiple of steps_per_noise
     const int64_t spn = ap.in.steps_per_noise;
@@ -469,10 +476,10 @@ __device__ __forceinline__ void astro_window(const DevNet& n, uint32_t a, int64_
     uint64_t pkey = rng_key(seed ^ kSaltAstroPoisson, static_cast<uint64_t>(T0) * kStreamStride + a);
     for (int k = 0; k < W; ++k, pkey += kStreamStride * kStreamMul) {
         const int row_n = ring_next(row, S);
-        const size_t slot = static_cast<size_t>(row) * NA + a;
+        const size_t slot = static_cast<size_t>(row) * LA + a;
         const double v = v_n;
         if (k + 1 < W) {
-            v_n = ring_astro[static_cast<size_t>(row_n) * NA + a];
+            v_n = ring_astro[static_cast<size_t>(row_n) * LA + a];
         }
         double ip3_in = 0.0;
         if (v != 0.0) {
@@ -511,7 +518,7 @@ __device__ __forceinline__ void astro_window(const DevNet& n, uint32_t a, int64_
         if (k == sic_next) {
             sic_next += sic_interval;
             const double f = sic_factor(Ca, ap.SIC_th, ap.SIC_scale);
-            fac[static_cast<size_t>(k) * NA + a] = f;
+            fac[static_cast<size_t>(k) * LA + a] = f;
             live_bits |= static_cast<uint32_t>(f != 0.0) << k;
         }
         row = row_n;
@@ -526,6 +533,7 @@ __device__ __forceinline__ void astro_window(const DevNet& n, uint32_t a, int64_
 
 __device__ __forceinline__ void neuron_window_f32(const DevNet& n, uint32_t i, int64_t T0, int W) {
     const uint32_t N = n.n_neur;
+    const size_t LN = n.ldn;  // row stride of the [rows][neurons] arrays
     const bool valid = i < N;
     const int pi = (i < n.n_exc) ? 0 : 1;
     const NeuronKf pf = n.pkf[pi];
@@ -566,7 +574,7 @@ __device__ __forceinline__ void neuron_window_f32(const DevNet& n, uint32_t i, i
         refr = n.refr[i];
         Vr = n.V_reset[i];
         b = n.b[i];
-        const size_t s0 = static_cast<size_t>(row) * N + i;
+        const size_t s0 = static_cast<size_t>(row) * LN + i;
         ex_n = ring_exc[s0];
         ih_n = ring_inh[s0];
         if (sic_live_n) sic_n = ring_sic[s0];
@@ -583,16 +591,16 @@ __device__ __forceinline__ void neuron_window_f32(const DevNet& n, uint32_t i, i
         }
         const bool next_live = (live_in >> (k + 1)) & 1u;
         if (valid) {
-            const size_t slot = static_cast<size_t>(row) * N + i;
+            const size_t slot = static_cast<size_t>(row) * LN + i;
             const float ex = ex_n, ih = ih_n, sic = sic_n;
             const int ev = ev_n;
             const float noise = noise_n;
             if (k + 1 < W) {
-                const size_t nx = static_cast<size_t>(row_n) * N + i;
+                const size_t nx = static_cast<size_t>(row_n) * LN + i;
                 ex_n = ring_exc[nx];
                 ih_n = ring_inh[nx];
                 sic_n = next_live ? ring_sic[nx] : 0.0f;
-                const size_t dk1 = static_cast<size_t>(k + 1) * N + i;
+                const size_t dk1 = static_cast<size_t>(k + 1) * LN + i;
                 if (has_poiss) ev_n = drive_n[dk1];
                 if (has_noise) noise_n = drive_noise[dk1];
             }
@@ -657,6 +665,7 @@ __device__ __forceinline__ void neuron_window_f32(const DevNet& n, uint32_t i, i
 
 __device__ __forceinline__ void astro_window_f32(const DevNet& n, uint32_t a, int64_t T0, int W) {
     const uint32_t NA = n.n_astro;
+    const size_t LA = n.lda;  // row stride of the [rows][astrocytes] arrays
     if (a >= NA) {
         return;
     }
@@ -675,7 +684,7 @@ __device__ __forceinline__ void astro_window_f32(const DevNet& n, uint32_t a, in
     const float Ca_tot = n.fCa_tot[a], IP3_0 = n.fIP3_0[a], inv_tau = n.finv_tau[a],
                 delta = n.fdelta[a];
     int row = static_cast<int>(T0 % S);  // ring row of the current step
-    float v_n = ring_astro[static_cast<size_t>(row) * NA + a];
+    float v_n = ring_astro[static_cast<size_t>(row) * LA + a];
     // noise index T / steps_per_noise, redrawn at the window start and at
     // every multiple of steps_per_noise
     const int64_t spn = ain.steps_per_noise;
@@ -689,10 +698,10 @@ __device__ __forceinline__ void astro_window_f32(const DevNet& n, uint32_t a, in
     uint64_t pkey = rng_key(seed ^ kSaltAstroPoisson, static_cast<uint64_t>(T0) * kStreamStride + a);
     for (int k = 0; k < W; ++k, pkey += kStreamStride * kStreamMul) {
         const int row_n = ring_next(row, S);
-        const size_t slot = static_cast<size_t>(row) * NA + a;
+        const size_t slot = static_cast<size_t>(row) * LA + a;
         const float v = v_n;
         if (k + 1 < W) {
-            v_n = ring_astro[static_cast<size_t>(row_n) * NA + a];
+            v_n = ring_astro[static_cast<size_t>(row_n) * LA + a];
         }
         float ip3_in = 0.0f;
         if (v != 0.0f) {
@@ -729,7 +738,7 @@ __device__ __forceinline__ void astro_window_f32(const DevNet& n, uint32_t a, in
         if (k == sic_next) {
             sic_next += sic_interval;
             const float f = sic_factor_mixed(Ca, SIC_th, SIC_scale);
-            fac[static_cast<size_t>(k) * NA + a] = f;
+            fac[static_cast<size_t>(k) * LA + a] = f;
             live_bits |= static_cast<uint32_t>(f != 0.0f) << k;
         }
         row = row_n;
@@ -744,7 +753,7 @@ __device__ __forceinline__ void astro_window_f32(const DevNet& n, uint32_t a, in
 // of m * w_a2n * factor, compensated.
 template <int GM>
 __device__ void deliver_sic_f32(const DevNet& n, uint32_t t, int g0, int G) {
-    const uint32_t N = n.n_neur;
+    const size_t LN = n.ldn;
     const int64_t T0 = n.win_T0;
     uint32_t send = 0;  // bit j: window step g0 + j sends an exchange
     for (int64_t j = first_hit(T0 + g0, 0, n.sic_interval); j < G; j += n.sic_interval) {
@@ -760,20 +769,20 @@ __device__ void deliver_sic_f32(const DevNet& n, uint32_t t, int g0, int G) {
     if (live != 0) {
         const uint32_t* __restrict__ src = n.sic_src;
         const uint32_t* __restrict__ mult = n.sic_mult;
-        const float* __restrict__ fac = n.ffac + static_cast<size_t>(g0) * n.n_astro;
-        const size_t NA = n.n_astro;
+        const float* __restrict__ fac = n.ffac + static_cast<size_t>(g0) * n.lda;
+        const size_t LA = n.lda;
         const float w_a2n = static_cast<float>(n.w_a2n);
         for (uint32_t q = 0; q < n.sic_width; ++q) {
-            const uint32_t m = mult[static_cast<size_t>(q) * N + t];
+            const uint32_t m = mult[static_cast<size_t>(q) * LN + t];
             if (m == 0) {
                 break;
             }
-            const uint32_t a = src[static_cast<size_t>(q) * N + t];
+            const uint32_t a = src[static_cast<size_t>(q) * LN + t];
             const float fm = static_cast<float>(m);
 #pragma unroll
             for (int j = 0; j < GM; ++j) {
                 if ((live >> j) & 1u) {
-                    const float fa = fac[j * NA + a];
+                    const float fa = fac[j * LA + a];
                     if (fa != 0.0f) {
                         comp_add(acc[j], comp[j], fm * (w_a2n * fa));
                     }
@@ -790,7 +799,7 @@ __device__ void deliver_sic_f32(const DevNet& n, uint32_t t, int g0, int G) {
         const float v = acc[j] + comp[j];
         if ((live >> j) & 1u) {
             const int row = row0 + j < S ? row0 + j : row0 + j - S;
-            n.fring_sic[static_cast<size_t>(row) * N + t] = v;
+            n.fring_sic[static_cast<size_t>(row) * LN + t] = v;
         }
     }
 }
@@ -826,6 +835,7 @@ __global__ __launch_bounds__(64) void k_drive(DevNet* nets, const CellBlock* nbl
         return;
     }
     const uint32_t N = n.n_neur;
+    const size_t LN = n.ldn;
     const uint32_t lane = threadIdx.x;
     const uint32_t i = cb.first + lane;
     const bool valid = i < N;
@@ -839,7 +849,7 @@ __global__ __launch_bounds__(64) void k_drive(DevNet* nets, const CellBlock* nbl
         if (n.in[0].lambda > 30.0 || n.in[1].lambda > 30.0) {
             // the normal approximation above lambda = 30: one lane per draw
             for (int k = 0; valid && k < W; ++k) {
-                out[static_cast<size_t>(k) * N + i] =
+                out[static_cast<size_t>(k) * LN + i] =
                     rng_poisson_fresh(pseed, static_cast<uint64_t>(T0 + k) * kStreamStride + i,
                                       in.lambda, in.limit);
             }
@@ -858,7 +868,7 @@ __global__ __launch_bounds__(64) void k_drive(DevNet* nets, const CellBlock* nbl
                     pend = p > in.limit;
                 }
                 if (valid && !pend) {
-                    out[static_cast<size_t>(k) * N + i] = 0;
+                    out[static_cast<size_t>(k) * LN + i] = 0;
                 }
                 const uint64_t b = __ballot(pend);
                 if (pend) {
@@ -890,7 +900,7 @@ __global__ __launch_bounds__(64) void k_drive(DevNet* nets, const CellBlock* nbl
                             pend = true;
                             key = (cnt << 10) | (kk << 6) | l;
                         } else {
-                            out[static_cast<size_t>(kk) * N + ii] = static_cast<int32_t>(cnt);
+                            out[static_cast<size_t>(kk) * LN + ii] = static_cast<int32_t>(cnt);
                         }
                     }
                     __syncthreads();  // this chunk is read before survivors overwrite it
@@ -931,7 +941,7 @@ __global__ __launch_bounds__(64) void k_drive(DevNet* nets, const CellBlock* nbl
                     key += idx_key;
                     z = rng_normal_fresh_f32(key);
                 }
-                n.fdrive_noise[static_cast<size_t>(k) * N + i] = sd * z;
+                n.fdrive_noise[static_cast<size_t>(k) * LN + i] = sd * z;
             }
         } else {
             uint64_t ctr = 0;
@@ -943,7 +953,7 @@ __global__ __launch_bounds__(64) void k_drive(DevNet* nets, const CellBlock* nbl
                     ctr = 0;
                     z = rng_normal_fresh_keyed(key, ctr);
                 }
-                n.drive_noise[static_cast<size_t>(k) * N + i] = 0.0 + in.noise_std * z;
+                n.drive_noise[static_cast<size_t>(k) * LN + i] = 0.0 + in.noise_std * z;
             }
         }
     }
@@ -1060,7 +1070,7 @@ __global__ __launch_bounds__(64) void k_compact(DevNet* nets, const int32_t* act
 // astrocytes ascending, written to ring_sic at the arrival slot.
 template <int GM>
 __device__ void deliver_sic(const DevNet& n, uint32_t t, int g0, int G) {
-    const uint32_t N = n.n_neur;
+    const size_t LN = n.ldn;
     const int64_t T0 = n.win_T0;
     uint32_t send = 0;  // bit j: window step g0 + j sends an exchange
     for (int64_t j = first_hit(T0 + g0, 0, n.sic_interval); j < G; j += n.sic_interval) {
@@ -1077,22 +1087,22 @@ __device__ void deliver_sic(const DevNet& n, uint32_t t, int g0, int G) {
     if (live != 0) {
         const uint32_t* __restrict__ src = n.sic_src;
         const uint32_t* __restrict__ mult = n.sic_mult;
-        const double* __restrict__ fac = n.fac + static_cast<size_t>(g0) * n.n_astro;
-        const size_t NA = n.n_astro;
+        const double* __restrict__ fac = n.fac + static_cast<size_t>(g0) * n.lda;
+        const size_t LA = n.lda;
         const double w_a2n = n.w_a2n;
         for (uint32_t q = 0; q < n.sic_width; ++q) {
-            const uint32_t m = mult[static_cast<size_t>(q) * N + t];
+            const uint32_t m = mult[static_cast<size_t>(q) * LN + t];
             if (m == 0) {
                 break;
             }
-            const uint32_t a = src[static_cast<size_t>(q) * N + t];
+            const uint32_t a = src[static_cast<size_t>(q) * LN + t];
             double v[GM];
             uint32_t nz = 0;
 #pragma unroll
             for (int j = 0; j < GM; ++j) {
                 v[j] = 0.0;
                 if ((live >> j) & 1u) {
-                    const double fa = fac[j * NA + a];
+                    const double fa = fac[j * LA + a];
                     v[j] = w_a2n * fa;
                     nz |= static_cast<uint32_t>(fa != 0.0) << j;
                 }
@@ -1117,7 +1127,7 @@ __device__ void deliver_sic(const DevNet& n, uint32_t t, int g0, int G) {
     for (int j = 0; j < GM; ++j) {
         if ((live >> j) & 1u) {
             const int row = row0 + j < S ? row0 + j : row0 + j - S;
-            n.ring_sic[static_cast<size_t>(row) * N + t] = acc[j];
+            n.ring_sic[static_cast<size_t>(row) * LN + t] = acc[j];
         }
     }
 }
@@ -1179,7 +1189,7 @@ struct StepWords {
 // of its rows. OR is order-independent, so wavefronts may mark in any order.
 // The window's words and the spiking neurons' row bounds are loaded once.
 __device__ void mark_steps(const DevNet& n, uint32_t first, uint32_t lane, int k0, int k1) {
-    const uint32_t N = n.n_neur, NE = n.n_exc;
+    const uint32_t NE = n.n_exc;
     const StepWords sw(n, first >> 6, lane, 0, n.win_W);
     uint32_t window_total = 0;
     const uint32_t before_k = wave_exclusive_scan(sw.tot, window_total);
@@ -1222,9 +1232,9 @@ __device__ void mark_steps(const DevNet& n, uint32_t first, uint32_t lane, int k
             const size_t chunk = jw >> 6;
             const uint32_t q0 = lane_u32(r0, L), q1 = lane_u32(r1, L);
             mark_row(first + static_cast<uint32_t>(L) < NE ? n.exc_tgt : n.inh_tgt, q0, q1,
-                     n.mask_n + chunk * N, bit, lane);
+                     n.mask_n + chunk * n.ldn, bit, lane);
             if (first + static_cast<uint32_t>(L) < NE) {
-                mark_row(n.n2a_tgt, lane_u32(a0, L), lane_u32(a1, L), n.mask_a + chunk * n.n_astro,
+                mark_row(n.n2a_tgt, lane_u32(a0, L), lane_u32(a1, L), n.mask_a + chunk * n.lda,
                          bit, lane);
             }
         }
@@ -1474,6 +1484,7 @@ __device__ __forceinline__ void sum_window(const DevNet& n, int kind, uint32_t i
     const uint32_t hi = min(k1 == W ? wc.before[kWindowCap] : wc.before_of(k1), 64 * n.cw);
     const bool neuron = kind == 0;
     const size_t targets = neuron ? N : static_cast<size_t>(n.n_astro);
+    const size_t ld = neuron ? static_cast<size_t>(n.ldn) : static_cast<size_t>(n.lda);
     if (i >= targets) {
         return;
     }
@@ -1487,7 +1498,7 @@ __device__ __forceinline__ void sum_window(const DevNet& n, int kind, uint32_t i
     size_t run_slot = 0;
     int run_key = -1;       // 2 * step + (inhibitory ? 1 : 0)
     for (uint32_t c = lo >> 6; lo < hi && c <= (hi - 1) >> 6; ++c) {
-        uint64_t m = mask[c * targets];
+        uint64_t m = mask[c * ld];
         const uint32_t b0 = max(lo, c * 64) - c * 64, b1 = min(hi, c * 64 + 64) - c * 64;
         m &= (b1 == 64 ? ~0ULL : (1ULL << b1) - 1ULL) & ~((1ULL << b0) - 1ULL);
         if (m == 0) {
@@ -1495,9 +1506,9 @@ __device__ __forceinline__ void sum_window(const DevNet& n, int kind, uint32_t i
         }
         const bool owned = (k0 == 0 || lo <= c * 64) && (k1 == W || hi >= c * 64 + 64);
         if (owned) {
-            mask[c * targets] = 0;
+            mask[c * ld] = 0;
         } else {
-            atomicAnd(reinterpret_cast<unsigned long long*>(mask + c * targets), ~m);
+            atomicAnd(reinterpret_cast<unsigned long long*>(mask + c * ld), ~m);
         }
         while (m) {
             const uint32_t jw = c * 64 + static_cast<uint32_t>(__ffsll(static_cast<long long>(m)) - 1);
@@ -1516,7 +1527,7 @@ __device__ __forceinline__ void sum_window(const DevNet& n, int kind, uint32_t i
                     const int row0 = inh ? rowI0 : rowE0;
                     const int row = row0 + k < S ? row0 + k : row0 + k - S;
                     run_ring = inh ? rings.inh : rings.exc;
-                    run_slot = static_cast<size_t>(row) * N + i;
+                    run_slot = static_cast<size_t>(row) * ld + i;
                     acc.start(run_ring[run_slot]);
                 }
                 acc.add(inh ? -spk_c[cv] : spk_c[cv]);
@@ -1528,7 +1539,7 @@ __device__ __forceinline__ void sum_window(const DevNet& n, int kind, uint32_t i
                     run_key = k;
                     const int row = rowA0 + k < S ? rowA0 + k : rowA0 + k - S;
                     run_ring = rings.astro;
-                    run_slot = static_cast<size_t>(row) * targets + i;
+                    run_slot = static_cast<size_t>(row) * ld + i;
                     acc.start(run_ring[run_slot]);
                 }
                 // spike j's source counts its n2a multiplicity m times: m
@@ -1765,13 +1776,15 @@ std::string validate(const asg_net_desc& d) {
     return "";
 }
 
+inline size_t round_up64(size_t n) { return (n + 63) / 64 * 64; }
+
 // Host-side derived connectivity: deduplicated n2a rows and the SIC gather
 // lists (a2n transposed to targets, astrocytes ascending, with multiplicity).
 struct DerivedConn {
     std::vector<uint32_t> n2a_row, n2a_tgt, n2a_mult;
     uint32_t sic_width = 0;
     int64_t sic_pairs = 0;
-    std::vector<uint32_t> sic_src, sic_mult;  // [width][N]
+    std::vector<uint32_t> sic_src, sic_mult;  // [width][round_up64(N)]
 };
 
 DerivedConn derive(const asg_net_desc& d) {
@@ -1809,8 +1822,9 @@ DerivedConn derive(const asg_net_desc& d) {
     for (uint32_t t = 0; t < N; ++t) width = std::max(width, count[t + 1]);
     out.sic_width = width;
     out.sic_pairs = 0;
-    out.sic_src.assign(static_cast<size_t>(width) * N, 0);
-    out.sic_mult.assign(static_cast<size_t>(width) * N, 0);
+    const size_t ld = round_up64(N);
+    out.sic_src.assign(static_cast<size_t>(width) * ld, 0);
+    out.sic_mult.assign(static_cast<size_t>(width) * ld, 0);
     std::vector<uint32_t> fill(N, 0);
     std::fill(last.begin(), last.end(), UINT32_MAX);
     for (uint32_t a = 0; a < d.n_astro; ++a) {
@@ -1818,13 +1832,13 @@ DerivedConn derive(const asg_net_desc& d) {
             const uint32_t t = d.a2n.target[q];
             if (last[t] != a) {
                 last[t] = a;
-                const size_t at = static_cast<size_t>(fill[t]) * N + t;
+                const size_t at = static_cast<size_t>(fill[t]) * ld + t;
                 out.sic_src[at] = a;
                 out.sic_mult[at] = 1;
                 ++fill[t];
                 ++out.sic_pairs;
             } else {
-                ++out.sic_mult[static_cast<size_t>(fill[t] - 1) * N + t];
+                ++out.sic_mult[static_cast<size_t>(fill[t] - 1) * ld + t];
             }
         }
     }
@@ -1840,7 +1854,7 @@ int window_of(const asg_net_desc& d) {
 // Byte plan of one network's device arrays, so the footprint query and the
 // allocation agree.
 struct Plan {
-    size_t N, NA, words, S, Wmax, cw, r_cap, hist_bits;
+    size_t N, NA, words, S, Wmax, cw, r_cap, hist_bits, ldn, lda;
     size_t n_exc_syn, n_inh_syn, n2a_dedup, sic_pairs;
     size_t bytes;
 };
@@ -1850,6 +1864,8 @@ Plan plan_of(const asg_net_desc& d, const DerivedConn* dc) {
     p.N = d.n_exc + d.n_inh;
     p.NA = d.n_astro;
     p.words = (p.N + 63) / 64;
+    p.ldn = round_up64(p.N);
+    p.lda = round_up64(p.NA);
     p.Wmax = static_cast<size_t>(window_of(d));
     const int dmax = std::max(std::max(d.exc.delay_steps, d.inh.delay_steps),
                               std::max(d.n2a.delay_steps, d.a2n.delay_steps));
@@ -1864,7 +1880,7 @@ Plan plan_of(const asg_net_desc& d, const DerivedConn* dc) {
     // delivered in part.
     p.cw = (p.Wmax * p.N + 63) / 64;
     {
-        const size_t per_chunk = (p.N + p.NA) * 8;
+        const size_t per_chunk = (round_up64(p.N) + round_up64(p.NA)) * 8;
         if (p.cw * per_chunk > kMaskBudget && per_chunk > 0) {
             p.cw = std::min(p.cw, std::max<size_t>(kMinChunks, kMaskBudget / per_chunk));
         }
@@ -1880,16 +1896,16 @@ Plan plan_of(const asg_net_desc& d, const DerivedConn* dc) {
     size_t b = 0;
     b += p.N * (7 * fb + 4 + 2 * 8 + 3 * 8);       // neuron state, V_reset/b, STP
     b += p.NA * (3 * fb + 4 * fb);                  // astrocyte state and parameters
-    b += p.S * p.N * 3 * fb + p.S * p.NA * fb + p.Wmax * p.NA * fb + p.hist_bits / 8;
+    b += p.S * p.ldn * 3 * fb + p.S * p.lda * fb + p.Wmax * p.lda * fb + p.hist_bits / 8;
     b += (d.n_exc + 1 + p.n_exc_syn + d.n_inh + 1 + p.n_inh_syn) * 4;
     b += (2 * (d.n_exc + 1) + 2 * p.n2a_dedup) * 4;
     b += 2 * p.sic_pairs * 4;
     b += p.Wmax * (p.words * 12 + p.N * 4 + 8 + p.N * 16);
-    b += p.cw * (p.N + p.NA) * 8;
+    b += p.cw * (p.ldn + p.lda) * 8;
     b += p.r_cap * 12;
     const bool poiss = d.input_exc.poiss_rate > 0.0 || d.input_inh.poiss_rate > 0.0;
     const bool noise = d.input_exc.gauss_noise_std > 0.0 || d.input_inh.gauss_noise_std > 0.0;
-    b += (poiss ? p.Wmax * p.N * 4 : 0) + (noise ? p.Wmax * p.N * fb : 0);
+    b += (poiss ? p.Wmax * p.ldn * 4 : 0) + (noise ? p.Wmax * p.ldn * fb : 0);
     p.bytes = b;
     return p;
 }
@@ -1973,6 +1989,8 @@ void build_slot(NetSlot& s, const asg_net_desc& d, int64_t total, hipStream_t st
     n.n_astro = d.n_astro;
     n.words = static_cast<uint32_t>(pl.words);
     n.cw = static_cast<uint32_t>(pl.cw);
+    n.ldn = static_cast<uint32_t>(pl.ldn);
+    n.lda = static_cast<uint32_t>(pl.lda);
     n.S = static_cast<int32_t>(pl.S);
     n.Wmax = static_cast<int32_t>(pl.Wmax);
     n.d_exc = d.exc.delay_steps;
@@ -2042,11 +2060,11 @@ void build_slot(NetSlot& s, const asg_net_desc& d, int64_t total, hipStream_t st
         n.fIP3_0 = dev_copy(s, IP3_0.data(), NA, st);
         n.finv_tau = dev_copy(s, inv_tau.data(), NA, st);
         n.fdelta = dev_copy(s, delta.data(), NA, st);
-        n.fring_exc = dev_zero<float>(s, pl.S * N, st);
-        n.fring_inh = dev_zero<float>(s, pl.S * N, st);
-        n.fring_sic = dev_zero<float>(s, pl.S * N, st);
-        n.fring_astro = dev_zero<float>(s, pl.S * NA, st);
-        n.ffac = dev_zero<float>(s, pl.Wmax * NA, st);
+        n.fring_exc = dev_zero<float>(s, pl.S * pl.ldn, st);
+        n.fring_inh = dev_zero<float>(s, pl.S * pl.ldn, st);
+        n.fring_sic = dev_zero<float>(s, pl.S * pl.ldn, st);
+        n.fring_astro = dev_zero<float>(s, pl.S * pl.lda, st);
+        n.ffac = dev_zero<float>(s, pl.Wmax * pl.lda, st);
         HIP_CHECK(hipStreamSynchronize(st));
     } else {
         n.V = dev_copy(s, V0.data(), N, st);
@@ -2063,11 +2081,11 @@ void build_slot(NetSlot& s, const asg_net_desc& d, int64_t total, hipStream_t st
         n.IP3_0 = dev_copy(s, d.IP3_0, NA, st);
         n.tau_IP3 = dev_copy(s, d.tau_IP3, NA, st);
         n.delta_IP3 = dev_copy(s, d.delta_IP3, NA, st);
-        n.ring_exc = dev_zero<double>(s, pl.S * N, st);
-        n.ring_inh = dev_zero<double>(s, pl.S * N, st);
-        n.ring_sic = dev_zero<double>(s, pl.S * N, st);
-        n.ring_astro = dev_zero<double>(s, pl.S * NA, st);
-        n.fac = dev_zero<double>(s, pl.Wmax * NA, st);
+        n.ring_exc = dev_zero<double>(s, pl.S * pl.ldn, st);
+        n.ring_inh = dev_zero<double>(s, pl.S * pl.ldn, st);
+        n.ring_sic = dev_zero<double>(s, pl.S * pl.ldn, st);
+        n.ring_astro = dev_zero<double>(s, pl.S * pl.lda, st);
+        n.fac = dev_zero<double>(s, pl.Wmax * pl.lda, st);
     }
     n.stp_x = dev_fill(s, 1.0, N, st);
     n.stp_u = dev_fill(s, d.stp.U, N, st);
@@ -2091,16 +2109,16 @@ void build_slot(NetSlot& s, const asg_net_desc& d, int64_t total, hipStream_t st
     n.spk_list = dev_zero<uint32_t>(s, pl.Wmax * N, st);
     n.spk_n = dev_zero<uint32_t>(s, pl.Wmax * 2, st);
     n.spk_c = dev_zero<double>(s, pl.Wmax * N * 2, st);
-    n.mask_n = dev_zero<uint64_t>(s, pl.cw * N, st);
-    n.mask_a = dev_zero<uint64_t>(s, pl.cw * NA, st);
+    n.mask_n = dev_zero<uint64_t>(s, pl.cw * pl.ldn, st);
+    n.mask_a = dev_zero<uint64_t>(s, pl.cw * pl.lda, st);
     if (n.in[0].poiss_rate > 0.0 || n.in[1].poiss_rate > 0.0) {
-        n.drive_n = dev_zero<int32_t>(s, pl.Wmax * N, st);
+        n.drive_n = dev_zero<int32_t>(s, pl.Wmax * pl.ldn, st);
     }
     if (n.in[0].noise_std > 0.0 || n.in[1].noise_std > 0.0) {
         if (d.precision == ASG_FP32) {
-            n.fdrive_noise = dev_zero<float>(s, pl.Wmax * N, st);
+            n.fdrive_noise = dev_zero<float>(s, pl.Wmax * pl.ldn, st);
         } else {
-            n.drive_noise = dev_zero<double>(s, pl.Wmax * N, st);
+            n.drive_noise = dev_zero<double>(s, pl.Wmax * pl.ldn, st);
         }
     }
     n.r_cap = static_cast<int64_t>(pl.r_cap);
