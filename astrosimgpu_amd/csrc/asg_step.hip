@@ -1334,8 +1334,10 @@ __device__ void deliver_sources(DevNet& n, uint32_t first, uint32_t lane) {
 // per group of G steps for the SIC sums of the block's neurons. G trades
 // re-reading the astrocyte lists (G = Wg reads them once per window) against
 // parallelism for small networks (G = 1); MG = Wg saves the dispatch of
-// wavefronts that mostly find no spike in a large batch.
-template <int G>
+// wavefronts that mostly find no spike in a large batch. One instantiation
+// per precision, so the FP32 networks' launch does not carry the FP64 SIC
+// accumulators' registers.
+template <int G, int P>
 __global__ __launch_bounds__(256) void k_deliver(DevNet* nets, const CellBlock* nblocks,
                                                  int32_t n_items, int32_t Wg, int32_t MG) {
     const int e = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x) * 4 +
@@ -1354,7 +1356,7 @@ __global__ __launch_bounds__(256) void k_deliver(DevNet* nets, const CellBlock* 
         const int len = min(G, n.win_W - g0);
         const uint32_t t = cb.first + lane;
         if (len > 0 && t < n.n_neur) {
-            if (n.precision == ASG_FP32) {
+            if constexpr (P == ASG_FP32) {
                 deliver_sic_f32<G>(n, t, g0, len);
             } else {
                 deliver_sic<G>(n, t, g0, len);
@@ -2195,6 +2197,8 @@ struct Engine {
     // d_blocks holds two ranges, [cell_range[r], cell_range[r + 1]): the
     // FP64 networks' blocks, then the FP32 networks'.
     int cell_range[3] = {0, 0, 0};
+    // d_nblocks likewise: the FP64 networks' neuron blocks, then the FP32's.
+    int nblock_range[3] = {0, 0, 0};
 
     explicit Engine(int max_n, int64_t budget_bytes) : max_nets(max_n), budget(budget_bytes) {
         HIP_CHECK(hipGetDevice(&device));
@@ -2320,6 +2324,7 @@ struct Engine {
                 }
             }
             cell_range[range + 1] = static_cast<int>(blocks.size());
+            nblock_range[range + 1] = static_cast<int>(nblocks.size());
         }
         n_blocks = static_cast<int>(blocks.size());
         n_nblocks = static_cast<int>(nblocks.size());
@@ -2379,24 +2384,29 @@ struct Engine {
     }
 
     // k_deliver for SIC group size G (its accumulators are registers).
-    void launch_deliver(int G, int items, int32_t Wg_, int MG) {
-        const dim3 grid((items + 3) / 4), block(256);
+    template <int G, int P>
+    void launch_deliver_range(int range, int per_block, int32_t Wg_, int MG) {
+        const int count = nblock_range[range + 1] - nblock_range[range];
+        if (count == 0) return;
+        const int items = count * per_block;
+        hipLaunchKernelGGL((k_deliver<G, P>), dim3((items + 3) / 4), dim3(256), 0, stream, d_nets,
+                           d_nblocks + nblock_range[range], items, Wg_, MG);
+    }
+
+    template <int G>
+    void launch_deliver_g(int per_block, int32_t Wg_, int MG) {
+        launch_deliver_range<G, ASG_FP64>(0, per_block, Wg_, MG);
+        launch_deliver_range<G, ASG_FP32>(1, per_block, Wg_, MG);
+    }
+
+    // Pass 3 over every neuron block: per_block wavefronts each.
+    void launch_deliver(int G, int per_block, int32_t Wg_, int MG) {
         switch (G) {
-            case 1:
-                hipLaunchKernelGGL(k_deliver<1>, grid, block, 0, stream, d_nets, d_nblocks, items, Wg_, MG);
-                break;
-            case 2:
-                hipLaunchKernelGGL(k_deliver<2>, grid, block, 0, stream, d_nets, d_nblocks, items, Wg_, MG);
-                break;
-            case 4:
-                hipLaunchKernelGGL(k_deliver<4>, grid, block, 0, stream, d_nets, d_nblocks, items, Wg_, MG);
-                break;
-            case 8:
-                hipLaunchKernelGGL(k_deliver<8>, grid, block, 0, stream, d_nets, d_nblocks, items, Wg_, MG);
-                break;
-            default:
-                hipLaunchKernelGGL(k_deliver<16>, grid, block, 0, stream, d_nets, d_nblocks, items, Wg_, MG);
-                break;
+            case 1: launch_deliver_g<1>(per_block, Wg_, MG); break;
+            case 2: launch_deliver_g<2>(per_block, Wg_, MG); break;
+            case 4: launch_deliver_g<4>(per_block, Wg_, MG); break;
+            case 8: launch_deliver_g<8>(per_block, Wg_, MG); break;
+            default: launch_deliver_g<16>(per_block, Wg_, MG); break;
         }
     }
 
@@ -2447,7 +2457,7 @@ struct Engine {
     // Launches `windows` windows; with `marks`, one window in prof_every is
     // bracketed by events (6 per timed window). Returns the windows timed.
     int64_t launch_windows(int64_t windows, int G, int wpb, int SG, bool marks) {
-        const int deliver_items = n_nblocks * ((Wg + SG - 1) / SG + (Wg + G - 1) / G);
+        const int deliver_per_block = (Wg + SG - 1) / SG + (Wg + G - 1) / G;
 
         int64_t timed = 0;
         for (int64_t wdx = 0; wdx < windows; ++wdx) {
@@ -2468,8 +2478,8 @@ struct Engine {
             hipLaunchKernelGGL(k_compact, dim3(n_active, Wg), dim3(64), 0, stream, d_nets,
                                d_active);
             mark(3);
-            if (deliver_items > 0) {
-                launch_deliver(G, deliver_items, Wg, SG);
+            if (n_nblocks > 0) {
+                launch_deliver(G, deliver_per_block, Wg, SG);
             }
             mark(4);
             const int sum_items = n_blocks * ((Wg + SG - 1) / SG);
