@@ -1721,10 +1721,15 @@ InputK make_input_k(const asg_input_params& in, double dt) {
 }
 
 NeuronKf make_neuron_kf(const asg_neuron_params& s) {
+    constexpr double kLog2e = 1.4426950408889634074;
     NeuronKf p{};
     p.inv_C_m = 1.0 / s.C_m;
-    p.inv_Delta_T = 1.0 / s.Delta_T;
     p.inv_tau_w = 1.0 / s.tau_w;
+    p.exp_a = kLog2e / s.Delta_T;
+    p.exp_b = -s.V_th * kLog2e / s.Delta_T;
+    p.gL_EL_Ie = s.g_L * s.E_L + s.I_e;
+    p.a_EL = s.a * s.E_L;
+    p.g_L = s.g_L;
     return p;
 }
 

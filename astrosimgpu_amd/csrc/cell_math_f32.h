@@ -22,9 +22,17 @@
 
 namespace asg {
 
-// FP64 reciprocals of the neuron parameters the FP32 step divides by.
+// Neuron constants of the FP32 step's FP64 integration, folded so that the
+// right-hand side needs no division and fewer operations:
+//   (V - V_th) / Delta_T * log2(e) = V * exp_a + exp_b, and
+//   -g_L (V - E_L) - g_ex (V - E_ex) - g_in (V - E_in) + I_e
+//       = -V (g_L + g_ex + g_in) + (g_L E_L + I_e) + g_ex E_ex + g_in E_in.
 struct NeuronKf {
-    double inv_C_m, inv_Delta_T, inv_tau_w;
+    double inv_C_m, inv_tau_w;
+    double exp_a, exp_b;
+    double gL_EL_Ie;  // g_L E_L + I_e
+    double a_EL;      // a E_L
+    double g_L;
 };
 
 struct AstroKf {
@@ -58,43 +66,48 @@ __device__ inline void comp_add(float& s, float& c, float x) {
     s = t;
 }
 
-// e^x in FP64 for the neuron integration (|x| below ~40 there):
-// x = n ln2 + r with n = rint(x / ln2) and a two-part ln2 (Cody-Waite), so
-// |r| <= ln2 / 2, then e^r by its degree-10 Taylor polynomial (truncation
-// below r^11/11! < 3e-13 relative) and the scale 2^n.
-__device__ inline double island_exp(double x) {
-    constexpr double kLog2e = 1.4426950408889634074;
-    constexpr double kLn2Hi = 6.93147180369123816490e-01;
-    constexpr double kLn2Lo = 1.90821492927058770002e-10;
-    const double n = rint(x * kLog2e);
-    const double r = fma(-n, kLn2Lo, fma(-n, kLn2Hi, x));
-    double p = 1.0 / 3628800.0;
-    p = fma(p, r, 1.0 / 362880.0);
-    p = fma(p, r, 1.0 / 40320.0);
-    p = fma(p, r, 1.0 / 5040.0);
-    p = fma(p, r, 1.0 / 720.0);
-    p = fma(p, r, 1.0 / 120.0);
-    p = fma(p, r, 1.0 / 24.0);
-    p = fma(p, r, 1.0 / 6.0);
-    p = fma(p, r, 0.5);
-    p = fma(p, r, 1.0);
-    p = fma(p, r, 1.0);
+// 2^y in FP64 for the neuron integration (|y| below ~60 there): n = rint(y),
+// f = y - n (exact, |f| <= 1/2), 2^f by its degree-10 Taylor polynomial in
+// f (coefficients ln2^k / k!; relative error below 3e-13, checked on the
+// host against exp2l over [-60, 10]) and the scale 2^n.
+__device__ inline double exp2_poly(double y) {
+    const double n = rint(y);
+    const double f = y - n;
+    double p = 7.054911620801123e-09;
+    p = fma(p, f, 1.01780860092397e-07);
+    p = fma(p, f, 1.321548679014431e-06);
+    p = fma(p, f, 1.5252733804059841e-05);
+    p = fma(p, f, 0.0001540353039338161);
+    p = fma(p, f, 0.0013333558146428443);
+    p = fma(p, f, 0.009618129107628477);
+    p = fma(p, f, 0.05550410866482158);
+    p = fma(p, f, 0.24022650695910072);
+    p = fma(p, f, 0.6931471805599453);
+    p = fma(p, f, 1.0);
     return ldexp(p, static_cast<int>(n));
 }
 
-// The AdEx right-hand side in FP64 (reference operation order,
-// neuron.cpp:85-99, with the divisions by C_m, Delta_T and tau_w taken as
-// products with their reciprocals).
-__device__ inline void island_rhs(const NeuronK& p, const NeuronKf& pf, double V, double w,
-                                  double g_ex, double g_in, double I_ext, double& dV,
-                                  double& dw) {
+// The conductance terms of the AdEx right-hand side for one (g_ex, g_in):
+// gsum = g_L + g_ex + g_in and gc = (g_L E_L + I_e + I_ext) + g_ex E_ex +
+// g_in E_in, shared by the RK4 stages that use the same conductances.
+struct AdexG {
+    double gsum, gc;
+};
+__device__ inline AdexG adex_g(const NeuronK& p, const NeuronKf& pf, double g_ex, double g_in,
+                               double I_ext) {
+    return AdexG{pf.g_L + g_ex + g_in,
+                 fma(g_in, p.E_in, fma(g_ex, p.E_ex, pf.gL_EL_Ie + I_ext))};
+}
+
+// The AdEx right-hand side (neuron.cpp:85-99) in FP64 on the folded
+// constants: dV = (gc - w - V gsum + g_L Delta_T 2^y) / C_m,
+// dw = (a V - a E_L - w) / tau_w, with V clamped at V_peak.
+__device__ inline void adex_rhs_folded(const NeuronK& p, const NeuronKf& pf, double V, double w,
+                                       const AdexG& g, double& dV, double& dw) {
     const double Vc = (p.V_peak < V) ? p.V_peak : V;
-    const double I_spike = p.gL_DeltaT * island_exp((Vc - p.V_th) * pf.inv_Delta_T);
-    const double I_syn_ex = g_ex * (Vc - p.E_ex);
-    const double I_syn_in = g_in * (Vc - p.E_in);
-    dV = (p.neg_g_L * (Vc - p.E_L) + I_spike - I_syn_ex - I_syn_in - w + p.I_e + I_ext) *
-         pf.inv_C_m;
-    dw = (p.a * (Vc - p.E_L) - w) * pf.inv_tau_w;
+    const double I_spike = p.gL_DeltaT * exp2_poly(fma(Vc, pf.exp_a, pf.exp_b));
+    dV = (fma(-Vc, g.gsum, g.gc - w) + I_spike) * pf.inv_C_m;
+    dw = (fma(p.a, Vc, -pf.a_EL) - w) * pf.inv_tau_w;
 }
 
 // One substep of the AdEx + alpha-conductance update (neuron.cpp:101-141)
@@ -110,15 +123,16 @@ __device__ inline bool adex_substep_reg(const NeuronK& pd, const NeuronKf& pf, d
     if (refr > 0) {
         V = V_reset;
         const double Vc = (pd.V_peak < V) ? pd.V_peak : V;
-        w += hs * ((pd.a * (Vc - pd.E_L) - w) * pf.inv_tau_w);
+        w += hs * ((fma(pd.a, Vc, -pf.a_EL) - w) * pf.inv_tau_w);
     } else {
-        const double ge_mid = 0.5 * (g_ex + ge_next);
-        const double gi_mid = 0.5 * (g_in + gi_next);
+        const AdexG g1 = adex_g(pd, pf, g_ex, g_in, I_ext);
+        const AdexG g2 = adex_g(pd, pf, 0.5 * (g_ex + ge_next), 0.5 * (g_in + gi_next), I_ext);
+        const AdexG g4 = adex_g(pd, pf, ge_next, gi_next, I_ext);
         double k1V, k1w, k2V, k2w, k3V, k3w, k4V, k4w;
-        island_rhs(pd, pf, V, w, g_ex, g_in, I_ext, k1V, k1w);
-        island_rhs(pd, pf, V + half_h * k1V, w + half_h * k1w, ge_mid, gi_mid, I_ext, k2V, k2w);
-        island_rhs(pd, pf, V + half_h * k2V, w + half_h * k2w, ge_mid, gi_mid, I_ext, k3V, k3w);
-        island_rhs(pd, pf, V + hs * k3V, w + hs * k3w, ge_next, gi_next, I_ext, k4V, k4w);
+        adex_rhs_folded(pd, pf, V, w, g1, k1V, k1w);
+        adex_rhs_folded(pd, pf, V + half_h * k1V, w + half_h * k1w, g2, k2V, k2w);
+        adex_rhs_folded(pd, pf, V + half_h * k2V, w + half_h * k2w, g2, k3V, k3w);
+        adex_rhs_folded(pd, pf, V + hs * k3V, w + hs * k3w, g4, k4V, k4w);
         V += h_sixth * (k1V + 2.0 * k2V + 2.0 * k3V + k4V);
         w += h_sixth * (k1w + 2.0 * k2w + 2.0 * k3w + k4w);
     }
