@@ -115,6 +115,8 @@ asg_status guarded(F&& f) {
 
 constexpr int kCellBlock = 64;     // one wavefront per block of the per-cell kernels
 constexpr int kWindowCap = 16;     // longest window (bounds scratch memory)
+constexpr double kFixScale = 4294967296.0;        // 2^32: fixed-point ring units
+constexpr double kFixUnit = 1.0 / 4294967296.0;   // 2^-32
 constexpr int64_t kSicWaves = 4096; // SIC wavefronts that fill the device (256 CUs x 16)
 constexpr int64_t kGroupWaves = 16384;  // per-step wavefronts beyond which steps are grouped
 constexpr size_t kMaskBudget = size_t(256) << 20;  // delivery masks per network
@@ -184,7 +186,11 @@ struct DevNet {
     // FP32 networks keep their state, parameters and rings here instead
     float *fV, *fw, *fgex, *fdgex, *fgin, *fdgin, *fIsic;
     float *fCa, *fIP3, *fhh, *fCa_tot, *fIP3_0, *finv_tau, *fdelta;
-    float *fring_exc, *fring_inh, *fring_sic, *fring_astro, *ffac;
+    float *fring_sic, *ffac;
+    // FP32 networks' exc, inh and n2a rings in fixed point (value * 2^32):
+    // delivered by 64-bit atomic adds, whose integer sums do not depend on
+    // their order
+    int64_t *qring_exc, *qring_inh, *qring_astro;
     double *stp_x, *stp_u, *stp_tl;
     // connectivity
     const uint32_t *exc_row, *exc_tgt, *inh_row, *inh_tgt;
@@ -545,8 +551,8 @@ __device__ __forceinline__ void neuron_window_f32(const DevNet& n, uint32_t i, i
     const int substeps = n.substeps;
     const double hs = n.h, half_h = n.half_h, h_sixth = n.h_sixth;
     const uint32_t words = n.words;
-    float* __restrict__ ring_exc = n.fring_exc;
-    float* __restrict__ ring_inh = n.fring_inh;
+    int64_t* __restrict__ ring_exc = n.qring_exc;
+    int64_t* __restrict__ ring_inh = n.qring_inh;
     const float* __restrict__ ring_sic = n.fring_sic;
     uint64_t* __restrict__ spk_bits = n.spk_bits;
     const int32_t* __restrict__ drive_n = n.drive_n;
@@ -559,7 +565,8 @@ __device__ __forceinline__ void neuron_window_f32(const DevNet& n, uint32_t i, i
     // FP64 registers for the window; FP32 in memory
     double V = 0, w = 0, gex = 0, dgex = 0, gin = 0, dgin = 0, Isic = 0, Vr = 0, b = 0;
     int refr = 0;
-    float ex_n = 0, ih_n = 0, sic_n = 0;
+    int64_t ex_n = 0, ih_n = 0;
+    float sic_n = 0;
     int ev_n = 0;
     float noise_n = 0.0f;
     const bool has_poiss = in.poiss_rate > 0.0, has_noise = in.noise_std > 0.0;
@@ -592,7 +599,8 @@ __device__ __forceinline__ void neuron_window_f32(const DevNet& n, uint32_t i, i
         const bool next_live = (live_in >> (k + 1)) & 1u;
         if (valid) {
             const size_t slot = static_cast<size_t>(row) * LN + i;
-            const float ex = ex_n, ih = ih_n, sic = sic_n;
+            const int64_t ex = ex_n, ih = ih_n;
+            const float sic = sic_n;
             const int ev = ev_n;
             const float noise = noise_n;
             if (k + 1 < W) {
@@ -605,21 +613,23 @@ __device__ __forceinline__ void neuron_window_f32(const DevNet& n, uint32_t i, i
                 if (has_noise) noise_n = drive_noise[dk1];
             }
             double exc_in = 0.0, inh_in = 0.0;
-            if (ex != 0.0f) {
-                if (ex > 0.0f) {
-                    exc_in += ex;
+            if (ex != 0) {
+                const double v = static_cast<double>(ex) * kFixUnit;
+                if (v > 0.0) {
+                    exc_in += v;
                 } else {
-                    inh_in -= ex;
+                    inh_in -= v;
                 }
-                ring_exc[slot] = 0.0f;
+                ring_exc[slot] = 0;
             }
-            if (ih != 0.0f) {
-                if (ih < 0.0f) {
-                    exc_in -= ih;
+            if (ih != 0) {
+                const double v = static_cast<double>(ih) * kFixUnit;
+                if (v < 0.0) {
+                    exc_in -= v;
                 } else {
-                    inh_in += ih;
+                    inh_in += v;
                 }
-                ring_inh[slot] = 0.0f;
+                ring_inh[slot] = 0;
             }
             if (sic_step) {
                 Isic = sic;
@@ -678,13 +688,13 @@ __device__ __forceinline__ void astro_window_f32(const DevNet& n, uint32_t a, in
     const uint64_t seed = n.seed;
     const float hs = static_cast<float>(n.h), half_h = static_cast<float>(n.half_h),
                 h_sixth = static_cast<float>(n.h_sixth);
-    float* __restrict__ ring_astro = n.fring_astro;
+    int64_t* __restrict__ ring_astro = n.qring_astro;
     float* __restrict__ fac = n.ffac;
     float Ca = n.fCa[a], IP3 = n.fIP3[a], hh = n.fhh[a];
     const float Ca_tot = n.fCa_tot[a], IP3_0 = n.fIP3_0[a], inv_tau = n.finv_tau[a],
                 delta = n.fdelta[a];
     int row = static_cast<int>(T0 % S);  // ring row of the current step
-    float v_n = ring_astro[static_cast<size_t>(row) * LA + a];
+    int64_t v_n = ring_astro[static_cast<size_t>(row) * LA + a];
     // noise index T / steps_per_noise, redrawn at the window start and at
     // every multiple of steps_per_noise
     const int64_t spn = ain.steps_per_noise;
@@ -699,14 +709,14 @@ __device__ __forceinline__ void astro_window_f32(const DevNet& n, uint32_t a, in
     for (int k = 0; k < W; ++k, pkey += kStreamStride * kStreamMul) {
         const int row_n = ring_next(row, S);
         const size_t slot = static_cast<size_t>(row) * LA + a;
-        const float v = v_n;
+        const int64_t v = v_n;
         if (k + 1 < W) {
             v_n = ring_astro[static_cast<size_t>(row_n) * LA + a];
         }
         float ip3_in = 0.0f;
-        if (v != 0.0f) {
-            ip3_in += v;
-            ring_astro[slot] = 0.0f;
+        if (v != 0) {
+            ip3_in += static_cast<float>(static_cast<double>(v) * kFixUnit);
+            ring_astro[slot] = 0;
         }
         if (ain.poiss_rate > 0.0) {
             const int ev = rng_poisson_fresh_keyed(pkey, ain.lambda, ain.limit);
@@ -1328,6 +1338,150 @@ __device__ void deliver_sources(DevNet& n, uint32_t first, uint32_t lane) {
 }
 
 // Window pass 3: one wavefront per (64 neurons, window step); four per block.
+// Add val (fixed point) into the ring slot of every target of one row, the
+// whole wavefront together, 4 target loads in flight per lane; with MULT,
+// target q's multiplicity mult[q] times val.
+template <bool MULT>
+__device__ inline void add_row(const uint32_t* __restrict__ tgt, const uint32_t* __restrict__ mult,
+                               uint32_t q0, uint32_t q1, int64_t* base, int64_t val, uint32_t lane) {
+    constexpr int U = 4;
+    for (uint32_t q = q0 + lane; q < q1; q += 64 * U) {
+        uint32_t tt[U], mm[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t qq = q + static_cast<uint32_t>(u) * 64;
+            tt[u] = qq < q1 ? tgt[qq] : UINT32_MAX;
+            if constexpr (MULT) {
+                mm[u] = qq < q1 ? mult[qq] : 0u;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (tt[u] != UINT32_MAX) {
+                const int64_t v = MULT ? val * static_cast<int64_t>(mm[u]) : val;
+                atomicAdd(reinterpret_cast<unsigned long long*>(base + tt[u]),
+                          static_cast<unsigned long long>(v));
+            }
+        }
+    }
+}
+
+// Pass 3 of an FP32 network for the 64 neurons of one wavefront, over the
+// whole window in time order. At each step the spiking lanes do their
+// Tsodyks-Markram update (network.cpp:230-247) and raster entry, then the
+// wavefront adds each spike's weights, in fixed point, into the arrival slot
+// of every target of its rows: w_exc * x * u into the exc ring, -w_inh * x * u
+// into the inh ring (the FP64 sums' sign convention), multiplicity * w_n2a *
+// x * u into the n2a ring. The integer sums are the
+// same in any order, so spikes need neither masks nor a summing pass.
+__device__ void deliver_fixed(DevNet& n, uint32_t first, uint32_t lane) {
+    const uint32_t N = n.n_neur, NE = n.n_exc;
+    const uint32_t t = first + lane;
+    const bool valid = t < N;
+    const int W = n.win_W;
+    const int64_t T0 = n.win_T0;
+    const int64_t r_head = n.r_head;
+    const int S = n.S;
+    const size_t LN = n.ldn, LA = n.lda;
+    const StepWords sw(n, first >> 6, lane, 0, W);
+    uint32_t window_total = 0;
+    const uint32_t before_k = wave_exclusive_scan(sw.tot, window_total);
+    const bool room = r_head + static_cast<int64_t>(window_total) <= n.r_cap;
+    uint32_t mine = 0;  // bit k: this lane's neuron spiked at window step k
+    for (int k2 = 0; k2 < W; ++k2) {
+        mine |= static_cast<uint32_t>((lane_u64(sw.word, k2) >> lane) & 1ULL) << k2;
+    }
+    const bool exc = t < NE;
+    const bool stp = n.stp_on != 0;
+    uint32_t r0 = 0, r1 = 0, a0 = 0, a1 = 0;
+    double x = 0.0, u = 0.0, tl = 0.0;
+    int64_t row_syn = 0;
+    if (valid && mine != 0) {
+        if (exc) {
+            r0 = n.exc_row[t];
+            r1 = n.exc_row[t + 1];
+            a0 = n.n2a_row[t];
+            a1 = n.n2a_row[t + 1];
+            row_syn = (r1 - r0) + (n.n2a_raw_row[t + 1] - n.n2a_raw_row[t]);
+        } else {
+            r0 = n.inh_row[t - NE];
+            r1 = n.inh_row[t - NE + 1];
+            row_syn = r1 - r0;
+        }
+        if (stp) {
+            x = n.stp_x[t];
+            u = n.stp_u[t];
+            tl = n.stp_tl[t];
+        }
+    }
+    const uint64_t below = (1ULL << lane) - 1ULL;
+    int rowE = static_cast<int>((T0 + n.d_exc) % S);
+    int rowI = static_cast<int>((T0 + n.d_inh) % S);
+    int rowA = static_cast<int>((T0 + n.d_n2a) % S);
+    int64_t syn = 0;
+    for (int k2 = 0; k2 < W; ++k2) {
+        const uint64_t word = lane_u64(sw.word, k2);
+        if (word != 0) {
+            int64_t q0 = 0, q1 = 0;  // this lane's spike weights, fixed point
+            if ((mine >> k2) & 1u) {
+                const uint32_t j = lane_u32(sw.off, k2) + static_cast<uint32_t>(__popcll(word & below));
+                double c0 = exc ? n.w_exc : n.w_inh;
+                double c1 = n.w_n2a;
+                const double t_now = static_cast<double>(T0 + k2) * n.dt;
+                if (stp) {
+                    const double dts = t_now - tl;
+                    const double x_decay = asg_exp(-dts / n.tau_rec);
+                    const double u_decay = n.fac_on ? asg_exp(-dts / n.tau_fac) : 0.0;
+                    x = 1.0 + (x - x * u - 1.0) * x_decay;
+                    u = n.U + u * n.one_minus_U * u_decay;
+                    tl = t_now;
+                    c0 = c0 * x * u;
+                    c1 = c1 * x * u;
+                }
+                q0 = llrint(c0 * kFixScale);
+                q1 = llrint(c1 * kFixScale);
+                syn += row_syn;
+                if (room) {
+                    const int64_t at = r_head + lane_u32(before_k, k2) + j;
+                    n.r_step[at] = T0 + k2;
+                    n.r_src[at] = t;
+                }
+            }
+            uint64_t m = word;
+            while (m) {
+                const int L = __ffsll(static_cast<long long>(m)) - 1;
+                m &= m - 1;
+                const int64_t w0 = static_cast<int64_t>(lane_u64(static_cast<uint64_t>(q0), L));
+                if (first + static_cast<uint32_t>(L) < NE) {
+                    add_row<false>(n.exc_tgt, nullptr, lane_u32(r0, L), lane_u32(r1, L),
+                            n.qring_exc + static_cast<size_t>(rowE) * LN, w0, lane);
+                    add_row<true>(n.n2a_tgt, n.n2a_mult, lane_u32(a0, L), lane_u32(a1, L),
+                            n.qring_astro + static_cast<size_t>(rowA) * LA,
+                            static_cast<int64_t>(lane_u64(static_cast<uint64_t>(q1), L)), lane);
+                } else {
+                    // the inh ring holds the negated weights (as the FP64 sums)
+                    add_row<false>(n.inh_tgt, nullptr, lane_u32(r0, L), lane_u32(r1, L),
+                                   n.qring_inh + static_cast<size_t>(rowI) * LN, -w0, lane);
+                }
+            }
+        }
+        rowE = ring_next(rowE, S);
+        rowI = ring_next(rowI, S);
+        rowA = ring_next(rowA, S);
+    }
+    if (valid && mine != 0 && stp) {
+        n.stp_x[t] = x;
+        n.stp_u[t] = u;
+        n.stp_tl[t] = tl;
+    }
+    if (syn > 0) {
+        add_i64(&n.c_syn, syn);
+    }
+    if (!room && window_total > 0 && t == 0) {
+        atomicOr(&n.overflow, 1);
+    }
+}
+
 // Window pass 3. Per neuron block, ceil(Wg / MG) + ceil(Wg / G) wavefronts:
 // one per group of MG window steps for the marking of those steps' spikes
 // (the first also does the sources' plasticity and raster entries), then one
@@ -1336,7 +1490,8 @@ __device__ void deliver_sources(DevNet& n, uint32_t first, uint32_t lane) {
 // parallelism for small networks (G = 1); MG = Wg saves the dispatch of
 // wavefronts that mostly find no spike in a large batch. One instantiation
 // per precision, so the FP32 networks' launch does not carry the FP64 SIC
-// accumulators' registers.
+// accumulators' registers; FP32 networks take one delivery wavefront per
+// block (deliver_fixed, MG = Wg) instead of marking.
 template <int G, int P>
 __global__ __launch_bounds__(256) void k_deliver(DevNet* nets, const CellBlock* nblocks,
                                                  int32_t n_items, int32_t Wg, int32_t MG) {
@@ -1365,12 +1520,19 @@ __global__ __launch_bounds__(256) void k_deliver(DevNet* nets, const CellBlock* 
         return;
     }
     const int k0 = k * MG;
-    const int k1 = min(k0 + MG, n.win_W);
-    if (k1 > k0) {
-        mark_steps(n, cb.first, lane, k0, k1);
-    }
-    if (k0 == 0 && n.win_W > 0) {
-        deliver_sources(n, cb.first, lane);
+    if constexpr (P == ASG_FP32) {
+        // launched with MG = Wg: one delivery wavefront per block
+        if (k0 == 0 && n.win_W > 0) {
+            deliver_fixed(n, cb.first, lane);
+        }
+    } else {
+        const int k1 = min(k0 + MG, n.win_W);
+        if (k1 > k0) {
+            mark_steps(n, cb.first, lane, k0, k1);
+        }
+        if (k0 == 0 && n.win_W > 0) {
+            deliver_sources(n, cb.first, lane);
+        }
     }
 }
 
@@ -1388,42 +1550,31 @@ __device__ inline uint32_t n2a_multiplicity(const DevNet& n, uint32_t s, uint32_
     return (lo < n.n2a_row[s + 1] && n.n2a_tgt[lo] == a) ? n.n2a_mult[lo] : 0u;
 }
 
-// One ordered sum of the window pass 4 in the precision of the network: FP64
-// adds in order; FP32 adds in order with Neumaier compensation and rounds
-// once at the end.
-template <int P>
-struct OrderedSum {
-    double acc;
-    __device__ __forceinline__ void start(double v) { acc = v; }
-    __device__ __forceinline__ void add(double c) { acc += c; }
-    __device__ __forceinline__ double value() const { return acc; }
-};
-template <>
-struct OrderedSum<ASG_FP32> {
-    float acc, comp;
-    __device__ __forceinline__ void start(float v) {
-        acc = v;
-        comp = 0.0f;
+// Closes the window of `total` spikes: the raster head, the liveness of the
+// window's SIC exchanges at their arrival steps (a step that sends no
+// exchange has sic_live 0), counters and the next window's first step.
+__device__ inline void close_window(DevNet& n, uint32_t total) {
+    const int W = n.win_W;
+    const int64_t T0 = n.win_T0;
+    if (n.r_head + total <= n.r_cap) {
+        n.r_head += total;
     }
-    __device__ __forceinline__ void add(double c) { comp_add(acc, comp, static_cast<float>(c)); }
-    __device__ __forceinline__ void add_f32(float c) { comp_add(acc, comp, c); }
-    __device__ __forceinline__ float value() const { return acc + comp; }
-};
-
-template <int P>
-struct Rings {
-    using T = double;
-    double *exc, *inh, *astro;
-    __device__ __forceinline__ explicit Rings(const DevNet& n)
-        : exc(n.ring_exc), inh(n.ring_inh), astro(n.ring_astro) {}
-};
-template <>
-struct Rings<ASG_FP32> {
-    using T = float;
-    float *exc, *inh, *astro;
-    __device__ __forceinline__ explicit Rings(const DevNet& n)
-        : exc(n.fring_exc), inh(n.fring_inh), astro(n.fring_astro) {}
-};
+    // the W liveness bits go to ring bits b0 .. b0 + W - 1 (W <= 16 < 32):
+    // at most two words, each read and written once
+    const uint32_t b0 = static_cast<uint32_t>(T0 + n.d_a2n) & n.hist_mask;
+    const uint32_t w0 = b0 >> 5, sh = b0 & 31, words = (n.hist_mask + 1) >> 5;
+    const uint64_t span = ((1ULL << W) - 1ULL) << sh;
+    const uint64_t live = static_cast<uint64_t>(n.sic_live & ((1u << W) - 1u)) << sh;
+    uint32_t* hist = n.sic_hist;
+    hist[w0] = (hist[w0] & ~static_cast<uint32_t>(span)) | static_cast<uint32_t>(live);
+    if ((span >> 32) != 0) {
+        const uint32_t w1 = (w0 + 1) & (words - 1);
+        hist[w1] = (hist[w1] & ~static_cast<uint32_t>(span >> 32)) | static_cast<uint32_t>(live >> 32);
+    }
+    n.c_steps += W;
+    n.step = T0 + W;
+    n.sic_live = 0;
+}
 
 // The window's per-step spike counts as uniform arrays: before[k] spikes
 // precede step k in window order, nE[k] of step k's are excitatory. Loaded
@@ -1475,11 +1626,8 @@ struct WindowCounts {
 // stored when the run ends. The bits read are cleared: a mask word that
 // lies wholly in this group's range with a store, a word shared with the
 // neighbouring group's range with an atomic AND.
-template <int P>
 __device__ __forceinline__ void sum_window(const DevNet& n, int kind, uint32_t i, int64_t T0, int W,
                                            int k0, int k1, const WindowCounts& wc) {
-    using R = typename Rings<P>::T;
-    const Rings<P> rings(n);
     const size_t N = n.n_neur;
     const int S = n.S;
     const uint32_t lo = wc.before_of(k0);
@@ -1495,8 +1643,8 @@ __device__ __forceinline__ void sum_window(const DevNet& n, int kind, uint32_t i
     const int rowE0 = static_cast<int>((T0 + n.d_exc) % S);
     const int rowI0 = static_cast<int>((T0 + n.d_inh) % S);
     const int rowA0 = static_cast<int>((T0 + n.d_n2a) % S);
-    OrderedSum<P> acc;
-    R* run_ring = nullptr;  // ring and slot of the open run
+    double acc = 0.0;
+    double* run_ring = nullptr;  // ring and slot of the open run
     size_t run_slot = 0;
     int run_key = -1;       // 2 * step + (inhibitory ? 1 : 0)
     for (uint32_t c = lo >> 6; lo < hi && c <= (hi - 1) >> 6; ++c) {
@@ -1523,44 +1671,40 @@ __device__ __forceinline__ void sum_window(const DevNet& n, int kind, uint32_t i
                 const int key = 2 * k + (inh ? 1 : 0);
                 if (key != run_key) {
                     if (run_ring != nullptr) {
-                        run_ring[run_slot] = acc.value();
+                        run_ring[run_slot] = acc;
                     }
                     run_key = key;
                     const int row0 = inh ? rowI0 : rowE0;
                     const int row = row0 + k < S ? row0 + k : row0 + k - S;
-                    run_ring = inh ? rings.inh : rings.exc;
+                    run_ring = inh ? n.ring_inh : n.ring_exc;
                     run_slot = static_cast<size_t>(row) * ld + i;
-                    acc.start(run_ring[run_slot]);
+                    acc = run_ring[run_slot];
                 }
-                acc.add(inh ? -spk_c[cv] : spk_c[cv]);
+                acc += inh ? -spk_c[cv] : spk_c[cv];
             } else {
                 if (k != run_key) {
                     if (run_ring != nullptr) {
-                        run_ring[run_slot] = acc.value();
+                        run_ring[run_slot] = acc;
                     }
                     run_key = k;
                     const int row = rowA0 + k < S ? rowA0 + k : rowA0 + k - S;
-                    run_ring = rings.astro;
+                    run_ring = n.ring_astro;
                     run_slot = static_cast<size_t>(row) * ld + i;
-                    acc.start(run_ring[run_slot]);
+                    acc = run_ring[run_slot];
                 }
                 // spike j's source counts its n2a multiplicity m times: m
-                // repeated FP64 additions (network.cpp:249-279 adds once
-                // per synapse), or one FP32 product m * weight
+                // repeated additions (network.cpp:249-279 adds once per
+                // synapse)
                 const uint32_t mult = n2a_multiplicity(n, n.spk_list[static_cast<size_t>(k) * N + j], i);
                 const double w = spk_c[cv + 1];
-                if constexpr (P == ASG_FP32) {
-                    acc.add_f32(static_cast<float>(mult) * static_cast<float>(w));
-                } else {
-                    for (uint32_t r = mult; r > 0; --r) {
-                        acc.add(w);
-                    }
+                for (uint32_t r = mult; r > 0; --r) {
+                    acc += w;
                 }
             }
         }
     }
     if (run_ring != nullptr) {
-        run_ring[run_slot] = acc.value();
+        run_ring[run_slot] = acc;
     }
 }
 
@@ -1590,31 +1734,30 @@ __global__ __launch_bounds__(256) void k_sum(DevNet* nets, const CellBlock* bloc
     uint32_t total = 0;
     const uint32_t l_before = wave_exclusive_scan(l_tot, total);
     if (cb.lead && k0 == 0 && lane == 0) {
-        if (n.r_head + total <= n.r_cap) {
-            n.r_head += total;
-        }
-        // liveness of the window's exchanges at their arrival steps (a
-        // step that sends no exchange has sic_live 0)
-        for (int k2 = 0; k2 < W; ++k2) {
-            const uint32_t b = static_cast<uint32_t>(T0 + n.d_a2n + k2) & n.hist_mask;
-            const uint32_t bit = 1u << (b & 31);
-            uint32_t& word = n.sic_hist[b >> 5];
-            word = ((n.sic_live >> k2) & 1u) ? (word | bit) : (word & ~bit);
-        }
-        n.c_steps += W;
-        n.step = T0 + W;
-        n.sic_live = 0;
+        close_window(n, total);
     }
     const int k1 = min(k0 + SG, W);
     if (total == 0 || k1 <= k0) {
         return;
     }
     const WindowCounts wc(l_before, l_nE, total);
-    const uint32_t i = cb.first + lane;
-    if (n.precision == ASG_FP32) {
-        sum_window<ASG_FP32>(n, cb.kind, i, T0, W, k0, k1, wc);
-    } else {
-        sum_window<ASG_FP64>(n, cb.kind, i, T0, W, k0, k1, wc);
+    sum_window(n, cb.kind, cb.first + lane, T0, W, k0, k1, wc);
+}
+
+// The window close of FP32 networks (their spikes were delivered in pass 3),
+// one wavefront per network: lanes k < W load the step totals together.
+__global__ __launch_bounds__(64) void k_close(DevNet* nets, const int32_t* active, int32_t count) {
+    const int j = static_cast<int>(blockIdx.x);
+    if (j >= count) {
+        return;
+    }
+    DevNet& n = nets[active[j]];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t tot = static_cast<int>(lane) < n.win_W ? n.spk_n[2 * lane] : 0u;
+    uint32_t total = 0;
+    wave_exclusive_scan(tot, total);
+    if (lane == 0) {
+        close_window(n, total);
     }
 }
 
@@ -1903,12 +2046,18 @@ Plan plan_of(const asg_net_desc& d, const DerivedConn* dc) {
     size_t b = 0;
     b += p.N * (7 * fb + 4 + 2 * 8 + 3 * 8);       // neuron state, V_reset/b, STP
     b += p.NA * (3 * fb + 4 * fb);                  // astrocyte state and parameters
-    b += p.S * p.ldn * 3 * fb + p.S * p.lda * fb + p.Wmax * p.lda * fb + p.hist_bits / 8;
+    const bool f32 = d.precision == ASG_FP32;
+    // rings: FP64 exc/inh/SIC and n2a; FP32 fixed-point exc/inh and n2a, FP32 SIC
+    b += f32 ? p.S * p.ldn * (8 + 8 + 4) + p.S * p.lda * 8 : p.S * p.ldn * 3 * 8 + p.S * p.lda * 8;
+    b += p.Wmax * p.lda * fb + p.hist_bits / 8;
     b += (d.n_exc + 1 + p.n_exc_syn + d.n_inh + 1 + p.n_inh_syn) * 4;
     b += (2 * (d.n_exc + 1) + 2 * p.n2a_dedup) * 4;
     b += 2 * p.sic_pairs * 4;
-    b += p.Wmax * (p.words * 12 + p.N * 4 + 8 + p.N * 16);
-    b += p.cw * (p.ldn + p.lda) * 8;
+    b += p.Wmax * (p.words * 12 + p.N * 4 + 8);
+    if (!f32) {
+        b += p.Wmax * p.N * 16;               // spike weights
+        b += p.cw * (p.ldn + p.lda) * 8;      // window masks
+    }
     b += p.r_cap * 12;
     const bool poiss = d.input_exc.poiss_rate > 0.0 || d.input_inh.poiss_rate > 0.0;
     const bool noise = d.input_exc.gauss_noise_std > 0.0 || d.input_inh.gauss_noise_std > 0.0;
@@ -2067,10 +2216,10 @@ void build_slot(NetSlot& s, const asg_net_desc& d, int64_t total, hipStream_t st
         n.fIP3_0 = dev_copy(s, IP3_0.data(), NA, st);
         n.finv_tau = dev_copy(s, inv_tau.data(), NA, st);
         n.fdelta = dev_copy(s, delta.data(), NA, st);
-        n.fring_exc = dev_zero<float>(s, pl.S * pl.ldn, st);
-        n.fring_inh = dev_zero<float>(s, pl.S * pl.ldn, st);
+        n.qring_exc = dev_zero<int64_t>(s, pl.S * pl.ldn, st);
+        n.qring_inh = dev_zero<int64_t>(s, pl.S * pl.ldn, st);
         n.fring_sic = dev_zero<float>(s, pl.S * pl.ldn, st);
-        n.fring_astro = dev_zero<float>(s, pl.S * pl.lda, st);
+        n.qring_astro = dev_zero<int64_t>(s, pl.S * pl.lda, st);
         n.ffac = dev_zero<float>(s, pl.Wmax * pl.lda, st);
         HIP_CHECK(hipStreamSynchronize(st));
     } else {
@@ -2115,9 +2264,12 @@ void build_slot(NetSlot& s, const asg_net_desc& d, int64_t total, hipStream_t st
     n.spk_woff = dev_zero<uint32_t>(s, pl.Wmax * pl.words, st);
     n.spk_list = dev_zero<uint32_t>(s, pl.Wmax * N, st);
     n.spk_n = dev_zero<uint32_t>(s, pl.Wmax * 2, st);
-    n.spk_c = dev_zero<double>(s, pl.Wmax * N * 2, st);
-    n.mask_n = dev_zero<uint64_t>(s, pl.cw * pl.ldn, st);
-    n.mask_a = dev_zero<uint64_t>(s, pl.cw * pl.lda, st);
+    if (d.precision != ASG_FP32) {
+        // FP64 delivery: spike weights and window masks for the ordered sums
+        n.spk_c = dev_zero<double>(s, pl.Wmax * N * 2, st);
+        n.mask_n = dev_zero<uint64_t>(s, pl.cw * pl.ldn, st);
+        n.mask_a = dev_zero<uint64_t>(s, pl.cw * pl.lda, st);
+    }
     if (n.in[0].poiss_rate > 0.0 || n.in[1].poiss_rate > 0.0) {
         n.drive_n = dev_zero<int32_t>(s, pl.Wmax * pl.ldn, st);
     }
@@ -2202,8 +2354,10 @@ struct Engine {
     // d_blocks holds two ranges, [cell_range[r], cell_range[r + 1]): the
     // FP64 networks' blocks, then the FP32 networks'.
     int cell_range[3] = {0, 0, 0};
-    // d_nblocks likewise: the FP64 networks' neuron blocks, then the FP32's.
+    // d_nblocks and d_active likewise: the FP64 networks' neuron blocks
+    // (active slots), then the FP32's.
     int nblock_range[3] = {0, 0, 0};
+    int active_range[3] = {0, 0, 0};
 
     explicit Engine(int max_n, int64_t budget_bytes) : max_nets(max_n), budget(budget_bytes) {
         HIP_CHECK(hipGetDevice(&device));
@@ -2330,6 +2484,7 @@ struct Engine {
             }
             cell_range[range + 1] = static_cast<int>(blocks.size());
             nblock_range[range + 1] = static_cast<int>(nblocks.size());
+            active_range[range + 1] = static_cast<int>(active.size());
         }
         n_blocks = static_cast<int>(blocks.size());
         n_nblocks = static_cast<int>(nblocks.size());
@@ -2390,28 +2545,29 @@ struct Engine {
 
     // k_deliver for SIC group size G (its accumulators are registers).
     template <int G, int P>
-    void launch_deliver_range(int range, int per_block, int32_t Wg_, int MG) {
+    void launch_deliver_range(int range, int32_t Wg_, int MG) {
         const int count = nblock_range[range + 1] - nblock_range[range];
         if (count == 0) return;
-        const int items = count * per_block;
+        const int items = count * ((Wg_ + MG - 1) / MG + (Wg_ + G - 1) / G);
         hipLaunchKernelGGL((k_deliver<G, P>), dim3((items + 3) / 4), dim3(256), 0, stream, d_nets,
                            d_nblocks + nblock_range[range], items, Wg_, MG);
     }
 
     template <int G>
-    void launch_deliver_g(int per_block, int32_t Wg_, int MG) {
-        launch_deliver_range<G, ASG_FP64>(0, per_block, Wg_, MG);
-        launch_deliver_range<G, ASG_FP32>(1, per_block, Wg_, MG);
+    void launch_deliver_g(int32_t Wg_, int MG) {
+        launch_deliver_range<G, ASG_FP64>(0, Wg_, MG);
+        launch_deliver_range<G, ASG_FP32>(1, Wg_, Wg_);
     }
 
-    // Pass 3 over every neuron block: per_block wavefronts each.
-    void launch_deliver(int G, int per_block, int32_t Wg_, int MG) {
+    // Pass 3 over every neuron block (FP64 networks' marking in groups of MG
+    // steps; FP32 networks' delivery one wavefront per block).
+    void launch_deliver(int G, int32_t Wg_, int MG) {
         switch (G) {
-            case 1: launch_deliver_g<1>(per_block, Wg_, MG); break;
-            case 2: launch_deliver_g<2>(per_block, Wg_, MG); break;
-            case 4: launch_deliver_g<4>(per_block, Wg_, MG); break;
-            case 8: launch_deliver_g<8>(per_block, Wg_, MG); break;
-            default: launch_deliver_g<16>(per_block, Wg_, MG); break;
+            case 1: launch_deliver_g<1>(Wg_, MG); break;
+            case 2: launch_deliver_g<2>(Wg_, MG); break;
+            case 4: launch_deliver_g<4>(Wg_, MG); break;
+            case 8: launch_deliver_g<8>(Wg_, MG); break;
+            default: launch_deliver_g<16>(Wg_, MG); break;
         }
     }
 
@@ -2462,7 +2618,6 @@ struct Engine {
     // Launches `windows` windows; with `marks`, one window in prof_every is
     // bracketed by events (6 per timed window). Returns the windows timed.
     int64_t launch_windows(int64_t windows, int G, int wpb, int SG, bool marks) {
-        const int deliver_per_block = (Wg + SG - 1) / SG + (Wg + G - 1) / G;
 
         int64_t timed = 0;
         for (int64_t wdx = 0; wdx < windows; ++wdx) {
@@ -2484,12 +2639,21 @@ struct Engine {
                                d_active);
             mark(3);
             if (n_nblocks > 0) {
-                launch_deliver(G, deliver_per_block, Wg, SG);
+                launch_deliver(G, Wg, SG);
             }
             mark(4);
-            const int sum_items = n_blocks * ((Wg + SG - 1) / SG);
-            hipLaunchKernelGGL(k_sum, dim3((sum_items + 3) / 4), dim3(256), 0, stream, d_nets,
-                               d_blocks, sum_items, Wg, SG);
+            // FP64 networks: ordered sums and window close; FP32 networks
+            // (delivered in pass 3): window close
+            const int sum_items = cell_range[1] * ((Wg + SG - 1) / SG);
+            if (sum_items > 0) {
+                hipLaunchKernelGGL(k_sum, dim3((sum_items + 3) / 4), dim3(256), 0, stream, d_nets,
+                                   d_blocks, sum_items, Wg, SG);
+            }
+            const int n_close = active_range[2] - active_range[1];
+            if (n_close > 0) {
+                hipLaunchKernelGGL(k_close, dim3(n_close), dim3(64), 0, stream, d_nets,
+                                   d_active + active_range[1], n_close);
+            }
             mark(5);
             if (timed_here) ++timed;
         }
