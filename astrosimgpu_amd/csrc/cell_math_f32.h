@@ -11,8 +11,8 @@
 // (Ca - SIC_th) * 1000 > 1, is evaluated in FP64 (Ca is a small difference
 // above a threshold). Random draws use the FP64 step's counter streams:
 // identical Poisson counts, and normals from the same two uniforms through
-// FP32 hardware functions. Synaptic input is summed with Neumaier's
-// compensated summation.
+// FP32 hardware functions. Synaptic input arrives through fixed-point rings
+// (exact integer sums); the SIC gather sums with Neumaier's compensation.
 #pragma once
 
 #include <hip/hip_runtime.h>
