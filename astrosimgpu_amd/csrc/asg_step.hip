@@ -222,7 +222,8 @@ struct DevNet {
     // counters
     int64_t c_steps, c_spikes, c_syn, c_sic;
     uint32_t sic_live;     // bit k: some astrocyte has a nonzero SIC factor at
-                           // window step k (set by k_cells, cleared by k_sum)
+                           // window step k (set by k_cells, cleared at the close)
+    int32_t open;          // 1 from a window's cell pass to its close
     // bit T mod (hist_mask + 1): the SIC exchange arriving at step T had a
     // nonzero factor, i.e. its ring_sic row was written; 0 means every
     // neuron's SIC is 0 (a power-of-two ring of at least S + 32 bits)
@@ -814,6 +815,59 @@ __device__ void deliver_sic_f32(const DevNet& n, uint32_t t, int g0, int G) {
     }
 }
 
+// Exclusive prefix sum over the 64 lanes of a wavefront (no barriers).
+__device__ inline uint32_t wave_exclusive_scan(uint32_t v, uint32_t& total) {
+    const int lane = static_cast<int>(threadIdx.x & 63);
+    uint32_t x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) {
+            x += y;
+        }
+    }
+    total = __shfl(x, 63, 64);
+    return x - v;
+}
+
+// Closes the window of `total` spikes: the raster head, the liveness of the
+// window's SIC exchanges at their arrival steps (a step that sends no
+// exchange has sic_live 0), counters and the next window's first step.
+__device__ inline void close_window(DevNet& n, uint32_t total) {
+    const int W = n.win_W;
+    const int64_t T0 = n.win_T0;
+    if (n.r_head + total <= n.r_cap) {
+        n.r_head += total;
+    }
+    // the W liveness bits go to ring bits b0 .. b0 + W - 1 (W <= 16 < 32):
+    // at most two words, each read and written once
+    const uint32_t b0 = static_cast<uint32_t>(T0 + n.d_a2n) & n.hist_mask;
+    const uint32_t w0 = b0 >> 5, sh = b0 & 31, words = (n.hist_mask + 1) >> 5;
+    const uint64_t span = ((1ULL << W) - 1ULL) << sh;
+    const uint64_t live = static_cast<uint64_t>(n.sic_live & ((1u << W) - 1u)) << sh;
+    uint32_t* hist = n.sic_hist;
+    hist[w0] = (hist[w0] & ~static_cast<uint32_t>(span)) | static_cast<uint32_t>(live);
+    if ((span >> 32) != 0) {
+        const uint32_t w1 = (w0 + 1) & (words - 1);
+        hist[w1] = (hist[w1] & ~static_cast<uint32_t>(span >> 32)) | static_cast<uint32_t>(live >> 32);
+    }
+    n.c_steps += W;
+    n.step = T0 + W;
+    n.sic_live = 0;
+    n.open = 0;
+}
+
+// Closes an open window from one whole wavefront: lanes k < W load the step
+// totals together.
+__device__ inline void close_open_window(DevNet& n) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t tot = static_cast<int>(lane) < n.win_W ? n.spk_n[2 * lane] : 0u;
+    uint32_t total = 0;
+    wave_exclusive_scan(tot, total);
+    if (lane == 0) {
+        close_window(n, total);
+    }
+}
+
 // Window pass 0: the external drive of every neuron for the window's steps,
 // from the reference's counter streams -- Poisson event counts of the
 // background input (neuron.cpp:143-151) and the Gaussian noise current held
@@ -839,8 +893,16 @@ __global__ __launch_bounds__(64) void k_drive(DevNet* nets, const CellBlock* nbl
         return;
     }
     const CellBlock cb = nblocks[item];
-    const DevNet& n = nets[cb.net];
-    const int W = window_len(n, Wg);
+    DevNet& n = nets[cb.net];
+    // The window follows the last one opened (win_T0 + win_W is the step
+    // its close sets): the lead wavefront may close that window here while
+    // the other blocks read only fields the close does not write.
+    const int64_t T0 = n.win_T0 + n.win_W;
+    const int64_t rem = n.target - T0;
+    const int W = rem <= 0 ? 0 : (rem < Wg ? static_cast<int>(rem) : Wg);
+    if (cb.lead && n.open) {
+        close_open_window(n);
+    }
     if (W == 0) {
         return;
     }
@@ -851,7 +913,6 @@ __global__ __launch_bounds__(64) void k_drive(DevNet* nets, const CellBlock* nbl
     const bool valid = i < N;
     const int pi = (i < n.n_exc) ? 0 : 1;
     const InputK in = n.in[pi];
-    const int64_t T0 = n.step;
     const uint64_t seed = n.seed;
     if (n.in[0].poiss_rate > 0.0 || n.in[1].poiss_rate > 0.0) {
         int32_t* __restrict__ out = n.drive_n;
@@ -993,6 +1054,7 @@ __global__ __launch_bounds__(kCellBlock * WPB) void k_cells(DevNet* nets, const 
     if (cb.lead && lane == 0) {
         n.win_T0 = T0;
         n.win_W = W;
+        n.open = 1;
     }
     if (W == 0) {
         return;
@@ -1013,19 +1075,6 @@ __global__ __launch_bounds__(kCellBlock * WPB) void k_cells(DevNet* nets, const 
     }
 }
 
-// Exclusive prefix sum over the 64 lanes of a wavefront (no barriers).
-__device__ inline uint32_t wave_exclusive_scan(uint32_t v, uint32_t& total) {
-    const int lane = static_cast<int>(threadIdx.x & 63);
-    uint32_t x = v;
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t y = __shfl_up(x, off, 64);
-        if (lane >= off) {
-            x += y;
-        }
-    }
-    total = __shfl(x, 63, 64);
-    return x - v;
-}
 
 // Window pass 2: one wavefront per (network, step) turns that step's spike
 // words into the source-ordered spike list (the reference sorts each step's
@@ -1550,32 +1599,6 @@ __device__ inline uint32_t n2a_multiplicity(const DevNet& n, uint32_t s, uint32_
     return (lo < n.n2a_row[s + 1] && n.n2a_tgt[lo] == a) ? n.n2a_mult[lo] : 0u;
 }
 
-// Closes the window of `total` spikes: the raster head, the liveness of the
-// window's SIC exchanges at their arrival steps (a step that sends no
-// exchange has sic_live 0), counters and the next window's first step.
-__device__ inline void close_window(DevNet& n, uint32_t total) {
-    const int W = n.win_W;
-    const int64_t T0 = n.win_T0;
-    if (n.r_head + total <= n.r_cap) {
-        n.r_head += total;
-    }
-    // the W liveness bits go to ring bits b0 .. b0 + W - 1 (W <= 16 < 32):
-    // at most two words, each read and written once
-    const uint32_t b0 = static_cast<uint32_t>(T0 + n.d_a2n) & n.hist_mask;
-    const uint32_t w0 = b0 >> 5, sh = b0 & 31, words = (n.hist_mask + 1) >> 5;
-    const uint64_t span = ((1ULL << W) - 1ULL) << sh;
-    const uint64_t live = static_cast<uint64_t>(n.sic_live & ((1u << W) - 1u)) << sh;
-    uint32_t* hist = n.sic_hist;
-    hist[w0] = (hist[w0] & ~static_cast<uint32_t>(span)) | static_cast<uint32_t>(live);
-    if ((span >> 32) != 0) {
-        const uint32_t w1 = (w0 + 1) & (words - 1);
-        hist[w1] = (hist[w1] & ~static_cast<uint32_t>(span >> 32)) | static_cast<uint32_t>(live >> 32);
-    }
-    n.c_steps += W;
-    n.step = T0 + W;
-    n.sic_live = 0;
-}
-
 // The window's per-step spike counts as uniform arrays: before[k] spikes
 // precede step k in window order, nE[k] of step k's are excitatory. Loaded
 // by lanes 0..W-1 and broadcast, so any lane can look up any step.
@@ -1744,20 +1767,18 @@ __global__ __launch_bounds__(256) void k_sum(DevNet* nets, const CellBlock* bloc
     sum_window(n, cb.kind, cb.first + lane, T0, W, k0, k1, wc);
 }
 
-// The window close of FP32 networks (their spikes were delivered in pass 3),
-// one wavefront per network: lanes k < W load the step totals together.
+// The window close of FP32 networks (their spikes were delivered in pass 3)
+// that are still open, one wavefront per network. The next window's k_drive
+// closes them otherwise; this runs after a call's last window, and after
+// every window for a batch with FP32 networks that have no neurons.
 __global__ __launch_bounds__(64) void k_close(DevNet* nets, const int32_t* active, int32_t count) {
     const int j = static_cast<int>(blockIdx.x);
     if (j >= count) {
         return;
     }
     DevNet& n = nets[active[j]];
-    const uint32_t lane = threadIdx.x;
-    const uint32_t tot = static_cast<int>(lane) < n.win_W ? n.spk_n[2 * lane] : 0u;
-    uint32_t total = 0;
-    wave_exclusive_scan(tot, total);
-    if (lane == 0) {
-        close_window(n, total);
+    if (n.open) {
+        close_open_window(n);
     }
 }
 
@@ -2358,6 +2379,7 @@ struct Engine {
     // (active slots), then the FP32's.
     int nblock_range[3] = {0, 0, 0};
     int active_range[3] = {0, 0, 0};
+    bool fp32_without_neurons = false;  // some FP32 network has no neuron blocks
 
     explicit Engine(int max_n, int64_t budget_bytes) : max_nets(max_n), budget(budget_bytes) {
         HIP_CHECK(hipGetDevice(&device));
@@ -2461,6 +2483,7 @@ struct Engine {
     void rebuild_tables() {
         std::vector<CellBlock> blocks, nblocks;
         std::vector<int32_t> active;
+        fp32_without_neurons = false;
         Wg = kWindowCap;
         // Blocks in two contiguous ranges, one per cell kernel: the FP64
         // networks' blocks, then the FP32 networks'. Within a network the
@@ -2470,6 +2493,7 @@ struct Engine {
                 const NetSlot& s = slots[sidx];
                 if (!s.used || (s.dn.precision == ASG_FP32) != (range == 1)) continue;
                 active.push_back(sidx);
+                if (range == 1 && s.dn.n_neur == 0) fp32_without_neurons = true;
                 Wg = std::min(Wg, s.W);
                 uint32_t lead = 1;
                 for (uint32_t f = 0; f < s.dn.n_neur; f += kCellBlock) {
@@ -2642,22 +2666,31 @@ struct Engine {
                 launch_deliver(G, Wg, SG);
             }
             mark(4);
-            // FP64 networks: ordered sums and window close; FP32 networks
-            // (delivered in pass 3): window close
+            // FP64 networks: ordered sums and window close. FP32 networks
+            // (delivered in pass 3) are closed by the next window's k_drive,
+            // or here when some have no neurons (no drive wavefront).
             const int sum_items = cell_range[1] * ((Wg + SG - 1) / SG);
             if (sum_items > 0) {
                 hipLaunchKernelGGL(k_sum, dim3((sum_items + 3) / 4), dim3(256), 0, stream, d_nets,
                                    d_blocks, sum_items, Wg, SG);
             }
-            const int n_close = active_range[2] - active_range[1];
-            if (n_close > 0) {
-                hipLaunchKernelGGL(k_close, dim3(n_close), dim3(64), 0, stream, d_nets,
-                                   d_active + active_range[1], n_close);
+            if (fp32_without_neurons) {
+                launch_close();
             }
             mark(5);
             if (timed_here) ++timed;
         }
+        launch_close();  // the last window's
         return timed;
+    }
+
+    // Closes every FP32 network's open window (k_close skips closed ones).
+    void launch_close() {
+        const int n_close = active_range[2] - active_range[1];
+        if (n_close > 0) {
+            hipLaunchKernelGGL(k_close, dim3(n_close), dim3(64), 0, stream, d_nets,
+                               d_active + active_range[1], n_close);
+        }
     }
 
     // Graph of `windows` windows for the current tables and schedule,
