@@ -1415,6 +1415,46 @@ __device__ inline void add_row(const uint32_t* __restrict__ tgt, const uint32_t*
     }
 }
 
+// An excitatory spike's two rows with one round of target loads: w_e into
+// the exc ring slot of every exc-row target, multiplicity * w_a into the n2a
+// ring slot of every (deduplicated) n2a-row target.
+__device__ inline void add_rows_exc(const DevNet& n, uint32_t e0, uint32_t e1, int64_t* base_e,
+                                    int64_t w_e, uint32_t a0, uint32_t a1, int64_t* base_a,
+                                    int64_t w_a, uint32_t lane) {
+    constexpr int UE = 4, UA = 2;
+    const uint32_t* __restrict__ et = n.exc_tgt;
+    const uint32_t* __restrict__ at = n.n2a_tgt;
+    const uint32_t* __restrict__ am = n.n2a_mult;
+    for (uint32_t qe = e0 + lane, qa = a0 + lane; qe < e1 || qa < a1; qe += 64 * UE, qa += 64 * UA) {
+        uint32_t te[UE], ta[UA], ma[UA];
+#pragma unroll
+        for (int u = 0; u < UE; ++u) {
+            const uint32_t q = qe + static_cast<uint32_t>(u) * 64;
+            te[u] = q < e1 ? et[q] : UINT32_MAX;
+        }
+#pragma unroll
+        for (int u = 0; u < UA; ++u) {
+            const uint32_t q = qa + static_cast<uint32_t>(u) * 64;
+            ta[u] = q < a1 ? at[q] : UINT32_MAX;
+            ma[u] = q < a1 ? am[q] : 0u;
+        }
+#pragma unroll
+        for (int u = 0; u < UE; ++u) {
+            if (te[u] != UINT32_MAX) {
+                atomicAdd(reinterpret_cast<unsigned long long*>(base_e + te[u]),
+                          static_cast<unsigned long long>(w_e));
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < UA; ++u) {
+            if (ta[u] != UINT32_MAX) {
+                atomicAdd(reinterpret_cast<unsigned long long*>(base_a + ta[u]),
+                          static_cast<unsigned long long>(w_a * static_cast<int64_t>(ma[u])));
+            }
+        }
+    }
+}
+
 // Pass 3 of an FP32 network for the 64 neurons of one wavefront, over the
 // whole window in time order. At each step the spiking lanes do their
 // Tsodyks-Markram update (network.cpp:230-247) and raster entry, then the
@@ -1502,11 +1542,10 @@ __device__ void deliver_fixed(DevNet& n, uint32_t first, uint32_t lane) {
                 m &= m - 1;
                 const int64_t w0 = static_cast<int64_t>(lane_u64(static_cast<uint64_t>(q0), L));
                 if (first + static_cast<uint32_t>(L) < NE) {
-                    add_row<false>(n.exc_tgt, nullptr, lane_u32(r0, L), lane_u32(r1, L),
-                            n.qring_exc + static_cast<size_t>(rowE) * LN, w0, lane);
-                    add_row<true>(n.n2a_tgt, n.n2a_mult, lane_u32(a0, L), lane_u32(a1, L),
-                            n.qring_astro + static_cast<size_t>(rowA) * LA,
-                            static_cast<int64_t>(lane_u64(static_cast<uint64_t>(q1), L)), lane);
+                    add_rows_exc(n, lane_u32(r0, L), lane_u32(r1, L),
+                                 n.qring_exc + static_cast<size_t>(rowE) * LN, w0, lane_u32(a0, L),
+                                 lane_u32(a1, L), n.qring_astro + static_cast<size_t>(rowA) * LA,
+                                 static_cast<int64_t>(lane_u64(static_cast<uint64_t>(q1), L)), lane);
                 } else {
                     // the inh ring holds the negated weights (as the FP64 sums)
                     add_row<false>(n.inh_tgt, nullptr, lane_u32(r0, L), lane_u32(r1, L),
