@@ -60,9 +60,10 @@ sys.path.insert(0, HERE)
 
 METRIC = "neuron-steps/s batched FP32 vs plain FP64 on 1 GPU; HBM GB/s; p99 job latency"
 # Kernel durations come from HIP events around one window in PROFILE_EVERY of
-# the timed region: events around every window would cost ~14% of the wall
-# time they are meant to explain.
-PROFILE_EVERY = 10
+# the timed region (200 of the 5,000 windows): events around every window
+# would cost ~14% of the wall time they are meant to explain, around one in
+# 25 about 1%.
+PROFILE_EVERY = 25
 SWEEP_CONFIG = os.path.join(HERE, "tests", "golden", "configs", "use_case.json")
 SPARSE_CONFIG = os.path.join(HERE, "tests", "golden", "configs", "paper_sparse.json")
 HBM_PEAK_GBS = 8000.0
