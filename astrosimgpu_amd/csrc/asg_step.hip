@@ -563,7 +563,7 @@ __device__ __forceinline__ void neuron_window_f32(const DevNet& n, uint32_t i, i
     int64_t sic_next = first_hit(T0, d_a2n, sic_interval);
     const uint32_t live_in = live_arrivals(n, T0);
     const bool sic_live_n = live_in & 1u;
-    // FP64 registers for the window; FP32 in memory
+    // FP64 registers, rounded to FP32 at the end of every step
     double V = 0, w = 0, gex = 0, dgex = 0, gin = 0, dgin = 0, Isic = 0, Vr = 0, b = 0;
     int refr = 0;
     int64_t ex_n = 0, ih_n = 0;
@@ -655,6 +655,14 @@ __device__ __forceinline__ void neuron_window_f32(const DevNet& n, uint32_t i, i
                                                  gex, dgex, gin, dgin, refr);
                 spiked = spiked || sp;
             }
+            // the state is FP32 between steps, so results do not depend on
+            // where windows (call lengths, the batch's shortest delay) fall
+            V = round_f32(V);
+            w = round_f32(w);
+            gex = round_f32(gex);
+            dgex = round_f32(dgex);
+            gin = round_f32(gin);
+            dgin = round_f32(dgin);
         }
         const uint64_t mask = __ballot(spiked);
         if (lane == 0 && (i & ~63u) < N) {

@@ -1,12 +1,12 @@
 // Per-cell math of the FP32 step (device only).
 //
-// State, conductances, rings and the drive are stored in FP32. A neuron's
-// state is loaded into FP64 registers for a window and integrated in FP64 --
-// FP64 vector instructions issue at the FP32 rate on this GPU, and near
-// threshold the exponential term e^((V - V_th)/Delta_T) decides the spike
-// time -- with the divisions by C_m, Delta_T and tau_w taken as products
-// with FP64 reciprocals and a table-free FP64 exp accurate to ~3e-13; it is
-// rounded to FP32 when stored at the end of the window. Astrocytes integrate
+// State, conductances and the drive are stored in FP32. A neuron's state is
+// integrated in FP64 registers -- FP64 vector instructions issue at the FP32
+// rate on this GPU, and near threshold the exponential term
+// e^((V - V_th)/Delta_T) decides the spike time -- with the divisions by
+// C_m, Delta_T and tau_w taken as products with FP64 reciprocals and a
+// table-free FP64 exp accurate to ~3e-13, and rounded to FP32 at the end of
+// every step, so results do not depend on window boundaries. Astrocytes integrate
 // in FP32 with hardware reciprocals; their Ca-threshold test of the SIC,
 // (Ca - SIC_th) * 1000 > 1, is evaluated in FP64 (Ca is a small difference
 // above a threshold). Random draws use the FP64 step's counter streams:
@@ -108,6 +108,11 @@ __device__ inline void adex_rhs_folded(const NeuronK& p, const NeuronKf& pf, dou
     const double I_spike = p.gL_DeltaT * exp2_poly(fma(Vc, pf.exp_a, pf.exp_b));
     dV = (fma(-Vc, g.gsum, g.gc - w) + I_spike) * pf.inv_C_m;
     dw = (fma(p.a, Vc, -pf.a_EL) - w) * pf.inv_tau_w;
+}
+
+// x rounded to FP32 and back.
+__device__ __forceinline__ double round_f32(double x) {
+    return static_cast<double>(static_cast<float>(x));
 }
 
 // One substep of the AdEx + alpha-conductance update (neuron.cpp:101-141)

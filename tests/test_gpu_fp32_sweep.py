@@ -102,3 +102,65 @@ def test_small_sweep_fp64_batch_equals_plain_and_fp32_is_close():
     assert cmp["networks"] == 12
     assert cmp["pooled_rate_rel_err"] < 0.05
     assert cmp["corr_abs_err_median"] < 0.1
+
+
+# Delays of 13, 21 and 17 steps (windows of 13), SIC every third step, a SIC
+# threshold just above the initial Ca.
+ODD_TIMING = {"synapse.d_e": 1.3, "synapse.d_i": 2.1, "synapse.d_a2n": 1.7,
+              "synapse.sic_interval": 3, "astrocyte.SIC_th": 0.076}
+
+
+def _fp32_model(seed, **assign):
+    m = A.Model(config_path("use_case"), seed=seed, sim_time=300.0, pre_time=0.0,
+                assignments=dict(ODD_TIMING, **assign))
+    m.desc.contents.precision = capi.ASG_FP32
+    return m
+
+
+def _fp32_run(m, chunks):
+    net = A.DeviceNetwork(m)
+    steps = m.info["sim_steps"]
+    done, k, parts = 0, 0, []
+    while done < steps:
+        c = min(chunks[k % len(chunks)], steps - done)
+        net.step(c)
+        done += c
+        k += 1
+        parts.append(net.spikes())
+    out = ({f: net.state(f) for f in ("V", "w", "g_ex", "Ca", "IP3", "I_sic")},
+           np.concatenate([p[0] for p in parts]), np.concatenate([p[1] for p in parts]))
+    net.close()
+    return out
+
+
+def test_fp32_results_do_not_depend_on_call_lengths():
+    """The FP32 state is FP32 between steps and the fixed-point sums are
+    order-free, so calls of 7/23 steps (windows cut at every call) and one
+    call give the same bits."""
+    _announce("fp32 call lengths")
+    m = _fp32_model(5)
+    a_state, a_s, a_n = _fp32_run(m, [m.info["sim_steps"]])
+    b_state, b_s, b_n = _fp32_run(m, [7, 23])
+    assert len(a_s) > 0
+    assert np.array_equal(a_s, b_s) and np.array_equal(a_n, b_n)
+    for f in a_state:
+        assert np.array_equal(a_state[f].view(np.uint64), b_state[f].view(np.uint64)), f
+
+
+def test_fp32_network_alone_equals_batch_with_shorter_windows():
+    """In a batch with a network of shorter delays (windows of 7 steps) an
+    FP32 network gives the same bits as alone (windows of 13)."""
+    _announce("fp32 alone vs shorter-window batch")
+    m = _fp32_model(5)
+    alone_state, alone_s, alone_n = _fp32_run(m, [m.info["sim_steps"]])
+    short = _fp32_model(8, **{"synapse.d_e": 0.7})
+    batch = A.Batch(4)
+    sa = batch.add(m, m.info["sim_steps"])
+    batch.add(short, short.info["sim_steps"])
+    batch.step(m.info["sim_steps"])
+    s, n = batch.spikes(sa)
+    assert np.array_equal(s, alone_s) and np.array_equal(n, alone_n)
+    for f in alone_state:
+        assert np.array_equal(batch.state(sa, f).view(np.uint64),
+                              alone_state[f].view(np.uint64)), f
+    batch.close()

@@ -143,6 +143,42 @@ def test_batch_members_equal_solo_runs():
             assert np.array_equal(batch.state(slot, f), o.state(f)), f
 
 
+# Delays that are multiples neither of each other nor of the window (exc
+# 13, inh 21, a2n 17 steps: windows of 13), the SIC exchanged every third
+# step, and a threshold just above the initial Ca so that exchanges turn
+# live during the run.
+ODD_TIMING = {"synapse.d_e": 1.3, "synapse.d_i": 2.1, "synapse.d_a2n": 1.7,
+              "synapse.sic_interval": 3, "astrocyte.SIC_th": 0.076}
+
+
+def test_odd_delays_and_sic_interval_equal_oracle():
+    """Calls of 7 and 23 steps cut windows short at every call, so ring rows,
+    arrival cadences and the SIC liveness ring are exercised at every
+    alignment; the device still equals the oracle bit for bit."""
+    _announce("odd timing")
+    m = A.Model(config_path("use_case"), seed=5, sim_time=300.0, pre_time=0.0,
+                assignments=ODD_TIMING)
+    steps = m.info["sim_steps"]
+    net = A.DeviceNetwork(m)
+    done, k, parts_s, parts_n = 0, 0, [], []
+    while done < steps:
+        c = min((7, 23)[k % 2], steps - done)
+        net.step(c)
+        done += c
+        k += 1
+        s, n = net.spikes()
+        parts_s.append(s)
+        parts_n.append(n)
+    s, n = np.concatenate(parts_s), np.concatenate(parts_n)
+    o = oracle.OracleRun(m.desc, oracle.LIBM_AMATH)
+    o.step(steps)
+    os_, on = o.spikes()
+    assert len(s) > 0 and np.count_nonzero(o.state("I_sic")) > 0
+    assert np.array_equal(s, os_) and np.array_equal(n, on)
+    for f in STATE_FIELDS:
+        assert np.array_equal(net.state(f).view(np.uint64), o.state(f).view(np.uint64)), f
+
+
 def test_astro_dropin_matches_engine_cell_math():
     """asg_astro_* (the astrocyte_cuda.hpp replacement) against the oracle's
     astrocyte update: an astrocyte-only network driven by its Poisson input,
