@@ -960,41 +960,58 @@ __global__ __launch_bounds__(64) void k_drive(DevNet* nets, const CellBlock* nbl
                 qn += static_cast<uint32_t>(__popcll(b));
             }
             __syncthreads();
-            while (qn > 0) {
-                uint32_t nn = 0;  // survivors, compacted to the front (nn <= off)
-                for (uint32_t off = 0; off < qn; off += 64) {
-                    const uint32_t idx = off + lane;
-                    bool pend = false;
-                    double p = 0.0;
-                    uint32_t key = 0;
-                    if (idx < qn) {
-                        p = q_prod[idx];
-                        key = q_key[idx];
-                        const uint32_t l = key & 63u, kk = (key >> 6) & 15u, cnt = (key >> 10) + 1;
-                        const uint32_t ii = cb.first + l;
-                        const double limit = n.in[ii < n.n_exc ? 0 : 1].limit;
-                        p *= rng_uniform(pseed, static_cast<uint64_t>(T0 + kk) * kStreamStride + ii,
-                                         cnt);
-                        if (p > limit) {
-                            pend = true;
-                            key = (cnt << 10) | (kk << 6) | l;
-                        } else {
-                            out[static_cast<size_t>(kk) * LN + ii] = static_cast<int32_t>(cnt);
-                        }
+            // The queue is a ring of 64 * Wg entries worked from its head 64
+            // at a time, survivors appended at its tail, so a draw's next
+            // uniform need not wait for the rest of its round. Each (step,
+            // lane) has at most one entry, so the live span tail - head
+            // never exceeds the ring.
+            const uint32_t cap = 64u * static_cast<uint32_t>(Wg);
+            // the stream key of (step kk, lane l) is base + (kk * stride + l) * phi
+            const uint64_t base = rng_key(pseed, static_cast<uint64_t>(T0) * kStreamStride + cb.first);
+            const double limit_e = n.in[0].limit, limit_i = n.in[1].limit;
+            const uint32_t n_exc = n.n_exc;
+            uint32_t head = 0, tail = qn;
+            while (head < tail) {
+                const uint32_t take = tail - head < 64u ? tail - head : 64u;
+                uint32_t at = head + lane;
+                at = at >= cap ? at - cap : at;
+                bool pend = false;
+                double p = 0.0;
+                uint32_t key = 0;
+                if (lane < take) {
+                    p = q_prod[at];
+                    key = q_key[at];
+                    const uint32_t l = key & 63u, kk = (key >> 6) & 15u, cnt = (key >> 10) + 1;
+                    const uint32_t ii = cb.first + l;
+                    const double limit = ii < n_exc ? limit_e : limit_i;
+                    const uint32_t soff = __umul24(kk, static_cast<uint32_t>(kStreamStride)) + l;
+                    const uint64_t z = base + static_cast<uint64_t>(soff) * kStreamMul +
+                                       static_cast<uint64_t>(cnt) * kCounterMul;
+                    p *= static_cast<double>(rng_bits_keyed(z, 0) >> 11) * (1.0 / 9007199254740992.0);
+                    if (p > limit) {
+                        pend = true;
+                        key = (cnt << 10) | (kk << 6) | l;
+                    } else {
+                        out[static_cast<size_t>(kk) * LN + ii] = static_cast<int32_t>(cnt);
                     }
-                    __syncthreads();  // this chunk is read before survivors overwrite it
-                    const uint64_t b = __ballot(pend);
-                    if (pend) {
-                        const uint32_t pos =
-                            nn + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(b >> 32),
-                                                           __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(b), 0));
-                        q_prod[pos] = p;
-                        q_key[pos] = key;
-                    }
-                    nn += static_cast<uint32_t>(__popcll(b));
-                    __syncthreads();
                 }
-                qn = nn;
+                head += take;
+                head = head >= cap ? head - cap : head;
+                tail = tail >= cap ? tail - cap : tail;
+                if (tail < head) {
+                    tail += cap;  // keep tail - head the live span
+                }
+                __syncthreads();  // this chunk is read before survivors overwrite it
+                const uint64_t b = __ballot(pend);
+                if (pend) {
+                    uint32_t pos = tail + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(b >> 32),
+                                                                    __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(b), 0));
+                    pos = pos >= cap ? pos - cap : pos;
+                    q_prod[pos] = p;
+                    q_key[pos] = key;
+                }
+                tail += static_cast<uint32_t>(__popcll(b));
+                __syncthreads();
             }
         }
     }
