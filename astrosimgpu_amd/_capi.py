@@ -180,6 +180,7 @@ PROTOTYPES = {
     "asg_batch_step_async": (C.c_int, [P, C.c_int64]),
     "asg_batch_net_step": (C.c_int64, [P, C.c_int32]),
     "asg_batch_spikes": (C.c_int, [P, C.c_int32, I64P, U32P, C.c_int64, I64P]),
+    "asg_batch_spikes_all": (C.c_int, [P, I64P, I64P, U32P, C.c_int64, I64P]),
     "asg_batch_read_state": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int64, C.c_int64, DBLP]),
     "asg_batch_counters": (C.c_int, [P, C.c_int32, C.POINTER(Counters)]),
     "asg_batch_bytes_in_use": (C.c_int64, [P]),

@@ -2973,6 +2973,25 @@ struct Engine {
 
     int64_t spikes_out(int slot, int64_t* step, uint32_t* src, int64_t cap) {
         drain_all();
+        return take_queued(slot, step, src, cap);
+    }
+
+    // Every slot's queued pairs, back to back in slot order (counts per slot).
+    int64_t spikes_out_all(int64_t* counts, int64_t* step, uint32_t* src, int64_t cap) {
+        drain_all();
+        int64_t done = 0;
+        for (int s = 0; s < max_nets; ++s) {
+            counts[s] = 0;
+            if (slots[s].used && done < cap) {
+                counts[s] = take_queued(s, step + done, src + done, cap - done);
+                done += counts[s];
+            }
+        }
+        return done;
+    }
+
+    // Up to cap of a slot's host-queued pairs, oldest first.
+    int64_t take_queued(int slot, int64_t* step, uint32_t* src, int64_t cap) {
         NetSlot& s = slots[slot];
         const int64_t avail = static_cast<int64_t>(s.q_step.size() - s.q_pos);
         const int64_t m = std::min(avail, cap);
@@ -3370,6 +3389,21 @@ asg_status asg_batch_spikes(asg_batch* batch, int32_t slot, int64_t* step, uint3
             return fail(ASG_ERR_OVERFLOW, overflow_message(bits));
         }
         *n_out = batch->eng->spikes_out(slot, step, source, capacity);
+        return ASG_OK;
+    });
+}
+
+asg_status asg_batch_spikes_all(asg_batch* batch, int64_t* counts, int64_t* step,
+                                uint32_t* source, int64_t capacity, int64_t* n_out) {
+    if (batch == nullptr || counts == nullptr || n_out == nullptr || capacity < 0 ||
+        (capacity > 0 && (step == nullptr || source == nullptr))) {
+        return fail(ASG_ERR_INVALID, "bad argument");
+    }
+    return guarded([&]() -> asg_status {
+        if (const int bits = batch->eng->overflow_bits_any()) {
+            return fail(ASG_ERR_OVERFLOW, overflow_message(bits));
+        }
+        *n_out = batch->eng->spikes_out_all(counts, step, source, capacity);
         return ASG_OK;
     });
 }

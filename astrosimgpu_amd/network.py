@@ -198,8 +198,10 @@ class Batch:
     def __init__(self, max_nets: int, budget_bytes: int = 0):
         self._h = C.c_void_p()
         check(lib().asg_batch_create(int(max_nets), int(budget_bytes), C.byref(self._h)))
+        self.max_nets = int(max_nets)
         self.models = {}
         self._rec_n = {}
+        self._drain = None  # staging arrays of spikes_all, kept between calls
 
     def add(self, model: Model, total_steps: int) -> int:
         slot = C.c_int32()
@@ -241,6 +243,39 @@ class Batch:
             if n.value < capacity:
                 break
         return np.concatenate(out_s), np.concatenate(out_n)
+
+    def spikes_all(self, capacity: int = 1 << 20) -> dict:
+        """Drain every network's raster in one call (asg_batch_spikes_all):
+        {slot: (steps int64, sources uint32)} for every occupied slot, each
+        slot's pairs in the order asg_batch_spikes would return them."""
+        if self._drain is None or len(self._drain[1]) != capacity:
+            self._drain = (np.empty(self.max_nets, np.int64), np.empty(capacity, np.int64),
+                           np.empty(capacity, np.uint32))
+        counts, step, src = self._drain
+        n = C.c_int64()
+        rounds = []
+        while True:
+            check(lib().asg_batch_spikes_all(self._h, counts.ctypes.data_as(capi.I64P),
+                                             step.ctypes.data_as(capi.I64P),
+                                             src.ctypes.data_as(capi.U32P), capacity,
+                                             C.byref(n)))
+            # one right-sized copy per round; the slots' arrays are views of it
+            ends = np.cumsum(counts)
+            rounds.append(((ends - counts).tolist(), ends.tolist(), step[:n.value].copy(),
+                           src[:n.value].copy()))
+            if n.value < capacity:
+                break
+        out = {}
+        for slot in self.models:
+            parts_s, parts_n = [], []
+            for starts, ends, st, sr in rounds:
+                parts_s.append(st[starts[slot]:ends[slot]])
+                parts_n.append(sr[starts[slot]:ends[slot]])
+            if len(rounds) == 1:
+                out[slot] = (parts_s[0], parts_n[0])
+            else:
+                out[slot] = (np.concatenate(parts_s), np.concatenate(parts_n))
+        return out
 
     def state(self, slot: int, field: str, offset: int = 0, count: Optional[int] = None):
         m = self.models[slot]

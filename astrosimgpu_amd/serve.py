@@ -184,8 +184,9 @@ class Server:
             # one chunk for every running job
             self.batch.step(self.chunk_steps)
             finished = []
+            drained = self.batch.spikes_all()
             for slot, j in running.items():
-                st, src = self.batch.spikes(slot, 1 << 16)
+                st, src = drained[slot]
                 j.spike_steps.append(st)
                 j.spike_src.append(src)
                 if self.batch.net_step(slot) >= j.total_steps:

@@ -103,9 +103,9 @@ def run_batched(models, precision: int, chunk_steps: int = 10000, budget_bytes: 
     while done < longest:
         batch.step(chunk_steps)
         done += chunk_steps
+        drained = batch.spikes_all()
         for k, s in enumerate(slots):
-            st, _ = batch.spikes(s)
-            spikes[k].append(st)
+            spikes[k].append(drained[s][0])
     elapsed = time.perf_counter() - t0
     traces = []
     for s in slots:

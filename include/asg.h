@@ -387,6 +387,18 @@ int64_t asg_batch_net_step(asg_batch* batch, int32_t slot);
 
 asg_status asg_batch_spikes(asg_batch* batch, int32_t slot, int64_t* step, uint32_t* source,
                             int64_t capacity, int64_t* n_out);
+
+/* The drain of asg_batch_spikes for every slot in one call (one device
+ * round trip instead of one call per network). The pending (step, source)
+ * pairs are written back to back in slot order, each slot's in its own
+ * order; counts[s] (max_nets entries) receives how many belong to slot s
+ * (0 for an empty slot) and *n_out the total. With fewer than the pending
+ * pairs' room, the first `capacity` are written (the last slot written may
+ * be partial) and the rest stay queued for the next call, so callers repeat
+ * while *n_out == capacity. ASG_ERR_OVERFLOW, and nothing drained, if any
+ * network's raster overflowed. */
+asg_status asg_batch_spikes_all(asg_batch* batch, int64_t* counts, int64_t* step,
+                                uint32_t* source, int64_t capacity, int64_t* n_out);
 asg_status asg_batch_read_state(asg_batch* batch, int32_t slot, int32_t field, int64_t offset,
                                 int64_t count, double* out);
 asg_status asg_batch_counters(asg_batch* batch, int32_t slot, asg_counters* out);

@@ -25,9 +25,10 @@ def _models():
     return ms
 
 
-def _run(configure, chunks=(700, 1000, 1000, 300), total=3000):
-    """Every chunk's spikes are drained and a state array read between the
-    calls (eager work between graph replays)."""
+def _run(configure, chunks=(700, 1000, 1000, 300), total=3000, drain_all=None):
+    """Every chunk's spikes are drained (slot by slot, or all slots in one
+    call with that capacity) and a state array read between the calls
+    (eager work between graph replays)."""
     models = _models()
     b = A.Batch(8)
     slots = [b.add(m, total) for m in models]
@@ -35,8 +36,9 @@ def _run(configure, chunks=(700, 1000, 1000, 300), total=3000):
     parts = {s: ([], []) for s in slots}
     for c in chunks:
         b.step(c)
+        drained = b.spikes_all(drain_all) if drain_all else None
         for s in slots:
-            st, src = b.spikes(s)
+            st, src = drained[s] if drain_all else b.spikes(s)
             parts[s][0].append(st)
             parts[s][1].append(src)
         b.state(slots[1], "V")
@@ -74,6 +76,17 @@ def test_graph_replays_equal_eager_launches():
     # uneven chunks: several graphs (one per window count)
     ref = _run(lambda b: None)
     _same(ref, _run(lambda b: b.set_graphs(True)))
+
+
+def test_drain_of_all_slots_equals_slot_by_slot():
+    """asg_batch_spikes_all: the same pairs per slot as asg_batch_spikes,
+    also when a small capacity splits the drain over several calls with
+    slots cut in the middle (4 occupied slots of 8)."""
+    print("[gpu case] drain all", flush=True)
+    ref = _run(lambda b: None)
+    assert sum(len(x[0]) for x in ref) > 200  # spikes enough to split
+    for cap in (1 << 20, 97):
+        _same(ref, _run(lambda b: None, drain_all=cap))
 
 
 def test_autotune_then_step_equals_plain_run():

@@ -68,6 +68,9 @@ class FakeBatch:
     def spikes(self, slot, capacity=0):
         return np.zeros(0, np.int64), np.zeros(0, np.uint32)
 
+    def spikes_all(self, capacity=0):
+        return {slot: self.spikes(slot) for slot in self.nets}
+
     def bytes_in_use(self):
         return sum(v[2] for v in self.nets.values())
 
