@@ -1898,6 +1898,7 @@ __global__ void k_math(int32_t fn, const double* x, double* y, int64_t n) {
             case ASG_MATH_LOG: r = asg_log(v); break;
             case ASG_MATH_SIN: r = asg_sin(v); break;
             case ASG_MATH_COS: r = asg_cos(v); break;
+            case ASG_MATH_EXP2_F32: r = spike_exp2(v); break;
             default: r = asg_sqrt(v); break;
         }
         y[i] = r;
@@ -3532,7 +3533,7 @@ asg_status asg_batch_kernel_times(asg_batch* batch, asg_kernel_times* out) {
 }
 
 asg_status asg_device_math(int32_t fn, const double* x, double* y, int64_t n) {
-    if (fn < ASG_MATH_EXP || fn > ASG_MATH_SQRT || n < 0 || (n > 0 && (x == nullptr || y == nullptr))) {
+    if (fn < ASG_MATH_EXP || fn > ASG_MATH_EXP2_F32 || n < 0 || (n > 0 && (x == nullptr || y == nullptr))) {
         return fail(ASG_ERR_INVALID, "bad argument");
     }
     if (n == 0) return ASG_OK;

@@ -66,25 +66,13 @@ __device__ inline void comp_add(float& s, float& c, float x) {
     s = t;
 }
 
-// 2^y in FP64 for the neuron integration (|y| below ~60 there): n = rint(y),
-// f = y - n (exact, |f| <= 1/2), 2^f by a degree-7 polynomial in f and the
-// scale 2^n. The coefficients are a near-minimax fit of the relative error
-// over [-1/2, 1/2] (Lawson's reweighted least squares on Chebyshev nodes);
-// relative error below 4.1e-11, checked on the host against exp2l over
-// [-60, 10] -- three orders below the FP32 rounding the state takes at
-// every step.
-__device__ inline double exp2_poly(double y) {
-    const double n = rint(y);
-    const double f = y - n;
-    double p = 1.5201918127723554e-05;
-    p = fma(p, f, 0.00015469291113459095);
-    p = fma(p, f, 0.0013333922578591716);
-    p = fma(p, f, 0.009618027253774001);
-    p = fma(p, f, 0.055504103534293775);
-    p = fma(p, f, 0.24022651198156603);
-    p = fma(p, f, 0.6931471807284527);
-    p = fma(p, f, 0.9999999999616821);
-    return ldexp(p, static_cast<int>(n));
+// 2^y of the AdEx spike current: y rounded to FP32 and the hardware
+// v_exp_f32 (about 1 ulp), with no range reduction. Its relative error,
+// ln2 |y| 2^-24 from rounding y plus the instruction's own, is of the order
+// of what the FP32 rounding of V at every step already puts into y
+// (ulp(V) / Delta_T), and half the issue cycles of an FP64 polynomial.
+__device__ __forceinline__ double spike_exp2(double y) {
+    return static_cast<double>(__builtin_amdgcn_exp2f(static_cast<float>(y)));
 }
 
 // The conductance terms of the AdEx right-hand side for one (g_ex, g_in):
@@ -105,7 +93,7 @@ __device__ inline AdexG adex_g(const NeuronK& p, const NeuronKf& pf, double g_ex
 __device__ inline void adex_rhs_folded(const NeuronK& p, const NeuronKf& pf, double V, double w,
                                        const AdexG& g, double& dV, double& dw) {
     const double Vc = (p.V_peak < V) ? p.V_peak : V;
-    const double I_spike = p.gL_DeltaT * exp2_poly(fma(Vc, pf.exp_a, pf.exp_b));
+    const double I_spike = p.gL_DeltaT * spike_exp2(fma(Vc, pf.exp_a, pf.exp_b));
     dV = (fma(-Vc, g.gsum, g.gc - w) + I_spike) * pf.inv_C_m;
     dw = (fma(p.a, Vc, -pf.a_EL) - w) * pf.inv_tau_w;
 }

@@ -365,8 +365,9 @@ def _read_trace(call, n_rec: int, max_records: int):
 
 
 def device_math(fn: str, x: np.ndarray) -> np.ndarray:
-    """Evaluate exp/log/sin/cos/sqrt of the device step on the device."""
-    codes = {"exp": 0, "log": 1, "sin": 2, "cos": 3, "sqrt": 4}
+    """Evaluate exp/log/sin/cos/sqrt of the device step, or the FP32 path's
+    2^y of the spike current ("exp2_f32"), on the device."""
+    codes = {"exp": 0, "log": 1, "sin": 2, "cos": 3, "sqrt": 4, "exp2_f32": 5}
     x = np.ascontiguousarray(x, np.float64)
     y = np.empty_like(x)
     check(lib().asg_device_math(codes[fn], x.ctypes.data_as(capi.DBLP),

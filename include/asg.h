@@ -462,11 +462,19 @@ asg_status asg_net_kernel_times(asg_net* net, asg_kernel_times* out);
 asg_status asg_batch_set_profiling(asg_batch* batch, int32_t every);
 asg_status asg_batch_kernel_times(asg_batch* batch, asg_kernel_times* out);
 
-enum { ASG_MATH_EXP = 0, ASG_MATH_LOG = 1, ASG_MATH_SIN = 2, ASG_MATH_COS = 3, ASG_MATH_SQRT = 4 };
+enum {
+    ASG_MATH_EXP = 0,
+    ASG_MATH_LOG = 1,
+    ASG_MATH_SIN = 2,
+    ASG_MATH_COS = 3,
+    ASG_MATH_SQRT = 4,
+    ASG_MATH_EXP2_F32 = 5 /* the FP32 path's 2^y of the AdEx spike current */
+};
 
-/* Evaluates one of the step's FP64 functions on the device for n host
- * inputs (no counterpart in the reference; used to show the device rounds
- * these functions exactly as the host build does). */
+/* Evaluates one of the step's functions on the device for n host inputs
+ * (no counterpart in the reference; used to show the device rounds the FP64
+ * functions exactly as the host build does, and to bound the FP32 path's
+ * 2^y). */
 asg_status asg_device_math(int32_t fn, const double* x, double* y, int64_t n);
 
 #ifdef __cplusplus

@@ -38,6 +38,25 @@ def test_device_math_is_host_amath_bit_for_bit(fn, lo, hi):
     assert np.array_equal(host.view(np.uint64), dev.view(np.uint64))
 
 
+def test_fp32_spike_exp2_error_bound():
+    """The FP32 path's 2^y of the AdEx spike current (y rounded to FP32,
+    hardware exp): relative error within ln2 |y| 2^-24 (rounding y) plus two
+    FP32 ulps, against 2^y in x87 extended precision, over the range the
+    exponent takes (V from far below rest to V_peak); below 2^-126 it only
+    has to stay that small."""
+    _announce("device math exp2_f32")
+    rng = np.random.default_rng(11)
+    y = np.concatenate([rng.uniform(-60.0, 40.0, 1 << 20), np.linspace(-1.0, 1.0, 4097),
+                        np.arange(-60.0, 41.0)])
+    got = A.device_math("exp2_f32", y)
+    exact = np.exp2(y.astype(np.longdouble))
+    rel = np.abs((got.astype(np.longdouble) - exact) / exact).astype(np.float64)
+    bound = np.log(2.0) * np.abs(y) * 2.0 ** -24 * 1.0001 + 2.0 ** -22
+    assert np.all(rel <= bound), (y[np.argmax(rel / bound)], rel.max())
+    tiny = A.device_math("exp2_f32", np.linspace(-150.0, -127.0, 1001))
+    assert np.all((tiny >= 0.0) & (tiny <= 2.0 ** -126))
+
+
 def test_device_exp_log_equal_c_library():
     rng = np.random.default_rng(7)
     x = rng.uniform(-30, 30, 1 << 20)
