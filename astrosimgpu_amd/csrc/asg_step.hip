@@ -119,6 +119,7 @@ constexpr double kFixScale = 4294967296.0;        // 2^32: fixed-point ring unit
 constexpr double kFixUnit = 1.0 / 4294967296.0;   // 2^-32
 constexpr int64_t kSicWaves = 4096; // SIC wavefronts that fill the device (256 CUs x 16)
 constexpr int64_t kGroupWaves = 16384;  // per-step wavefronts beyond which steps are grouped
+constexpr int kDeliverWaves = 512;  // FP32 delivery wavefronts sharing out the spiking blocks
 constexpr size_t kMaskBudget = size_t(256) << 20;  // delivery masks per network
 constexpr size_t kMinChunks = 64;
 
@@ -538,7 +539,8 @@ __device__ __forceinline__ void astro_window(const DevNet& n, uint32_t a, int64_
 
 // ----------------------------------------------------- FP32 cell windows ---
 
-__device__ __forceinline__ void neuron_window_f32(const DevNet& n, uint32_t i, int64_t T0, int W) {
+__device__ __forceinline__ uint64_t neuron_window_f32(const DevNet& n, uint32_t i, int64_t T0,
+                                                      int W) {
     const uint32_t N = n.n_neur;
     const size_t LN = n.ldn;  // row stride of the [rows][neurons] arrays
     const bool valid = i < N;
@@ -590,6 +592,7 @@ __device__ __forceinline__ void neuron_window_f32(const DevNet& n, uint32_t i, i
         if (has_noise) noise_n = drive_noise[i];
     }
     const uint32_t lane = threadIdx.x & 63;
+    uint64_t spiked_any = 0;  // OR of the window's spike words (uniform)
     for (int k = 0; k < W; ++k) {
         bool spiked = false;
         const int row_n = ring_next(row, S);
@@ -668,6 +671,7 @@ __device__ __forceinline__ void neuron_window_f32(const DevNet& n, uint32_t i, i
         if (lane == 0 && (i & ~63u) < N) {
             spk_bits[static_cast<size_t>(k) * words + (i >> 6)] = mask;
         }
+        spiked_any |= mask;
         row = row_n;
     }
     if (valid) {
@@ -680,6 +684,7 @@ __device__ __forceinline__ void neuron_window_f32(const DevNet& n, uint32_t i, i
         n.fIsic[i] = static_cast<float>(Isic);
         n.refr[i] = refr;
     }
+    return spiked_any;
 }
 
 __device__ __forceinline__ void astro_window_f32(const DevNet& n, uint32_t a, int64_t T0, int W) {
@@ -892,8 +897,11 @@ __device__ inline void close_open_window(DevNet& n) {
 // same (seed, stream, counter) as in the sequential loop, so the counts are
 // identical.
 __global__ __launch_bounds__(64) void k_drive(DevNet* nets, const CellBlock* nblocks,
-                                              int32_t n_items, int32_t Wg) {
+                                              int32_t n_items, int32_t Wg, int32_t* work_n) {
     extern __shared__ double drive_lds[];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *work_n = 0;  // the window's FP32 delivery list starts empty
+    }
     double* q_prod = drive_lds;                                             // [64 * Wg]
     uint32_t* q_key = reinterpret_cast<uint32_t*>(drive_lds + 64 * Wg);     // [64 * Wg]
     const int item = static_cast<int>(blockIdx.x);
@@ -1060,9 +1068,12 @@ __global__ __launch_bounds__(64) void k_drive(DevNet* nets, const CellBlock* nbl
 // astrocytes share the launch, so the short astrocyte blocks overlap the
 // neuron blocks.
 // WPB wavefronts (64-cell blocks) per workgroup, a launch-schedule knob.
+// An FP32 neuron block that spiked in the window appends itself to the
+// delivery list `work` (count in *work_n, zeroed by the window's k_drive).
 template <int P, int WPB>
 __global__ __launch_bounds__(kCellBlock * WPB) void k_cells(DevNet* nets, const CellBlock* blocks,
-                                                            int32_t count, int32_t Wg) {
+                                                            int32_t count, int32_t Wg,
+                                                            CellBlock* work, int32_t* work_n) {
     // The wavefront's block index is uniform across its lanes: said so
     // explicitly, so the block and network descriptors stay in scalar
     // registers.
@@ -1087,7 +1098,9 @@ __global__ __launch_bounds__(kCellBlock * WPB) void k_cells(DevNet* nets, const 
     const uint32_t i = cb.first + lane;
     if constexpr (P == ASG_FP32) {
         if (cb.kind == 0) {
-            neuron_window_f32(n, i, T0, W);
+            if (neuron_window_f32(n, i, T0, W) != 0 && lane == 0) {
+                work[atomicAdd(work_n, 1)] = cb;
+            }
         } else {
             astro_window_f32(n, i, T0, W);
         }
@@ -1590,8 +1603,8 @@ __device__ void deliver_fixed(DevNet& n, uint32_t first, uint32_t lane) {
     if (syn > 0) {
         add_i64(&n.c_syn, syn);
     }
-    if (!room && window_total > 0 && t == 0) {
-        atomicOr(&n.overflow, 1);
+    if (!room && window_total > 0 && lane == 0) {
+        atomicOr(&n.overflow, 1);  // every delivering wavefront of the network agrees
     }
 }
 
@@ -1603,14 +1616,41 @@ __device__ void deliver_fixed(DevNet& n, uint32_t first, uint32_t lane) {
 // parallelism for small networks (G = 1); MG = Wg saves the dispatch of
 // wavefronts that mostly find no spike in a large batch. One instantiation
 // per precision, so the FP32 networks' launch does not carry the FP64 SIC
-// accumulators' registers; FP32 networks take one delivery wavefront per
-// block (deliver_fixed, MG = Wg) instead of marking.
+// accumulators' registers. FP32 networks need no marking: their launch
+// holds MG delivery wavefronts (deliver_fixed), which share out the blocks
+// the cell pass listed as having spiked, and then the SIC wavefronts.
 template <int G, int P>
 __global__ __launch_bounds__(256) void k_deliver(DevNet* nets, const CellBlock* nblocks,
-                                                 int32_t n_items, int32_t Wg, int32_t MG) {
+                                                 int32_t n_items, int32_t Wg, int32_t MG,
+                                                 const CellBlock* work, const int32_t* work_n) {
     const int e = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x) * 4 +
                                                  static_cast<int>(threadIdx.x >> 6));
     if (e >= n_items) {
+        return;
+    }
+    const uint32_t lane = threadIdx.x & 63;
+    if constexpr (P == ASG_FP32) {
+        // The first MG items are delivery wavefronts, each taking every
+        // MG-th block of the list of blocks that spiked in the window (first,
+        // so that their chains of dependent loads start with the launch); the
+        // others are the SIC wavefronts, ceil(Wg / G) per block.
+        if (e < MG) {
+            const int listed = *work_n;
+            for (int j = e; j < listed; j += MG) {
+                const CellBlock cb = work[j];
+                deliver_fixed(nets[cb.net], cb.first, lane);
+            }
+            return;
+        }
+        const int per_block = (Wg + G - 1) / G;
+        const CellBlock cb = nblocks[(e - MG) / per_block];
+        DevNet& n = nets[cb.net];
+        const int g0 = ((e - MG) % per_block) * G;
+        const int len = min(G, n.win_W - g0);
+        const uint32_t t = cb.first + lane;
+        if (len > 0 && t < n.n_neur) {
+            deliver_sic_f32<G>(n, t, g0, len);
+        }
         return;
     }
     const int mark_waves = (Wg + MG - 1) / MG;
@@ -1618,34 +1658,22 @@ __global__ __launch_bounds__(256) void k_deliver(DevNet* nets, const CellBlock* 
     const CellBlock cb = nblocks[e / per_block];
     const int k = e % per_block;
     DevNet& n = nets[cb.net];
-    const uint32_t lane = threadIdx.x & 63;
     if (k >= mark_waves) {
         const int g0 = (k - mark_waves) * G;
         const int len = min(G, n.win_W - g0);
         const uint32_t t = cb.first + lane;
         if (len > 0 && t < n.n_neur) {
-            if constexpr (P == ASG_FP32) {
-                deliver_sic_f32<G>(n, t, g0, len);
-            } else {
-                deliver_sic<G>(n, t, g0, len);
-            }
+            deliver_sic<G>(n, t, g0, len);
         }
         return;
     }
     const int k0 = k * MG;
-    if constexpr (P == ASG_FP32) {
-        // launched with MG = Wg: one delivery wavefront per block
-        if (k0 == 0 && n.win_W > 0) {
-            deliver_fixed(n, cb.first, lane);
-        }
-    } else {
-        const int k1 = min(k0 + MG, n.win_W);
-        if (k1 > k0) {
-            mark_steps(n, cb.first, lane, k0, k1);
-        }
-        if (k0 == 0 && n.win_W > 0) {
-            deliver_sources(n, cb.first, lane);
-        }
+    const int k1 = min(k0 + MG, n.win_W);
+    if (k1 > k0) {
+        mark_steps(n, cb.first, lane, k0, k1);
+    }
+    if (k0 == 0 && n.win_W > 0) {
+        deliver_sources(n, cb.first, lane);
     }
 }
 
@@ -2392,7 +2420,11 @@ struct Engine {
     CellBlock* d_blocks = nullptr;
     CellBlock* d_nblocks = nullptr;   // neuron blocks only
     int32_t* d_active = nullptr;
-    size_t blocks_cap = 0, nblocks_cap = 0, active_cap = 0;
+    // The window's FP32 delivery list: neuron blocks that spiked, appended by
+    // k_cells, counted in *d_work_n (zeroed by k_drive).
+    CellBlock* d_work = nullptr;
+    int32_t* d_work_n = nullptr;
+    size_t blocks_cap = 0, nblocks_cap = 0, active_cap = 0, work_cap = 0;
     int n_blocks = 0, n_nblocks = 0, n_active = 0, Wg = 1;
     std::vector<int32_t> h_active;     // host copy of d_active
     bool dirty = true;
@@ -2471,6 +2503,8 @@ struct Engine {
         (void)hipFree(d_blocks);
         (void)hipFree(d_nblocks);
         (void)hipFree(d_active);
+        (void)hipFree(d_work);
+        (void)hipFree(d_work_n);
         (void)hipFree(d_targets);
         (void)hipFree(d_heads);
         (void)hipFree(d_flags);
@@ -2588,6 +2622,12 @@ struct Engine {
             grow(d_nblocks, nblocks_cap, nblocks.size());
             HIP_CHECK(hipMemcpyAsync(d_nblocks, nblocks.data(), nblocks.size() * sizeof(CellBlock),
                                      hipMemcpyHostToDevice, stream));
+            // each FP32 neuron block is listed at most once per window
+            grow(d_work, work_cap, nblocks.size());
+            if (d_work_n == nullptr) {
+                HIP_CHECK(hipMalloc(&d_work_n, sizeof(int32_t)));
+                HIP_CHECK(hipMemsetAsync(d_work_n, 0, sizeof(int32_t), stream));
+            }
         }
         if (n_active > 0) {
             grow(d_active, active_cap, active.size());
@@ -2637,15 +2677,21 @@ struct Engine {
     void launch_deliver_range(int range, int32_t Wg_, int MG) {
         const int count = nblock_range[range + 1] - nblock_range[range];
         if (count == 0) return;
-        const int items = count * ((Wg_ + MG - 1) / MG + (Wg_ + G - 1) / G);
+        int items;
+        if constexpr (P == ASG_FP32) {
+            MG = std::min(count, kDeliverWaves);  // delivery wavefronts before the SIC ones
+            items = count * ((Wg_ + G - 1) / G) + MG;
+        } else {
+            items = count * ((Wg_ + MG - 1) / MG + (Wg_ + G - 1) / G);
+        }
         hipLaunchKernelGGL((k_deliver<G, P>), dim3((items + 3) / 4), dim3(256), 0, stream, d_nets,
-                           d_nblocks + nblock_range[range], items, Wg_, MG);
+                           d_nblocks + nblock_range[range], items, Wg_, MG, d_work, d_work_n);
     }
 
     template <int G>
     void launch_deliver_g(int32_t Wg_, int MG) {
         launch_deliver_range<G, ASG_FP64>(0, Wg_, MG);
-        launch_deliver_range<G, ASG_FP32>(1, Wg_, Wg_);
+        launch_deliver_range<G, ASG_FP32>(1, Wg_, 0);
     }
 
     // Pass 3 over every neuron block (FP64 networks' marking in groups of MG
@@ -2676,15 +2722,15 @@ struct Engine {
         switch (wpb) {
             case 4:
                 hipLaunchKernelGGL((k_cells<P, 4>), grid, dim3(4 * kCellBlock), 0, stream, d_nets,
-                                   blocks, count, Wg_);
+                                   blocks, count, Wg_, d_work, d_work_n);
                 break;
             case 2:
                 hipLaunchKernelGGL((k_cells<P, 2>), grid, dim3(2 * kCellBlock), 0, stream, d_nets,
-                                   blocks, count, Wg_);
+                                   blocks, count, Wg_, d_work, d_work_n);
                 break;
             default:
                 hipLaunchKernelGGL((k_cells<P, 1>), grid, dim3(kCellBlock), 0, stream, d_nets,
-                                   blocks, count, Wg_);
+                                   blocks, count, Wg_, d_work, d_work_n);
                 break;
         }
     }
@@ -2718,7 +2764,7 @@ struct Engine {
             if (n_nblocks > 0) {
                 hipLaunchKernelGGL(k_drive, dim3(n_nblocks), dim3(64),
                                    static_cast<size_t>(64 * Wg) * (sizeof(double) + sizeof(uint32_t)),
-                                   stream, d_nets, d_nblocks, n_nblocks, Wg);
+                                   stream, d_nets, d_nblocks, n_nblocks, Wg, d_work_n);
             }
             mark(1);
             launch_cells<ASG_FP64>(0, Wg, wpb);
