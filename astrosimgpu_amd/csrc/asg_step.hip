@@ -1424,30 +1424,23 @@ __device__ void deliver_sources(DevNet& n, uint32_t first, uint32_t lane) {
     }
 }
 
-// Window pass 3: one wavefront per (64 neurons, window step); four per block.
-// Add val (fixed point) into the ring slot of every target of one row, the
-// whole wavefront together, 4 target loads in flight per lane; with MULT,
-// target q's multiplicity mult[q] times val.
-template <bool MULT>
-__device__ inline void add_row(const uint32_t* __restrict__ tgt, const uint32_t* __restrict__ mult,
-                               uint32_t q0, uint32_t q1, int64_t* base, int64_t val, uint32_t lane) {
+// Adds val (fixed point) into the ring slot of every target of one row, the
+// whole wavefront together, 4 target loads in flight per lane.
+__device__ inline void add_row(const uint32_t* __restrict__ tgt, uint32_t q0, uint32_t q1,
+                               int64_t* base, int64_t val, uint32_t lane) {
     constexpr int U = 4;
     for (uint32_t q = q0 + lane; q < q1; q += 64 * U) {
-        uint32_t tt[U], mm[U];
+        uint32_t tt[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const uint32_t qq = q + static_cast<uint32_t>(u) * 64;
             tt[u] = qq < q1 ? tgt[qq] : UINT32_MAX;
-            if constexpr (MULT) {
-                mm[u] = qq < q1 ? mult[qq] : 0u;
-            }
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             if (tt[u] != UINT32_MAX) {
-                const int64_t v = MULT ? val * static_cast<int64_t>(mm[u]) : val;
                 atomicAdd(reinterpret_cast<unsigned long long*>(base + tt[u]),
-                          static_cast<unsigned long long>(v));
+                          static_cast<unsigned long long>(val));
             }
         }
     }
@@ -1586,7 +1579,7 @@ __device__ void deliver_fixed(DevNet& n, uint32_t first, uint32_t lane) {
                                  static_cast<int64_t>(lane_u64(static_cast<uint64_t>(q1), L)), lane);
                 } else {
                     // the inh ring holds the negated weights (as the FP64 sums)
-                    add_row<false>(n.inh_tgt, nullptr, lane_u32(r0, L), lane_u32(r1, L),
+                    add_row(n.inh_tgt, lane_u32(r0, L), lane_u32(r1, L),
                                    n.qring_inh + static_cast<size_t>(rowI) * LN, -w0, lane);
                 }
             }
