@@ -541,6 +541,8 @@ __device__ __forceinline__ void astro_window(const DevNet& n, uint32_t a, int64_
 
 __device__ __forceinline__ uint64_t neuron_window_f32(const DevNet& n, uint32_t i, int64_t T0,
                                                       int W) {
+    // FP32 path: mul-add pairs may fuse (it has no host build to match)
+    #pragma clang fp contract(fast)
     const uint32_t N = n.n_neur;
     const size_t LN = n.ldn;  // row stride of the [rows][neurons] arrays
     const bool valid = i < N;
@@ -688,6 +690,8 @@ __device__ __forceinline__ uint64_t neuron_window_f32(const DevNet& n, uint32_t 
 }
 
 __device__ __forceinline__ void astro_window_f32(const DevNet& n, uint32_t a, int64_t T0, int W) {
+    // FP32 path: mul-add pairs may fuse (it has no host build to match)
+    #pragma clang fp contract(fast)
     const uint32_t NA = n.n_astro;
     const size_t LA = n.lda;  // row stride of the [rows][astrocytes] arrays
     if (a >= NA) {

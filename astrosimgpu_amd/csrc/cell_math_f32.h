@@ -83,6 +83,8 @@ struct AdexG {
 };
 __device__ inline AdexG adex_g(const NeuronK& p, const NeuronKf& pf, double g_ex, double g_in,
                                double I_ext) {
+    // FP32 path: mul-add pairs may fuse (it has no host build to match)
+    #pragma clang fp contract(fast)
     return AdexG{pf.g_L + g_ex + g_in,
                  fma(g_in, p.E_in, fma(g_ex, p.E_ex, pf.gL_EL_Ie + I_ext))};
 }
@@ -92,6 +94,8 @@ __device__ inline AdexG adex_g(const NeuronK& p, const NeuronKf& pf, double g_ex
 // dw = (a V - a E_L - w) / tau_w, with V clamped at V_peak.
 __device__ inline void adex_rhs_folded(const NeuronK& p, const NeuronKf& pf, double V, double w,
                                        const AdexG& g, double& dV, double& dw) {
+    // FP32 path: mul-add pairs may fuse (it has no host build to match)
+    #pragma clang fp contract(fast)
     const double Vc = (p.V_peak < V) ? p.V_peak : V;
     const double I_spike = p.gL_DeltaT * spike_exp2(fma(Vc, pf.exp_a, pf.exp_b));
     dV = (fma(-Vc, g.gsum, g.gc - w) + I_spike) * pf.inv_C_m;
@@ -111,6 +115,8 @@ __device__ inline bool adex_substep_reg(const NeuronK& pd, const NeuronKf& pf, d
                                         double b, double I_ext, double hs, double half_h,
                                         double h_sixth, double& V, double& w, double& g_ex,
                                         double& dg_ex, double& g_in, double& dg_in, int& refr) {
+    // FP32 path: mul-add pairs may fuse (it has no host build to match)
+    #pragma clang fp contract(fast)
     const double ge_next = pd.ex_decay * (g_ex + hs * dg_ex);
     const double gi_next = pd.in_decay * (g_in + hs * dg_in);
     if (refr > 0) {
@@ -182,6 +188,8 @@ __device__ __forceinline__ float rcp_div(float a, float b) { return a * __builti
 __device__ inline void astro_rhs_f32(const AstroKf& c, float Ca_tot, float IP3_0, float inv_tau,
                                      float Ca, float IP3, float h, float noise, float& dCa,
                                      float& dIP3, float& dh) {
+    // FP32 path: mul-add pairs may fuse (it has no host build to match)
+    #pragma clang fp contract(fast)
     const float Ca_ER = (Ca_tot - Ca) * c.inv_ratio_ER_cyt;
     const float m_inf = rcp_div(IP3, IP3 + c.Kd_IP3_1);
     const float n_inf = rcp_div(Ca, Ca + c.Kd_act);
@@ -202,6 +210,8 @@ __device__ inline void astro_step_f32(const AstroKf& c, float Ca_tot, float IP3_
                                       float delta_IP3, float ip3_input, float noise, float hs,
                                       float half_h, float h_sixth, int substeps, float& Ca,
                                       float& IP3, float& h) {
+    // FP32 path: mul-add pairs may fuse (it has no host build to match)
+    #pragma clang fp contract(fast)
     if (ip3_input != 0.0f) {
         IP3 += delta_IP3 * ip3_input;
     }
