@@ -96,7 +96,7 @@ __device__ inline void adex_rhs_folded(const NeuronK& p, const NeuronKf& pf, dou
                                        const AdexG& g, double& dV, double& dw) {
     // FP32 path: mul-add pairs may fuse (it has no host build to match)
     #pragma clang fp contract(fast)
-    const double Vc = (p.V_peak < V) ? p.V_peak : V;
+    const double Vc = fmin(V, p.V_peak);  // one v_min_f64 (V is never NaN)
     const double I_spike = p.gL_DeltaT * spike_exp2(fma(Vc, pf.exp_a, pf.exp_b));
     dV = (fma(-Vc, g.gsum, g.gc - w) + I_spike) * pf.inv_C_m;
     dw = (fma(p.a, Vc, -pf.a_EL) - w) * pf.inv_tau_w;
@@ -121,12 +121,14 @@ __device__ inline bool adex_substep_reg(const NeuronK& pd, const NeuronKf& pf, d
     const double gi_next = pd.in_decay * (g_in + hs * dg_in);
     if (refr > 0) {
         V = V_reset;
-        const double Vc = (pd.V_peak < V) ? pd.V_peak : V;
+        const double Vc = fmin(V, pd.V_peak);
         w += hs * ((fma(pd.a, Vc, -pf.a_EL) - w) * pf.inv_tau_w);
     } else {
         const AdexG g1 = adex_g(pd, pf, g_ex, g_in, I_ext);
-        const AdexG g2 = adex_g(pd, pf, 0.5 * (g_ex + ge_next), 0.5 * (g_in + gi_next), I_ext);
         const AdexG g4 = adex_g(pd, pf, ge_next, gi_next, I_ext);
+        // the midpoint conductances' terms: gsum and gc are affine in
+        // (g_ex, g_in), so they are the means of the end points'
+        const AdexG g2{0.5 * (g1.gsum + g4.gsum), 0.5 * (g1.gc + g4.gc)};
         double k1V, k1w, k2V, k2w, k3V, k3w, k4V, k4w;
         adex_rhs_folded(pd, pf, V, w, g1, k1V, k1w);
         adex_rhs_folded(pd, pf, V + half_h * k1V, w + half_h * k1w, g2, k2V, k2w);
