@@ -183,27 +183,31 @@ __device__ __forceinline__ float rng_normal_clamped_f32(std::uint64_t key) {
     return box_muller_f32(rng_uniform_keyed(key, 0), rng_uniform_keyed(key, 1));
 }
 
-// a / b through the hardware reciprocal (v_rcp_f32, 1 ulp): the quotients of
-// the FP32 astrocyte RHS do not need the correctly rounded divide.
-__device__ __forceinline__ float rcp_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
 __device__ inline void astro_rhs_f32(const AstroKf& c, float Ca_tot, float IP3_0, float inv_tau,
                                      float Ca, float IP3, float h, float noise, float& dCa,
                                      float& dIP3, float& dh) {
     // FP32 path: mul-add pairs may fuse (it has no host build to match)
     #pragma clang fp contract(fast)
-    const float Ca_ER = (Ca_tot - Ca) * c.inv_ratio_ER_cyt;
-    const float m_inf = rcp_div(IP3, IP3 + c.Kd_IP3_1);
-    const float n_inf = rcp_div(Ca, Ca + c.Kd_act);
-    const float m3 = m_inf * m_inf * m_inf;
-    const float n3 = n_inf * n_inf * n_inf;
-    const float h3 = h * h * h;
-    const float J_channel = c.ratio_rate_IP3R * m3 * n3 * h3 * (Ca_ER - Ca);
-    const float J_pump = rcp_div(c.rate_SERCA * Ca * Ca, c.Km_SERCA_sq + Ca * Ca);
-    const float J_leak = c.ratio_rate_L * (Ca_ER - Ca);
-    const float alpha_h = rcp_div(c.k_Kd_inh * (IP3 + c.Kd_IP3_1), IP3 + c.Kd_IP3_2);
+    // The Li-Rinzel terms of astro_derivatives, restated for fewer
+    // instructions: m_inf^3 n_inf^3 h^3 = (m_inf n_inf h)^3; the four
+    // quotients take two hardware reciprocals, one per pair of denominators
+    // (d1 d2 and d3 d4, all positive); J_channel + J_leak share the factor
+    // (Ca_ER - Ca).
+    const float d1 = IP3 + c.Kd_IP3_1;
+    const float d2 = Ca + c.Kd_act;
+    const float Ca2 = Ca * Ca;
+    const float d3 = c.Km_SERCA_sq + Ca2;
+    const float d4 = IP3 + c.Kd_IP3_2;
+    const float r12 = __builtin_amdgcn_rcpf(d1 * d2);
+    const float r34 = __builtin_amdgcn_rcpf(d3 * d4);
+    const float mnh = IP3 * Ca * r12 * h;  // m_inf n_inf h
+    const float gap = (Ca_tot - Ca) * c.inv_ratio_ER_cyt - Ca;  // Ca_ER - Ca
+    const float J_ch_leak = (c.ratio_rate_IP3R * (mnh * mnh * mnh) + c.ratio_rate_L) * gap;
+    const float J_pump = c.rate_SERCA * Ca2 * (r34 * d4);
+    const float alpha_h = c.k_Kd_inh * d1 * (r34 * d3);
     const float beta_h = c.k_IP3R * Ca;
-    dCa = J_channel - J_pump + J_leak + noise;
+    dCa = J_ch_leak - J_pump + noise;
     dIP3 = (IP3_0 - IP3) * inv_tau;
     dh = alpha_h * (1.0f - h) - beta_h * h;
 }
