@@ -164,3 +164,37 @@ def test_fp32_network_alone_equals_batch_with_shorter_windows():
         assert np.array_equal(batch.state(sa, f).view(np.uint64),
                               alone_state[f].view(np.uint64)), f
     batch.close()
+
+
+def test_fp32_delivery_list_longer_than_its_wavefronts():
+    """Fourteen FP32 Sparse networks in one batch: in some windows more
+    64-neuron blocks spike than there are delivery wavefronts (512), so a
+    wavefront delivers several listed blocks. Each network's raster and
+    state equal its solo run, where every listed block has a wavefront of
+    its own."""
+    _announce("fp32 delivery list beyond 512 blocks")
+    steps, window = 2000, 10  # Sparse: shortest delay 1 ms = 10 steps
+    models = []
+    for seed in range(1, 15):
+        m = A.Model(config_path("paper_sparse"), seed=seed)
+        m.desc.contents.precision = capi.ASG_FP32
+        models.append(m)
+    batch = A.Batch(len(models))
+    slots = [batch.add(m, steps) for m in models]
+    batch.step(steps)
+    drained = batch.spikes_all()
+    together = [(drained[s], batch.state(s, "V"), batch.state(s, "Ca")) for s in slots]
+    batch.close()
+    listed = np.zeros(steps // window, np.int64)
+    for (st, src), _, _ in together:
+        blocks = np.unique(np.stack([st // window, src // 64], axis=1), axis=0)
+        listed += np.bincount(blocks[:, 0], minlength=steps // window)
+    assert listed.max() > 512, listed.max()
+    for m, ((st, src), v, ca) in zip(models, together):
+        net = A.DeviceNetwork(m)
+        net.step(steps)
+        s1, n1 = net.spikes()
+        assert np.array_equal(st, s1) and np.array_equal(src, n1)
+        assert np.array_equal(v.view(np.uint64), net.state("V").view(np.uint64))
+        assert np.array_equal(ca.view(np.uint64), net.state("Ca").view(np.uint64))
+        net.close()
