@@ -318,7 +318,7 @@ __device__ inline void add_i64(int64_t* p, int64_t v) {
 
 // --------------------------------------------------------------- kernels ---
 
-__device__ __forceinline__ void neuron_window(const DevNet& n, uint32_t i, int64_t T0, int W) {
+__device__ __forceinline__ uint64_t neuron_window(const DevNet& n, uint32_t i, int64_t T0, int W) {
     // Everything the loop reads is copied into registers once; the rings are
     // reached through restrict pointers so their stores cannot force reloads.
     const uint32_t N = n.n_neur;
@@ -369,6 +369,7 @@ __device__ __forceinline__ void neuron_window(const DevNet& n, uint32_t i, int64
         if (has_noise) noise_n = drive_noise[i];
     }
     const uint32_t lane = threadIdx.x & 63;
+    uint64_t spiked_any = 0;  // OR of the window's spike words (uniform)
     for (int k = 0; k < W; ++k) {
         bool spiked = false;
         const int row_n = ring_next(row, S);
@@ -438,6 +439,7 @@ __device__ __forceinline__ void neuron_window(const DevNet& n, uint32_t i, int64
         if (lane == 0 && (i & ~63u) < N) {
             spk_bits[static_cast<size_t>(k) * words + (i >> 6)] = mask;
         }
+        spiked_any |= mask;
         row = row_n;
     }
     if (valid) {
@@ -450,6 +452,7 @@ __device__ __forceinline__ void neuron_window(const DevNet& n, uint32_t i, int64
         n.Isic[i] = Isic;
         n.refr[i] = refr;
     }
+    return spiked_any;
 }
 
 __device__ __forceinline__ void astro_window(const DevNet& n, uint32_t a, int64_t T0, int W) {
@@ -903,8 +906,8 @@ __device__ inline void close_open_window(DevNet& n) {
 __global__ __launch_bounds__(64) void k_drive(DevNet* nets, const CellBlock* nblocks,
                                               int32_t n_items, int32_t Wg, int32_t* work_n) {
     extern __shared__ double drive_lds[];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        *work_n = 0;  // the window's FP32 delivery list starts empty
+    if (blockIdx.x == 0 && threadIdx.x < 2) {
+        work_n[threadIdx.x] = 0;  // the window's delivery lists start empty
     }
     double* q_prod = drive_lds;                                             // [64 * Wg]
     uint32_t* q_key = reinterpret_cast<uint32_t*>(drive_lds + 64 * Wg);     // [64 * Wg]
@@ -1072,12 +1075,14 @@ __global__ __launch_bounds__(64) void k_drive(DevNet* nets, const CellBlock* nbl
 // astrocytes share the launch, so the short astrocyte blocks overlap the
 // neuron blocks.
 // WPB wavefronts (64-cell blocks) per workgroup, a launch-schedule knob.
-// An FP32 neuron block that spiked in the window appends itself to the
-// delivery list `work` (count in *work_n, zeroed by the window's k_drive).
+// A neuron block that spiked in the window appends itself to its
+// precision's delivery list `work` (P = 0 from entry 0, P = 1 from entry
+// fp32_base), counted in work_n[P] (zeroed by the window's k_drive).
 template <int P, int WPB>
 __global__ __launch_bounds__(kCellBlock * WPB) void k_cells(DevNet* nets, const CellBlock* blocks,
                                                             int32_t count, int32_t Wg,
-                                                            CellBlock* work, int32_t* work_n) {
+                                                            CellBlock* work, int32_t* work_n,
+                                                            int32_t fp32_base) {
     // The wavefront's block index is uniform across its lanes: said so
     // explicitly, so the block and network descriptors stay in scalar
     // registers.
@@ -1103,14 +1108,16 @@ __global__ __launch_bounds__(kCellBlock * WPB) void k_cells(DevNet* nets, const 
     if constexpr (P == ASG_FP32) {
         if (cb.kind == 0) {
             if (neuron_window_f32(n, i, T0, W) != 0 && lane == 0) {
-                work[atomicAdd(work_n, 1)] = cb;
+                work[fp32_base + atomicAdd(work_n + 1, 1)] = cb;
             }
         } else {
             astro_window_f32(n, i, T0, W);
         }
     } else {
         if (cb.kind == 0) {
-            neuron_window(n, i, T0, W);
+            if (neuron_window(n, i, T0, W) != 0 && lane == 0) {
+                work[atomicAdd(work_n, 1)] = cb;
+            }
         } else {
             astro_window(n, i, T0, W);
         }
@@ -1420,10 +1427,11 @@ __device__ void deliver_sources(DevNet& n, uint32_t first, uint32_t lane) {
     if (syn > 0) {
         add_i64(&n.c_syn, syn);
     }
-    if (!room && window_total > 0 && t == 0) {
+    // every delivering wavefront of the network agrees on these
+    if (!room && window_total > 0 && lane == 0) {
         atomicOr(&n.overflow, 1);
     }
-    if (window_total > 64 * n.cw && t == 0) {
+    if (window_total > 64 * n.cw && lane == 0) {
         atomicOr(&n.overflow, 2);  // spikes beyond the delivery masks' capacity
     }
 }
@@ -1605,72 +1613,64 @@ __device__ void deliver_fixed(DevNet& n, uint32_t first, uint32_t lane) {
     }
 }
 
-// Window pass 3. Per neuron block, ceil(Wg / MG) + ceil(Wg / G) wavefronts:
-// one per group of MG window steps for the marking of those steps' spikes
-// (the first also does the sources' plasticity and raster entries), then one
-// per group of G steps for the SIC sums of the block's neurons. G trades
-// re-reading the astrocyte lists (G = Wg reads them once per window) against
-// parallelism for small networks (G = 1); MG = Wg saves the dispatch of
-// wavefronts that mostly find no spike in a large batch. One instantiation
-// per precision, so the FP32 networks' launch does not carry the FP64 SIC
-// accumulators' registers. FP32 networks need no marking: their launch
-// holds MG delivery wavefronts (deliver_fixed), which share out the blocks
-// the cell pass listed as having spiked, and then the SIC wavefronts.
+// Window pass 3, one launch per precision (so the FP32 networks' launch
+// does not carry the FP64 SIC accumulators' registers): D wavefronts deliver
+// the blocks the cell pass listed as having spiked (FP64: marking in groups
+// of SG steps, the first group also doing the sources' plasticity and
+// raster entries; FP32: deliver_fixed), then ceil(Wg / G) wavefronts per
+// block sum the SIC of its neurons. G trades re-reading the astrocyte lists
+// (G = Wg reads them once per window) against parallelism for small
+// networks (G = 1).
 template <int G, int P>
 __global__ __launch_bounds__(256) void k_deliver(DevNet* nets, const CellBlock* nblocks,
-                                                 int32_t n_items, int32_t Wg, int32_t MG,
-                                                 const CellBlock* work, const int32_t* work_n) {
+                                                 int32_t n_items, int32_t Wg, int32_t D,
+                                                 int32_t SG, const CellBlock* work,
+                                                 const int32_t* work_n) {
     const int e = __builtin_amdgcn_readfirstlane(static_cast<int>(blockIdx.x) * 4 +
                                                  static_cast<int>(threadIdx.x >> 6));
     if (e >= n_items) {
         return;
     }
     const uint32_t lane = threadIdx.x & 63;
-    if constexpr (P == ASG_FP32) {
-        // The first MG items are delivery wavefronts, each taking every
-        // MG-th block of the list of blocks that spiked in the window (first,
-        // so that their chains of dependent loads start with the launch); the
-        // others are the SIC wavefronts, ceil(Wg / G) per block.
-        if (e < MG) {
-            const int listed = *work_n;
-            for (int j = e; j < listed; j += MG) {
-                const CellBlock cb = work[j];
-                deliver_fixed(nets[cb.net], cb.first, lane);
+    // The first D items are delivery wavefronts that share out the list of
+    // blocks that spiked in the window (first, so that their chains of
+    // dependent loads start with the launch): an FP32 block is one item
+    // (deliver_fixed); an FP64 block is ceil(Wg / SG) items, one per group
+    // of SG steps to mark, the first also delivering the block's sources.
+    if (e < D) {
+        const int groups = P == ASG_FP32 ? 1 : (Wg + SG - 1) / SG;
+        const int items = *work_n * groups;
+        for (int j = e; j < items; j += D) {
+            const CellBlock cb = work[j / groups];
+            DevNet& n = nets[cb.net];
+            if constexpr (P == ASG_FP32) {
+                deliver_fixed(n, cb.first, lane);
+            } else {
+                const int k0 = (j % groups) * SG;
+                const int k1 = min(k0 + SG, n.win_W);
+                if (k1 > k0) {
+                    mark_steps(n, cb.first, lane, k0, k1);
+                }
+                if (k0 == 0) {
+                    deliver_sources(n, cb.first, lane);
+                }
             }
-            return;
-        }
-        const int per_block = (Wg + G - 1) / G;
-        const CellBlock cb = nblocks[(e - MG) / per_block];
-        DevNet& n = nets[cb.net];
-        const int g0 = ((e - MG) % per_block) * G;
-        const int len = min(G, n.win_W - g0);
-        const uint32_t t = cb.first + lane;
-        if (len > 0 && t < n.n_neur) {
-            deliver_sic_f32<G>(n, t, g0, len);
         }
         return;
     }
-    const int mark_waves = (Wg + MG - 1) / MG;
-    const int per_block = mark_waves + (Wg + G - 1) / G;
-    const CellBlock cb = nblocks[e / per_block];
-    const int k = e % per_block;
+    // the others: the SIC wavefronts, ceil(Wg / G) per block
+    const int per_block = (Wg + G - 1) / G;
+    const CellBlock cb = nblocks[(e - D) / per_block];
     DevNet& n = nets[cb.net];
-    if (k >= mark_waves) {
-        const int g0 = (k - mark_waves) * G;
-        const int len = min(G, n.win_W - g0);
-        const uint32_t t = cb.first + lane;
-        if (len > 0 && t < n.n_neur) {
+    const int g0 = ((e - D) % per_block) * G;
+    const int len = min(G, n.win_W - g0);
+    const uint32_t t = cb.first + lane;
+    if (len > 0 && t < n.n_neur) {
+        if constexpr (P == ASG_FP32) {
+            deliver_sic_f32<G>(n, t, g0, len);
+        } else {
             deliver_sic<G>(n, t, g0, len);
         }
-        return;
-    }
-    const int k0 = k * MG;
-    const int k1 = min(k0 + MG, n.win_W);
-    if (k1 > k0) {
-        mark_steps(n, cb.first, lane, k0, k1);
-    }
-    if (k0 == 0 && n.win_W > 0) {
-        deliver_sources(n, cb.first, lane);
     }
 }
 
@@ -2417,8 +2417,10 @@ struct Engine {
     CellBlock* d_blocks = nullptr;
     CellBlock* d_nblocks = nullptr;   // neuron blocks only
     int32_t* d_active = nullptr;
-    // The window's FP32 delivery list: neuron blocks that spiked, appended by
-    // k_cells, counted in *d_work_n (zeroed by k_drive).
+    // The window's delivery lists: neuron blocks that spiked, appended by
+    // k_cells, the FP64 networks' from entry 0 and the FP32 networks' from
+    // entry nblock_range[1], counted in d_work_n[0] and d_work_n[1] (zeroed
+    // by k_drive).
     CellBlock* d_work = nullptr;
     int32_t* d_work_n = nullptr;
     size_t blocks_cap = 0, nblocks_cap = 0, active_cap = 0, work_cap = 0;
@@ -2622,8 +2624,8 @@ struct Engine {
             // each FP32 neuron block is listed at most once per window
             grow(d_work, work_cap, nblocks.size());
             if (d_work_n == nullptr) {
-                HIP_CHECK(hipMalloc(&d_work_n, sizeof(int32_t)));
-                HIP_CHECK(hipMemsetAsync(d_work_n, 0, sizeof(int32_t), stream));
+                HIP_CHECK(hipMalloc(&d_work_n, 2 * sizeof(int32_t)));
+                HIP_CHECK(hipMemsetAsync(d_work_n, 0, 2 * sizeof(int32_t), stream));
             }
         }
         if (n_active > 0) {
@@ -2674,25 +2676,24 @@ struct Engine {
     void launch_deliver_range(int range, int32_t Wg_, int MG) {
         const int count = nblock_range[range + 1] - nblock_range[range];
         if (count == 0) return;
-        int items;
-        if constexpr (P == ASG_FP32) {
-            MG = std::min(count, kDeliverWaves);  // delivery wavefronts before the SIC ones
-            items = count * ((Wg_ + G - 1) / G) + MG;
-        } else {
-            items = count * ((Wg_ + MG - 1) / MG + (Wg_ + G - 1) / G);
-        }
+        // delivery wavefronts (before the SIC ones) over the listed blocks
+        const int groups = P == ASG_FP32 ? 1 : (Wg_ + MG - 1) / MG;
+        const int D = std::min(count * groups, kDeliverWaves);
+        const int items = count * ((Wg_ + G - 1) / G) + D;
         hipLaunchKernelGGL((k_deliver<G, P>), dim3((items + 3) / 4), dim3(256), 0, stream, d_nets,
-                           d_nblocks + nblock_range[range], items, Wg_, MG, d_work, d_work_n);
+                           d_nblocks + nblock_range[range], items, Wg_, D, MG,
+                           d_work + nblock_range[range], d_work_n + range);
     }
 
     template <int G>
     void launch_deliver_g(int32_t Wg_, int MG) {
         launch_deliver_range<G, ASG_FP64>(0, Wg_, MG);
-        launch_deliver_range<G, ASG_FP32>(1, Wg_, 0);
+        launch_deliver_range<G, ASG_FP32>(1, Wg_, 1);
     }
 
-    // Pass 3 over every neuron block (FP64 networks' marking in groups of MG
-    // steps; FP32 networks' delivery one wavefront per block).
+    // Pass 3: delivery of the blocks that spiked (FP64 networks' marking in
+    // groups of MG steps; FP32 networks' fixed-point delivery) and the SIC
+    // sums of every block.
     void launch_deliver(int G, int32_t Wg_, int MG) {
         switch (G) {
             case 1: launch_deliver_g<1>(Wg_, MG); break;
@@ -2719,15 +2720,15 @@ struct Engine {
         switch (wpb) {
             case 4:
                 hipLaunchKernelGGL((k_cells<P, 4>), grid, dim3(4 * kCellBlock), 0, stream, d_nets,
-                                   blocks, count, Wg_, d_work, d_work_n);
+                                   blocks, count, Wg_, d_work, d_work_n, nblock_range[1]);
                 break;
             case 2:
                 hipLaunchKernelGGL((k_cells<P, 2>), grid, dim3(2 * kCellBlock), 0, stream, d_nets,
-                                   blocks, count, Wg_, d_work, d_work_n);
+                                   blocks, count, Wg_, d_work, d_work_n, nblock_range[1]);
                 break;
             default:
                 hipLaunchKernelGGL((k_cells<P, 1>), grid, dim3(kCellBlock), 0, stream, d_nets,
-                                   blocks, count, Wg_, d_work, d_work_n);
+                                   blocks, count, Wg_, d_work, d_work_n, nblock_range[1]);
                 break;
         }
     }
