@@ -350,6 +350,11 @@ __device__ __forceinline__ uint64_t neuron_window(const DevNet& n, uint32_t i, i
     int ev_n = 0;                           // drive of the next step
     double noise_n = 0.0;
     const bool has_poiss = in.poiss_rate > 0.0, has_noise = in.noise_std > 0.0;
+    // The drive holds one noise value per noise interval of the window:
+    // value nz_j from window step nz_next - steps_per_noise on.
+    const int64_t nz_period = has_noise ? in.steps_per_noise : 1;
+    int64_t nz_next = first_hit(T0 + 1, 0, nz_period) + 1;
+    uint32_t nz_j = 0;
     if (valid) {
         V = n.V[i];
         w = n.w[i];
@@ -366,7 +371,7 @@ __device__ __forceinline__ uint64_t neuron_window(const DevNet& n, uint32_t i, i
         ih_n = ring_inh[s0];
         if (sic_live_n) sic_n = ring_sic[s0];
         if (has_poiss) ev_n = drive_n[i];
-        if (has_noise) noise_n = drive_noise[i];
+        if (has_noise) noise_n = drive_noise[i];  // the window's first noise interval
     }
     const uint32_t lane = threadIdx.x & 63;
     uint64_t spiked_any = 0;  // OR of the window's spike words (uniform)
@@ -390,7 +395,12 @@ __device__ __forceinline__ uint64_t neuron_window(const DevNet& n, uint32_t i, i
                 sic_n = next_live ? ring_sic[nx] : 0.0;
                 const size_t dk1 = static_cast<size_t>(k + 1) * LN + i;
                 if (has_poiss) ev_n = drive_n[dk1];
-                if (has_noise) noise_n = drive_noise[dk1];
+                if (has_noise && k + 1 == nz_next) {
+                    // a new noise interval: its value is the next one stored
+                    nz_next += nz_period;
+                    ++nz_j;
+                    noise_n = drive_noise[static_cast<size_t>(nz_j) * LN + i];
+                }
             }
             double exc_in = 0.0, inh_in = 0.0;
             // apply_arrivals, network.cpp:301-327
@@ -578,6 +588,11 @@ __device__ __forceinline__ uint64_t neuron_window_f32(const DevNet& n, uint32_t 
     int ev_n = 0;
     float noise_n = 0.0f;
     const bool has_poiss = in.poiss_rate > 0.0, has_noise = in.noise_std > 0.0;
+    // The drive holds one noise value per noise interval of the window:
+    // value nz_j from window step nz_next - steps_per_noise on.
+    const int64_t nz_period = has_noise ? in.steps_per_noise : 1;
+    int64_t nz_next = first_hit(T0 + 1, 0, nz_period) + 1;
+    uint32_t nz_j = 0;
     if (valid) {
         V = n.fV[i];
         w = n.fw[i];
@@ -594,7 +609,7 @@ __device__ __forceinline__ uint64_t neuron_window_f32(const DevNet& n, uint32_t 
         ih_n = ring_inh[s0];
         if (sic_live_n) sic_n = ring_sic[s0];
         if (has_poiss) ev_n = drive_n[i];
-        if (has_noise) noise_n = drive_noise[i];
+        if (has_noise) noise_n = drive_noise[i];  // the window's first noise interval
     }
     const uint32_t lane = threadIdx.x & 63;
     uint64_t spiked_any = 0;  // OR of the window's spike words (uniform)
@@ -619,7 +634,12 @@ __device__ __forceinline__ uint64_t neuron_window_f32(const DevNet& n, uint32_t 
                 sic_n = next_live ? ring_sic[nx] : 0.0f;
                 const size_t dk1 = static_cast<size_t>(k + 1) * LN + i;
                 if (has_poiss) ev_n = drive_n[dk1];
-                if (has_noise) noise_n = drive_noise[dk1];
+                if (has_noise && k + 1 == nz_next) {
+                    // a new noise interval: its value is the next one stored
+                    nz_next += nz_period;
+                    ++nz_j;
+                    noise_n = drive_noise[static_cast<size_t>(nz_j) * LN + i];
+                }
             }
             double exc_in = 0.0, inh_in = 0.0;
             if (ex != 0) {
@@ -1031,7 +1051,8 @@ __global__ __launch_bounds__(64) void k_drive(DevNet* nets, const CellBlock* nbl
         }
     }
     if (valid && in.noise_std > 0.0) {
-        // One draw per noise interval (neuron.cpp:152-166), held across it.
+        // One draw per noise interval (neuron.cpp:152-166), held across it:
+        // value j of the window's noise drive holds its j-th interval.
         const uint64_t salt_shared = pi == 0 ? kSaltNoiseSharedExc : kSaltNoiseSharedInh;
         const uint64_t nseed = in.independent ? seed ^ kSaltNeuronNoise : seed ^ salt_shared;
         const int64_t spn = in.steps_per_noise;
@@ -1045,26 +1066,18 @@ __global__ __launch_bounds__(64) void k_drive(DevNet* nets, const CellBlock* nbl
         if (n.precision == ASG_FP32) {
             // FP32 path: the same uniforms through the FP32 Box-Muller.
             const float sd = static_cast<float>(in.noise_std);
-            float z = rng_normal_fresh_f32(key);
-            for (int k = 0; k < W; ++k) {
-                if (k == next) {
-                    next += spn;
-                    key += idx_key;
-                    z = rng_normal_fresh_f32(key);
-                }
-                n.fdrive_noise[static_cast<size_t>(k) * LN + i] = sd * z;
+            n.fdrive_noise[i] = sd * rng_normal_fresh_f32(key);
+            for (size_t j = 1; next < W; next += spn, ++j) {
+                key += idx_key;
+                n.fdrive_noise[j * LN + i] = sd * rng_normal_fresh_f32(key);
             }
         } else {
             uint64_t ctr = 0;
-            double z = rng_normal_fresh_keyed(key, ctr);
-            for (int k = 0; k < W; ++k) {
-                if (k == next) {
-                    next += spn;
-                    key += idx_key;
-                    ctr = 0;
-                    z = rng_normal_fresh_keyed(key, ctr);
-                }
-                n.drive_noise[static_cast<size_t>(k) * LN + i] = 0.0 + in.noise_std * z;
+            n.drive_noise[i] = 0.0 + in.noise_std * rng_normal_fresh_keyed(key, ctr);
+            for (size_t j = 1; next < W; next += spn, ++j) {
+                key += idx_key;
+                ctr = 0;
+                n.drive_noise[j * LN + i] = 0.0 + in.noise_std * rng_normal_fresh_keyed(key, ctr);
             }
         }
     }

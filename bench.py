@@ -91,18 +91,20 @@ def log(*a):
 #     and cleared (2 s) and the SIC factor written (s).
 # s = 8 bytes in FP64, 4 in FP32.
 def cell_bytes(n_neurons: int, n_astro: int, window: int, state_bytes: int,
-               ring_bytes: int, noise: bool = True) -> float:
+               ring_bytes: int, noise_values: int = 1) -> float:
     """Bytes one cell-kernel launch must move for a window: per neuron the 7
     state values read and written (14 s), the refractory counter (8) and
-    V_reset, b (16), and per step the exc and inh ring slots (2 r) and the
-    drive (4, + s with noise); per astrocyte 3 state values read and written
-    and 4 parameters (10 s), and per step the n2a ring slot (r) and the SIC
-    factor (s). s is the state value size, r the ring slot size (8 for the
-    FP32 path's fixed-point rings). Reads of SIC slots of live exchanges and
-    clears of nonzero ring slots depend on the activity and are not counted,
-    so the bandwidth derived from these bytes is a lower bound."""
+    V_reset, b (16), per step the exc and inh ring slots (2 r) and the
+    Poisson count (4), and per noise interval of the window its noise value
+    (s; noise_values intervals, one when the noise interval is the window);
+    per astrocyte 3 state values read and written and 4 parameters (10 s),
+    and per step the n2a ring slot (r) and the SIC factor (s). s is the state
+    value size, r the ring slot size (8 for the FP32 path's fixed-point
+    rings). Reads of SIC slots of live exchanges and clears of nonzero ring
+    slots depend on the activity and are not counted, so the bandwidth
+    derived from these bytes is a lower bound."""
     s, r = state_bytes, ring_bytes
-    neuron = 14 * s + 8 + 16 + window * (2 * r + 4 + (s if noise else 0))
+    neuron = 14 * s + 8 + 16 + window * (2 * r + 4) + noise_values * s
     astro = 10 * s + window * (r + s)
     return n_neurons * neuron + n_astro * astro
 
@@ -394,7 +396,8 @@ def main():
     window_len = 10  # min(d_e, d_i, d_a2n) of use_case.json = 1 ms = 10 steps
     if dominant == "cells":
         # use_case.json drives neurons with Poisson input and noise
-        bytes_launch = cell_bytes(n_total, na_total, window_len, 4, 8, noise=True)
+        # noise_dt of use_case.json = 1 ms = the window: one noise value each
+        bytes_launch = cell_bytes(n_total, na_total, window_len, 4, 8, noise_values=1)
         avg_s = kt32["cells"] / 1000.0 / max(windows, 1)
         achieved = bytes_launch / avg_s / 1e9
         roof = {"kernel": "k_cells (every neuron and astrocyte of the batch, one window of "
