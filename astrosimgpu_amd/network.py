@@ -248,6 +248,13 @@ class Batch:
         """Drain every network's raster in one call (asg_batch_spikes_all):
         {slot: (steps int64, sources uint32)} for every occupied slot, each
         slot's pairs in the order asg_batch_spikes would return them."""
+        return self.split_spikes(self.spikes_all_raw(capacity))
+
+    def spikes_all_raw(self, capacity: int = 1 << 20) -> list:
+        """The drain of spikes_all without the split per slot: a list of
+        (starts, ends, steps, sources) rounds, each slot's pairs at
+        [starts[slot], ends[slot]) of its round. split_spikes turns it into
+        spikes_all's dict, e.g. while the next steps run."""
         if self._drain is None or len(self._drain[1]) != capacity:
             self._drain = (np.empty(self.max_nets, np.int64), np.empty(capacity, np.int64),
                            np.empty(capacity, np.uint32))
@@ -265,6 +272,9 @@ class Batch:
                            src[:n.value].copy()))
             if n.value < capacity:
                 break
+        return rounds
+
+    def split_spikes(self, rounds: list) -> dict:
         out = {}
         for slot in self.models:
             parts_s, parts_n = [], []
