@@ -13,6 +13,7 @@ from conftest import config_path, golden, golden_exists
 
 import astrosimgpu_amd as A
 import oracle
+from astrosimgpu_amd import _capi as capi
 
 pytestmark = pytest.mark.gpu
 
@@ -160,6 +161,47 @@ def test_batch_members_equal_solo_runs():
         assert np.array_equal(bs, os_) and np.array_equal(bn, on)
         for f in ("V", "Ca", "IP3", "I_sic"):
             assert np.array_equal(batch.state(slot, f), o.state(f)), f
+
+
+def test_mixed_precision_batch_with_partial_networks_equals_solo_runs():
+    """FP64 and FP32 networks in one batch, among them an FP32 network of
+    astrocytes only (no drive, delivery or spike list; its window closes
+    every window) and an FP64 network without astrocytes: every member
+    gives the bits of its solo run (the FP64 members also the oracle's),
+    over calls that end mid-window. Each precision keeps its own delivery
+    list and compaction set."""
+    _announce("mixed batch with partial networks")
+    specs = [("quick", 3, capi.ASG_FP64, {}), ("use_case", 4, capi.ASG_FP32, {}),
+             ("quick", 5, capi.ASG_FP32, {"N.N_E": 0, "N.N_I": 0}),
+             ("quick", 6, capi.ASG_FP64, {"N.N_A": 0}), ("quick", 7, capi.ASG_FP32, {})]
+    models = []
+    for cfg, seed, prec, assign in specs:
+        m = A.Model(config_path(cfg), seed=seed, assignments=assign or None)
+        m.desc.contents.precision = prec
+        models.append(m)
+    total, chunks = 2400, (333, 1000, 1067)
+    batch = A.Batch(8)
+    slots = [batch.add(m, total) for m in models]
+    for c in chunks:
+        batch.step(c)
+    drained = batch.spikes_all()
+    for (cfg, seed, prec, _), m, slot in zip(specs, models, slots):
+        net = A.DeviceNetwork(m)
+        net.step(total)
+        s1, n1 = net.spikes()
+        bs, bn = drained[slot]
+        assert np.array_equal(bs, s1) and np.array_equal(bn, n1), (cfg, seed)
+        fields = (("V",) if m.n_neurons else ()) + (("Ca", "IP3") if m.n_astro else ())
+        for f in fields:
+            assert np.array_equal(batch.state(slot, f).view(np.uint64),
+                                  net.state(f).view(np.uint64)), (cfg, seed, f)
+        net.close()
+        if prec == capi.ASG_FP64:
+            o = oracle.OracleRun(m.desc, oracle.LIBM_AMATH)
+            o.step(total)
+            os_, on = o.spikes()
+            assert np.array_equal(bs, os_) and np.array_equal(bn, on), (cfg, seed)
+    batch.close()
 
 
 # Delays that are multiples neither of each other nor of the window (exc
