@@ -189,3 +189,29 @@ def test_cli_usage_and_errors_without_a_device(tmp_path):
     bad.write_text('{"N": {"N_E": 10, "N_I": 2, "N_A": 3}, "simulation": {"nope": 1}}')
     r = subprocess.run([cli, "-c", str(bad)], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "error:" in r.stderr
+
+
+def test_split_spikes_reassembles_each_slot_across_drain_rounds():
+    """Batch.split_spikes (host side of asg_batch_spikes_all): pairs written
+    back to back in slot order, over rounds of a small capacity that cut a
+    slot in the middle, come back per occupied slot in order; empty slots
+    and slots with no pairs give empty arrays."""
+    rng = np.random.default_rng(3)
+    occupied = [0, 2, 3, 6]
+    want = {s: (np.sort(rng.integers(0, 1000, n)).astype(np.int64),
+                rng.integers(0, 500, n).astype(np.uint32))
+            for s, n in zip(occupied, (5, 0, 9, 4))}
+    flat_s = np.concatenate([want[s][0] for s in occupied])
+    flat_n = np.concatenate([want[s][1] for s in occupied])
+    owner = np.concatenate([np.full(len(want[s][0]), s) for s in occupied])
+    cap, rounds = 7, []
+    for at in range(0, len(flat_s) + 1, cap):  # the C side's rounds of `cap` pairs
+        piece = slice(at, min(at + cap, len(flat_s)))
+        counts = np.bincount(owner[piece], minlength=8).astype(np.int64)
+        ends = np.cumsum(counts)
+        rounds.append(((ends - counts).tolist(), ends.tolist(), flat_s[piece], flat_n[piece]))
+    fake = type("FakeBatch", (), {"models": {s: None for s in occupied}})()
+    got = A.Batch.split_spikes(fake, rounds)
+    assert sorted(got) == occupied
+    for s in occupied:
+        assert np.array_equal(got[s][0], want[s][0]) and np.array_equal(got[s][1], want[s][1])
