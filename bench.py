@@ -474,7 +474,7 @@ def main():
         "config": {
             "workload": f"{len(models)}-network sweep (N log-uniform in [200, 5000], 80/20 E/I, "
                         "0.2 N astrocytes, w_n2a U[0.10, 0.40], in-degree 100), batched FP32 "
-                        f"with FP64 islands, {transient_ms / 1000:g} s transient untimed + "
+                        f"path, {transient_ms / 1000:g} s transient untimed + "
                         f"{timed_ms / 1000:g} s timed, 1000 time steps (dt 0.1 ms) per bench step",
             "networks": len(models), "neurons": n_total, "astrocytes": na_total,
             "time_steps_per_bench_step": STEPS_PER_BENCH_STEP,
