@@ -16,12 +16,11 @@
 //    r = z/c - 1, and a degree-11 polynomial path for x within 2^-4 of 1),
 //    again with the FMA variant's fusion pattern. Equal to glibc's log on all
 //    inputs tested.
-//  * asg_sin / asg_cos use fdlibm's kernels (Sun Microsystems, 1993:
-//    k_sin.c, k_cos.c and the medium-range Cody-Waite reduction of
-//    e_rem_pio2.c). They are accurate to < 1 ulp but are NOT glibc's sin/cos
-//    (IBM Accurate Mathematical Library), so results that pass through them
-//    (Box-Muller noise) are bit-identical host-vs-device but only
-//    statistically identical to a glibc build.
+//  * asg_sin / asg_cos follow glibc 2.35's sin/cos (the IBM Accurate
+//    Mathematical Library's table-driven method, s_sin.c), again with the
+//    FMA variant's fusion pattern; the sin/cos table of k/128 is recomputed
+//    from mpmath. Equal to glibc's cos on every Box-Muller angle tested
+//    (5e7 draws of 2 pi u) and to glibc's sin on the inputs tested.
 //
 // Every function here is written with explicit fma() where one is intended
 // and must be compiled with -ffp-contract=off everywhere else.
@@ -192,111 +191,168 @@ ASG_HD double asg_log(double x) {
 }
 
 // ------------------------------------------------------- sin and cos ----
+//
+// glibc 2.35's sin and cos for |x| < 105414350 (sysdeps/ieee754/dbl-64/
+// s_sin.c, the IBM Accurate Mathematical Library method as simplified in
+// glibc 2.28), restated:
+//  * |x| < 2^-27 (cos) or 2^-26 (sin): the first Taylor term;
+//  * |x| < 0.85546875: x = k/128 + r with k = nearest multiple (found by
+//    adding 1.5 * 2^45, whose ulp is 1/128), then
+//    sin/cos(x) from the double-double table of sin/cos(k/128) and short
+//    polynomials in r (sn3, sn5; cs2, cs4, cs6); |x| < 0.126 uses a degree-11
+//    odd Taylor-like polynomial (s1..s5) instead, with a first-order
+//    correction for the argument's low part;
+//  * |x| < 2.426265: the cofunction of pi/2 - |x|, carried as a double-double;
+//  * larger: Cody-Waite reduction by multiples of pi/2 in four parts
+//    (mp1, mp2, pp3, pp4) to a double-double argument and its quadrant.
+// The x86-64 C library selects the variant compiled with -mfma at run time,
+// in which GCC contracts each a*b + c of the C source into one fused
+// multiply-add (each product whose every use is an add or subtract); the
+// fma_() calls below reproduce those contractions. On 5e7 Box-Muller angles
+// 2 pi u, u on the 53-bit draw grid, this equals the host C library's cos
+// bit for bit. |x| >= 105414350 (glibc's Payne-Hanek range) is outside the
+// domain here and returns NaN; the step only evaluates [0, 2 pi).
 
-ASG_HD double kernel_sin(double x, double y, int iy) {
-    constexpr double S1 = -1.66666666666666324348e-01, S2 = 8.33333333332248946124e-03,
-                     S3 = -1.98412698298579493134e-04, S4 = 2.75573137070700676789e-06,
-                     S5 = -2.50507602534068634195e-08, S6 = 1.58969099521155010221e-10;
-    const double z = x * x;
-    const double v = z * x;
-    const double r = S2 + z * (S3 + z * (S4 + z * (S5 + z * S6)));
-    if (iy == 0) {
-        return x + v * (S1 + z * r);
-    }
-    return x - ((z * (0.5 * y - v * r) - y) - v * S1);
+struct SinCosConst {
+    static constexpr double sn3 = -1.66666666666664880952546298448555E-01;
+    static constexpr double sn5 = 8.33333214285722277379541354343671E-03;
+    static constexpr double cs2 = 4.99999999999999999999950396842453E-01;
+    static constexpr double cs4 = -4.16666666666664434524222570944589E-02;
+    static constexpr double cs6 = 1.38888874007937613028114285595617E-03;
+    static constexpr double s1 = -0x1.5555555555555p-3;
+    static constexpr double s2 = 0x1.1111111110ecep-7;
+    static constexpr double s3 = -0x1.a01a019db08b8p-13;
+    static constexpr double s4 = 0x1.71de27b9a7ed9p-19;
+    static constexpr double s5 = -0x1.addffc2fcdf59p-26;
+    static constexpr double big = 0x1.8p45;
+    static constexpr double hp0 = 0x1.921fb54442d18p0;   // pi/2 high part
+    static constexpr double hp1 = 0x1.1a62633145c07p-54;  // pi/2 - hp0
+    static constexpr double mp1 = 0x1.921fb58p0;          // pi/2 in four parts
+    static constexpr double mp2 = -0x1.dde973cp-27;
+    static constexpr double pp3 = -0x1.cb3b398p-55;
+    static constexpr double pp4 = -0x1.d747f23e32ed7p-83;
+    static constexpr double hpinv = 0x1.45f306dc9c883p-1;  // 2/pi
+    static constexpr double toint = 0x1.8p52;
+};
+
+// Nearest table point k/128 of |x| < 109.5/128 and the remainder |x| - k/128.
+ASG_HD int sincos_index(double ax, double& r) {
+    const double u = SinCosConst::big + ax;
+    r = ax - (u - SinCosConst::big);
+    return static_cast<int>(static_cast<std::uint32_t>(as_u64(u))) * 4;
 }
 
-ASG_HD double kernel_cos(double x, double y) {
-    constexpr double C1 = 4.16666666666666019037e-02, C2 = -1.38888888888741095749e-03,
-                     C3 = 2.48015872894767294178e-05, C4 = -2.75573143513906633035e-07,
-                     C5 = 2.08757232129817482790e-09, C6 = -1.13596475577881948265e-11;
-    const std::uint32_t ix = static_cast<std::uint32_t>(as_u64(x) >> 32) & 0x7fffffffu;
-    if (ix < 0x3e400000u) {
-        return 1.0;  // |x| < 2^-27
-    }
-    const double z = x * x;
-    const double r = z * (C1 + z * (C2 + z * (C3 + z * (C4 + z * (C5 + z * C6)))));
-    if (ix < 0x3fd33333u) {
-        return 1.0 - (0.5 * z - (z * r - x * y));
-    }
-    const double qx = ix > 0x3fe90000u
-                          ? 0.28125
-                          : as_f64(static_cast<std::uint64_t>(ix - 0x00200000u) << 32);
-    const double hz = 0.5 * z - qx;
-    const double a = 1.0 - qx;
-    return a - (hz - (z * r - x * y));
+// sin(a + da) for |a| < 0.126.
+ASG_HD double sincos_taylor_sin(double xx, double a, double da) {
+    using C = SinCosConst;
+    const double p = fma_(fma_(fma_(fma_(C::s5, xx, C::s4), xx, C::s3), xx, C::s2), xx, C::s1);
+    const double t = fma_(fma_(p, a, -(0.5 * da)), xx, da);
+    return a + t;
 }
 
-// Reduces |x| < 2^19 * pi/2 to y0 + y1 in [-pi/4, pi/4]; returns the quadrant.
-ASG_HD int rem_pio2_medium(double x, double& y0, double& y1) {
-    constexpr double kInvPio2 = 6.36619772367581382433e-01;
-    constexpr double kPio2_1 = 1.57079632673412561417e+00, kPio2_1t = 6.07710050650619224932e-11;
-    constexpr double kPio2_2 = 6.07710050630396597660e-11, kPio2_2t = 2.02226624879595063154e-21;
-    constexpr double kPio2_3 = 2.02226624871116645580e-21, kPio2_3t = 8.47842766036889956997e-32;
-    const double t = x < 0.0 ? -x : x;
-    const int n = static_cast<int>(t * kInvPio2 + 0.5);
-    const double fn = static_cast<double>(n);
-    double r = t - fn * kPio2_1;
-    double w = fn * kPio2_1t;
-    y0 = r - w;
-    const int j = static_cast<int>((as_u64(t) >> 52) & 0x7ff);
-    int i = j - static_cast<int>((as_u64(y0) >> 52) & 0x7ff);
-    if (i > 16) {
-        double tt = r;
-        w = fn * kPio2_2;
-        r = tt - w;
-        w = fn * kPio2_2t - ((tt - r) - w);
-        y0 = r - w;
-        i = j - static_cast<int>((as_u64(y0) >> 52) & 0x7ff);
-        if (i > 49) {
-            tt = r;
-            w = fn * kPio2_3;
-            r = tt - w;
-            w = fn * kPio2_3t - ((tt - r) - w);
-            y0 = r - w;
-        }
-    }
-    y1 = (r - y0) - w;
+// cos(x + dx), |x| < 0.85546875.
+ASG_HD double sincos_do_cos(double x, double dx) {
+    using C = SinCosConst;
     if (x < 0.0) {
-        y0 = -y0;
-        y1 = -y1;
-        return -n;
+        dx = -dx;
     }
+    double r;
+    const int k = sincos_index(__builtin_fabs(x), r);
+    x = r + dx;
+    const double xx = x * x;
+    const double s = fma_(x * xx, fma_(xx, C::sn5, C::sn3), x);
+    const double c = xx * fma_(xx, fma_(xx, C::cs6, C::cs4), C::cs2);
+    const double sn = ASG_SINCOS_TAB[k], ssn = ASG_SINCOS_TAB[k + 1];
+    const double cs = ASG_SINCOS_TAB[k + 2], ccs = ASG_SINCOS_TAB[k + 3];
+    const double cor = fma_(-sn, s, fma_(-cs, c, fma_(-s, ssn, ccs)));
+    return cs + cor;
+}
+
+// sin(x + dx), |x| < 0.85546875.
+ASG_HD double sincos_do_sin(double x, double dx) {
+    using C = SinCosConst;
+    const double xold = x;
+    if (__builtin_fabs(x) < 0.126) {
+        return sincos_taylor_sin(x * x, x, dx);
+    }
+    if (x <= 0.0) {
+        dx = -dx;
+    }
+    double r;
+    const int k = sincos_index(__builtin_fabs(x), r);
+    x = r;
+    const double xx = x * x;
+    const double s = x + fma_(x * xx, fma_(xx, C::sn5, C::sn3), dx);
+    const double c = fma_(x, dx, xx * fma_(xx, fma_(xx, C::cs6, C::cs4), C::cs2));
+    const double sn = ASG_SINCOS_TAB[k], ssn = ASG_SINCOS_TAB[k + 1];
+    const double cs = ASG_SINCOS_TAB[k + 2], ccs = ASG_SINCOS_TAB[k + 3];
+    const double cor = fma_(cs, s, fma_(-sn, c, fma_(s, ccs, ssn)));
+    return __builtin_copysign(sn + cor, xold);
+}
+
+// x = n pi/2 + (a + da), |a + da| <= pi/4; returns n mod 4.
+ASG_HD int sincos_reduce(double x, double& a, double& da) {
+    using C = SinCosConst;
+    const double t = fma_(x, C::hpinv, C::toint);
+    const double xn = t - C::toint;
+    const double y = fma_(-xn, C::mp2, fma_(-xn, C::mp1, x));
+    const int n = static_cast<int>(as_u64(t) & 3);
+    const double t2 = fma_(-xn, C::pp3, y);
+    double db = fma_(-xn, C::pp3, y - t2);
+    const double b = fma_(-xn, C::pp4, t2);
+    db = db + fma_(-xn, C::pp4, t2 - b);
+    a = b;
+    da = db;
     return n;
 }
 
-// Both need only |x| < 2^19 * pi/2 (the Box-Muller angle is in [0, 2 pi)).
+// sin (n even) or cos (n odd) of a + da, negated for quadrants n & 2.
+ASG_HD double sincos_quadrant(double a, double da, int n) {
+    const double r = (n & 1) ? sincos_do_cos(a, da) : sincos_do_sin(a, da);
+    return (n & 2) ? -r : r;
+}
+
 ASG_HD double asg_sin(double x) {
-    const std::uint32_t ix = static_cast<std::uint32_t>(as_u64(x) >> 32) & 0x7fffffffu;
-    if (ix <= 0x3fe921fbu) {
-        if (ix < 0x3e400000u) {
-            return x;
-        }
-        return kernel_sin(x, 0.0, 0);
+    using C = SinCosConst;
+    const std::uint32_t k = static_cast<std::uint32_t>(as_u64(x) >> 32) & 0x7fffffffu;
+    if (k < 0x3e500000u) {
+        return x;
     }
-    double y0, y1;
-    const int n = rem_pio2_medium(x, y0, y1);
-    switch (n & 3) {
-        case 0: return kernel_sin(y0, y1, 1);
-        case 1: return kernel_cos(y0, y1);
-        case 2: return -kernel_sin(y0, y1, 1);
-        default: return -kernel_cos(y0, y1);
+    if (k < 0x3feb6000u) {
+        return sincos_do_sin(x, 0.0);
     }
+    if (k < 0x400368fdu) {
+        return __builtin_copysign(sincos_do_cos(C::hp0 - __builtin_fabs(x), C::hp1), x);
+    }
+    if (k < 0x419921fbu) {
+        double a, da;
+        const int n = sincos_reduce(x, a, da);
+        return sincos_quadrant(a, da, n);
+    }
+    return __builtin_nan("");
 }
 
 ASG_HD double asg_cos(double x) {
-    const std::uint32_t ix = static_cast<std::uint32_t>(as_u64(x) >> 32) & 0x7fffffffu;
-    if (ix <= 0x3fe921fbu) {
-        return kernel_cos(x, 0.0);
+    using C = SinCosConst;
+    const std::uint32_t k = static_cast<std::uint32_t>(as_u64(x) >> 32) & 0x7fffffffu;
+    if (k < 0x3e400000u) {
+        return 1.0;
     }
-    double y0, y1;
-    const int n = rem_pio2_medium(x, y0, y1);
-    switch (n & 3) {
-        case 0: return kernel_cos(y0, y1);
-        case 1: return -kernel_sin(y0, y1, 1);
-        case 2: return -kernel_cos(y0, y1);
-        default: return kernel_sin(y0, y1, 1);
+    if (k < 0x3feb6000u) {
+        return sincos_do_cos(x, 0.0);
     }
+    if (k < 0x400368fdu) {
+        const double y = C::hp0 - __builtin_fabs(x);
+        const double a = y + C::hp1;
+        const double da = (y - a) + C::hp1;
+        return sincos_do_sin(a, da);
+    }
+    if (k < 0x419921fbu) {
+        double a, da;
+        const int n = sincos_reduce(x, a, da);
+        return sincos_quadrant(a, da, n + 1);
+    }
+    return __builtin_nan("");
 }
 
 ASG_HD double asg_sqrt(double x) { return __builtin_sqrt(x); }

@@ -1,4 +1,5 @@
-"""Generate amath_tables.h: the lookup tables of the FP64 exp and log in amath.h.
+"""Generate amath_tables.h: the lookup tables of the FP64 exp, log, sin and cos
+in amath.h.
 
 Both functions follow the table-driven method of glibc 2.35's exp and log
 (Szabolcs Nagy's algorithms, shipped in glibc since 2.28 and in Arm's
@@ -17,6 +18,13 @@ exact) and |log(1/invc) - logc| < 2^-66. Several doubles meet that rule per
 subinterval; gen_log_table.c enumerates them in quad precision and keeps the
 one whose results equal the host C library's log on that subinterval (about a
 minute on 8 cores). Its output file is read here.
+
+sin/cos, 110 entries for k in [0, 110) (glibc 2.35's table-driven sin/cos,
+the IBM Accurate Mathematical Library method, looks up the nearest multiple
+k/128 of the reduced argument, which stays below 109.5/128):
+    tab[4k] = RN(sin(k/128)), tab[4k+1] = RN(sin(k/128) - tab[4k]),
+    tab[4k+2] = RN(cos(k/128)), tab[4k+3] = RN(cos(k/128) - tab[4k+2]),
+i.e. each value as an unevaluated double-double sum, from 300-bit mpmath.
 
 Usage:  python gen_amath_tables.py LOGTAB_FILE > amath_tables.h
 """
@@ -40,6 +48,18 @@ def exp_table(n: int = 128):
         h = float(v)
         t = float((v - mpmath.mpf(h)) / mpmath.mpf(h))
         out.append((bits(t), (bits(h) - (k << 45)) & 0xFFFFFFFFFFFFFFFF))
+    return out
+
+
+def sincos_table(n: int = 110):
+    out = []
+    for k in range(n):
+        x = mpmath.mpf(k) / 128
+        row = []
+        for v in (mpmath.sin(x), mpmath.cos(x)):
+            hi = float(v)
+            row += [hi, float(v - mpmath.mpf(hi))]
+        out.append(row)
     return out
 
 
@@ -76,6 +96,12 @@ def main() -> None:
     print("ASG_TABLE double ASG_LOG_TAB[256] = {")
     for invc, logc in log_table(sys.argv[1]):
         print(f"    {invc.hex()}, {logc.hex()},")
+    print("};")
+    print()
+    print("// sin/cos: {sin hi, sin lo, cos hi, cos lo} of k/128, k = 0..109")
+    print("ASG_TABLE double ASG_SINCOS_TAB[440] = {")
+    for row in sincos_table():
+        print("    " + ", ".join(v.hex() for v in row) + ",")
     print("};")
 
 

@@ -17,6 +17,14 @@ Outputs (data only):
 
 --full also runs the 10 s Sparse network for seeds 1, 2 and 3 (about a
 minute each on 8 cores).
+
+Variant cases exercise the reference's branches that no shipped config
+reaches. Each writes configs/<case>.json = a shipped config's values with a
+few keys overridden (listed in VARIANTS), and runs the reference on that file:
+substeps = 2 (neuron.cpp:183-229), a refractory period (neuron.cpp:193-197),
+STP facilitation (network.cpp:237-243), shared noise for neurons and
+astrocytes (neuron.cpp:117-124, astrocyte.cpp:247-249), autapses and one
+astrocyte output per pair (network.cpp:154-204).
 """
 
 from __future__ import annotations
@@ -49,6 +57,31 @@ SHORT_CASES = {
     "random_pool_1s": ("random_pool", 1, 1000.0, 0.0, True),
     "sparse_200ms": ("paper_sparse", 1, 200.0, 0.0, True),
     "sparse_1s_s2": ("paper_sparse", 2, 1000.0, 0.0, True),
+}
+# case -> (base config, {"section.key": value}, seed, sim ms, pre ms, full raster)
+VARIANTS = {
+    "sparse_sub2": ("paper_sparse", {"simulation.substeps": 2}, 1, 300.0, 0.0, True),
+    "quick_tref": ("quick", {"neuron_exc.t_ref": 2.0, "neuron_inh.t_ref": 1.0},
+                   1, 1500.0, 0.0, True),
+    "quick_fac": ("quick", {"synapse.stp.tau_fac": 600.0, "synapse.stp.U": 0.2},
+                  1, 1500.0, 0.0, True),
+    "use_case_shared": ("use_case", {"input_exc.independent_noise": False,
+                                     "input_inh.independent_noise": False,
+                                     "input_astro.independent_noise": False},
+                        1, 2000.0, 0.0, True),
+    "quick_autapse_unique": ("quick", {"connectivity.allow_autapses": True,
+                                       "connectivity.unique_third_out": True,
+                                       "connectivity.p_third_if_primary": 0.5},
+                             3, 1500.0, 0.0, True),
+    "bursting_sub2_tref": ("bursting", {"simulation.substeps": 2, "neuron_exc.t_ref": 1.5,
+                                        "synapse.stp.tau_fac": 300.0},
+                           2, 1000.0, 0.0, True),
+}
+# Long noise runs: the use_case network (Gaussian noise on every cell) for
+# 10 s simulated with no transient, and as configured (20 s transient).
+LONG_CASES = {
+    "use_case_10s": ("use_case", 1, 10000.0, 0.0, False),
+    "use_case_10s_pre20s": ("use_case", 1, 10000.0, 20000.0, False),
 }
 FULL_CASES = {
     "sparse_10s_s1": ("paper_sparse", 1, 10000.0, 1000.0, False),
@@ -93,8 +126,23 @@ def parse_run_txt(path: str) -> dict:
     return out
 
 
+def write_variant(name, base, overrides):
+    with open(os.path.join(HERE, "configs", base + ".json")) as f:
+        values = json.load(f)
+    for dotted, v in overrides.items():
+        node = values
+        keys = dotted.split(".")
+        for k in keys[:-1]:
+            node = node.setdefault(k, {})
+        node[keys[-1]] = v
+    with open(os.path.join(HERE, "configs", name + ".json"), "w") as f:
+        json.dump(values, f, indent=1)
+
+
 def run_case(name, cfg, seed, sim, pre, full_raster, threads):
     cfg_path = os.path.join(REF_ROOT, "config", cfg + ".json")
+    if not os.path.exists(cfg_path):
+        cfg_path = os.path.join(HERE, "configs", cfg + ".json")
     with tempfile.TemporaryDirectory() as out:
         cmd = [REF_BIN, "-c", cfg_path, "-s", str(seed), "-t", str(sim), "-p", str(pre),
                "-o", out, "--no-analysis"]
@@ -139,6 +187,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--full", action="store_true")
     ap.add_argument("--threads", type=int, default=8)
+    ap.add_argument("--only", nargs="*", help="run only these cases")
     args = ap.parse_args()
     if not os.path.exists(REF_BIN):
         sys.exit(f"{REF_BIN} missing: run `make -C oracle ref` first")
@@ -149,9 +198,15 @@ def main():
         with open(os.path.join(HERE, "configs", c + ".json"), "w") as f:
             json.dump(values, f, indent=1)
     cases = dict(SHORT_CASES)
+    cases.update(LONG_CASES)
+    for name, (base, overrides, seed, sim, pre, full) in VARIANTS.items():
+        write_variant(name, base, overrides)
+        cases[name] = (name, seed, sim, pre, full)
     if args.full:
         cases.update(FULL_CASES)
     for name, (cfg, seed, sim, pre, full) in cases.items():
+        if args.only and name not in args.only:
+            continue
         run_case(name, cfg, seed, sim, pre, full, args.threads)
 
 

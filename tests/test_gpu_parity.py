@@ -83,9 +83,16 @@ def _run_device(m, steps, chunk):
 
 
 CASES = ["quick_s1", "quick_s7", "use_case_2s", "bursting_2s", "random_pool_1s", "sparse_200ms"]
+# Reference branches no shipped config reaches: substeps = 2
+# (neuron.cpp:183-229), the refractory clamp (neuron.cpp:193-197), STP
+# facilitation (network.cpp:237-243), shared noise (neuron.cpp:117-124,
+# astrocyte.cpp:247-249), autapses and unique astrocyte outputs
+# (network.cpp:154-204).
+VARIANTS = ["sparse_sub2", "quick_tref", "quick_fac", "use_case_shared",
+            "quick_autapse_unique", "bursting_sub2_tref"]
 
 
-@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("case", CASES + VARIANTS)
 def test_device_equals_oracle_bitwise(case):
     _announce(case)
     m, g = model_for(case)
@@ -101,7 +108,7 @@ def test_device_equals_oracle_bitwise(case):
     assert c["steps"] == steps and c["spikes"] == len(s)
 
 
-@pytest.mark.parametrize("case", CASES + ["sparse_1s_s2"])
+@pytest.mark.parametrize("case", CASES + VARIANTS + ["sparse_1s_s2"])
 def test_device_equals_reference_raster(case):
     _announce(case)
     m, g = model_for(case)
@@ -120,6 +127,23 @@ def test_sparse_10s_raster_equals_reference_build(case):
     _announce(case)
     m, g = model_for(case)
     _, s, n = _run_device(m, g["total_steps"], chunk=10000)
+    s, n = recorded(s, n, g["first_recorded_step"])
+    assert len(s) == g["n_spikes"]
+    counts = np.bincount((s - g["first_recorded_step"]) // 1000,
+                         minlength=len(g["counts_per_1000"]))
+    assert counts.tolist() == g["counts_per_1000"]
+    assert raster_hash(s, n) == g["raster_sha256"]
+
+
+@pytest.mark.parametrize("case", ["use_case_10s", "use_case_10s_pre20s"])
+def test_use_case_10s_raster_equals_reference_build(case):
+    """The use_case network (Gaussian noise on every cell, one Box-Muller
+    cosine per cell per noise interval) for 10 s simulated, without and with
+    its configured 20 s transient: the device raster hashes to the reference
+    build's, so the device cosine equals the C library's on every draw."""
+    _announce(case)
+    m, g = model_for(case)
+    _, s, n = _run_device(m, g["total_steps"], chunk=25000)
     s, n = recorded(s, n, g["first_recorded_step"])
     assert len(s) == g["n_spikes"]
     counts = np.bincount((s - g["first_recorded_step"]) // 1000,

@@ -11,15 +11,19 @@ import astrosimgpu_amd as A
 import oracle
 
 SHORT = ["quick_s1", "quick_s7", "use_case_2s", "bursting_2s", "random_pool_1s", "sparse_200ms"]
+# Branches no shipped config reaches: substeps = 2, refractory clamp, STP
+# facilitation, shared noise, autapses + unique astrocyte outputs.
+VARIANTS = ["sparse_sub2", "quick_tref", "quick_fac", "use_case_shared",
+            "quick_autapse_unique", "bursting_sub2_tref"]
 
 
-@pytest.mark.parametrize("case", SHORT)
+@pytest.mark.parametrize("case", SHORT + VARIANTS)
 def test_network_stats_equal_reference(case):
     m, g = model_for(case)
     assert stats_match(m, g), (m.stats(), g["stats"])
 
 
-@pytest.mark.parametrize("case", SHORT)
+@pytest.mark.parametrize("case", SHORT + VARIANTS)
 def test_oracle_with_c_library_equals_reference_raster(case):
     """libm = the host C library, as the reference build: bit-identical raster."""
     m, g = model_for(case)
@@ -31,17 +35,31 @@ def test_oracle_with_c_library_equals_reference_raster(case):
     assert np.array_equal(n, np.array(g["neurons"], np.uint32))
 
 
-@pytest.mark.parametrize("case", ["quick_s1", "sparse_200ms", "random_pool_1s"])
+@pytest.mark.parametrize("case", SHORT + VARIANTS)
 def test_oracle_with_amath_equals_reference_raster(case):
-    """libm = amath (what the device runs). exp and log are the C library's
-    bit for bit; these cases also happen to stay identical through the
-    Box-Muller sin/cos, which amath does not reproduce bit for bit."""
+    """libm = amath (what the device runs): exp, log and cos are the C
+    library's bit for bit, so the rasters are the reference build's."""
     m, g = model_for(case)
     o = oracle.OracleRun(m.desc, oracle.LIBM_AMATH)
     o.step(g["total_steps"])
     s, n = recorded(*o.spikes(), g["first_recorded_step"])
     assert np.array_equal(s, np.array(g["steps"], np.int64))
     assert np.array_equal(n, np.array(g["neurons"], np.uint32))
+
+
+@pytest.mark.parametrize("case", ["use_case_10s", "use_case_10s_pre20s"])
+def test_oracle_amath_use_case_10s_hash_equals_reference(case):
+    """The use_case network (Gaussian noise current on every neuron and
+    astrocyte, a Box-Muller cosine per cell per 1 ms) for 10 s, with no
+    transient and with the configured 20 s transient (300,000 steps): the
+    amath oracle's raster hashes to the reference build's."""
+    from _cases import raster_hash
+    m, g = model_for(case)
+    o = oracle.OracleRun(m.desc, oracle.LIBM_AMATH)
+    o.step(g["total_steps"])
+    s, n = recorded(*o.spikes(), g["first_recorded_step"])
+    assert len(s) == g["n_spikes"]
+    assert raster_hash(s, n) == g["raster_sha256"]
 
 
 def test_oracle_sparse_1s_seed2_equals_reference():
@@ -126,17 +144,32 @@ def test_amath_equals_c_library_on_the_draw_grid(fn):
     assert np.array_equal(a.view(np.uint64), g.view(np.uint64)), int(np.sum(a != g))
 
 
-def test_amath_cos_is_within_one_ulp_of_c_library():
-    """cos(2 pi u) of Box-Muller (rng.hpp:49) is where amath and the C library
-    part: within one ulp on every draw of 4M, equal on about 97% of them
-    (neither is correctly rounded everywhere; the C library's own result is
-    the one to reproduce and its algorithm is not restated here), so
-    Gaussian-noise runs match the reference build statistically, not bit
-    for bit."""
+@pytest.mark.parametrize("fn,lo,hi", [
+    ("cos", -1e-7, 1e-7), ("cos", -1.0, 1.0), ("cos", -7.0, 7.0), ("cos", -1e5, 1e5),
+    ("cos", -1e8, 1e8), ("sin", -1e-7, 1e-7), ("sin", -1.0, 1.0), ("sin", -7.0, 7.0),
+    ("sin", -1e5, 1e5),
+])
+def test_amath_sin_cos_equal_c_library(fn, lo, hi):
+    """amath's sin/cos (glibc's table-driven method restated) against the
+    host C library, bit for bit on 1M arguments per range, every branch of
+    the method included (first-order, table, cofunction, reduction)."""
+    rng = np.random.default_rng({"cos": 11, "sin": 12}[fn] + int(abs(lo)) % 97)
+    x = rng.uniform(lo, hi, 1 << 20)
+    a = oracle.math(fn, x, oracle.LIBM_AMATH)
+    g = oracle.math(fn, x, oracle.LIBM_GLIBC)
+    assert np.array_equal(a.view(np.uint64), g.view(np.uint64)), int(np.sum(a != g))
+
+
+def test_amath_cos_equals_c_library_on_box_muller_angles():
+    """cos(2 pi u) of Box-Muller (rng.hpp:49), u on the 53-bit draw grid:
+    equal to the C library on all of 4M draws. A plain mistake in the
+    restatement (a dropped table low part, an unfused product, a wrong
+    polynomial coefficient) breaks equality on a fraction of the draws."""
     u, _ = _draw_grid(1)
     x = 6.283185307179586476925286766559 * u
     a = oracle.math("cos", x, oracle.LIBM_AMATH)
     g = oracle.math("cos", x, oracle.LIBM_GLIBC)
-    ulps = np.abs(a - g) / np.spacing(np.abs(g))
-    assert ulps.max() <= 1.0
-    assert 0.9 < float(np.mean(a == g)) < 1.0
+    assert np.array_equal(a.view(np.uint64), g.view(np.uint64)), int(np.sum(a != g))
+    # Closed forms the table fixes: cos(k/128) for table points is the
+    # table's rounded value; cos(0) = 1, cos(pi) = -1.
+    assert oracle.math("cos", np.array([0.0, np.pi]), oracle.LIBM_AMATH).tolist() == [1.0, -1.0]
