@@ -184,9 +184,10 @@ struct DevNet {
     int32_t* drive_n;
     double* drive_noise;
     float* fdrive_noise;
-    // FP32 networks keep their state, parameters and rings here instead
-    float *fV, *fw, *fgex, *fdgex, *fgin, *fdgin, *fIsic;
-    float *fCa, *fIP3, *fhh, *fCa_tot, *fIP3_0, *finv_tau, *fdelta;
+    // FP32 networks keep their conductances, SIC current, parameters and
+    // rings here instead (their V, w, Ca, IP3 and h are the FP64 arrays)
+    float *fgex, *fdgex, *fgin, *fdgin, *fIsic;
+    float *fCa_tot, *fIP3_0, *finv_tau, *fdelta;
     float *fring_sic, *ffac;
     // FP32 networks' exc, inh and n2a rings in fixed point (value * 2^32):
     // delivered by 64-bit atomic adds, whose integer sums do not depend on
@@ -580,7 +581,8 @@ __device__ __forceinline__ uint64_t neuron_window_f32(const DevNet& n, uint32_t 
     int64_t sic_next = first_hit(T0, d_a2n, sic_interval);
     const uint32_t live_in = live_arrivals(n, T0);
     const bool sic_live_n = live_in & 1u;
-    // FP64 registers, rounded to FP32 at the end of every step
+    // FP64 registers; the conductances are rounded to FP32 at the end of
+    // every step, V and w stay FP64
     double V = 0, w = 0, gex = 0, dgex = 0, gin = 0, dgin = 0, Isic = 0, Vr = 0, b = 0;
     int refr = 0;
     int64_t ex_n = 0, ih_n = 0;
@@ -594,8 +596,8 @@ __device__ __forceinline__ uint64_t neuron_window_f32(const DevNet& n, uint32_t 
     int64_t nz_next = first_hit(T0 + 1, 0, nz_period) + 1;
     uint32_t nz_j = 0;
     if (valid) {
-        V = n.fV[i];
-        w = n.fw[i];
+        V = n.V[i];
+        w = n.w[i];
         gex = n.fgex[i];
         dgex = n.fdgex[i];
         gin = n.fgin[i];
@@ -683,10 +685,9 @@ __device__ __forceinline__ uint64_t neuron_window_f32(const DevNet& n, uint32_t 
                                                  gex, dgex, gin, dgin, refr);
                 spiked = spiked || sp;
             }
-            // the state is FP32 between steps, so results do not depend on
-            // where windows (call lengths, the batch's shortest delay) fall
-            V = round_f32(V);
-            w = round_f32(w);
+            // the conductances are FP32 between steps, so results do not
+            // depend on where windows (call lengths, the batch's shortest
+            // delay) fall
             gex = round_f32(gex);
             dgex = round_f32(dgex);
             gin = round_f32(gin);
@@ -700,8 +701,8 @@ __device__ __forceinline__ uint64_t neuron_window_f32(const DevNet& n, uint32_t 
         row = row_n;
     }
     if (valid) {
-        n.fV[i] = static_cast<float>(V);
-        n.fw[i] = static_cast<float>(w);
+        n.V[i] = V;
+        n.w[i] = w;
         n.fgex[i] = static_cast<float>(gex);
         n.fdgex[i] = static_cast<float>(dgex);
         n.fgin[i] = static_cast<float>(gin);
@@ -731,7 +732,7 @@ __device__ __forceinline__ void astro_window_f32(const DevNet& n, uint32_t a, in
                 h_sixth = static_cast<float>(n.h_sixth);
     int64_t* __restrict__ ring_astro = n.qring_astro;
     float* __restrict__ fac = n.ffac;
-    float Ca = n.fCa[a], IP3 = n.fIP3[a], hh = n.fhh[a];
+    double Ca = n.Ca[a], IP3 = n.IP3[a], hh = n.hh[a];
     const float Ca_tot = n.fCa_tot[a], IP3_0 = n.fIP3_0[a], inv_tau = n.finv_tau[a],
                 delta = n.fdelta[a];
     int row = static_cast<int>(T0 % S);  // ring row of the current step
@@ -794,9 +795,9 @@ __device__ __forceinline__ void astro_window_f32(const DevNet& n, uint32_t a, in
         }
         row = row_n;
     }
-    n.fCa[a] = Ca;
-    n.fIP3[a] = IP3;
-    n.fhh[a] = hh;
+    n.Ca[a] = Ca;
+    n.IP3[a] = IP3;
+    n.hh[a] = hh;
     mark_sic_live(n, live_bits, W);
 }
 
@@ -2168,8 +2169,9 @@ Plan plan_of(const asg_net_desc& d, const DerivedConn* dc) {
     p.sic_pairs = dc ? dc->sic_src.size() : a2n_raw;  // ELL entries
     const size_t fb = d.precision == ASG_FP32 ? 4 : 8;  // bytes of a state value
     size_t b = 0;
-    b += p.N * (7 * fb + 4 + 2 * 8 + 3 * 8);       // neuron state, V_reset/b, STP
-    b += p.NA * (3 * fb + 4 * fb);                  // astrocyte state and parameters
+    // neuron state (V and w FP64 in both precisions), V_reset/b, STP
+    b += p.N * (2 * 8 + 5 * fb + 4 + 2 * 8 + 3 * 8);
+    b += p.NA * (3 * 8 + 4 * fb);                   // astrocyte state (FP64) and parameters
     const bool f32 = d.precision == ASG_FP32;
     // rings: FP64 exc/inh/SIC and n2a; FP32 fixed-point exc/inh and n2a, FP32 SIC
     b += f32 ? p.S * p.ldn * (8 + 8 + 4) + p.S * p.lda * 8 : p.S * p.ldn * 3 * 8 + p.S * p.lda * 8;
@@ -2321,17 +2323,16 @@ void build_slot(NetSlot& s, const asg_net_desc& d, int64_t total, hipStream_t st
             for (size_t q = 0; q < cnt; ++q) v[q] = static_cast<float>(src[q]);
             return v;
         };
-        const std::vector<float> V0f = to_f(V0.data(), N);
-        n.fV = dev_copy(s, V0f.data(), N, st);
-        n.fw = dev_zero<float>(s, N, st);
+        n.V = dev_copy(s, V0.data(), N, st);
+        n.w = dev_zero<double>(s, N, st);
         n.fgex = dev_zero<float>(s, N, st);
         n.fdgex = dev_zero<float>(s, N, st);
         n.fgin = dev_zero<float>(s, N, st);
         n.fdgin = dev_zero<float>(s, N, st);
         n.fIsic = dev_zero<float>(s, N, st);
-        n.fCa = dev_fill(s, static_cast<float>(d.astro.Ca_init), NA, st);
-        n.fIP3 = dev_fill(s, static_cast<float>(d.astro.IP3_init), NA, st);
-        n.fhh = dev_fill(s, static_cast<float>(d.astro.h_init), NA, st);
+        n.Ca = dev_fill(s, d.astro.Ca_init, NA, st);
+        n.IP3 = dev_fill(s, d.astro.IP3_init, NA, st);
+        n.hh = dev_fill(s, d.astro.h_init, NA, st);
         const std::vector<float> Ca_tot = to_f(d.Ca_tot, NA), IP3_0 = to_f(d.IP3_0, NA),
                                  delta = to_f(d.delta_IP3, NA);
         std::vector<float> inv_tau(NA);
@@ -3129,16 +3130,16 @@ struct Engine {
         const float* fsrc = nullptr;
         const bool f32 = n.precision == ASG_FP32;
         switch (field) {
-            case ASG_FIELD_V: src = n.V; fsrc = n.fV; break;
-            case ASG_FIELD_W: src = n.w; fsrc = n.fw; break;
+            case ASG_FIELD_V: src = n.V; break;
+            case ASG_FIELD_W: src = n.w; break;
             case ASG_FIELD_G_EX: src = n.gex; fsrc = n.fgex; break;
             case ASG_FIELD_DG_EX: src = n.dgex; fsrc = n.fdgex; break;
             case ASG_FIELD_G_IN: src = n.gin; fsrc = n.fgin; break;
             case ASG_FIELD_DG_IN: src = n.dgin; fsrc = n.fdgin; break;
             case ASG_FIELD_I_SIC: src = n.Isic; fsrc = n.fIsic; break;
-            case ASG_FIELD_CA: src = n.Ca; fsrc = n.fCa; break;
-            case ASG_FIELD_IP3: src = n.IP3; fsrc = n.fIP3; break;
-            case ASG_FIELD_H: src = n.hh; fsrc = n.fhh; break;
+            case ASG_FIELD_CA: src = n.Ca; break;
+            case ASG_FIELD_IP3: src = n.IP3; break;
+            case ASG_FIELD_H: src = n.hh; break;
             case ASG_FIELD_STP_X: src = n.stp_x; fsrc = nullptr; break;
             case ASG_FIELD_STP_U: src = n.stp_u; fsrc = nullptr; break;
             case ASG_FIELD_STP_TLAST: src = n.stp_tl; fsrc = nullptr; break;

@@ -1,17 +1,19 @@
 // Per-cell math of the FP32 step (device only).
 //
-// State, conductances and the drive are stored in FP32. A neuron's state is
-// integrated in FP64 registers -- FP64 vector instructions issue at the FP32
-// rate on this GPU, and near threshold the exponential term
-// e^((V - V_th)/Delta_T) decides the spike time -- with the divisions by
-// C_m, Delta_T and tau_w taken as products with FP64 reciprocals and a
-// table-free FP64 exp accurate to ~3e-13, and rounded to FP32 at the end of
-// every step, so results do not depend on window boundaries. Astrocytes integrate
-// in FP32 with hardware reciprocals; their Ca-threshold test of the SIC,
-// (Ca - SIC_th) * 1000 > 1, is evaluated in FP64 (Ca is a small difference
-// above a threshold). Random draws use the FP64 step's counter streams:
-// identical Poisson counts, and normals from the same two uniforms through
-// FP32 hardware functions. Synaptic input arrives through fixed-point rings
+// Conductances, the SIC current, parameters and the drive are stored in
+// FP32; the slow state -- V and w of a neuron, Ca, IP3 and h of an astrocyte
+// -- is carried in FP64 between steps. A neuron's step is integrated in FP64
+// registers -- FP64 vector instructions issue at the FP32 rate on this GPU,
+// and near threshold the exponential term e^((V - V_th)/Delta_T) decides the
+// spike time -- with the divisions by C_m, Delta_T and tau_w taken as
+// products with FP64 reciprocals; the conductances are rounded to FP32 at
+// the end of every step, so results do not depend on window boundaries.
+// Astrocytes evaluate their right-hand sides in FP32 with hardware
+// reciprocals and add each step's increment to the FP64 state; their
+// Ca-threshold test of the SIC, (Ca - SIC_th) * 1000 > 1, is evaluated in
+// FP64 (Ca is a small difference above a threshold). Random draws use the
+// FP64 step's counter streams: identical Poisson counts, and normals from the
+// same two uniforms through FP32 hardware functions. Synaptic input arrives through fixed-point rings
 // (exact integer sums); the SIC gather sums with Neumaier's compensation.
 #pragma once
 
@@ -67,10 +69,11 @@ __device__ inline void comp_add(float& s, float& c, float x) {
 }
 
 // 2^y of the AdEx spike current: y rounded to FP32 and the hardware
-// v_exp_f32 (about 1 ulp), with no range reduction. Its relative error,
-// ln2 |y| 2^-24 from rounding y plus the instruction's own, is of the order
-// of what the FP32 rounding of V at every step already puts into y
-// (ulp(V) / Delta_T), and half the issue cycles of an FP64 polynomial.
+// v_exp_f32 (about 1 ulp), with no range reduction, at half the issue cycles
+// of an FP64 polynomial. Its relative error, ln2 |y| 2^-24 from rounding y
+// plus the instruction's own, is about 1e-7 near threshold (|y| < 2), where
+// the current is g_L Delta_T ~ 40 pA: a change of V of ~1e-9 mV per step,
+// three orders below an FP32 ulp of V.
 __device__ __forceinline__ double spike_exp2(double y) {
     return static_cast<double>(__builtin_amdgcn_exp2f(static_cast<float>(y)));
 }
@@ -212,36 +215,49 @@ __device__ inline void astro_rhs_f32(const AstroKf& c, float Ca_tot, float IP3_0
     dh = alpha_h * (1.0f - h) - beta_h * h;
 }
 
+// One astrocyte step with FP32 right-hand sides. S is the type the state
+// (Ca, IP3, h) is carried in between steps: float for the FP32-state
+// experiment of the isolated update, double for the FP32 step of the
+// simulation. Each RK4 stage is evaluated in FP32 from the state rounded to
+// FP32; the step's increment h/6 (k1 + 2 k2 + 2 k3 + k4), formed in FP32, is
+// then added to the state in S. With S = double no increment is lost to the
+// state's rounding: near equilibrium an increment can fall below half an
+// FP32 ulp of the state (h relaxes with a time constant of seconds), which
+// FP32 state would drop at every step.
+template <typename S>
 __device__ inline void astro_step_f32(const AstroKf& c, float Ca_tot, float IP3_0, float inv_tau,
                                       float delta_IP3, float ip3_input, float noise, float hs,
-                                      float half_h, float h_sixth, int substeps, float& Ca,
-                                      float& IP3, float& h) {
+                                      float half_h, float h_sixth, int substeps, S& Ca, S& IP3,
+                                      S& h) {
     // FP32 path: mul-add pairs may fuse (it has no host build to match)
     #pragma clang fp contract(fast)
     if (ip3_input != 0.0f) {
-        IP3 += delta_IP3 * ip3_input;
+        IP3 += static_cast<S>(delta_IP3) * static_cast<S>(ip3_input);
     }
     for (int s = 0; s < substeps; ++s) {
+        const float Ca0 = static_cast<float>(Ca), IP30 = static_cast<float>(IP3),
+                    h0 = static_cast<float>(h);
         float k1C, k1I, k1h, k2C, k2I, k2h, k3C, k3I, k3h, k4C, k4I, k4h;
-        astro_rhs_f32(c, Ca_tot, IP3_0, inv_tau, Ca, IP3, h, noise, k1C, k1I, k1h);
-        astro_rhs_f32(c, Ca_tot, IP3_0, inv_tau, Ca + half_h * k1C, IP3 + half_h * k1I,
-                      h + half_h * k1h, noise, k2C, k2I, k2h);
-        astro_rhs_f32(c, Ca_tot, IP3_0, inv_tau, Ca + half_h * k2C, IP3 + half_h * k2I,
-                      h + half_h * k2h, noise, k3C, k3I, k3h);
-        astro_rhs_f32(c, Ca_tot, IP3_0, inv_tau, Ca + hs * k3C, IP3 + hs * k3I, h + hs * k3h,
+        astro_rhs_f32(c, Ca_tot, IP3_0, inv_tau, Ca0, IP30, h0, noise, k1C, k1I, k1h);
+        astro_rhs_f32(c, Ca_tot, IP3_0, inv_tau, Ca0 + half_h * k1C, IP30 + half_h * k1I,
+                      h0 + half_h * k1h, noise, k2C, k2I, k2h);
+        astro_rhs_f32(c, Ca_tot, IP3_0, inv_tau, Ca0 + half_h * k2C, IP30 + half_h * k2I,
+                      h0 + half_h * k2h, noise, k3C, k3I, k3h);
+        astro_rhs_f32(c, Ca_tot, IP3_0, inv_tau, Ca0 + hs * k3C, IP30 + hs * k3I, h0 + hs * k3h,
                       noise, k4C, k4I, k4h);
-        Ca += h_sixth * (k1C + 2.0f * k2C + 2.0f * k3C + k4C);
-        IP3 += h_sixth * (k1I + 2.0f * k2I + 2.0f * k3I + k4I);
-        h += h_sixth * (k1h + 2.0f * k2h + 2.0f * k3h + k4h);
-        Ca = Ca < 0.0f ? 0.0f : (Ca > Ca_tot ? Ca_tot : Ca);
-        h = h < 0.0f ? 0.0f : (h > 1.0f ? 1.0f : h);
-        IP3 = IP3 < 0.0f ? 0.0f : IP3;
+        Ca += static_cast<S>(h_sixth * (k1C + 2.0f * k2C + 2.0f * k3C + k4C));
+        IP3 += static_cast<S>(h_sixth * (k1I + 2.0f * k2I + 2.0f * k3I + k4I));
+        h += static_cast<S>(h_sixth * (k1h + 2.0f * k2h + 2.0f * k3h + k4h));
+        const S Ct = static_cast<S>(Ca_tot);
+        Ca = Ca < S(0) ? S(0) : (Ca > Ct ? Ct : Ca);
+        h = h < S(0) ? S(0) : (h > S(1) ? S(1) : h);
+        IP3 = IP3 < S(0) ? S(0) : IP3;
     }
 }
 
 // The SIC factor with the threshold test in FP64 (astrocyte.cpp:331-339).
-__device__ inline float sic_factor_mixed(float Ca, double SIC_th, double SIC_scale) {
-    const double y = (static_cast<double>(Ca) - SIC_th) * 1000.0;
+__device__ inline float sic_factor_mixed(double Ca, double SIC_th, double SIC_scale) {
+    const double y = (Ca - SIC_th) * 1000.0;
     if (y <= 1.0) {
         return 0.0f;
     }
