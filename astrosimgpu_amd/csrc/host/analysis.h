@@ -1,6 +1,14 @@
-// Post-run calcium analysis (reference src/analysis.cpp, include/astrosimgpu/
-// analysis.hpp): threshold-crossing event detection with merging, sliding-
-// window onset rates, Pearson correlation, pairwise mean, summary statistics.
+// Post-run calcium analysis: what the reference's command line prints under
+// "Calcium analysis" (src/main.cpp:264-313, on the functions of
+// src/analysis.cpp). Its results are reproduced bit for bit; the passes are
+// this file's own:
+//  * transients of every cell in one sweep over the [samples][cells] record
+//    (a run detector per cell, merging as it goes);
+//  * onset-rate bins by a two-pointer sweep of each cell's sorted onsets;
+//  * the pairwise Pearson coefficients from rows of centred bins computed
+//    once per cell, so a pair costs one dot product. Every sum keeps the
+//    reference's order (bins ascending, pairs in (i, j) order), which is what
+//    the bits depend on.
 #pragma once
 
 #include <cstddef>
@@ -9,34 +17,27 @@
 
 namespace asg_host {
 
-struct Events {
-    std::vector<double> onset, offset, duration;
+// Threshold-crossing events of one trace (analysis.cpp:10-64): an event is a
+// maximal run of samples at or above the threshold, from the time of its
+// first sample to the time of its last; with merge, a run that starts less
+// than max_gap after the previous one ended extends it. Fewer than two
+// samples give no events.
+struct EventList {
+    std::vector<double> onset, offset;
 };
+EventList find_events(const double* times, const double* values, int64_t n, double threshold,
+                      bool merge, double max_gap);
 
-// analysis.cpp:10-64.
-Events detect_events(const std::vector<double>& times, const std::vector<double>& values,
-                     double threshold, bool merge, double max_gap);
-// analysis.cpp:66-96 (window [start, end]).
-std::vector<double> sliding_window_rate(const std::vector<double>& events, double start, double end,
-                                        double width, double shift);
-// analysis.cpp:98-123.
-double pearson(const std::vector<double>& a, const std::vector<double>& b);
-// analysis.cpp:136-166; returns every coefficient, the mean is taken by the caller.
-std::vector<double> pairwise_correlation(const std::vector<std::vector<double>>& per_cell_events,
-                                         double start, double end, double width, double shift);
-// analysis.cpp:168-199.
-std::vector<double> nearest_event_distances(const std::vector<std::vector<double>>& per_cell);
+// Onset rate in sliding windows [start + i shift, start + i shift + width]
+// (both ends included), i < floor((end - start - width) / shift), in events
+// per ms (analysis.cpp:66-96).
+std::vector<double> onset_rate_bins(std::vector<double> onsets, double start, double end,
+                                    double width, double shift);
 
-struct Summary {
-    double mean = 0.0, stddev = 0.0;
-    size_t count = 0;
-};
-// analysis.cpp:201-220.
-Summary summarise(const std::vector<double>& values);
+// Pearson's r of two series of n values; 0 when n < 2 or either has no
+// variance (analysis.cpp:98-123).
+double pearson(const double* a, const double* b, int64_t n);
 
-// What the reference prints under "Calcium analysis" (main.cpp:266-317) for
-// sampled traces: per cell events with merging, then the pairwise
-// correlation of windowed onset rates, with the short-run window fallback.
 struct CalciumSummary {
     int32_t cells = 0, active = 0;
     int64_t transients = 0, pairs = 0;

@@ -4,7 +4,7 @@
 #include <stdexcept>
 #include <vector>
 
-#include "json.h"
+#include "config_doc.h"
 
 namespace asg_host {
 
@@ -42,76 +42,76 @@ asg_input_params input_defaults() {  // parameters.hpp:132-138
     return p;
 }
 
-void load_astro(const JsonValue& j, ModelConfig& c) {  // parameters.cpp:13-35
+void load_astro(ConfigDoc::Node j, ModelConfig& c) {  // parameters.cpp:13-35
     if (!j.is_object()) return;
     asg_astro_params& a = c.astro;
-    j.read("Ca_tot", c.astro_Ca_tot);
-    j.read("IP3_0", c.astro_IP3_0);
-    j.read("Kd_IP3_1", a.c.Kd_IP3_1);
-    j.read("Kd_IP3_2", a.c.Kd_IP3_2);
-    j.read("Kd_act", a.c.Kd_act);
-    j.read("Kd_inh", a.c.Kd_inh);
-    j.read("Km_SERCA", a.c.Km_SERCA);
-    j.read("SIC_scale", a.SIC_scale);
-    j.read("SIC_th", a.SIC_th);
-    j.read("delta_IP3", c.astro_delta_IP3);
-    j.read("k_IP3R", a.c.k_IP3R);
-    j.read("rate_IP3R", a.c.rate_IP3R);
-    j.read("rate_L", a.c.rate_L);
-    j.read("rate_SERCA", a.c.rate_SERCA);
-    j.read("ratio_ER_cyt", a.c.ratio_ER_cyt);
-    j.read("tau_IP3", c.astro_tau_IP3);
-    j.read("Ca_init", a.Ca_init);
-    j.read("h_init", a.h_init);
-    j.read("IP3", a.IP3_init);        // alias, then the long name wins
-    j.read("IP3_init", a.IP3_init);
+    j.take("Ca_tot", c.astro_Ca_tot);
+    j.take("IP3_0", c.astro_IP3_0);
+    j.take("Kd_IP3_1", a.c.Kd_IP3_1);
+    j.take("Kd_IP3_2", a.c.Kd_IP3_2);
+    j.take("Kd_act", a.c.Kd_act);
+    j.take("Kd_inh", a.c.Kd_inh);
+    j.take("Km_SERCA", a.c.Km_SERCA);
+    j.take("SIC_scale", a.SIC_scale);
+    j.take("SIC_th", a.SIC_th);
+    j.take("delta_IP3", c.astro_delta_IP3);
+    j.take("k_IP3R", a.c.k_IP3R);
+    j.take("rate_IP3R", a.c.rate_IP3R);
+    j.take("rate_L", a.c.rate_L);
+    j.take("rate_SERCA", a.c.rate_SERCA);
+    j.take("ratio_ER_cyt", a.c.ratio_ER_cyt);
+    j.take("tau_IP3", c.astro_tau_IP3);
+    j.take("Ca_init", a.Ca_init);
+    j.take("h_init", a.h_init);
+    j.take("IP3", a.IP3_init);        // alias, then the long name wins
+    j.take("IP3_init", a.IP3_init);
 }
 
-void load_neuron(const JsonValue& j, asg_neuron_params& p) {  // parameters.cpp:37-57
+void load_neuron(ConfigDoc::Node j, asg_neuron_params& p) {  // parameters.cpp:37-57
     if (!j.is_object()) return;
-    j.read("C_m", p.C_m);
-    j.read("g_L", p.g_L);
-    j.read("E_L", p.E_L);
-    j.read("V_th", p.V_th);
-    j.read("Delta_T", p.Delta_T);
-    j.read("a", p.a);
-    j.read("b", p.b);
-    j.read("tau_w", p.tau_w);
-    j.read("V_reset", p.V_reset);
-    j.read("V_peak", p.V_peak);
-    j.read("t_ref", p.t_ref);
-    j.read("E_ex", p.E_ex);
-    j.read("E_in", p.E_in);
-    j.read("tau_syn_ex", p.tau_syn_ex);
-    j.read("tau_syn_in", p.tau_syn_in);
-    j.read("I_e", p.I_e);
-    j.read("V_m", p.V_m_init);
+    j.take("C_m", p.C_m);
+    j.take("g_L", p.g_L);
+    j.take("E_L", p.E_L);
+    j.take("V_th", p.V_th);
+    j.take("Delta_T", p.Delta_T);
+    j.take("a", p.a);
+    j.take("b", p.b);
+    j.take("tau_w", p.tau_w);
+    j.take("V_reset", p.V_reset);
+    j.take("V_peak", p.V_peak);
+    j.take("t_ref", p.t_ref);
+    j.take("E_ex", p.E_ex);
+    j.take("E_in", p.E_in);
+    j.take("tau_syn_ex", p.tau_syn_ex);
+    j.take("tau_syn_in", p.tau_syn_in);
+    j.take("I_e", p.I_e);
+    j.take("V_m", p.V_m_init);
 }
 
-void load_input(const JsonValue& j, asg_input_params& p) {  // parameters.cpp:59-69
+void load_input(ConfigDoc::Node j, asg_input_params& p) {  // parameters.cpp:59-69
     if (!j.is_object()) return;
-    j.read("poiss_rate", p.poiss_rate);
-    j.read("poiss_weight", p.poiss_weight);
+    j.take("poiss_rate", p.poiss_rate);
+    j.take("poiss_weight", p.poiss_weight);
     // Both names set the standard deviation; the second wins if both exist.
-    j.read("gauss_noise_var", p.gauss_noise_std);
-    j.read("gauss_noise_std", p.gauss_noise_std);
-    j.read("noise_dt", p.noise_dt);
+    j.take("gauss_noise_var", p.gauss_noise_std);
+    j.take("gauss_noise_std", p.gauss_noise_std);
+    j.take("noise_dt", p.noise_dt);
     bool indep = p.independent_noise != 0;
-    j.read("independent_noise", indep);
+    j.take("independent_noise", indep);
     p.independent_noise = indep ? 1 : 0;
 }
 
-void load_randomize(const JsonValue& j, RandomizeSpec& r) {  // parameters.cpp:71-81
+void load_randomize(ConfigDoc::Node j, RandomizeSpec& r) {  // parameters.cpp:71-81
     if (!j.is_object()) return;
     r.enabled = true;
-    j.read("enabled", r.enabled);
-    j.read("lower", r.lower);
-    j.read("upper", r.upper);
-    j.read("var", r.var);
+    j.take("enabled", r.enabled);
+    j.take("lower", r.lower);
+    j.take("upper", r.upper);
+    j.take("var", r.var);
 }
 
 // "a.b.c=value;d.e=value": value is parsed as a JSON literal.
-void apply_assignments(JsonValue& root, const std::string& spec) {
+void apply_assignments(ConfigDoc& doc, const std::string& spec) {
     std::stringstream ss(spec);
     std::string item;
     while (std::getline(ss, item, ';')) {
@@ -120,86 +120,76 @@ void apply_assignments(JsonValue& root, const std::string& spec) {
         if (eq == std::string::npos) {
             throw std::runtime_error("override '" + item + "' is not of the form key=value");
         }
-        const std::string path = item.substr(0, eq);
-        const JsonValue value = JsonValue::parse(item.substr(eq + 1));
-        JsonValue* node = &root;
-        size_t start = 0;
-        for (;;) {
-            const size_t dot = path.find('.', start);
-            const std::string key = path.substr(start, dot == std::string::npos ? dot : dot - start);
-            node = &node->member(key);
-            if (dot == std::string::npos) break;
-            start = dot + 1;
-        }
-        node->set_parsed(value);
+        doc.assign(item.substr(0, eq), item.substr(eq + 1));
     }
 }
 
-ModelConfig map_config(const JsonValue& root, const std::string& name, const Overrides& ov) {
+ModelConfig map_config(const ConfigDoc& doc, const std::string& name, const Overrides& ov) {
+    const ConfigDoc::Node root = doc.root();
     if (!root.is_object()) {
         throw std::runtime_error("configuration root must be a JSON object: " + name);
     }
     ModelConfig cfg;
-    if (const JsonValue& n = root.get("N"); n.is_object()) {
-        n.read("N_A", cfg.N_astro);
-        n.read("N_E", cfg.N_exc);
-        n.read("N_I", cfg.N_inh);
+    if (const ConfigDoc::Node n = root.at("N"); n.is_object()) {
+        n.take("N_A", cfg.N_astro);
+        n.take("N_E", cfg.N_exc);
+        n.take("N_I", cfg.N_inh);
     }
-    if (const JsonValue& s = root.get("simulation"); s.is_object()) {
-        s.read("dt", cfg.dt);
-        s.read("substeps", cfg.substeps);
-        s.read("pre_sim_time", cfg.pre_sim_time);
-        s.read("sim_time", cfg.sim_time);
+    if (const ConfigDoc::Node s = root.at("simulation"); s.is_object()) {
+        s.take("dt", cfg.dt);
+        s.take("substeps", cfg.substeps);
+        s.take("pre_sim_time", cfg.pre_sim_time);
+        s.take("sim_time", cfg.sim_time);
         unsigned long long seed = cfg.seed;
-        s.read("seed", seed);
+        s.take("seed", seed);
         cfg.seed = seed;
-        s.read("output_dir", cfg.output_dir);
-        s.read("record_spikes", cfg.record_spikes);
-        s.read("record_astro", cfg.record_astro);
-        s.read("record_neuron", cfg.record_neuron);
-        s.read("record_every", cfg.record_every);
-        s.read("record_astro_max", cfg.record_astro_max);
-        s.read("record_neuron_max", cfg.record_neuron_max);
+        s.take("output_dir", cfg.output_dir);
+        s.take("record_spikes", cfg.record_spikes);
+        s.take("record_astro", cfg.record_astro);
+        s.take("record_neuron", cfg.record_neuron);
+        s.take("record_every", cfg.record_every);
+        s.take("record_astro_max", cfg.record_astro_max);
+        s.take("record_neuron_max", cfg.record_neuron_max);
     }
-    load_astro(root.get("astrocyte"), cfg);
-    load_neuron(root.get("neuron_exc"), cfg.neuron_exc);
-    load_neuron(root.get("neuron_inh"), cfg.neuron_inh);
-    load_input(root.get("input_astro"), cfg.input_astro);
-    load_input(root.get("input_exc"), cfg.input_exc);
-    load_input(root.get("input_inh"), cfg.input_inh);
-    load_randomize(root.get("randomize_astro"), cfg.randomize_astro);
-    load_randomize(root.get("randomize_neuron"), cfg.randomize_neuron);
-    if (const JsonValue& s = root.get("synapse"); s.is_object()) {
-        s.read("w_e", cfg.syn.w_e);
-        s.read("w_i", cfg.syn.w_i);
-        s.read("w_n2a", cfg.syn.w_n2a);
-        s.read("w_a2n", cfg.syn.w_a2n);
-        s.read("d_e", cfg.syn.d_e);
-        s.read("d_i", cfg.syn.d_i);
-        s.read("d_a2n", cfg.syn.d_a2n);
-        s.read("sic_interval", cfg.syn.sic_interval);
-        if (const JsonValue& stp = s.get("stp"); stp.is_object()) {
-            stp.read("enabled", cfg.syn.stp_enabled);
-            stp.read("U", cfg.syn.U);
-            stp.read("tau_rec", cfg.syn.tau_rec);
-            stp.read("tau_fac", cfg.syn.tau_fac);
+    load_astro(root.at("astrocyte"), cfg);
+    load_neuron(root.at("neuron_exc"), cfg.neuron_exc);
+    load_neuron(root.at("neuron_inh"), cfg.neuron_inh);
+    load_input(root.at("input_astro"), cfg.input_astro);
+    load_input(root.at("input_exc"), cfg.input_exc);
+    load_input(root.at("input_inh"), cfg.input_inh);
+    load_randomize(root.at("randomize_astro"), cfg.randomize_astro);
+    load_randomize(root.at("randomize_neuron"), cfg.randomize_neuron);
+    if (const ConfigDoc::Node s = root.at("synapse"); s.is_object()) {
+        s.take("w_e", cfg.syn.w_e);
+        s.take("w_i", cfg.syn.w_i);
+        s.take("w_n2a", cfg.syn.w_n2a);
+        s.take("w_a2n", cfg.syn.w_a2n);
+        s.take("d_e", cfg.syn.d_e);
+        s.take("d_i", cfg.syn.d_i);
+        s.take("d_a2n", cfg.syn.d_a2n);
+        s.take("sic_interval", cfg.syn.sic_interval);
+        if (const ConfigDoc::Node stp = s.at("stp"); stp.is_object()) {
+            stp.take("enabled", cfg.syn.stp_enabled);
+            stp.take("U", cfg.syn.U);
+            stp.take("tau_rec", cfg.syn.tau_rec);
+            stp.take("tau_fac", cfg.syn.tau_fac);
         }
     }
-    if (const JsonValue& a = root.get("analysis"); a.is_object()) {
-        a.read("enabled", cfg.analysis.enabled);
-        a.read("ca_threshold", cfg.analysis.ca_threshold);
-        a.read("merge_gap", cfg.analysis.merge_gap);
-        a.read("hist_window", cfg.analysis.hist_window);
-        a.read("hist_shift", cfg.analysis.hist_shift);
+    if (const ConfigDoc::Node a = root.at("analysis"); a.is_object()) {
+        a.take("enabled", cfg.analysis.enabled);
+        a.take("ca_threshold", cfg.analysis.ca_threshold);
+        a.take("merge_gap", cfg.analysis.merge_gap);
+        a.take("hist_window", cfg.analysis.hist_window);
+        a.take("hist_shift", cfg.analysis.hist_shift);
     }
-    if (const JsonValue& c = root.get("connectivity"); c.is_object()) {
-        c.read("p_primary", cfg.conn.p_primary);
-        c.read("p_third_if_primary", cfg.conn.p_third_if_primary);
-        c.read("pool_size", cfg.conn.pool_size);
-        c.read("allow_autapses", cfg.conn.allow_autapses);
-        c.read("unique_third_out", cfg.conn.unique_third_out);
+    if (const ConfigDoc::Node c = root.at("connectivity"); c.is_object()) {
+        c.take("p_primary", cfg.conn.p_primary);
+        c.take("p_third_if_primary", cfg.conn.p_third_if_primary);
+        c.take("pool_size", cfg.conn.pool_size);
+        c.take("allow_autapses", cfg.conn.allow_autapses);
+        c.take("unique_third_out", cfg.conn.unique_third_out);
         std::string generator = "sequential";
-        c.read("generator", generator);
+        c.take("generator", generator);
         if (generator == "per_row") {
             cfg.conn.per_row = true;
         } else if (generator != "sequential") {
@@ -207,7 +197,7 @@ ModelConfig map_config(const JsonValue& root, const std::string& name, const Ove
                                      "' (expected \"sequential\" or \"per_row\")");
         }
         std::string pool_type = "block";
-        c.read("pool_type", pool_type);
+        c.take("pool_type", pool_type);
         if (pool_type == "block") {
             cfg.conn.pool_type = PoolType::Block;
         } else if (pool_type == "random") {
@@ -227,8 +217,7 @@ ModelConfig map_config(const JsonValue& root, const std::string& name, const Ove
     if (cfg.syn.sic_interval < 1) throw std::runtime_error("synapse.sic_interval must be at least 1");
     if (cfg.conn.pool_size < 1) throw std::runtime_error("connectivity.pool_size must be at least 1");
 
-    std::vector<std::string> unused;
-    root.unused_keys("", unused);
+    const std::vector<std::string> unused = doc.unread_paths();
     if (!unused.empty()) {
         std::string msg = "unrecognised keys in " + name + ":";
         for (const std::string& k : unused) msg += "\n  " + k;
@@ -272,15 +261,15 @@ ModelConfig::ModelConfig() {
 
 ModelConfig load_config_text(const std::string& text, const std::string& name,
                              const Overrides& ov) {
-    JsonValue root = JsonValue::parse(text);
-    if (!ov.assignments.empty()) apply_assignments(root, ov.assignments);
-    return map_config(root, name, ov);
+    ConfigDoc doc = ConfigDoc::parse(text);
+    if (!ov.assignments.empty()) apply_assignments(doc, ov.assignments);
+    return map_config(doc, name, ov);
 }
 
 ModelConfig load_config(const std::string& path, const Overrides& ov) {
-    JsonValue root = JsonValue::parse_file(path);
-    if (!ov.assignments.empty()) apply_assignments(root, ov.assignments);
-    return map_config(root, path, ov);
+    ConfigDoc doc = ConfigDoc::load(path);
+    if (!ov.assignments.empty()) apply_assignments(doc, ov.assignments);
+    return map_config(doc, path, ov);
 }
 
 }  // namespace asg_host

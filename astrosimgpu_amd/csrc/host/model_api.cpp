@@ -173,21 +173,19 @@ asg_status asg_detect_events(const double* times, const double* values, int64_t 
     if (n < 0 || n_events == nullptr || (n > 0 && (times == nullptr || values == nullptr))) {
         return asg_host_fail(ASG_ERR_INVALID, "bad argument");
     }
-    const std::vector<double> t(times, times + n), v(values, values + n);
-    const asg_host::Events ev = asg_host::detect_events(t, v, threshold, merge != 0, max_gap);
-    const int64_t m = static_cast<int64_t>(std::max(ev.onset.size(), ev.offset.size()));
+    const asg_host::EventList ev =
+        asg_host::find_events(times, values, n, threshold, merge != 0, max_gap);
+    const int64_t m = static_cast<int64_t>(ev.onset.size());
     *n_events = m;
     if (m > capacity) return asg_host_fail(ASG_ERR_CAPACITY, "output arrays too small");
-    for (int64_t i = 0; i < m; ++i) {
-        onset[i] = i < static_cast<int64_t>(ev.onset.size()) ? ev.onset[i] : -1.0;
-        offset[i] = i < static_cast<int64_t>(ev.offset.size()) ? ev.offset[i] : -1.0;
-    }
+    std::copy(ev.onset.begin(), ev.onset.end(), onset);
+    std::copy(ev.offset.begin(), ev.offset.end(), offset);
     return ASG_OK;
 }
 
 double asg_pearson(const double* a, const double* b, int64_t n) {
     if (n <= 0 || a == nullptr || b == nullptr) return 0.0;
-    return asg_host::pearson(std::vector<double>(a, a + n), std::vector<double>(b, b + n));
+    return asg_host::pearson(a, b, n);
 }
 
 asg_status asg_sliding_window_rate(const double* events, int64_t n, double start, double end,
@@ -196,8 +194,8 @@ asg_status asg_sliding_window_rate(const double* events, int64_t n, double start
     if (n < 0 || n_out == nullptr || (n > 0 && events == nullptr)) {
         return asg_host_fail(ASG_ERR_INVALID, "bad argument");
     }
-    const std::vector<double> h =
-        asg_host::sliding_window_rate(std::vector<double>(events, events + n), start, end, width, shift);
+    const std::vector<double> h = asg_host::onset_rate_bins(
+        std::vector<double>(events, events + n), start, end, width, shift);
     *n_out = static_cast<int64_t>(h.size());
     if (*n_out > capacity) return asg_host_fail(ASG_ERR_CAPACITY, "output array too small");
     for (size_t i = 0; i < h.size(); ++i) out[i] = h[i];

@@ -100,3 +100,66 @@ def test_matrix_and_per_cell_forms_agree():
                                   e["threshold"], e["merge_gap"], e["pre_time"], e["sim_time"],
                                   e["hist_window"], e["hist_shift"], round6=round6)
         assert a == b
+
+
+REF_DRIVER = os.path.join(os.path.dirname(GOLDEN), "..", "oracle", "_ref", "analysis_driver")
+
+
+def _reference_analysis(times, ca, params, tmp_path):
+    """The reference's analysis functions (its analysis.cpp behind
+    oracle/analysis_driver.cpp) on the same per-cell traces, at full
+    precision."""
+    import subprocess
+    off = np.concatenate([[0], np.cumsum([len(x) for x in ca])]).astype(np.int64)
+    path = tmp_path / "traces.bin"
+    with open(path, "wb") as f:
+        f.write(np.int32(len(ca)).tobytes())
+        f.write(np.array(params, np.float64).tobytes())
+        f.write(off.tobytes())
+        f.write(np.concatenate([np.zeros(0)] + list(times)).astype(np.float64).tobytes())
+        f.write(np.concatenate([np.zeros(0)] + list(ca)).astype(np.float64).tobytes())
+    out = subprocess.run([REF_DRIVER, str(path)], check=True, capture_output=True,
+                         text=True).stdout.split()
+    return {"active": int(out[0]), "transients": int(out[1]), "pairs": int(out[2]),
+            "mean_duration": float.fromhex(out[3]), "sd_duration": float.fromhex(out[4]),
+            "mean_correlation": float.fromhex(out[5]), "width": float.fromhex(out[6]),
+            "shift": float.fromhex(out[7])}
+
+
+def _random_traces(rng, n_cells, threshold):
+    times, ca = [], []
+    for c in range(n_cells):
+        kind = rng.integers(0, 6)
+        n = {0: 0, 1: 1}.get(kind, int(rng.integers(2, 400)))
+        t = np.cumsum(rng.uniform(0.5, 1.5, n)) + rng.uniform(0, 5)
+        if kind == 2:       # values exactly at the threshold in places
+            v = np.where(rng.random(n) < 0.4, threshold, threshold - 0.01)
+        elif kind == 3:     # a plain pulse train with gaps of exactly 5 samples
+            v = np.where((np.arange(n) // 5) % 2 == 0, threshold + 0.1, threshold - 0.1)
+            t = np.arange(n, dtype=np.float64)
+        else:               # a random walk around the threshold
+            v = threshold + np.cumsum(rng.normal(0, 0.02, n))
+        times.append(t)
+        ca.append(v)
+    return times, ca
+
+
+@pytest.mark.skipif(not os.path.exists(REF_DRIVER), reason="reference build absent")
+@pytest.mark.parametrize("seed", range(12))
+def test_calcium_analysis_bit_equals_reference_functions(seed, tmp_path):
+    """Random traces with empty and one-sample cells, values exactly at the
+    threshold, gaps exactly max_gap and the short-window fallback: every
+    field of the product's analysis equals the reference's, bit for bit."""
+    rng = np.random.default_rng(seed)
+    threshold = 0.2
+    times, ca = _random_traces(rng, int(rng.integers(0, 30)), threshold)
+    sim = float(rng.choice([50.0, 200.0, 400.0]))
+    params = [threshold, float(rng.choice([0.0, 2.0, 5.0, 6.0, 20.0])), float(rng.uniform(0, 20)),
+              sim, float(rng.choice([20.0, 100.0, 1000.0])), float(rng.choice([2.0, 5.0, 10.0]))]
+    ref = _reference_analysis(times, ca, params, tmp_path)
+    got = analyze_calcium_cells(times, ca, *params)
+    for k, v in ref.items():
+        if isinstance(v, float):
+            assert float(got[k]).hex() == v.hex(), (k, got[k], v)
+        else:
+            assert got[k] == v, (k, got[k], v)

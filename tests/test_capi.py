@@ -215,3 +215,44 @@ def test_split_spikes_reassembles_each_slot_across_drain_rounds():
     assert sorted(got) == occupied
     for s in occupied:
         assert np.array_equal(got[s][0], want[s][0]) and np.array_equal(got[s][1], want[s][1])
+
+
+@pytest.mark.parametrize("text,where", [
+    ('{"a": }', "byte 6"), ('{"a" 1}', "byte 5"), ('{"a": 1,}', "byte 8"),
+    ('[1, 2', "byte 5"), ('{"N": {}} x', "byte 10"), ('"\\q"', "byte 1"),
+    ('{"a": tru}', "byte 6"), ('{"a": "\\u12"}', "byte 11"), ("", "byte 0"),
+    ('{"a": "open', "byte 6"),
+])
+def test_malformed_documents_name_the_byte(text, where):
+    with pytest.raises(A.AsgError) as e:
+        A.Model(text=text)
+    assert e.value.status == capi.ASG_ERR_CONFIG
+    assert "JSON parse error at " + where in str(e.value), str(e.value)
+
+
+def test_document_rules_of_the_reference_loader():
+    """A later duplicate key wins (json.cpp builds a std::map); a comment may
+    end the text without a newline; numbers take strtod's longest prefix of
+    the sign/digit/point/exponent run ("12e" is 12); unread keys are listed
+    at every level in byte order (parameters.cpp:191-202)."""
+    m = A.Model(text='{"N": {"N_A": 4, "N_E": 8, "N_I": 2, "N_A": 6},\n'
+                     ' "simulation": {"seed": 12e}} // end')
+    assert m.info["n_astro"] == 6 and m.info["seed"] == 12
+    with pytest.raises(A.AsgError) as e:
+        A.Model(text='{"N": {"N_A": 4, "zz": 1, "N_E": 8, "N_I": 2}, "bogus": {"x": 1},'
+                     ' "simulation": {"b": 1, "a": 2}}')
+    msg = str(e.value)
+    assert msg.index("N.zz") < msg.index("bogus") < msg.index("simulation.a") < msg.index("simulation.b")
+    assert "bogus.x" not in msg
+
+
+def test_assignments_create_and_replace_members():
+    """Overrides "a.b=literal" parse the literal as JSON, create missing
+    objects on the path and replace what was there."""
+    m = A.Model(config_path("quick"), assignments={"N.N_A": 7, "synapse.stp.tau_fac": 250.0,
+                                                   "connectivity.pool_type": "random"})
+    assert m.info["n_astro"] == 7
+    assert m.desc.contents.stp.tau_fac == 250.0
+    text = '{"N": {"N_A": 4, "N_E": 8, "N_I": 2}, "input_exc": {"poiss_rate": 5.0}}'
+    m = A.Model(text=text, assignments={"input_exc.independent_noise": False})
+    assert m.desc.contents.input_exc.independent_noise == 0

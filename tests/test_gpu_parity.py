@@ -264,16 +264,21 @@ def test_odd_delays_and_sic_interval_equal_oracle():
         assert np.array_equal(net.state(f).view(np.uint64), o.state(f).view(np.uint64)), f
 
 
-def test_astro_dropin_matches_engine_cell_math():
+@pytest.mark.parametrize("noise,substeps", [(0.0, 1), (0.003, 1), (0.003, 2)])
+def test_astro_dropin_matches_engine_cell_math(noise, substeps):
     """asg_astro_* (the astrocyte_cuda.hpp replacement) against the oracle's
-    astrocyte update: an astrocyte-only network driven by its Poisson input,
-    replayed through the drop-in with the same per-step inputs."""
-    _announce("astro drop-in")
+    astrocyte update: an astrocyte-only network driven by its Poisson input
+    (and, where set, an independent Gaussian noise current held for 5
+    steps), replayed through the drop-in with the same per-step inputs, at
+    one and two RK4 substeps per step."""
+    _announce(f"astro drop-in noise {noise} substeps {substeps}")
     text = """{ "N": {"N_A": 3000, "N_E": 0, "N_I": 0},
+      "simulation": {"substeps": %d},
       "astrocyte": {"IP3": 0.3, "Ca_tot": 1.8958264782990153, "IP3_0": 0.010925583871830699,
                     "tau_IP3": 1058.7589568400845, "delta_IP3": 0.04911592657233997},
       "randomize_astro": {"enabled": true, "lower": 0.9, "upper": 1.05, "var": 0.1},
-      "input_astro": {"poiss_rate": 300.0, "poiss_weight": 1.0, "gauss_noise_std": 0.0} }"""
+      "input_astro": {"poiss_rate": 300.0, "poiss_weight": 1.0, "gauss_noise_std": %r,
+                      "noise_dt": 0.5} }""" % (substeps, noise)
     m = A.Model(text=text)
     d = m.desc.contents
     n = d.n_astro
@@ -291,17 +296,39 @@ def test_astro_dropin_matches_engine_cell_math():
     h = lib.asg_astro_create(n, P(Ca), P(IP3), P(hh), P(Ca_tot), P(IP3_0), P(tau), P(delta))
     lam = d.input_astro.poiss_rate * d.dt * 1e-3
     seed = d.seed ^ 0xC2B2AE3D27D4EB4F
+    noise_seed = d.seed ^ 0x9E3779B97F4A7C15
+    spn = max(1, int(round(d.input_astro.noise_dt / d.dt)))
     for step in range(steps):
         ev = _poisson_counts(seed, step, n, lam)
         ip3_in = d.input_astro.poiss_weight * ev.astype(np.float64)
         ip3_in[ev == 0] = 0.0
         lib.asg_astro_push_input(h, P(ip3_in))
-        lib.asg_astro_update(h, C.byref(d.astro.c), d.dt, 1, 0.0, 1, 0.0, 0, 0)
+        lib.asg_astro_update(h, C.byref(d.astro.c), d.dt / substeps, substeps, noise, 1, 0.0,
+                             noise_seed, step // spn)
     lib.asg_astro_destroy(h, P(Ca), P(IP3), P(hh))
     o.step(steps)
     assert np.array_equal(Ca, o.state("Ca"))
     assert np.array_equal(IP3, o.state("IP3"))
     assert np.array_equal(hh, o.state("h"))
+
+
+def test_astro_dropin_empty_and_null_handles_are_no_ops():
+    """n == 0 gives a handle with no buffers; every call on it, and on a null
+    handle, returns without touching its arguments (astrocyte_cuda.cu:90-93,
+    108-110, 125-127)."""
+    _announce("astro drop-in n = 0")
+    import ctypes as C
+    lib = A.lib()
+    h = lib.asg_astro_create(0, None, None, None, None, None, None, None)
+    assert h
+    lib.asg_astro_push_input(h, None)
+    lib.asg_astro_update(h, None, 0.1, 1, 0.0, 1, 0.0, 0, 0)
+    lib.asg_astro_pull_calcium(h, None)
+    lib.asg_astro_destroy(h, None, None, None)
+    lib.asg_astro_push_input(None, None)
+    lib.asg_astro_update(None, None, 0.1, 1, 0.0, 1, 0.0, 0, 0)
+    lib.asg_astro_pull_calcium(None, None)
+    lib.asg_astro_destroy(None, None, None, None)
 
 
 def _poisson_counts(seed, step, n, lam):
