@@ -119,7 +119,8 @@ constexpr double kFixScale = 4294967296.0;        // 2^32: fixed-point ring unit
 constexpr double kFixUnit = 1.0 / 4294967296.0;   // 2^-32
 constexpr int64_t kSicWaves = 4096; // SIC wavefronts that fill the device (256 CUs x 16)
 constexpr int64_t kGroupWaves = 16384;  // per-step wavefronts beyond which steps are grouped
-constexpr int kDeliverWaves = 512;  // FP32 delivery wavefronts sharing out the spiking blocks
+constexpr int kDeliverWaves = 512;  // delivery wavefronts sharing out the spiking blocks
+constexpr int kSicWorkers = 2048;   // FP32 SIC wavefronts sharing out the live networks' blocks
 constexpr size_t kMaskBudget = size_t(256) << 20;  // delivery masks per network
 constexpr size_t kMinChunks = 64;
 
@@ -225,6 +226,8 @@ struct DevNet {
     int64_t c_steps, c_spikes, c_syn, c_sic;
     uint32_t sic_live;     // bit k: some astrocyte has a nonzero SIC factor at
                            // window step k (set by k_cells, cleared at the close)
+    uint32_t sic_live_p[2];  // FP32 networks: the same, one word per window
+                             // parity (the close empties the other parity's)
     int32_t open;          // 1 from a window's cell pass to its close
     // bit T mod (hist_mask + 1): the SIC exchange arriving at step T had a
     // nonzero factor, i.e. its ring_sic row was written; 0 means every
@@ -272,8 +275,11 @@ __device__ inline void record_astro(const DevNet& n, uint32_t a, int64_t slot, d
 }
 
 // Bit k of `bits`: this lane's SIC factor at window step k is nonzero. One
-// atomic per wavefront publishes the OR over its active lanes.
-__device__ inline void mark_sic_live(const DevNet& n, uint32_t bits, int W) {
+// atomic per wavefront publishes the OR over its active lanes into the
+// network's live word of the window. Returns true on the one lane (of one
+// wavefront per network and window) whose atomic turned it from 0 to
+// nonzero.
+__device__ inline bool mark_sic_live(uint32_t* live_word, uint32_t bits, int W) {
     uint32_t any = 0;
     for (int k = 0; k < W; ++k) {
         if (__ballot((bits >> k) & 1u) != 0) {
@@ -283,9 +289,17 @@ __device__ inline void mark_sic_live(const DevNet& n, uint32_t bits, int W) {
     const uint64_t active = __ballot(true);
     if (any != 0 &&
         (threadIdx.x & 63) == static_cast<uint32_t>(__ffsll(static_cast<long long>(active)) - 1)) {
-        atomicOr(const_cast<uint32_t*>(&n.sic_live), any);
+        return atomicOr(live_word, any) == 0u;
     }
+    return false;
 }
+
+// Per-window state of the FP32 post pass (k_post), one per window parity:
+// the networks whose SIC exchange is live in the window, as the cell pass
+// lists them (emptied by the post pass of the window before).
+struct PostState {
+    uint64_t sic_ctr;  // (networks listed << 32) | SIC items (neuron blocks) listed
+};
 
 struct CellBlock {
     int32_t net;
@@ -548,7 +562,7 @@ __device__ __forceinline__ void astro_window(const DevNet& n, uint32_t a, int64_
     n.Ca[a] = Ca;
     n.IP3[a] = IP3;
     n.hh[a] = hh;
-    mark_sic_live(n, live_bits, W);
+    mark_sic_live(const_cast<uint32_t*>(&n.sic_live), live_bits, W);
 }
 
 // ----------------------------------------------------- FP32 cell windows ---
@@ -713,13 +727,16 @@ __device__ __forceinline__ uint64_t neuron_window_f32(const DevNet& n, uint32_t 
     return spiked_any;
 }
 
-__device__ __forceinline__ void astro_window_f32(const DevNet& n, uint32_t a, int64_t T0, int W) {
+// Returns true on the lane whose update made the network's SIC exchange
+// live for the window (see mark_sic_live).
+__device__ __forceinline__ bool astro_window_f32(const DevNet& n, uint32_t a, int64_t T0, int W,
+                                                 int par) {
     // FP32 path: mul-add pairs may fuse (it has no host build to match)
     #pragma clang fp contract(fast)
     const uint32_t NA = n.n_astro;
     const size_t LA = n.lda;  // row stride of the [rows][astrocytes] arrays
     if (a >= NA) {
-        return;
+        return false;
     }
     const AstroKf kf = n.akf;
     const InputK ain = n.ap.in;
@@ -798,20 +815,21 @@ __device__ __forceinline__ void astro_window_f32(const DevNet& n, uint32_t a, in
     n.Ca[a] = Ca;
     n.IP3[a] = IP3;
     n.hh[a] = hh;
-    mark_sic_live(n, live_bits, W);
+    return mark_sic_live(const_cast<uint32_t*>(&n.sic_live_p[par]), live_bits, W);
 }
 
 // FP32 SIC of one (neuron, window step): sum over (astrocyte, multiplicity)
 // of m * w_a2n * factor, compensated.
 template <int GM>
-__device__ void deliver_sic_f32(const DevNet& n, uint32_t t, int g0, int G) {
+__device__ __forceinline__ void deliver_sic_f32(const DevNet& n, uint32_t t, int g0, int G,
+                                                uint32_t sic_live) {
     const size_t LN = n.ldn;
     const int64_t T0 = n.win_T0;
     uint32_t send = 0;  // bit j: window step g0 + j sends an exchange
     for (int64_t j = first_hit(T0 + g0, 0, n.sic_interval); j < G; j += n.sic_interval) {
         send |= 1u << j;
     }
-    const uint32_t live = send & (n.sic_live >> g0);
+    const uint32_t live = send & (sic_live >> g0);
     float acc[GM], comp[GM];
 #pragma unroll
     for (int j = 0; j < GM; ++j) {
@@ -870,21 +888,19 @@ __device__ inline uint32_t wave_exclusive_scan(uint32_t v, uint32_t& total) {
     return x - v;
 }
 
-// Closes the window of `total` spikes: the raster head, the liveness of the
-// window's SIC exchanges at their arrival steps (a step that sends no
-// exchange has sic_live 0), counters and the next window's first step.
-__device__ inline void close_window(DevNet& n, uint32_t total) {
+// Closes the current window whose SIC exchanges were live at the window
+// steps of the bits of `sic_live`: their liveness at their arrival steps (a
+// step that sends no exchange has a 0 bit), counters and the next window's
+// first step.
+__device__ inline void close_window_with(DevNet& n, uint32_t sic_live) {
     const int W = n.win_W;
     const int64_t T0 = n.win_T0;
-    if (n.r_head + total <= n.r_cap) {
-        n.r_head += total;
-    }
     // the W liveness bits go to ring bits b0 .. b0 + W - 1 (W <= 16 < 32):
     // at most two words, each read and written once
     const uint32_t b0 = static_cast<uint32_t>(T0 + n.d_a2n) & n.hist_mask;
     const uint32_t w0 = b0 >> 5, sh = b0 & 31, words = (n.hist_mask + 1) >> 5;
     const uint64_t span = ((1ULL << W) - 1ULL) << sh;
-    const uint64_t live = static_cast<uint64_t>(n.sic_live & ((1u << W) - 1u)) << sh;
+    const uint64_t live = static_cast<uint64_t>(sic_live & ((1u << W) - 1u)) << sh;
     uint32_t* hist = n.sic_hist;
     hist[w0] = (hist[w0] & ~static_cast<uint32_t>(span)) | static_cast<uint32_t>(live);
     if ((span >> 32) != 0) {
@@ -893,20 +909,21 @@ __device__ inline void close_window(DevNet& n, uint32_t total) {
     }
     n.c_steps += W;
     n.step = T0 + W;
-    n.sic_live = 0;
     n.open = 0;
 }
 
-// Closes an open window from one whole wavefront: lanes k < W load the step
-// totals together.
-__device__ inline void close_open_window(DevNet& n) {
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t tot = static_cast<int>(lane) < n.win_W ? n.spk_n[2 * lane] : 0u;
-    uint32_t total = 0;
-    wave_exclusive_scan(tot, total);
-    if (lane == 0) {
-        close_window(n, total);
+__device__ inline void close_window_state(DevNet& n) {
+    close_window_with(n, n.sic_live);
+    n.sic_live = 0;
+}
+
+// The close of an FP64 window of `total` spikes: its raster entries (written
+// at r_head by the delivery pass) become part of the raster if they all fit.
+__device__ inline void close_window(DevNet& n, uint32_t total) {
+    if (n.r_head + total <= n.r_cap) {
+        n.r_head += total;
     }
+    close_window_state(n);
 }
 
 // Window pass 0: the external drive of every neuron for the window's steps,
@@ -924,29 +941,18 @@ __device__ inline void close_open_window(DevNet& n) {
 // compacting survivors in place, until it is empty. Every uniform is the
 // same (seed, stream, counter) as in the sequential loop, so the counts are
 // identical.
-__global__ __launch_bounds__(64) void k_drive(DevNet* nets, const CellBlock* nblocks,
-                                              int32_t n_items, int32_t Wg, int32_t* work_n) {
-    extern __shared__ double drive_lds[];
-    if (blockIdx.x == 0 && threadIdx.x < 2) {
-        work_n[threadIdx.x] = 0;  // the window's delivery lists start empty
-    }
+//
+// drive_block computes the drive of the window after the last one opened
+// (win_T0 + win_W is the step that window's close sets) for the 64 neurons
+// of block cb, with the dynamic LDS ring drive_lds (64 * Wg products and
+// keys). All 64 lanes of a one-wavefront workgroup call it together.
+__device__ __forceinline__ void drive_block(DevNet& n, const CellBlock& cb, int32_t Wg,
+                                            double* drive_lds) {
     double* q_prod = drive_lds;                                             // [64 * Wg]
     uint32_t* q_key = reinterpret_cast<uint32_t*>(drive_lds + 64 * Wg);     // [64 * Wg]
-    const int item = static_cast<int>(blockIdx.x);
-    if (item >= n_items) {
-        return;
-    }
-    const CellBlock cb = nblocks[item];
-    DevNet& n = nets[cb.net];
-    // The window follows the last one opened (win_T0 + win_W is the step
-    // its close sets): the lead wavefront may close that window here while
-    // the other blocks read only fields the close does not write.
     const int64_t T0 = n.win_T0 + n.win_W;
     const int64_t rem = n.target - T0;
     const int W = rem <= 0 ? 0 : (rem < Wg ? static_cast<int>(rem) : Wg);
-    if (cb.lead && n.open) {
-        close_open_window(n);
-    }
     if (W == 0) {
         return;
     }
@@ -1084,19 +1090,44 @@ __global__ __launch_bounds__(64) void k_drive(DevNet* nets, const CellBlock* nbl
     }
 }
 
+// Window pass 0 as a launch of its own: the drive of the next window of
+// every network whose neuron blocks are [nblocks, nblocks + n_items), one
+// wavefront per block; block 0 first empties the delivery list count
+// work_n (when given) for the window's cell pass.
+__global__ __launch_bounds__(64) void k_drive(DevNet* nets, const CellBlock* nblocks,
+                                              int32_t n_items, int32_t Wg, int32_t* work_n) {
+    extern __shared__ double drive_lds[];
+    if (blockIdx.x == 0 && threadIdx.x == 0 && work_n != nullptr) {
+        *work_n = 0;
+    }
+    const int item = static_cast<int>(blockIdx.x);
+    if (item >= n_items) {
+        return;
+    }
+    const CellBlock cb = nblocks[item];
+    drive_block(nets[cb.net], cb, Wg, drive_lds);
+}
+
 // Window pass 1: every cell of every network advances W steps. One kernel per
 // precision, so neither carries the other's registers; neurons and
 // astrocytes share the launch, so the short astrocyte blocks overlap the
 // neuron blocks.
 // WPB wavefronts (64-cell blocks) per workgroup, a launch-schedule knob.
 // A neuron block that spiked in the window appends itself to its
-// precision's delivery list `work` (P = 0 from entry 0, P = 1 from entry
-// fp32_base), counted in work_n[P] (zeroed by the window's k_drive).
+// precision's delivery list `work`, counted in *work_n. An FP32 network
+// whose SIC exchange becomes live in the window is listed for the post
+// pass: its slot in sic_net, the first of its SIC items (its neuron blocks,
+// nbr[2 * slot + 1] of them) in sic_start. FP32 windows alternate between
+// two sets of these lists and live words by parity `par` (the host passes
+// the set of the window's parity), so that the post pass can empty the
+// other set while it reads this one.
 template <int P, int WPB>
 __global__ __launch_bounds__(kCellBlock * WPB) void k_cells(DevNet* nets, const CellBlock* blocks,
                                                             int32_t count, int32_t Wg,
                                                             CellBlock* work, int32_t* work_n,
-                                                            int32_t fp32_base) {
+                                                            int32_t par, PostState* post,
+                                                            int32_t* sic_net, uint32_t* sic_start,
+                                                            const int32_t* nbr) {
     // The wavefront's block index is uniform across its lanes: said so
     // explicitly, so the block and network descriptors stay in scalar
     // registers.
@@ -1122,10 +1153,13 @@ __global__ __launch_bounds__(kCellBlock * WPB) void k_cells(DevNet* nets, const 
     if constexpr (P == ASG_FP32) {
         if (cb.kind == 0) {
             if (neuron_window_f32(n, i, T0, W) != 0 && lane == 0) {
-                work[fp32_base + atomicAdd(work_n + 1, 1)] = cb;
+                work[atomicAdd(work_n, 1)] = cb;
             }
-        } else {
-            astro_window_f32(n, i, T0, W);
+        } else if (astro_window_f32(n, i, T0, W, par)) {
+            const uint64_t old = atomicAdd(reinterpret_cast<unsigned long long*>(&post->sic_ctr),
+                                           (1ULL << 32) | static_cast<uint32_t>(nbr[2 * cb.net + 1]));
+            sic_net[old >> 32] = cb.net;
+            sic_start[old >> 32] = static_cast<uint32_t>(old);
         }
     } else {
         if (cb.kind == 0) {
@@ -1518,24 +1552,25 @@ __device__ inline void add_rows_exc(const DevNet& n, uint32_t e0, uint32_t e1, i
 // wavefront adds each spike's weights, in fixed point, into the arrival slot
 // of every target of its rows: w_exc * x * u into the exc ring, -w_inh * x * u
 // into the inh ring (the FP64 sums' sign convention), multiplicity * w_n2a *
-// x * u into the n2a ring. The integer sums are the
-// same in any order, so spikes need neither masks nor a summing pass.
-__device__ void deliver_fixed(DevNet& n, uint32_t first, uint32_t lane) {
+// x * u into the n2a ring. The integer sums are the same in any order, so
+// spikes need neither masks nor a summing pass. Raster entries are appended
+// at r_head, reserved by one atomic per (wavefront, step) with spikes; their
+// order within a window is restored when the raster is drained (sorted by
+// step, then source).
+__device__ __forceinline__ void deliver_fixed(DevNet& n, uint32_t first, uint32_t lane) {
     const uint32_t N = n.n_neur, NE = n.n_exc;
     const uint32_t t = first + lane;
     const bool valid = t < N;
     const int W = n.win_W;
     const int64_t T0 = n.win_T0;
-    const int64_t r_head = n.r_head;
     const int S = n.S;
     const size_t LN = n.ldn, LA = n.lda;
-    const StepWords sw(n, first >> 6, lane, 0, W);
-    uint32_t window_total = 0;
-    const uint32_t before_k = wave_exclusive_scan(sw.tot, window_total);
-    const bool room = r_head + static_cast<int64_t>(window_total) <= n.r_cap;
+    // lane k < W holds the block's spike word of window step k
+    const uint64_t l_word =
+        static_cast<int>(lane) < W ? n.spk_bits[static_cast<size_t>(lane) * n.words + (first >> 6)] : 0;
     uint32_t mine = 0;  // bit k: this lane's neuron spiked at window step k
     for (int k2 = 0; k2 < W; ++k2) {
-        mine |= static_cast<uint32_t>((lane_u64(sw.word, k2) >> lane) & 1ULL) << k2;
+        mine |= static_cast<uint32_t>((lane_u64(l_word, k2) >> lane) & 1ULL) << k2;
     }
     const bool exc = t < NE;
     const bool stp = n.stp_on != 0;
@@ -1566,11 +1601,22 @@ __device__ void deliver_fixed(DevNet& n, uint32_t first, uint32_t lane) {
     int rowA = static_cast<int>((T0 + n.d_n2a) % S);
     int64_t syn = 0;
     for (int k2 = 0; k2 < W; ++k2) {
-        const uint64_t word = lane_u64(sw.word, k2);
+        const uint64_t word = lane_u64(l_word, k2);
         if (word != 0) {
+            // this step's raster entries: one reservation for the wavefront
+            const uint32_t cnt = static_cast<uint32_t>(__popcll(word));
+            int64_t base = 0;
+            if (lane == 0) {
+                base = static_cast<int64_t>(atomicAdd(reinterpret_cast<unsigned long long*>(&n.r_head),
+                                                      static_cast<unsigned long long>(cnt)));
+                add_i64(&n.c_spikes, cnt);
+                if (base + cnt > n.r_cap) {
+                    atomicOr(&n.overflow, 1);  // raster buffer full
+                }
+            }
+            base = static_cast<int64_t>(lane_u64(static_cast<uint64_t>(base), 0));
             int64_t q0 = 0, q1 = 0;  // this lane's spike weights, fixed point
             if ((mine >> k2) & 1u) {
-                const uint32_t j = lane_u32(sw.off, k2) + static_cast<uint32_t>(__popcll(word & below));
                 double c0 = exc ? n.w_exc : n.w_inh;
                 double c1 = n.w_n2a;
                 const double t_now = static_cast<double>(T0 + k2) * n.dt;
@@ -1587,8 +1633,8 @@ __device__ void deliver_fixed(DevNet& n, uint32_t first, uint32_t lane) {
                 q0 = llrint(c0 * kFixScale);
                 q1 = llrint(c1 * kFixScale);
                 syn += row_syn;
-                if (room) {
-                    const int64_t at = r_head + lane_u32(before_k, k2) + j;
+                const int64_t at = base + __popcll(word & below);
+                if (at < n.r_cap) {
                     n.r_step[at] = T0 + k2;
                     n.r_src[at] = t;
                 }
@@ -1606,7 +1652,7 @@ __device__ void deliver_fixed(DevNet& n, uint32_t first, uint32_t lane) {
                 } else {
                     // the inh ring holds the negated weights (as the FP64 sums)
                     add_row(n.inh_tgt, lane_u32(r0, L), lane_u32(r1, L),
-                                   n.qring_inh + static_cast<size_t>(rowI) * LN, -w0, lane);
+                            n.qring_inh + static_cast<size_t>(rowI) * LN, -w0, lane);
                 }
             }
         }
@@ -1622,20 +1668,15 @@ __device__ void deliver_fixed(DevNet& n, uint32_t first, uint32_t lane) {
     if (syn > 0) {
         add_i64(&n.c_syn, syn);
     }
-    if (!room && window_total > 0 && lane == 0) {
-        atomicOr(&n.overflow, 1);  // every delivering wavefront of the network agrees
-    }
 }
 
-// Window pass 3, one launch per precision (so the FP32 networks' launch
-// does not carry the FP64 SIC accumulators' registers): D wavefronts deliver
-// the blocks the cell pass listed as having spiked (FP64: marking in groups
-// of SG steps, the first group also doing the sources' plasticity and
-// raster entries; FP32: deliver_fixed), then ceil(Wg / G) wavefronts per
-// block sum the SIC of its neurons. G trades re-reading the astrocyte lists
-// (G = Wg reads them once per window) against parallelism for small
-// networks (G = 1).
-template <int G, int P>
+// Window pass 3 of the FP64 networks: D wavefronts deliver the blocks the
+// cell pass listed as having spiked (marking in groups of SG steps, the
+// first group also doing the sources' plasticity and raster entries), then
+// ceil(Wg / G) wavefronts per block sum the SIC of its neurons. G trades
+// re-reading the astrocyte lists (G = Wg reads them once per window) against
+// parallelism for small networks (G = 1).
+template <int G>
 __global__ __launch_bounds__(256) void k_deliver(DevNet* nets, const CellBlock* nblocks,
                                                  int32_t n_items, int32_t Wg, int32_t D,
                                                  int32_t SG, const CellBlock* work,
@@ -1648,26 +1689,22 @@ __global__ __launch_bounds__(256) void k_deliver(DevNet* nets, const CellBlock* 
     const uint32_t lane = threadIdx.x & 63;
     // The first D items are delivery wavefronts that share out the list of
     // blocks that spiked in the window (first, so that their chains of
-    // dependent loads start with the launch): an FP32 block is one item
-    // (deliver_fixed); an FP64 block is ceil(Wg / SG) items, one per group
-    // of SG steps to mark, the first also delivering the block's sources.
+    // dependent loads start with the launch): a block is ceil(Wg / SG)
+    // items, one per group of SG steps to mark, the first also delivering
+    // the block's sources.
     if (e < D) {
-        const int groups = P == ASG_FP32 ? 1 : (Wg + SG - 1) / SG;
+        const int groups = (Wg + SG - 1) / SG;
         const int items = *work_n * groups;
         for (int j = e; j < items; j += D) {
             const CellBlock cb = work[j / groups];
             DevNet& n = nets[cb.net];
-            if constexpr (P == ASG_FP32) {
-                deliver_fixed(n, cb.first, lane);
-            } else {
-                const int k0 = (j % groups) * SG;
-                const int k1 = min(k0 + SG, n.win_W);
-                if (k1 > k0) {
-                    mark_steps(n, cb.first, lane, k0, k1);
-                }
-                if (k0 == 0) {
-                    deliver_sources(n, cb.first, lane);
-                }
+            const int k0 = (j % groups) * SG;
+            const int k1 = min(k0 + SG, n.win_W);
+            if (k1 > k0) {
+                mark_steps(n, cb.first, lane, k0, k1);
+            }
+            if (k0 == 0) {
+                deliver_sources(n, cb.first, lane);
             }
         }
         return;
@@ -1680,11 +1717,84 @@ __global__ __launch_bounds__(256) void k_deliver(DevNet* nets, const CellBlock* 
     const int len = min(G, n.win_W - g0);
     const uint32_t t = cb.first + lane;
     if (len > 0 && t < n.n_neur) {
-        if constexpr (P == ASG_FP32) {
-            deliver_sic_f32<G>(n, t, g0, len);
-        } else {
-            deliver_sic<G>(n, t, g0, len);
+        deliver_sic<G>(n, t, g0, len);
+    }
+}
+
+// Window pass 3 of the FP32 networks, with this window's close and the next
+// window's drive, in one launch of one-wavefront workgroups:
+//  * workgroups [0, C): the close, one lane per network: the liveness bits
+//    of the window's SIC exchanges at their arrival steps, counters, the
+//    next window's first step; the other parity's live word and lists are
+//    emptied for the next window (nothing here reads them);
+//  * [C, C + D): the delivery wavefronts, sharing out the list of neuron
+//    blocks that spiked (deliver_fixed);
+//  * [C + D, C + D + E): SIC wavefronts, sharing out the neuron blocks of
+//    the networks the cell pass listed as having a live exchange (a network
+//    whose astrocytes all have a zero SIC factor has nothing to sum);
+//  * [C + D + E, C + D + E + n_nblocks): the drive of the next window, one
+//    wavefront per neuron block -- it depends on nothing this window
+//    computes, and fills the device while the delivery chains wait on
+//    memory.
+// The close writes only what no other workgroup of the launch reads (the
+// live word it reads is this parity's, which nothing writes now), so the
+// workgroups need no ordering among themselves.
+__global__ __launch_bounds__(64) void k_post(DevNet* nets, const CellBlock* nblocks,
+                                             int32_t n_nblocks, int32_t Wg, int32_t C, int32_t D,
+                                             int32_t E, const CellBlock* work,
+                                             const int32_t* work_n, const PostState* post,
+                                             const int32_t* sic_net, const uint32_t* sic_start,
+                                             const int32_t* nbr, const int32_t* active,
+                                             int32_t n_active, int32_t par,
+                                             int32_t* other_work_n, PostState* other_post) {
+    extern __shared__ double drive_lds[];
+    const int e = static_cast<int>(blockIdx.x);
+    const uint32_t lane = threadIdx.x;
+    if (e < C) {
+        const int j = e * 64 + static_cast<int>(lane);
+        if (j < n_active) {
+            DevNet& n = nets[active[j]];
+            if (n.open) {
+                close_window_with(n, n.sic_live_p[par]);
+            }
+            n.sic_live_p[par ^ 1] = 0;
         }
+        if (j == 0) {
+            *other_work_n = 0;
+            other_post->sic_ctr = 0;
+        }
+    } else if (e < C + D) {
+        const int items = *work_n;
+        for (int j = e - C; j < items; j += D) {
+            const CellBlock cb = work[j];
+            deliver_fixed(nets[cb.net], cb.first, lane);
+        }
+    } else if (e < C + D + E) {
+        const uint64_t ctr = post->sic_ctr;
+        const int listed = static_cast<int>(ctr >> 32);
+        const uint32_t items = static_cast<uint32_t>(ctr);
+        for (uint32_t j = static_cast<uint32_t>(e - C - D); j < items; j += static_cast<uint32_t>(E)) {
+            // the listed network whose items hold j: the last with start <= j
+            int lo = 0, hi = listed - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (sic_start[mid] <= j) {
+                    lo = mid;
+                } else {
+                    hi = mid - 1;
+                }
+            }
+            const int net = sic_net[lo];
+            const CellBlock cb = nblocks[nbr[2 * net] + static_cast<int>(j - sic_start[lo])];
+            DevNet& n = nets[net];
+            const uint32_t t = cb.first + lane;
+            if (t < n.n_neur) {
+                deliver_sic_f32<kWindowCap>(n, t, 0, n.win_W, n.sic_live_p[par]);
+            }
+        }
+    } else if (e - C - D - E < n_nblocks) {
+        const CellBlock cb = nblocks[e - C - D - E];
+        drive_block(nets[cb.net], cb, Wg, drive_lds);
     }
 }
 
@@ -1870,21 +1980,6 @@ __global__ __launch_bounds__(256) void k_sum(DevNet* nets, const CellBlock* bloc
     sum_window(n, cb.kind, cb.first + lane, T0, W, k0, k1, wc);
 }
 
-// The window close of FP32 networks (their spikes were delivered in pass 3)
-// that are still open, one wavefront per network. The next window's k_drive
-// closes them otherwise; this runs after a call's last window, and after
-// every window for a batch with FP32 networks that have no neurons.
-__global__ __launch_bounds__(64) void k_close(DevNet* nets, const int32_t* active, int32_t count) {
-    const int j = static_cast<int>(blockIdx.x);
-    if (j >= count) {
-        return;
-    }
-    DevNet& n = nets[active[j]];
-    if (n.open) {
-        close_open_window(n);
-    }
-}
-
 // Element-wise evaluation of the step's FP64 functions (diagnostics).
 // Per-call bookkeeping of a batch, one launch each instead of one copy per
 // network: the new run targets of the active networks, and the raster heads
@@ -1902,7 +1997,8 @@ __global__ __launch_bounds__(256) void k_raster_heads(const DevNet* nets, const 
                                                       int32_t* flags) {
     const int j = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     if (j < n_active) {
-        heads[j] = nets[active[j]].r_head;
+        // an FP32 network's reservations may run past a full buffer
+        heads[j] = min(nets[active[j]].r_head, nets[active[j]].r_cap);
         flags[j] = nets[active[j]].overflow;
     }
 }
@@ -2432,11 +2528,22 @@ struct Engine {
     CellBlock* d_nblocks = nullptr;   // neuron blocks only
     int32_t* d_active = nullptr;
     // The window's delivery lists: neuron blocks that spiked, appended by
-    // k_cells, the FP64 networks' from entry 0 and the FP32 networks' from
-    // entry nblock_range[1], counted in d_work_n[0] and d_work_n[1] (zeroed
-    // by k_drive).
+    // k_cells: the FP64 networks' from entry 0, counted in d_work_n[0]
+    // (zeroed by the FP64 networks' k_drive); the FP32 networks' in two
+    // lists by window parity p, from entry nblock_range[1] + p * (FP32
+    // neuron blocks), counted in d_work_n[1 + p] (zeroed by the post pass of
+    // the window before).
     CellBlock* d_work = nullptr;
     int32_t* d_work_n = nullptr;
+    // The FP32 post pass's per-window state and its list of networks with a
+    // live SIC exchange (by parity: 2 x max_nets entries), and per slot the
+    // first of its neuron blocks in the FP32 range of d_nblocks and their
+    // count.
+    PostState* d_post = nullptr;
+    int32_t* d_sic_net = nullptr;
+    uint32_t* d_sic_start = nullptr;
+    int32_t* d_nbr = nullptr;
+    std::vector<int32_t> h_nbr;
     size_t blocks_cap = 0, nblocks_cap = 0, active_cap = 0, work_cap = 0;
     int n_blocks = 0, n_nblocks = 0, n_active = 0, Wg = 1;
     std::vector<int32_t> h_active;     // host copy of d_active
@@ -2469,9 +2576,13 @@ struct Engine {
     bool use_graphs = false;
     struct GraphEntry {
         int64_t windows;
-        int Wg, G, wpb, SG;
+        int Wg, G, wpb, SG, parity;  // parity of the first window
         hipGraphExec_t exec;
     };
+    // Windows launched so far: FP32 windows alternate their lists by the
+    // parity of this count, across calls (a graph replays the parities it
+    // was captured with, so graphs are kept per starting parity).
+    int64_t win_seq = 0;
     std::vector<GraphEntry> graphs;
     bool pending = false;  // async work in flight
     // Per-kernel event timing (off unless asked for).
@@ -2489,7 +2600,6 @@ struct Engine {
     // (active slots), then the FP32's.
     int nblock_range[3] = {0, 0, 0};
     int active_range[3] = {0, 0, 0};
-    bool fp32_without_neurons = false;  // some FP32 network has no neuron blocks
 
     explicit Engine(int max_n, int64_t budget_bytes) : max_nets(max_n), budget(budget_bytes) {
         HIP_CHECK(hipGetDevice(&device));
@@ -2505,6 +2615,19 @@ struct Engine {
         HIP_CHECK(hipMalloc(&d_heads, sizeof(int64_t) * m));
         HIP_CHECK(hipMalloc(&d_flags, sizeof(int32_t) * m));
         HIP_CHECK(hipMalloc(&d_offsets, sizeof(int64_t) * m));
+        // The delivery list counts and the post pass's state exist for every
+        // batch: k_post empties them even when no network has neurons.
+        HIP_CHECK(hipMalloc(&d_work_n, 3 * sizeof(int32_t)));
+        HIP_CHECK(hipMemsetAsync(d_work_n, 0, 3 * sizeof(int32_t), stream));
+        HIP_CHECK(hipMalloc(&d_post, 2 * sizeof(PostState)));
+        HIP_CHECK(hipMemsetAsync(d_post, 0, 2 * sizeof(PostState), stream));
+        HIP_CHECK(hipMalloc(&d_sic_net, 2 * sizeof(int32_t) * m));
+        HIP_CHECK(hipMalloc(&d_sic_start, 2 * sizeof(uint32_t) * m));
+        HIP_CHECK(hipMalloc(&d_nbr, 2 * sizeof(int32_t) * m));
+        h_nbr.assign(2 * m, 0);
+        HIP_CHECK(hipMemcpyAsync(d_nbr, h_nbr.data(), 2 * sizeof(int32_t) * m,
+                                 hipMemcpyHostToDevice, stream));
+        HIP_CHECK(hipStreamSynchronize(stream));
     }
 
     ~Engine() {
@@ -2522,6 +2645,10 @@ struct Engine {
         (void)hipFree(d_heads);
         (void)hipFree(d_flags);
         (void)hipFree(d_offsets);
+        (void)hipFree(d_post);
+        (void)hipFree(d_sic_net);
+        (void)hipFree(d_sic_start);
+        (void)hipFree(d_nbr);
         (void)hipFree(d_stage_step);
         (void)hipFree(d_stage_src);
         (void)hipHostFree(h_targets);
@@ -2595,8 +2722,8 @@ struct Engine {
     void rebuild_tables() {
         std::vector<CellBlock> blocks, nblocks;
         std::vector<int32_t> active;
-        fp32_without_neurons = false;
         Wg = kWindowCap;
+        std::fill(h_nbr.begin(), h_nbr.end(), 0);
         // Blocks in two contiguous ranges, one per cell kernel: the FP64
         // networks' blocks, then the FP32 networks'. Within a network the
         // neuron blocks come first; its first block is its lead block.
@@ -2605,8 +2732,12 @@ struct Engine {
                 const NetSlot& s = slots[sidx];
                 if (!s.used || (s.dn.precision == ASG_FP32) != (range == 1)) continue;
                 active.push_back(sidx);
-                if (range == 1 && s.dn.n_neur == 0) fp32_without_neurons = true;
                 Wg = std::min(Wg, s.W);
+                if (range == 1) {
+                    h_nbr[2 * sidx] = static_cast<int32_t>(nblocks.size()) - nblock_range[1];
+                    h_nbr[2 * sidx + 1] =
+                        static_cast<int32_t>((s.dn.n_neur + kCellBlock - 1) / kCellBlock);
+                }
                 uint32_t lead = 1;
                 for (uint32_t f = 0; f < s.dn.n_neur; f += kCellBlock) {
                     blocks.push_back(CellBlock{sidx, 0, f, lead});
@@ -2635,18 +2766,17 @@ struct Engine {
             grow(d_nblocks, nblocks_cap, nblocks.size());
             HIP_CHECK(hipMemcpyAsync(d_nblocks, nblocks.data(), nblocks.size() * sizeof(CellBlock),
                                      hipMemcpyHostToDevice, stream));
-            // each FP32 neuron block is listed at most once per window
-            grow(d_work, work_cap, nblocks.size());
-            if (d_work_n == nullptr) {
-                HIP_CHECK(hipMalloc(&d_work_n, 2 * sizeof(int32_t)));
-                HIP_CHECK(hipMemsetAsync(d_work_n, 0, 2 * sizeof(int32_t), stream));
-            }
+            // each neuron block is listed at most once per window (FP32: in
+            // the list of the window's parity)
+            grow(d_work, work_cap, nblocks.size() + (nblocks.size() - nblock_range[1]));
         }
         if (n_active > 0) {
             grow(d_active, active_cap, active.size());
             HIP_CHECK(hipMemcpyAsync(d_active, active.data(), active.size() * sizeof(int32_t),
                                      hipMemcpyHostToDevice, stream));
         }
+        HIP_CHECK(hipMemcpyAsync(d_nbr, h_nbr.data(), h_nbr.size() * sizeof(int32_t),
+                                 hipMemcpyHostToDevice, stream));
         HIP_CHECK(hipStreamSynchronize(stream));
         dirty = false;
         clear_graphs();
@@ -2685,30 +2815,23 @@ struct Engine {
         dirty = true;
     }
 
-    // k_deliver for SIC group size G (its accumulators are registers).
-    template <int G, int P>
-    void launch_deliver_range(int range, int32_t Wg_, int MG) {
-        const int count = nblock_range[range + 1] - nblock_range[range];
-        if (count == 0) return;
-        // delivery wavefronts (before the SIC ones) over the listed blocks
-        const int groups = P == ASG_FP32 ? 1 : (Wg_ + MG - 1) / MG;
-        const int D = std::min(count * groups, kDeliverWaves);
-        const int items = count * ((Wg_ + G - 1) / G) + D;
-        hipLaunchKernelGGL((k_deliver<G, P>), dim3((items + 3) / 4), dim3(256), 0, stream, d_nets,
-                           d_nblocks + nblock_range[range], items, Wg_, D, MG,
-                           d_work + nblock_range[range], d_work_n + range);
-    }
-
+    // k_deliver of the FP64 networks for SIC group size G (its accumulators
+    // are registers).
     template <int G>
     void launch_deliver_g(int32_t Wg_, int MG) {
-        launch_deliver_range<G, ASG_FP64>(0, Wg_, MG);
-        launch_deliver_range<G, ASG_FP32>(1, Wg_, 1);
+        const int count = nblock_range[1];
+        if (count == 0) return;
+        // delivery wavefronts (before the SIC ones) over the listed blocks
+        const int groups = (Wg_ + MG - 1) / MG;
+        const int D = std::min(count * groups, kDeliverWaves);
+        const int items = count * ((Wg_ + G - 1) / G) + D;
+        hipLaunchKernelGGL(k_deliver<G>, dim3((items + 3) / 4), dim3(256), 0, stream, d_nets,
+                           d_nblocks, items, Wg_, D, MG, d_work, d_work_n);
     }
 
-    // Pass 3: delivery of the blocks that spiked (FP64 networks' marking in
-    // groups of MG steps; FP32 networks' fixed-point delivery) and the SIC
+    // Pass 3 of the FP64 networks: marking in groups of MG steps and the SIC
     // sums of every block.
-    void launch_deliver(int G, int32_t Wg_, int MG) {
+    void launch_deliver64(int G, int32_t Wg_, int MG) {
         switch (G) {
             case 1: launch_deliver_g<1>(Wg_, MG); break;
             case 2: launch_deliver_g<2>(Wg_, MG); break;
@@ -2718,6 +2841,44 @@ struct Engine {
         }
     }
 
+    size_t drive_lds_bytes() const {
+        return static_cast<size_t>(64 * Wg) * (sizeof(double) + sizeof(uint32_t));
+    }
+
+    // The drive of the next window of the networks of precision range
+    // `range` as a launch of its own; the FP64 range also empties its
+    // delivery list count.
+    void launch_drive(int range) {
+        const int count = nblock_range[range + 1] - nblock_range[range];
+        if (count == 0) return;
+        hipLaunchKernelGGL(k_drive, dim3(count), dim3(64), drive_lds_bytes(), stream, d_nets,
+                           d_nblocks + nblock_range[range], count, Wg,
+                           range == 0 ? d_work_n : nullptr);
+    }
+
+    // The FP32 delivery list, its count and the post pass's state of window
+    // parity p.
+    CellBlock* work32(int p) const {
+        return d_work + nblock_range[1] + p * (nblock_range[2] - nblock_range[1]);
+    }
+    int32_t* work_n32(int p) const { return d_work_n + 1 + p; }
+
+    // Pass 3 of the FP32 networks with the window's close and the next
+    // window's drive (window parity p).
+    void launch_post(int p) {
+        const int n32 = active_range[2] - active_range[1];
+        if (n32 == 0) return;
+        const int count = nblock_range[2] - nblock_range[1];
+        const int C = (n32 + 63) / 64;
+        const int D = std::min(count, kDeliverWaves);
+        const int E = std::min(count, kSicWorkers);
+        hipLaunchKernelGGL(k_post, dim3(C + D + E + count), dim3(64), drive_lds_bytes(), stream,
+                           d_nets, d_nblocks + nblock_range[1], count, Wg, C, D, E, work32(p),
+                           work_n32(p), d_post + p, d_sic_net + p * max_nets,
+                           d_sic_start + p * max_nets, d_nbr, d_active + active_range[1], n32, p,
+                           work_n32(p ^ 1), d_post + (p ^ 1));
+    }
+
     // Window steps per marking / summing wavefront: as requested, or (0) the
     // whole window once per-step wavefronts would number beyond kGroupWaves.
     int step_group_for(int requested) const {
@@ -2725,24 +2886,33 @@ struct Engine {
         return static_cast<int64_t>(n_blocks) * Wg > kGroupWaves ? Wg : 1;
     }
 
+    // The cell pass of precision range `range` (FP32: window parity p).
     template <int P>
-    void launch_cells(int range, int32_t Wg_, int wpb) {
+    void launch_cells(int range, int32_t Wg_, int wpb, int p) {
         const int count = cell_range[range + 1] - cell_range[range];
         if (count == 0) return;
         const CellBlock* blocks = d_blocks + cell_range[range];
         const dim3 grid((count + wpb - 1) / wpb);
+        CellBlock* work = P == ASG_FP32 ? work32(p) : d_work;
+        int32_t* work_n = P == ASG_FP32 ? work_n32(p) : d_work_n;
+        PostState* post = d_post + p;
+        int32_t* sic_net = d_sic_net + p * max_nets;
+        uint32_t* sic_start = d_sic_start + p * max_nets;
         switch (wpb) {
             case 4:
                 hipLaunchKernelGGL((k_cells<P, 4>), grid, dim3(4 * kCellBlock), 0, stream, d_nets,
-                                   blocks, count, Wg_, d_work, d_work_n, nblock_range[1]);
+                                   blocks, count, Wg_, work, work_n, p, post, sic_net, sic_start,
+                                   d_nbr);
                 break;
             case 2:
                 hipLaunchKernelGGL((k_cells<P, 2>), grid, dim3(2 * kCellBlock), 0, stream, d_nets,
-                                   blocks, count, Wg_, d_work, d_work_n, nblock_range[1]);
+                                   blocks, count, Wg_, work, work_n, p, post, sic_net, sic_start,
+                                   d_nbr);
                 break;
             default:
                 hipLaunchKernelGGL((k_cells<P, 1>), grid, dim3(kCellBlock), 0, stream, d_nets,
-                                   blocks, count, Wg_, d_work, d_work_n, nblock_range[1]);
+                                   blocks, count, Wg_, work, work_n, p, post, sic_net, sic_start,
+                                   d_nbr);
                 break;
         }
     }
@@ -2761,66 +2931,60 @@ struct Engine {
     }
 
     // The launches of `windows` windows on the stream (eager, or recorded
-    // into a graph when the stream is capturing).
-    // Launches `windows` windows; with `marks`, one window in prof_every is
-    // bracketed by events (6 per timed window). Returns the windows timed.
+    // into a graph when the stream is capturing); with `marks`, one window
+    // in prof_every is bracketed by events (6 per timed window). Returns the
+    // windows timed.
+    //
+    // FP64 networks: drive, cells, compaction, delivery (marking + SIC),
+    // ordered sums (whose lead wavefront closes the window). FP32 networks:
+    // cells, then k_post (delivery, SIC of the live networks, the next
+    // window's drive, close); the first window's drive is launched before
+    // the loop.
     int64_t launch_windows(int64_t windows, int G, int wpb, int SG, bool marks) {
-
         int64_t timed = 0;
+        if (windows > 0) {
+            launch_drive(1);
+        }
+        const int n64 = active_range[1];
         for (int64_t wdx = 0; wdx < windows; ++wdx) {
+            const int p = static_cast<int>(win_seq++ & 1);  // the window's parity
             const bool timed_here = marks && (prof_seq++ % prof_every) == 0;
             auto mark = [&](int k) {
                 if (timed_here) HIP_CHECK(hipEventRecord(events[6 * timed + k], stream));
             };
             mark(0);
-            if (n_nblocks > 0) {
-                hipLaunchKernelGGL(k_drive, dim3(n_nblocks), dim3(64),
-                                   static_cast<size_t>(64 * Wg) * (sizeof(double) + sizeof(uint32_t)),
-                                   stream, d_nets, d_nblocks, n_nblocks, Wg, d_work_n);
-            }
+            launch_drive(0);
             mark(1);
-            launch_cells<ASG_FP64>(0, Wg, wpb);
-            launch_cells<ASG_FP32>(1, Wg, wpb);
+            launch_cells<ASG_FP64>(0, Wg, wpb, p);
+            launch_cells<ASG_FP32>(1, Wg, wpb, p);
             mark(2);
-            hipLaunchKernelGGL(k_compact, dim3(n_active, Wg), dim3(64), 0, stream, d_nets,
-                               d_active);
-            mark(3);
-            if (n_nblocks > 0) {
-                launch_deliver(G, Wg, SG);
+            if (n64 > 0) {
+                hipLaunchKernelGGL(k_compact, dim3(n64, Wg), dim3(64), 0, stream, d_nets,
+                                   d_active);
             }
+            mark(3);
+            launch_deliver64(G, Wg, SG);
+            launch_post(p);
             mark(4);
-            // FP64 networks: ordered sums and window close. FP32 networks
-            // (delivered in pass 3) are closed by the next window's k_drive,
-            // or here when some have no neurons (no drive wavefront).
             const int sum_items = cell_range[1] * ((Wg + SG - 1) / SG);
             if (sum_items > 0) {
                 hipLaunchKernelGGL(k_sum, dim3((sum_items + 3) / 4), dim3(256), 0, stream, d_nets,
                                    d_blocks, sum_items, Wg, SG);
             }
-            if (fp32_without_neurons) {
-                launch_close();
-            }
             mark(5);
             if (timed_here) ++timed;
         }
-        launch_close();  // the last window's
         return timed;
-    }
-
-    // Closes every FP32 network's open window (k_close skips closed ones).
-    void launch_close() {
-        const int n_close = active_range[2] - active_range[1];
-        if (n_close > 0) {
-            hipLaunchKernelGGL(k_close, dim3(n_close), dim3(64), 0, stream, d_nets,
-                               d_active + active_range[1], n_close);
-        }
     }
 
     // Graph of `windows` windows for the current tables and schedule,
     // captured on first use.
     hipGraphExec_t graph_for(int64_t windows, int G, int wpb, int SG) {
+        const int parity = static_cast<int>(win_seq & 1);
         for (const GraphEntry& g : graphs) {
-            if (g.windows == windows && g.G == G && g.wpb == wpb && g.SG == SG && g.Wg == Wg) {
+            if (g.windows == windows && g.G == G && g.wpb == wpb && g.SG == SG && g.Wg == Wg &&
+                g.parity == parity) {
+                win_seq += windows;  // what launching them would have counted
                 return g.exec;
             }
         }
@@ -2832,7 +2996,7 @@ struct Engine {
         const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         HIP_CHECK(e);
-        graphs.push_back(GraphEntry{windows, Wg, G, wpb, SG, exec});
+        graphs.push_back(GraphEntry{windows, Wg, G, wpb, SG, parity, exec});
         return exec;
     }
 
@@ -3009,8 +3173,50 @@ struct Engine {
             NetSlot& s = slots[h_active[j]];
             const int64_t* ps = h_stage_step + h_offsets[j];
             const uint32_t* pr = h_stage_src + h_offsets[j];
+            const size_t at = s.q_step.size();
             s.q_step.insert(s.q_step.end(), ps, ps + nr);
             s.q_src.insert(s.q_src.end(), pr, pr + nr);
+            if (s.dn.precision == ASG_FP32) {
+                // FP32 delivery appends a window's entries in the order its
+                // wavefronts reserve them: restore the reference's order,
+                // step then source. Windows arrive in order, so entries are
+                // out of place only within a window: an insertion sort costs
+                // about one comparison per entry. Windows with many spikes
+                // make it quadratic; past a budget of moves the range is
+                // sorted outright instead.
+                sort_raster(s.q_step, s.q_src, at);
+            }
+        }
+    }
+
+    // Orders q_step/q_src[at ..] by step, then source.
+    static void sort_raster(std::vector<int64_t>& q_step, std::vector<uint32_t>& q_src, size_t at) {
+        int64_t* qs = q_step.data();
+        uint32_t* qr = q_src.data();
+        const size_t end = q_step.size();
+        size_t budget = 16 * (end - at) + 1024;  // element moves before giving up
+        for (size_t a = at + 1; a < end; ++a) {
+            const int64_t ks = qs[a];
+            const uint32_t kr = qr[a];
+            size_t b = a;
+            while (b > at && (qs[b - 1] > ks || (qs[b - 1] == ks && qr[b - 1] > kr))) {
+                qs[b] = qs[b - 1];
+                qr[b] = qr[b - 1];
+                --b;
+            }
+            qs[b] = ks;
+            qr[b] = kr;
+            budget -= std::min(budget, a - b);
+            if (budget == 0) {
+                std::vector<std::pair<int64_t, uint32_t>> e(end - at);
+                for (size_t q = at; q < end; ++q) e[q - at] = {qs[q], qr[q]};
+                std::sort(e.begin(), e.end());
+                for (size_t q = at; q < end; ++q) {
+                    qs[q] = e[q - at].first;
+                    qr[q] = e[q - at].second;
+                }
+                return;
+            }
         }
     }
 
