@@ -120,7 +120,7 @@ constexpr double kFixUnit = 1.0 / 4294967296.0;   // 2^-32
 constexpr int64_t kSicWaves = 4096; // SIC wavefronts that fill the device (256 CUs x 16)
 constexpr int64_t kGroupWaves = 16384;  // per-step wavefronts beyond which steps are grouped
 constexpr int kDeliverWaves = 512;  // delivery wavefronts sharing out the spiking blocks
-constexpr int kSicWorkers = 2048;   // FP32 SIC wavefronts sharing out the live networks' blocks
+constexpr int kSicWorkers = 0;      // FP32 SIC wavefronts (0: one per FP32 neuron block)
 constexpr size_t kMaskBudget = size_t(256) << 20;  // delivery masks per network
 constexpr size_t kMinChunks = 64;
 
@@ -295,10 +295,12 @@ __device__ inline bool mark_sic_live(uint32_t* live_word, uint32_t bits, int W) 
 }
 
 // Per-window state of the FP32 post pass (k_post), one per window parity:
-// the networks whose SIC exchange is live in the window, as the cell pass
-// lists them (emptied by the post pass of the window before).
+// the count of SIC items -- neuron blocks of networks whose SIC exchange is
+// live in the window -- the cell pass listed (emptied by the post pass of
+// the window before).
 struct PostState {
-    uint64_t sic_ctr;  // (networks listed << 32) | SIC items (neuron blocks) listed
+    uint32_t sic_items;
+    uint32_t pad;
 };
 
 struct CellBlock {
@@ -1115,19 +1117,18 @@ __global__ __launch_bounds__(64) void k_drive(DevNet* nets, const CellBlock* nbl
 // WPB wavefronts (64-cell blocks) per workgroup, a launch-schedule knob.
 // A neuron block that spiked in the window appends itself to its
 // precision's delivery list `work`, counted in *work_n. An FP32 network
-// whose SIC exchange becomes live in the window is listed for the post
-// pass: its slot in sic_net, the first of its SIC items (its neuron blocks,
-// nbr[2 * slot + 1] of them) in sic_start. FP32 windows alternate between
-// two sets of these lists and live words by parity `par` (the host passes
-// the set of the window's parity), so that the post pass can empty the
-// other set while it reads this one.
+// whose SIC exchange becomes live in the window lists its neuron blocks
+// (nbr[2 * slot + 1] of them from index nbr[2 * slot] of the FP32 neuron
+// blocks) as the post pass's SIC items in sic_item. FP32 windows alternate
+// between two sets of these lists and live words by parity `par` (the host
+// passes the set of the window's parity), so that the post pass can empty
+// the other set while it reads this one.
 template <int P, int WPB>
 __global__ __launch_bounds__(kCellBlock * WPB) void k_cells(DevNet* nets, const CellBlock* blocks,
                                                             int32_t count, int32_t Wg,
                                                             CellBlock* work, int32_t* work_n,
                                                             int32_t par, PostState* post,
-                                                            int32_t* sic_net, uint32_t* sic_start,
-                                                            const int32_t* nbr) {
+                                                            int32_t* sic_item, const int32_t* nbr) {
     // The wavefront's block index is uniform across its lanes: said so
     // explicitly, so the block and network descriptors stay in scalar
     // registers.
@@ -1155,11 +1156,21 @@ __global__ __launch_bounds__(kCellBlock * WPB) void k_cells(DevNet* nets, const 
             if (neuron_window_f32(n, i, T0, W) != 0 && lane == 0) {
                 work[atomicAdd(work_n, 1)] = cb;
             }
-        } else if (astro_window_f32(n, i, T0, W, par)) {
-            const uint64_t old = atomicAdd(reinterpret_cast<unsigned long long*>(&post->sic_ctr),
-                                           (1ULL << 32) | static_cast<uint32_t>(nbr[2 * cb.net + 1]));
-            sic_net[old >> 32] = cb.net;
-            sic_start[old >> 32] = static_cast<uint32_t>(old);
+        } else {
+            // the wavefront whose update made the exchange live lists the
+            // network's blocks, its lanes together
+            const uint64_t made = __ballot(astro_window_f32(n, i, T0, W, par));
+            if (made != 0) {
+                const int32_t nb = nbr[2 * cb.net + 1], first = nbr[2 * cb.net];
+                uint32_t at = 0;
+                if (lane == static_cast<uint32_t>(__ffsll(static_cast<long long>(made)) - 1)) {
+                    at = atomicAdd(&post->sic_items, static_cast<uint32_t>(nb));
+                }
+                at = __builtin_amdgcn_readlane(at, __ffsll(static_cast<long long>(made)) - 1);
+                for (int32_t q = static_cast<int32_t>(lane); q < nb; q += 64) {
+                    sic_item[at + q] = first + q;
+                }
+            }
         }
     } else {
         if (cb.kind == 0) {
@@ -1743,8 +1754,7 @@ __global__ __launch_bounds__(64) void k_post(DevNet* nets, const CellBlock* nblo
                                              int32_t n_nblocks, int32_t Wg, int32_t C, int32_t D,
                                              int32_t E, const CellBlock* work,
                                              const int32_t* work_n, const PostState* post,
-                                             const int32_t* sic_net, const uint32_t* sic_start,
-                                             const int32_t* nbr, const int32_t* active,
+                                             const int32_t* sic_item, const int32_t* active,
                                              int32_t n_active, int32_t par,
                                              int32_t* other_work_n, PostState* other_post) {
     extern __shared__ double drive_lds[];
@@ -1761,7 +1771,7 @@ __global__ __launch_bounds__(64) void k_post(DevNet* nets, const CellBlock* nblo
         }
         if (j == 0) {
             *other_work_n = 0;
-            other_post->sic_ctr = 0;
+            other_post->sic_items = 0;
         }
     } else if (e < C + D) {
         const int items = *work_n;
@@ -1770,23 +1780,10 @@ __global__ __launch_bounds__(64) void k_post(DevNet* nets, const CellBlock* nblo
             deliver_fixed(nets[cb.net], cb.first, lane);
         }
     } else if (e < C + D + E) {
-        const uint64_t ctr = post->sic_ctr;
-        const int listed = static_cast<int>(ctr >> 32);
-        const uint32_t items = static_cast<uint32_t>(ctr);
+        const uint32_t items = post->sic_items;
         for (uint32_t j = static_cast<uint32_t>(e - C - D); j < items; j += static_cast<uint32_t>(E)) {
-            // the listed network whose items hold j: the last with start <= j
-            int lo = 0, hi = listed - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (sic_start[mid] <= j) {
-                    lo = mid;
-                } else {
-                    hi = mid - 1;
-                }
-            }
-            const int net = sic_net[lo];
-            const CellBlock cb = nblocks[nbr[2 * net] + static_cast<int>(j - sic_start[lo])];
-            DevNet& n = nets[net];
+            const CellBlock cb = nblocks[sic_item[j]];
+            DevNet& n = nets[cb.net];
             const uint32_t t = cb.first + lane;
             if (t < n.n_neur) {
                 deliver_sic_f32<kWindowCap>(n, t, 0, n.win_W, n.sic_live_p[par]);
@@ -2535,13 +2532,13 @@ struct Engine {
     // the window before).
     CellBlock* d_work = nullptr;
     int32_t* d_work_n = nullptr;
-    // The FP32 post pass's per-window state and its list of networks with a
-    // live SIC exchange (by parity: 2 x max_nets entries), and per slot the
-    // first of its neuron blocks in the FP32 range of d_nblocks and their
-    // count.
+    // The FP32 post pass's per-window state and its SIC items, the neuron
+    // blocks of the networks with a live SIC exchange (by parity: two lists
+    // of the FP32 neuron blocks' count), and per slot the first of its
+    // neuron blocks in the FP32 range of d_nblocks and their count.
     PostState* d_post = nullptr;
-    int32_t* d_sic_net = nullptr;
-    uint32_t* d_sic_start = nullptr;
+    int32_t* d_sic_item = nullptr;
+    size_t sic_item_cap = 0;
     int32_t* d_nbr = nullptr;
     std::vector<int32_t> h_nbr;
     size_t blocks_cap = 0, nblocks_cap = 0, active_cap = 0, work_cap = 0;
@@ -2570,6 +2567,8 @@ struct Engine {
     int sched_cell_waves = 1;
     int sched_sic_group = 0;
     int sched_step_group = 0;
+    int sched_deliver_waves = kDeliverWaves;
+    int sched_sic_workers = kSicWorkers;
     float tuned_ms_per_window = 0.0f;   // the autotuner's best median
     // Window sequences replayed from captured graphs (when use_graphs and
     // not profiling); cleared whenever the tables change.
@@ -2621,8 +2620,6 @@ struct Engine {
         HIP_CHECK(hipMemsetAsync(d_work_n, 0, 3 * sizeof(int32_t), stream));
         HIP_CHECK(hipMalloc(&d_post, 2 * sizeof(PostState)));
         HIP_CHECK(hipMemsetAsync(d_post, 0, 2 * sizeof(PostState), stream));
-        HIP_CHECK(hipMalloc(&d_sic_net, 2 * sizeof(int32_t) * m));
-        HIP_CHECK(hipMalloc(&d_sic_start, 2 * sizeof(uint32_t) * m));
         HIP_CHECK(hipMalloc(&d_nbr, 2 * sizeof(int32_t) * m));
         h_nbr.assign(2 * m, 0);
         HIP_CHECK(hipMemcpyAsync(d_nbr, h_nbr.data(), 2 * sizeof(int32_t) * m,
@@ -2646,8 +2643,7 @@ struct Engine {
         (void)hipFree(d_flags);
         (void)hipFree(d_offsets);
         (void)hipFree(d_post);
-        (void)hipFree(d_sic_net);
-        (void)hipFree(d_sic_start);
+        (void)hipFree(d_sic_item);
         (void)hipFree(d_nbr);
         (void)hipFree(d_stage_step);
         (void)hipFree(d_stage_src);
@@ -2767,8 +2763,9 @@ struct Engine {
             HIP_CHECK(hipMemcpyAsync(d_nblocks, nblocks.data(), nblocks.size() * sizeof(CellBlock),
                                      hipMemcpyHostToDevice, stream));
             // each neuron block is listed at most once per window (FP32: in
-            // the list of the window's parity)
+            // the lists of the window's parity)
             grow(d_work, work_cap, nblocks.size() + (nblocks.size() - nblock_range[1]));
+            grow(d_sic_item, sic_item_cap, 2 * (nblocks.size() - nblock_range[1]));
         }
         if (n_active > 0) {
             grow(d_active, active_cap, active.size());
@@ -2823,7 +2820,7 @@ struct Engine {
         if (count == 0) return;
         // delivery wavefronts (before the SIC ones) over the listed blocks
         const int groups = (Wg_ + MG - 1) / MG;
-        const int D = std::min(count * groups, kDeliverWaves);
+        const int D = std::min(count * groups, sched_deliver_waves);
         const int items = count * ((Wg_ + G - 1) / G) + D;
         hipLaunchKernelGGL(k_deliver<G>, dim3((items + 3) / 4), dim3(256), 0, stream, d_nets,
                            d_nblocks, items, Wg_, D, MG, d_work, d_work_n);
@@ -2862,6 +2859,9 @@ struct Engine {
         return d_work + nblock_range[1] + p * (nblock_range[2] - nblock_range[1]);
     }
     int32_t* work_n32(int p) const { return d_work_n + 1 + p; }
+    int32_t* sic_items32(int p) const {
+        return d_sic_item + p * (nblock_range[2] - nblock_range[1]);
+    }
 
     // Pass 3 of the FP32 networks with the window's close and the next
     // window's drive (window parity p).
@@ -2870,13 +2870,15 @@ struct Engine {
         if (n32 == 0) return;
         const int count = nblock_range[2] - nblock_range[1];
         const int C = (n32 + 63) / 64;
-        const int D = std::min(count, kDeliverWaves);
-        const int E = std::min(count, kSicWorkers);
+        const int D = std::min(count, sched_deliver_waves);
+        // SIC wavefronts: by default one per neuron block, so that every
+        // live block has one however many networks are live (those beyond
+        // the window's items exit at once)
+        const int E = sched_sic_workers > 0 ? std::min(count, sched_sic_workers) : count;
         hipLaunchKernelGGL(k_post, dim3(C + D + E + count), dim3(64), drive_lds_bytes(), stream,
                            d_nets, d_nblocks + nblock_range[1], count, Wg, C, D, E, work32(p),
-                           work_n32(p), d_post + p, d_sic_net + p * max_nets,
-                           d_sic_start + p * max_nets, d_nbr, d_active + active_range[1], n32, p,
-                           work_n32(p ^ 1), d_post + (p ^ 1));
+                           work_n32(p), d_post + p, sic_items32(p), d_active + active_range[1],
+                           n32, p, work_n32(p ^ 1), d_post + (p ^ 1));
     }
 
     // Window steps per marking / summing wavefront: as requested, or (0) the
@@ -2896,23 +2898,19 @@ struct Engine {
         CellBlock* work = P == ASG_FP32 ? work32(p) : d_work;
         int32_t* work_n = P == ASG_FP32 ? work_n32(p) : d_work_n;
         PostState* post = d_post + p;
-        int32_t* sic_net = d_sic_net + p * max_nets;
-        uint32_t* sic_start = d_sic_start + p * max_nets;
+        int32_t* sic_item = sic_items32(p);
         switch (wpb) {
             case 4:
                 hipLaunchKernelGGL((k_cells<P, 4>), grid, dim3(4 * kCellBlock), 0, stream, d_nets,
-                                   blocks, count, Wg_, work, work_n, p, post, sic_net, sic_start,
-                                   d_nbr);
+                                   blocks, count, Wg_, work, work_n, p, post, sic_item, d_nbr);
                 break;
             case 2:
                 hipLaunchKernelGGL((k_cells<P, 2>), grid, dim3(2 * kCellBlock), 0, stream, d_nets,
-                                   blocks, count, Wg_, work, work_n, p, post, sic_net, sic_start,
-                                   d_nbr);
+                                   blocks, count, Wg_, work, work_n, p, post, sic_item, d_nbr);
                 break;
             default:
                 hipLaunchKernelGGL((k_cells<P, 1>), grid, dim3(kCellBlock), 0, stream, d_nets,
-                                   blocks, count, Wg_, work, work_n, p, post, sic_net, sic_start,
-                                   d_nbr);
+                                   blocks, count, Wg_, work, work_n, p, post, sic_item, d_nbr);
                 break;
         }
     }
@@ -3044,61 +3042,45 @@ struct Engine {
         if (!async) wait();
     }
 
-    // Autotuner: every candidate schedule (cell wavefronts per workgroup x
-    // SIC group) runs `wpt` windows of the live batch, `rounds` times in
-    // round-robin order so slow drifts in activity hit all alike; the one
-    // with the smallest median time is kept. The schedule changes launch
-    // geometry only, never results, so tuning advances the simulation like
-    // any other call (by rounds x candidates x wpt windows).
-    void autotune(int wpt, int rounds) {
-        wait();
-        if (dirty) rebuild_tables();
-        if (n_active == 0 || wpt <= 0 || rounds <= 0) return;
-        int top = 1;
-        while (top < Wg) top *= 2;
-        struct Cand {
-            int wpb, g, sg;
-        };
-        std::vector<Cand> cand;
-        for (int wpb : {1, 2, 4}) {
-            for (int g = 1; g <= top; g *= 2) {
-                cand.push_back({wpb, g, 1});
-                if (Wg > 1) cand.push_back({wpb, g, Wg});
-            }
-        }
+    // Autotuner: one knob at a time -- cell wavefronts per workgroup; then,
+    // with FP64 networks, their SIC group and step group; then, with FP32
+    // networks, the delivery wavefront count -- each candidate run
+    // as step calls of `wpt` windows, `rounds` times round-robin so slow
+    // drifts in activity hit all alike, keeping the smallest median. The
+    // schedule changes launch geometry only, never results, so tuning
+    // advances the simulation like any other call.
+    struct Sched {
+        int wpb, g, sg, dw, sw;
+    };
+    Sched current_schedule() const {
+        return Sched{sched_cell_waves, sched_sic_group, sched_step_group, sched_deliver_waves,
+                     sched_sic_workers};
+    }
+    void apply_schedule(const Sched& c) {
+        sched_cell_waves = c.wpb;
+        sched_sic_group = c.g;
+        sched_step_group = c.sg;
+        sched_deliver_waves = c.dw;
+        sched_sic_workers = c.sw;
+    }
+
+    // Times every candidate; applies and returns the fastest (median ms per
+    // call of wpt windows).
+    float tune_among(const std::vector<Sched>& cand, int wpt, int rounds, hipEvent_t e0,
+                     hipEvent_t e1) {
         std::vector<std::vector<float>> times(cand.size());
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        HIP_CHECK(hipEventCreate(&e0));
-        HIP_CHECK(hipEventCreate(&e1));
-        const bool prof = profiling, graphs_on = use_graphs;
-        profiling = false;
-        use_graphs = false;
-        try {
-            for (int r = 0; r < rounds; ++r) {
-                for (size_t c = 0; c < cand.size(); ++c) {
-                    sched_cell_waves = cand[c].wpb;
-                    sched_sic_group = cand[c].g;
-                    sched_step_group = cand[c].sg;
-                    HIP_CHECK(hipEventRecord(e0, stream));
-                    step(static_cast<int64_t>(wpt) * Wg, true);
-                    HIP_CHECK(hipEventRecord(e1, stream));
-                    wait();
-                    float ms = 0.0f;
-                    HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-                    times[c].push_back(ms);
-                }
+        for (int r = 0; r < rounds; ++r) {
+            for (size_t c = 0; c < cand.size(); ++c) {
+                apply_schedule(cand[c]);
+                HIP_CHECK(hipEventRecord(e0, stream));
+                step(static_cast<int64_t>(wpt) * Wg, true);
+                HIP_CHECK(hipEventRecord(e1, stream));
+                wait();
+                float ms = 0.0f;
+                HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                times[c].push_back(ms);
             }
-        } catch (...) {
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
-            profiling = prof;
-            use_graphs = graphs_on;
-            throw;
         }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        profiling = prof;
-        use_graphs = graphs_on;
         size_t best = 0;
         float best_t = 0.0f;
         for (size_t c = 0; c < cand.size(); ++c) {
@@ -3110,9 +3092,64 @@ struct Engine {
                 best_t = med;
             }
         }
-        sched_cell_waves = cand[best].wpb;
-        sched_sic_group = cand[best].g;
-        sched_step_group = cand[best].sg;
+        apply_schedule(cand[best]);
+        return best_t;
+    }
+
+    void autotune(int wpt, int rounds) {
+        wait();
+        if (dirty) rebuild_tables();
+        if (n_active == 0 || wpt <= 0 || rounds <= 0) return;
+        int top = 1;
+        while (top < Wg) top *= 2;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        HIP_CHECK(hipEventCreate(&e0));
+        HIP_CHECK(hipEventCreate(&e1));
+        const bool prof = profiling;
+        profiling = false;
+        float best_t = 0.0f;
+        try {
+            std::vector<Sched> cand;
+            for (int wpb : {1, 2, 4}) {
+                Sched c = current_schedule();
+                c.wpb = wpb;
+                cand.push_back(c);
+            }
+            best_t = tune_among(cand, wpt, rounds, e0, e1);
+            if (active_range[1] > 0) {  // FP64 networks
+                cand.clear();
+                for (int g = 1; g <= top; g *= 2) {
+                    for (int sg : {1, Wg}) {
+                        Sched c = current_schedule();
+                        c.g = g;
+                        c.sg = sg;
+                        cand.push_back(c);
+                    }
+                }
+                best_t = tune_among(cand, wpt, rounds, e0, e1);
+            }
+            if (active_range[2] > active_range[1]) {  // FP32 networks
+                cand.clear();
+                // Not the SIC wavefront count: the SIC load grows with the
+                // activity (networks whose astrocytes pass the threshold),
+                // so a count tuned early in a run can starve it later; the
+                // default gives every neuron block a wavefront.
+                for (int dw : {512, 1024, 2048}) {
+                    Sched c = current_schedule();
+                    c.dw = dw;
+                    cand.push_back(c);
+                }
+                best_t = tune_among(cand, wpt, rounds, e0, e1);
+            }
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            profiling = prof;
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        profiling = prof;
         tuned_ms_per_window = best_t / static_cast<float>(wpt);
     }
 
@@ -3712,16 +3749,21 @@ asg_status asg_batch_set_graphs(asg_batch* batch, int32_t on) {
 asg_status asg_batch_set_schedule(asg_batch* batch, const asg_schedule* sched) {
     if (batch == nullptr || sched == nullptr) return fail(ASG_ERR_INVALID, "bad argument");
     const int w = sched->cell_waves, g = sched->sic_group, sg = sched->step_group;
+    const int dw = sched->deliver_waves, sw = sched->sic_workers;
     if ((w != 1 && w != 2 && w != 4) || (g != 0 && g != 1 && g != 2 && g != 4 && g != 8 && g != 16) ||
-        sg < 0 || sg > asg::kWindowCap) {
+        sg < 0 || sg > asg::kWindowCap || (dw != 0 && (dw < 64 || dw > 4096)) ||
+        (sw != 0 && (sw < 64 || sw > 8192))) {
         return fail(ASG_ERR_INVALID,
-                    "cell_waves must be 1, 2 or 4; sic_group 0, 1, 2, 4, 8 or 16; step_group 0-16");
+                    "cell_waves must be 1, 2 or 4; sic_group 0, 1, 2, 4, 8 or 16; step_group 0-16; "
+                    "deliver_waves 0 or 64-4096; sic_workers 0 or 64-8192");
     }
     return guarded([&]() -> asg_status {
         batch->eng->wait();
         batch->eng->sched_cell_waves = w;
         batch->eng->sched_sic_group = g;
         batch->eng->sched_step_group = sg;
+        batch->eng->sched_deliver_waves = dw == 0 ? asg::kDeliverWaves : dw;
+        batch->eng->sched_sic_workers = sw;
         return ASG_OK;
     });
 }
@@ -3731,6 +3773,8 @@ asg_status asg_batch_get_schedule(asg_batch* batch, asg_schedule* out) {
     out->cell_waves = batch->eng->sched_cell_waves;
     out->sic_group = batch->eng->sched_sic_group;
     out->step_group = batch->eng->sched_step_group;
+    out->deliver_waves = batch->eng->sched_deliver_waves;
+    out->sic_workers = batch->eng->sched_sic_workers;
     out->ms_per_window = batch->eng->tuned_ms_per_window;
     return ASG_OK;
 }
@@ -3746,10 +3790,7 @@ asg_status asg_batch_autotune(asg_batch* batch, int32_t windows_per_try, int32_t
             return fail(ASG_ERR_OVERFLOW, overflow_message(bits));
         }
         if (chosen != nullptr) {
-            chosen->cell_waves = batch->eng->sched_cell_waves;
-            chosen->sic_group = batch->eng->sched_sic_group;
-            chosen->step_group = batch->eng->sched_step_group;
-            chosen->ms_per_window = batch->eng->tuned_ms_per_window;
+            return asg_batch_get_schedule(batch, chosen);
         }
         return ASG_OK;
     });

@@ -325,22 +325,28 @@ class Batch:
     def bytes_in_use(self) -> int:
         return int(lib().asg_batch_bytes_in_use(self._h))
 
-    def autotune(self, windows_per_try: int = 2, rounds: int = 3) -> dict:
+    @staticmethod
+    def _schedule_dict(sch) -> dict:
+        return {"cell_waves": sch.cell_waves, "sic_group": sch.sic_group,
+                "step_group": sch.step_group, "deliver_waves": sch.deliver_waves,
+                "sic_workers": sch.sic_workers, "ms_per_window": sch.ms_per_window}
+
+    def autotune(self, windows_per_try: int = 20, rounds: int = 3) -> dict:
         """Pick the fastest launch schedule on the live batch (advances every
         network; results do not depend on the schedule)."""
         sch = capi.Schedule()
         check(lib().asg_batch_autotune(self._h, int(windows_per_try), int(rounds), C.byref(sch)))
-        return {"cell_waves": sch.cell_waves, "sic_group": sch.sic_group,
-                "step_group": sch.step_group, "ms_per_window": sch.ms_per_window}
+        return self._schedule_dict(sch)
 
     def schedule(self) -> dict:
         sch = capi.Schedule()
         check(lib().asg_batch_get_schedule(self._h, C.byref(sch)))
-        return {"cell_waves": sch.cell_waves, "sic_group": sch.sic_group,
-                "step_group": sch.step_group, "ms_per_window": sch.ms_per_window}
+        return self._schedule_dict(sch)
 
-    def set_schedule(self, cell_waves: int = 1, sic_group: int = 0, step_group: int = 0) -> None:
-        sch = capi.Schedule(int(cell_waves), int(sic_group), int(step_group), 0.0)
+    def set_schedule(self, cell_waves: int = 1, sic_group: int = 0, step_group: int = 0,
+                     deliver_waves: int = 0, sic_workers: int = 0) -> None:
+        sch = capi.Schedule(int(cell_waves), int(sic_group), int(step_group), int(deliver_waves),
+                            int(sic_workers), 0.0)
         check(lib().asg_batch_set_schedule(self._h, C.byref(sch)))
 
     def set_graphs(self, on: bool) -> None:

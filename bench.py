@@ -8,43 +8,47 @@ round(0.2 N) astrocytes, IP3 weight w_n2a uniform in [0.10, 0.40], seed
 1000 + index, every other parameter from use_case.json, fixed mean in-degree
 (p_primary = 100 / N) -- built on the host once, resident on the device.
 
-  value        batched FP32 (state FP32, FP64 near threshold and for the Ca
-               threshold test): all 256 networks in one asg_batch, every
-               window one launch per kernel for all of them. Neuron-steps/s =
-               sum over networks of N x time steps / wall seconds of the timed
-               bench steps. One bench step = 1,000 time steps (0.1 s
-               simulated) of every network plus draining their spike rasters;
-               --warmup bench steps run first as the untimed transient, then
-               --steps are timed (default 5 + 50: 0.5 s transient, 5 s timed).
+  value        batched FP32 (FP32 conductances, rings in fixed point, the slow
+               state V, w, Ca, IP3, h carried in FP64): all 256 networks in
+               one asg_batch, schedule picked by the autotuner, each call's
+               windows replayed from a captured hipGraph. Neuron-steps/s =
+               sum over networks of N x time steps / wall seconds of the
+               timed bench steps. One bench step = 1,000 time steps (0.1 s
+               simulated) of every network plus draining their spike
+               rasters; --warmup bench steps run first, untimed, then
+               --steps are timed.
+  batched_fp64 the same batch in FP64 (bit-identical to the plain runs), timed
+               over the accuracy run below.
   plain_fp64   the drop-in path: each network alone on the device in FP64,
-               one after the other (every 8th network of the sweep, timed
-               over the same time steps).
-  batched_fp64 the same batch in FP64; it is bit-identical to the plain runs
-               and serves as the FP64 reference of the accuracy figures.
-  accuracy     per network, FP32 against FP64: firing rate over the timed
-               window, active-astrocyte fraction, mean pairwise Ca
-               correlation (the reference's analysis of Ca recorded every
-               record_every steps).
-  roofline     the kernel with the largest share of device time: algorithmic
-               bytes per launch / mean launch time from HIP events recorded on
-               the batch's stream.
+               one after the other (every 8th network of the sweep).
+  accuracy     always over 0.5 s transient + 5 s recorded (the sweep's
+               duration), per network, FP32 against FP64: firing rate,
+               active-astrocyte fraction, mean pairwise Ca correlation (the
+               reference's analysis of Ca recorded every record_every steps),
+               and the same figures for FP64 against FP64 with every initial
+               V moved by 1e-6 mV (the chaos floor).
+  roofline     the kernel with the largest share of device time in a
+               profiled eager run of the same batch: algorithmic bytes per
+               launch / mean launch time from HIP events on the batch's
+               stream; traffic from the committed rocprofv3 counter summary.
   cpu_baseline the reference simulator built from its own sources
-               (oracle/_ref) on this host's cores, on a sample of the sweep's
-               networks.
+               (oracle/_ref) on this host: the sweep sample with one network
+               per core on every core of the box's share, the same runs' one-
+               thread rate, and the C1 / C2 configurations on all cores and on
+               one thread.
   sparse_fp64  the reference's 'Sparse' network (10,000 neurons, 10,000
                astrocytes, paper_sparse.json), FP64, seeds 1-3 side by side,
-               1 s transient + --sparse-seconds timed.
-  astro_scaling the isolated astrocyte update at 1e7 cells, 1,000 steps, one
-               launch per step: HBM GB/s of the update launches, FP64 and FP32,
-               and the FP32-vs-FP64 state error.
+               1 s transient + --sparse-seconds timed, with its cell kernel's
+               roofline.
+  astro_scaling the isolated astrocyte update at 1e7 cells, 1,000 steps.
   serving      a bounded sample of the serving loop (--serve-jobs jobs,
-               Poisson arrivals at 0.8 x the capacity measured first): job
-               latency p50/p99 and throughput.
+               Poisson arrivals at 0.8 x the capacity measured first).
 """
 
 from __future__ import annotations
 
 import argparse
+import concurrent.futures as cf
 import json
 import os
 import re
@@ -60,14 +64,16 @@ sys.path.insert(0, HERE)
 
 METRIC = "neuron-steps/s batched FP32 vs plain FP64 on 1 GPU; HBM GB/s; p99 job latency"
 # Kernel durations come from HIP events around one window in PROFILE_EVERY of
-# the timed region (200 of the 5,000 windows): events around every window
-# would cost ~14% of the wall time they are meant to explain, around one in
-# 25 about 1%.
+# the profiled run: events around every window would cost a noticeable share
+# of the time they are meant to explain, around one in 25 about 1%.
 PROFILE_EVERY = 25
 SWEEP_CONFIG = os.path.join(HERE, "tests", "golden", "configs", "use_case.json")
 SPARSE_CONFIG = os.path.join(HERE, "tests", "golden", "configs", "paper_sparse.json")
+REF_BIN = os.path.join(HERE, "oracle", "_ref", "astrosimgpu_ref")
 HBM_PEAK_GBS = 8000.0
 STEPS_PER_BENCH_STEP = 1000
+WINDOW = 10  # min(d_e, d_i, d_a2n) of use_case.json and paper_sparse.json = 1 ms = 10 steps
+ACC_TRANSIENT_BS, ACC_TIMED_BS = 5, 50  # accuracy run: 0.5 s + 5 s, in bench steps
 SPARSE_SEEDS = (1, 2, 3)
 # Published mean rates of the Sparse seeds over 1 s transient + 10 s
 # (reference docs/validation.md:83-89).
@@ -78,34 +84,25 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-# Algorithmic HBM bytes of the cell-update kernel. It keeps each cell's
-# state in registers for a whole window of W steps, so state moves once per
-# window and ring slots every step.
-#   neuron, per window: V, w, g_ex, dg_ex, g_in, dg_in, I_sic read + written
-#     (7 x 2 x s), refractory counter read + written (8), V_reset and b read
-#     (16); per step: exc, inh and SIC ring slots read (3 s), the exc and inh
-#     slots cleared (2 s), and the external drive k_drive wrote for it: the
-#     int32 Poisson count (4) and, with noise on, the noise current (s).
-#   astrocyte, per window: Ca, IP3, h read + written (6 s), Ca_tot, IP3_0,
-#     1/tau_IP3 and delta_IP3 read (4 s); per step: the IP3 ring slot read
-#     and cleared (2 s) and the SIC factor written (s).
-# s = 8 bytes in FP64, 4 in FP32.
-def cell_bytes(n_neurons: int, n_astro: int, window: int, state_bytes: int,
-               ring_bytes: int, noise_values: int = 1) -> float:
-    """Bytes one cell-kernel launch must move for a window: per neuron the 7
-    state values read and written (14 s), the refractory counter (8) and
-    V_reset, b (16), per step the exc and inh ring slots (2 r) and the
-    Poisson count (4), and per noise interval of the window its noise value
-    (s; noise_values intervals, one when the noise interval is the window);
-    per astrocyte 3 state values read and written and 4 parameters (10 s),
-    and per step the n2a ring slot (r) and the SIC factor (s). s is the state
-    value size, r the ring slot size (8 for the FP32 path's fixed-point
-    rings). Reads of SIC slots of live exchanges and clears of nonzero ring
-    slots depend on the activity and are not counted, so the bandwidth
-    derived from these bytes is a lower bound."""
-    s, r = state_bytes, ring_bytes
-    neuron = 14 * s + 8 + 16 + window * (2 * r + 4) + noise_values * s
-    astro = 10 * s + window * (r + s)
+def cell_bytes(n_neurons: int, n_astro: int, window: int, fp32: bool, noise_values: int = 1,
+               per_cell_params: bool = True) -> float:
+    """Algorithmic HBM bytes of one cell-kernel launch (one window of
+    `window` steps; the cell state stays in registers across the window).
+
+    Neuron, per window: V and w read + written (FP64 in both paths, 32),
+    g_ex, dg_ex, g_in, dg_in, I_sic read + written (5 x 2 x s), refractory
+    counter read + written (8), V_reset and b read (16); per step the exc and
+    inh ring slots read (2 x 8: FP64 sums, or 64-bit fixed point); per noise
+    interval of the window its noise value (s). Astrocyte, per window: Ca,
+    IP3, h read + written (FP64 in both paths, 48) and 4 parameters (4 s);
+    per step the n2a ring slot read (8) and the SIC factor written (s). s is
+    8 (FP64) or 4 (FP32). Not counted: clears and SIC reads of nonzero slots
+    (activity-dependent, so the bandwidth derived is a lower bound), and the
+    Poisson counts the drive pass stages for the cell pass (an artefact of
+    splitting the drive off, not of the step)."""
+    s = 4 if fp32 else 8
+    neuron = 32 + 10 * s + 8 + (16 if per_cell_params else 0) + window * 16 + noise_values * s
+    astro = 48 + 4 * s + window * (8 + s)
     return n_neurons * neuron + n_astro * astro
 
 
@@ -115,35 +112,37 @@ def build_sweep(count: int, transient_ms: float, timed_ms: float):
     return specs, [sweep.build(SWEEP_CONFIG, s, timed_ms, transient_ms) for s in specs]
 
 
-def run_batch(models, precision, warmup: int, steps: int, profile: bool):
-    """One asg_batch of all models; returns elapsed seconds of the timed
-    bench steps, per-network spike steps, Ca traces and kernel times."""
+def new_batch(models, precision, total_steps, record=True):
     import astrosimgpu_amd as A
     from astrosimgpu_amd import sweep
     batch = A.Batch(len(models))
     slots = []
     for m in models:
         m.desc.contents.precision = precision
-        start, every, n_rec, cap, total = sweep._record_plan(m)
-        s = batch.add(m, total)
-        batch.set_recording(s, start, every, n_rec, cap)
+        s = batch.add(m, total_steps)
+        if record:
+            start, every, n_rec, cap, _ = sweep._record_plan(m)
+            batch.set_recording(s, start, every, n_rec, cap)
         slots.append(s)
-    spikes = [[] for _ in models]
+    return batch, slots
+
+
+def timed_steps(batch, slots, warmup: int, steps: int, keep_spikes: bool):
+    """warmup untimed bench steps, then `steps` timed ones. Each launches
+    its steps without waiting, splits the previous step's raster per network
+    on the host while they run, then drains its own. Returns the timed
+    seconds and, with keep_spikes, each network's spike steps."""
+    spikes = [[] for _ in slots]
 
     def keep(raw):
-        drained = batch.split_spikes(raw)
-        for k, s in enumerate(slots):
-            spikes[k].append(drained[s][0])
+        if keep_spikes:
+            drained = batch.split_spikes(raw)
+            for k, s in enumerate(slots):
+                spikes[k].append(drained[s][0])
 
     for _ in range(warmup):
         batch.step(STEPS_PER_BENCH_STEP)
         keep(batch.spikes_all_raw())
-    if profile:
-        batch.set_profiling(True, every=PROFILE_EVERY)
-    # Each bench step launches its steps without waiting, splits the previous
-    # step's raster per network on the host while they run, then drains its
-    # own (the drain waits for the device: hipStreamSynchronize on the
-    # batch's stream). The clock brackets exactly the timed work.
     t0 = time.perf_counter()
     raw = None
     for i in range(steps):
@@ -155,19 +154,27 @@ def run_batch(models, precision, warmup: int, steps: int, profile: bool):
             log(f"    bench step {i + 1}/{steps}")
     keep(raw)
     elapsed = time.perf_counter() - t0
-    kt = batch.kernel_times() if profile else None
-    traces = []
-    for s in slots:
-        st, ca, _ = batch.read_trace(s)
-        traces.append((st, ca))
-    batch.close()
     spk = [np.concatenate(x) if x else np.zeros(0, np.int64) for x in spikes]
+    return elapsed, spk
+
+
+def accuracy_run(models, precision, profile=False):
+    """0.5 s + 5 s of the whole batch, eager launches, default schedule:
+    timed seconds of the 5 s, spikes, Ca traces, per-window kernel times."""
+    batch, slots = new_batch(models, precision, (ACC_TRANSIENT_BS + ACC_TIMED_BS) * 1000)
+    if profile:
+        batch.set_profiling(True, every=PROFILE_EVERY)
+    elapsed, spk = timed_steps(batch, slots, ACC_TRANSIENT_BS, ACC_TIMED_BS, True)
+    kt = batch.kernel_times() if profile else None
+    traces = [batch.read_trace(s)[:2] for s in slots]
+    batch.close()
     return elapsed, spk, traces, kt
 
 
 def per_window_us(kt: dict) -> dict:
-    """Mean device microseconds per window of each kernel over the timed
-    windows, and how many windows were timed."""
+    """Mean device microseconds per window of each pass over the timed
+    windows (FP32 networks: the drive rides in the post pass, reported as
+    "deliver"), and how many windows were timed."""
     w = max(kt["windows"], 1)
     out = {k: round(v * 1000.0 / w, 3) for k, v in kt.items() if k != "windows"}
     out["windows_timed"] = kt["windows"]
@@ -198,7 +205,8 @@ def run_plain(models, total_steps: int):
 
 def sparse_fp64(transient_bench_steps: int, timed_bench_steps: int):
     """The Sparse network, FP64, seeds 1-3 resident together, each on its
-    own stream."""
+    own stream; then seed 1 alone with per-kernel event timing for the
+    roofline of its cell kernel."""
     import astrosimgpu_amd as A
     models = [A.Model(SPARSE_CONFIG, seed=s) for s in SPARSE_SEEDS]
     nets = [A.DeviceNetwork(m) for m in models]
@@ -226,65 +234,106 @@ def sparse_fp64(transient_bench_steps: int, timed_bench_steps: int):
         rates[str(s)] = round(int((st >= first).sum()) * 1000.0 / (steps * 0.1 * 10000), 4)
     for net in nets:
         net.close()
+    # seed 1 alone, profiled: its cell kernel's bytes per window / time
+    m = models[0]
+    net = A.DeviceNetwork(m)
+    net.set_profiling(True, every=1)
+    net.step(10 * STEPS_PER_BENCH_STEP)
+    kt = net.kernel_times()
+    net.close()
+    d = m.desc.contents
+    noise = d.input_exc.gauss_noise_std > 0 or d.input_inh.gauss_noise_std > 0
+    by = cell_bytes(m.n_neurons, m.n_astro, WINDOW, False, 1 if noise else 0)
+    us = kt["cells"] * 1000.0 / max(kt["windows"], 1)
     out = {"value": round(3 * 10000 * steps / elapsed, 1), "unit": "neuron-steps/s",
-           "seconds_timed": steps / 10000.0, "mean_rate_hz": rates}
+           "seconds_timed": steps / 10000.0, "mean_rate_hz": rates,
+           "kernel_us_per_window_seed1": per_window_us(kt),
+           "roofline": {"kernel": "k_cells<FP64> (one Sparse network, one window of 10 steps)",
+                        "bound": "hbm", "algorithmic_bytes_per_launch": round(by),
+                        "mean_launch_us": round(us, 3),
+                        "achieved": round(by / us / 1e3, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(by / us / 1e3 / HBM_PEAK_GBS, 5),
+                        "note": "one 10,000-neuron network launches ~20k cells per window: "
+                                "far too few to fill 256 CUs, so latency-bound by construction"}}
     if transient_bench_steps == 10 and timed_bench_steps == 100:
         out["reference_rate_hz"] = {str(s): r for s, r in SPARSE_REFERENCE_RATE_HZ.items()}
     return out
 
 
-def reference_cpu_baseline(specs, models, sample_ms: float, every: int, transient_ms: float):
-    """Time the reference build (oracle/_ref) on every `every`-th sweep
-    network: `transient_ms` untimed, then `sample_ms` timed by the reference's
-    own phase profile (it times the recorded window only), on min(16, cpus)
-    OpenMP threads. Falls back to the CPU oracle when the reference build is
-    absent."""
-    threads = min(16, os.cpu_count() or 1)
-    ref = os.path.join(HERE, "oracle", "_ref", "astrosimgpu_ref")
+def _run_reference(cfg_path, seed, sim_ms, pre_ms, threads, out_dir):
+    """One reference run; returns (neurons, recorded steps, step-loop seconds
+    as its run.txt reports them)."""
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    subprocess.run([REF_BIN, "-c", cfg_path, "-s", str(seed), "-t", str(sim_ms), "-p", str(pre_ms),
+                    "-o", out_dir, "--no-analysis"], check=True, env=env,
+                   stdout=subprocess.DEVNULL)
+    with open(os.path.join(out_dir, "run.txt")) as f:
+        text = f.read()
+    loop = float(re.search(r"^total\s+([0-9.]+)", text, flags=re.M).group(1))
+    ne = int(re.search(r"^excitatory neurons\s+(\d+)", text, flags=re.M).group(1))
+    ni = int(re.search(r"^inhibitory neurons\s+(\d+)", text, flags=re.M).group(1))
+    return ne + ni, int(sim_ms / 0.1 + 0.5), loop
+
+
+def reference_cpu_baseline(specs, sample_ms: float, every: int, transient_ms: float):
+    """The reference build (oracle/_ref) on this host.
+
+    Sweep sample: every `every`-th sweep network, `transient_ms` untimed then
+    `sample_ms` timed by the reference's own step-loop timer, one network
+    per core (single-threaded processes, as many at once as the cores of the
+    box's share). value = the sum of the per-core rates (what the cores
+    deliver while each holds a network); one_thread = the same runs' total
+    neuron-steps / total loop seconds.
+
+    C1 (use_case, 10 s, no transient) and C2 (paper_sparse seed 1, a 200 ms
+    sample of its 10 s run): on one thread and on all cores."""
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    cores = max(1, min(cores, os.cpu_count() or 1))
+    if not os.path.exists(REF_BIN):
+        return None
     picked = list(range(0, len(specs), every))
-    steps = int(sample_ms / 0.1 + 0.5)
-    sample = (f"sweep networks {', '.join(str(i) for i in picked)} "
-              f"({sum(specs[i].n for i in picked)} neurons), {sample_ms:g} ms each after a "
-              f"{transient_ms:g} ms transient, step-loop time of the recorded window")
-    ns = 0
-    loop_s = 0.0
-    if os.path.exists(ref):
-        with open(SWEEP_CONFIG) as f:
-            base = json.load(f)
-        with tempfile.TemporaryDirectory() as tmp:
-            for i in picked:
-                sp = specs[i]
-                cfg = json.loads(json.dumps(base))
-                cfg["N"].update({"N_E": sp.n_exc, "N_I": sp.n_inh, "N_A": sp.n_astro})
-                cfg["synapse"]["w_n2a"] = sp.w_n2a
-                cfg["connectivity"]["p_primary"] = min(1.0, 100.0 / sp.n)
-                cfg["simulation"]["record_astro"] = False
-                path = os.path.join(tmp, f"net{i}.json")
-                with open(path, "w") as f:
-                    json.dump(cfg, f)
-                out = os.path.join(tmp, f"out{i}")
-                env = dict(os.environ, OMP_NUM_THREADS=str(threads))
-                subprocess.run([ref, "-c", path, "-s", str(sp.seed), "-t", str(sample_ms), "-p", str(transient_ms),
-                                "-o", out, "--no-analysis"], check=True, env=env,
-                               stdout=subprocess.DEVNULL)
-                with open(os.path.join(out, "run.txt")) as f:
-                    m = re.search(r"^total\s+([0-9.]+)", f.read(), flags=re.M)
-                loop_s += float(m.group(1))
-                ns += sp.n * steps
-        kind = "reference"
-    else:
-        import oracle
-        os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    with open(SWEEP_CONFIG) as f:
+        base = json.load(f)
+    out = {"unit": "neuron-steps/s", "cores": cores, "kind": "reference"}
+    with tempfile.TemporaryDirectory() as tmp:
+        jobs = []
         for i in picked:
-            o = oracle.OracleRun(models[i].desc, oracle.LIBM_GLIBC)
-            o.step(int(transient_ms / 0.1 + 0.5))
-            t0 = time.time()
-            o.step(steps)
-            loop_s += time.time() - t0
-            ns += specs[i].n * steps
-        kind = "port"
-    return {"value": round(ns / loop_s, 1), "unit": "neuron-steps/s", "cores": threads,
-            "kind": kind, "sample": sample, "seconds": round(loop_s, 3)}
+            sp = specs[i]
+            cfg = json.loads(json.dumps(base))
+            cfg["N"].update({"N_E": sp.n_exc, "N_I": sp.n_inh, "N_A": sp.n_astro})
+            cfg["synapse"]["w_n2a"] = sp.w_n2a
+            cfg["connectivity"]["p_primary"] = min(1.0, 100.0 / sp.n)
+            cfg["simulation"]["record_astro"] = False
+            path = os.path.join(tmp, f"net{i}.json")
+            with open(path, "w") as f:
+                json.dump(cfg, f)
+            jobs.append((path, sp.seed, os.path.join(tmp, f"out{i}")))
+        with cf.ThreadPoolExecutor(max_workers=cores) as ex:
+            runs = list(ex.map(lambda j: _run_reference(j[0], j[1], sample_ms, transient_ms, 1,
+                                                        j[2]), jobs))
+        out["value"] = round(sum(n * st / loop for n, st, loop in runs), 1)
+        out["one_thread"] = round(sum(n * st for n, st, _ in runs) / sum(l for _, _, l in runs), 1)
+        out["sample"] = (f"sweep networks {picked[0]}, {picked[1]}, ... every {every}th "
+                         f"({len(picked)} networks, {sum(specs[i].n for i in picked)} neurons), "
+                         f"{sample_ms:g} ms each after a {transient_ms:g} ms transient, one "
+                         f"single-threaded reference process per core on {cores} cores; "
+                         f"step-loop time of the recorded window (run.txt 'total')")
+        use_case = os.path.join(HERE, "tests", "golden", "configs", "use_case.json")
+        c1 = {}
+        for threads in (1, cores):
+            n, st, loop = _run_reference(use_case, 1, 10000.0, 0.0, threads,
+                                         os.path.join(tmp, f"c1_{threads}"))
+            c1[f"threads_{threads}"] = round(n * st / loop, 1)
+        c1["workload"] = "use_case.json seed 1, 10 s simulated, no transient"
+        out["C1_use_case"] = c1
+        c2 = {}
+        for threads in (1, cores):
+            n, st, loop = _run_reference(SPARSE_CONFIG, 1, 200.0, 0.0, threads,
+                                         os.path.join(tmp, f"c2_{threads}"))
+            c2[f"threads_{threads}"] = round(n * st / loop, 1)
+        c2["workload"] = "paper_sparse.json seed 1, the first 200 ms (2,000 steps) of its run"
+        out["C2_sparse"] = c2
+    return out
 
 
 def load_traffic(kernel: str):
@@ -293,6 +342,11 @@ def load_traffic(kernel: str):
         with open(path) as f:
             return json.load(f).get("hbm_bytes_per_launch")
     return None
+
+
+def rounded(d):
+    return None if d is None else {k: (round(v, 5) if isinstance(v, float) else v)
+                                   for k, v in d.items()}
 
 
 def main():
@@ -307,6 +361,7 @@ def main():
     ap.add_argument("--cpu-sample-ms", type=float, default=1500.0)
     ap.add_argument("--cpu-every", type=int, default=16)
     ap.add_argument("--no-sparse", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="default schedule, eager launches")
     ap.add_argument("--fp32-only", action="store_true",
                     help="run only the batched FP32 sweep (for kernel profiles)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -340,73 +395,88 @@ def main():
     from astrosimgpu_amd import sweep
 
     t_build = time.time()
-    transient_ms = args.warmup * STEPS_PER_BENCH_STEP * 0.1
-    timed_ms = args.steps * STEPS_PER_BENCH_STEP * 0.1
-    specs, models = build_sweep(args.networks, transient_ms, timed_ms)
-    total_steps = (args.warmup + args.steps) * STEPS_PER_BENCH_STEP
+    specs, models = build_sweep(args.networks, ACC_TRANSIENT_BS * 100.0, ACC_TIMED_BS * 100.0)
     n_total = sum(m.n_neurons for m in models)
     na_total = sum(m.n_astro for m in models)
     log(f"built {len(models)} networks ({n_total} neurons, {na_total} astrocytes) "
         f"in {time.time() - t_build:.1f} s")
 
+    # Headline: batched FP32, autotuned schedule, graph replay.
     log("batched FP32 ...")
-    e32, spk32, tr32, kt32 = run_batch(models, capi.ASG_FP32, args.warmup, args.steps, True)
+    tune_steps = 0 if args.no_tune else 40 * 3 * 20 * WINDOW  # <= 40 candidates x 3 rounds x 20 windows
+    batch, slots = new_batch(models, capi.ASG_FP32,
+                             tune_steps + (args.warmup + args.steps + 10) * 1000, record=False)
+    schedule = None
+    # The transient first: the activity the schedule is tuned on (how many
+    # networks have a live SIC exchange, how many blocks spike) should be
+    # the timed run's.
+    for _ in range(args.warmup):
+        batch.step(STEPS_PER_BENCH_STEP)
+        batch.spikes_all_raw()
+    if not args.no_tune:
+        batch.set_graphs(True)
+        schedule = batch.autotune(windows_per_try=20, rounds=3)
+        batch.spikes_all_raw()
+        log(f"  autotuned schedule {schedule}")
+    e32, _ = timed_steps(batch, slots, 0, args.steps, False)
     timed_ns = n_total * args.steps * STEPS_PER_BENCH_STEP
     value = timed_ns / e32
-    log(f"  {value:.3e} neuron-steps/s  kernel us per window {per_window_us(kt32)}")
+    log(f"  {value:.3e} neuron-steps/s")
+    # the same batch, eager with per-kernel events, for the roofline
+    batch.set_graphs(False)
+    batch.set_profiling(True, every=4)
+    timed_steps(batch, slots, 0, 10, False)
+    kt32 = batch.kernel_times()
+    batch.close()
+    log(f"  kernel us per window {per_window_us(kt32)}")
 
-    if args.fp32_only:
-        args.no_sparse = args.no_cpu_baseline = args.no_astro_scaling = True
-        args.serve_jobs = 0
-        e64 = kt64 = plain64 = cmp = floor = sub = None
-    else:
-        log("batched FP64 ...")
-        e64, spk64, tr64, kt64 = run_batch(models, capi.ASG_FP64, args.warmup, args.steps, True)
-        log(f"  {timed_ns / e64:.3e} neuron-steps/s")
-
-        log("plain FP64 (one network at a time) ...")
-        sub = models[::args.plain_every]
-        p_ns, p_s = run_plain(sub, total_steps)
-        plain64 = p_ns / p_s
-        log(f"  {plain64:.3e} neuron-steps/s over {len(sub)} networks")
-
-        r64 = sweep.RunResult(e64, timed_ns, spk64, tr64)
-        r32 = sweep.RunResult(e32, timed_ns, spk32, tr32)
-        cmp = sweep.compare(models, r64, r32)
+    cmp = floor = None
+    e64 = kt64 = plain64 = sub = None
+    if not args.fp32_only:
+        log("accuracy, 0.5 s + 5 s: batched FP64 ...")
+        e64, spk64, tr64, kt64 = accuracy_run(models, capi.ASG_FP64, profile=True)
+        log("  batched FP32 ...")
+        a32, spk32, tr32, _ = accuracy_run(models, capi.ASG_FP32)
+        acc_ns = n_total * ACC_TIMED_BS * STEPS_PER_BENCH_STEP
+        r64 = sweep.RunResult(e64, acc_ns, spk64, tr64)
+        cmp = sweep.compare(models, r64, sweep.RunResult(a32, acc_ns, spk32, tr32))
         cmp.pop("rows")
-
         # The networks are chaotic: any perturbation, however small, changes
         # individual spike times and then whole trajectories. The floor of
         # per-network agreement is what FP64 itself does when every initial
-        # membrane potential moves by 1e-6 mV, a quarter of one FP32 ulp of V
-        # (a one-ulp FP64 change is absorbed by the first step's rounding).
-        log("FP64 with the initial V 1e-6 mV higher (chaos floor) ...")
+        # membrane potential moves by 1e-6 mV, a quarter of one FP32 ulp of V.
+        log("  FP64 with the initial V 1e-6 mV higher (chaos floor) ...")
         saved = []
         for m in models:
             d = m.desc.contents
             saved.append((d.neuron[0].V_m_init, d.neuron[1].V_m_init))
             for k in (0, 1):
                 d.neuron[k].V_m_init = d.neuron[k].V_m_init + 1e-6
-        _, spkp, trp, _ = run_batch(models, capi.ASG_FP64, args.warmup, args.steps, False)
+        _, spkp, trp, _ = accuracy_run(models, capi.ASG_FP64)
         for m, (v0, v1) in zip(models, saved):
             m.desc.contents.neuron[0].V_m_init = v0
             m.desc.contents.neuron[1].V_m_init = v1
-        floor = sweep.compare(models, r64, sweep.RunResult(0.0, timed_ns, spkp, trp))
+        floor = sweep.compare(models, r64, sweep.RunResult(0.0, acc_ns, spkp, trp))
         floor.pop("rows")
+        e64 = acc_ns / e64  # neuron-steps/s of the batched FP64 run
+
+        log("plain FP64 (one network at a time) ...")
+        sub = models[::args.plain_every]
+        p_ns, p_s = run_plain(sub, (args.warmup + args.steps) * STEPS_PER_BENCH_STEP)
+        plain64 = p_ns / p_s
+        log(f"  {plain64:.3e} neuron-steps/s over {len(sub)} networks")
 
     # Roofline of the kernel with the largest share of device time.
     windows = kt32["windows"]
     share = {k: kt32[k] for k in ("cells", "compact", "deliver", "sum")}
     dominant = max(share, key=share.get)
-    window_len = 10  # min(d_e, d_i, d_a2n) of use_case.json = 1 ms = 10 steps
     if dominant == "cells":
-        # use_case.json drives neurons with Poisson input and noise
-        # noise_dt of use_case.json = 1 ms = the window: one noise value each
-        bytes_launch = cell_bytes(n_total, na_total, window_len, 4, 8, noise_values=1)
+        # use_case.json: noise_dt = 1 ms = the window, one noise value per neuron
+        bytes_launch = cell_bytes(n_total, na_total, WINDOW, True, noise_values=1)
         avg_s = kt32["cells"] / 1000.0 / max(windows, 1)
         achieved = bytes_launch / avg_s / 1e9
-        roof = {"kernel": "k_cells (every neuron and astrocyte of the batch, one window of "
-                          "10 steps, FP32)",
+        roof = {"kernel": "k_cells<FP32> (every neuron and astrocyte of the batch, one window "
+                          "of 10 steps)",
                 "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                 "traffic": load_traffic("k_cells"),
@@ -417,14 +487,21 @@ def main():
                 "unit": "GB/s", "frac": None, "traffic": None,
                 "mean_launch_us": round(share[dominant] * 1000.0 / max(windows, 1), 3),
                 "launches": windows}
+    fp64_roof = None
+    if kt64 is not None:
+        b64 = cell_bytes(n_total, na_total, WINDOW, False, noise_values=1)
+        us64 = kt64["cells"] * 1000.0 / max(kt64["windows"], 1)
+        fp64_roof = {"kernel": "k_cells<FP64>", "algorithmic_bytes_per_launch": round(b64),
+                     "mean_launch_us": round(us64, 3), "achieved": round(b64 / us64 / 1e3, 2),
+                     "frac": round(b64 / us64 / 1e3 / HBM_PEAK_GBS, 5)}
 
     sparse = None
-    if not args.no_sparse:
+    if not args.no_sparse and not args.fp32_only:
         log("Sparse FP64 ...")
         sparse = sparse_fp64(10, 10 * args.sparse_seconds)
 
     astro = None
-    if not args.no_astro_scaling:
+    if not args.no_astro_scaling and not args.fp32_only:
         log("astrocyte update at 1e7 cells ...")
         from astrosimgpu_amd import astro_bench
         row = astro_bench.scaling([10_000_000], 1000)[0]
@@ -434,7 +511,7 @@ def main():
                  "fp32_rel_err_max": {k: v["max"] for k, v in row["fp32_vs_fp64_rel_err"].items()}}
 
     serving = None
-    if args.serve_jobs > 0:
+    if args.serve_jobs > 0 and not args.fp32_only:
         log(f"serving sample, {args.serve_jobs} jobs ...")
         from astrosimgpu_amd import serve
         cap = serve.calibrate(serve.DEFAULT_CONFIG, 48, capi.ASG_FP32, 2000, 4)
@@ -452,10 +529,10 @@ def main():
                                "astrocytes U[0.2, 1] N, 1-10 s each, FP32"}
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and not args.fp32_only:
         log("timing the reference build on the host cores ...")
-        cpu = reference_cpu_baseline(specs, models, args.cpu_sample_ms, args.cpu_every,
-                                     transient_ms)
+        cpu = reference_cpu_baseline(specs, args.cpu_sample_ms, args.cpu_every,
+                                     ACC_TRANSIENT_BS * 100.0)
 
     line = {
         "metric": METRIC,
@@ -474,21 +551,23 @@ def main():
         "config": {
             "workload": f"{len(models)}-network sweep (N log-uniform in [200, 5000], 80/20 E/I, "
                         "0.2 N astrocytes, w_n2a U[0.10, 0.40], in-degree 100), batched FP32 "
-                        f"path, {transient_ms / 1000:g} s transient untimed + "
-                        f"{timed_ms / 1000:g} s timed, 1000 time steps (dt 0.1 ms) per bench step",
+                        "path (FP32 conductances, fixed-point rings, V/w/Ca/IP3/h carried in "
+                        f"FP64), {'autotuned schedule, hipGraph replay, ' if schedule else ''}"
+                        f"{args.warmup} bench steps untimed + {args.steps} timed, 1000 time "
+                        "steps (dt 0.1 ms) per bench step; accuracy over 0.5 s + 5 s",
             "networks": len(models), "neurons": n_total, "astrocytes": na_total,
             "time_steps_per_bench_step": STEPS_PER_BENCH_STEP,
         },
+        "schedule": schedule,
         "plain_fp64": None if plain64 is None else
         {"value": round(plain64, 1), "unit": "neuron-steps/s", "networks_timed": len(sub)},
-        "batched_fp64": None if e64 is None else
-        {"value": round(timed_ns / e64, 1), "unit": "neuron-steps/s"},
+        "batched_fp64": None if e64 is None else {"value": round(e64, 1), "unit": "neuron-steps/s"},
         "speedup_batched_fp32_vs_plain_fp64": None if plain64 is None else round(value / plain64, 2),
-        "accuracy_fp32_vs_fp64": None if cmp is None else
-        {k: (round(v, 5) if isinstance(v, float) else v) for k, v in cmp.items()},
-        "chaos_floor_fp64_vs_fp64_dV_1e-6_mV": None if floor is None else
-        {k: (round(v, 5) if isinstance(v, float) else v) for k, v in floor.items()},
+        "speedup_batched_fp32_vs_batched_fp64": None if e64 is None else round(value / e64, 2),
+        "accuracy_fp32_vs_fp64_5s": rounded(cmp),
+        "chaos_floor_fp64_vs_fp64_dV_1e-6_mV_5s": rounded(floor),
         "roofline": roof,
+        "roofline_fp64_cells": fp64_roof,
         "kernel_us_per_window_fp32": per_window_us(kt32),
         "kernel_us_per_window_fp64": None if kt64 is None else per_window_us(kt64),
         "sparse_fp64": sparse,

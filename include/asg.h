@@ -411,24 +411,32 @@ asg_status asg_batch_read_trace(asg_batch* batch, int32_t slot, double* out, int
 int64_t asg_batch_bytes_in_use(const asg_batch* batch);
 
 /* Launch schedule of the window pipeline: cell-kernel workgroup size in
- * wavefronts (1, 2 or 4), SIC group (window steps per SIC wavefront: 1, 2, 4,
- * 8 or 16; 0 = the largest that still fills the device) and step group
- * (window steps per spike-marking and per summing wavefront, 1-16; 0 = the
- * whole window for large batches, 1 otherwise). Results do not depend on it.
- * ms_per_window is the autotuner's measured time (output only). */
+ * wavefronts (1, 2 or 4), SIC group of the FP64 networks (window steps per
+ * SIC wavefront: 1, 2, 4, 8 or 16; 0 = the largest that still fills the
+ * device), step group of the FP64 networks (window steps per spike-marking
+ * and per summing wavefront, 1-16; 0 = the whole window for large batches,
+ * 1 otherwise), the wavefronts that share out the delivery list (64-4096,
+ * 0 = 512) and the FP32 post pass's SIC wavefronts (64-8192; 0 = one per
+ * neuron block, the default).
+ * Results do not depend on it. ms_per_window is the autotuner's measured
+ * time (output only). */
 typedef struct asg_schedule {
     int32_t cell_waves;
     int32_t sic_group;
     int32_t step_group;
+    int32_t deliver_waves;
+    int32_t sic_workers;
     float ms_per_window;
 } asg_schedule;
 asg_status asg_batch_set_schedule(asg_batch* batch, const asg_schedule* sched);
 asg_status asg_batch_get_schedule(asg_batch* batch, asg_schedule* out);
 
-/* Autotuner: runs every candidate schedule for windows_per_try windows of
- * the batch, `rounds` times round-robin, and keeps the one with the smallest
- * median time (written to *chosen if not NULL). The networks advance by
- * rounds x candidates x windows_per_try windows, exactly as a step call of
+/* Autotuner: tunes the schedule one knob at a time (cell waves, then the
+ * FP64 networks' SIC and step groups when the batch has FP64 networks, then
+ * the delivery wavefront count when it has FP32 networks), timing
+ * each candidate as step calls of windows_per_try windows, `rounds` times
+ * round-robin, and keeping the smallest median (written to *chosen if not
+ * NULL). The networks advance by the windows run, exactly as step calls of
  * that length would advance them. */
 asg_status asg_batch_autotune(asg_batch* batch, int32_t windows_per_try, int32_t rounds,
                               asg_schedule* chosen);

@@ -64,6 +64,10 @@ def test_every_schedule_gives_the_same_bits():
         for group in (1, 2, 4, 8, 16):
             for steps in (1, 3, 16):   # 3 does not divide the window; 16 clamps to it
                 _same(ref, _run(lambda b, w=waves, g=group, sg=steps: b.set_schedule(w, g, sg)))
+    # the delivery and SIC wavefront counts that share out the lists
+    for dw in (64, 512, 4096):
+        for sw in (64, 2048, 8192):
+            _same(ref, _run(lambda b, d=dw, e=sw: b.set_schedule(1, 0, 0, d, e)))
 
 
 def test_graph_replays_equal_eager_launches():
@@ -97,6 +101,8 @@ def test_autotune_then_step_equals_plain_run():
     chosen = b.autotune(windows_per_try=1, rounds=2)
     assert chosen["cell_waves"] in (1, 2, 4) and chosen["sic_group"] in (1, 2, 4, 8, 16)
     assert chosen["step_group"] >= 1
+    assert chosen["deliver_waves"] in (512, 1024, 2048)
+    assert chosen["sic_workers"] == 0
     assert chosen["ms_per_window"] > 0.0
     done = b.net_step(slots[0])
     assert 0 < done < 4000 and all(b.net_step(s) == done for s in slots)
