@@ -186,6 +186,7 @@ PROTOTYPES = {
     "asg_batch_counters": (C.c_int, [P, C.c_int32, C.POINTER(Counters)]),
     "asg_batch_bytes_in_use": (C.c_int64, [P]),
     "asg_device_math": (C.c_int, [C.c_int32, DBLP, DBLP, C.c_int64]),
+    "asg_device_math_noalloc": (C.c_int, [C.c_int32, DBLP, DBLP, C.c_int64]),
     "asg_analyze_calcium": (C.c_int, [DBLP, DBLP, C.c_int32, C.c_int64, C.c_double, C.c_double,
                                       C.c_double, C.c_double, C.c_double, C.c_double, C.c_int32,
                                       C.POINTER(CalciumSummary)]),

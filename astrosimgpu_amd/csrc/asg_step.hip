@@ -51,6 +51,7 @@
 #include <cstddef>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -3835,6 +3836,78 @@ asg_status asg_batch_set_profiling(asg_batch* batch, int32_t on) {
 asg_status asg_batch_kernel_times(asg_batch* batch, asg_kernel_times* out) {
     if (batch == nullptr) return fail(ASG_ERR_INVALID, "NULL handle");
     return kernel_times(*batch->eng, out);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------- no-allocation variant --
+
+namespace {
+
+// Staging of asg_device_math_noalloc: the inputs (first half) and outputs
+// (second half) of one call, in a file-scope device array every GPU the
+// library runs on holds until the process ends (16 MiB).
+constexpr int64_t kNoAllocMaxValues = int64_t(1) << 20;
+static __device__ __align__(256) double scratch[2 * kNoAllocMaxValues];
+
+// The address of `scratch` on each device, looked up once per device with
+// that device current (host code must not use the array's name as an
+// address: in host code it is only a stand-in for the symbol).
+constexpr int kMaxDevices = 64;
+double* g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+asg_status scratch_of_current_device(double** out) {
+    int dev = 0;
+    const hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        return fail(ASG_ERR_DEVICE, std::string("hipGetDevice: ") + hipGetErrorString(e));
+    }
+    if (dev < 0 || dev >= kMaxDevices) {
+        return fail(ASG_ERR_DEVICE, "device index beyond the scratch table");
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* p = nullptr;
+        const hipError_t g = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (g != hipSuccess || p == nullptr) {
+            return fail(ASG_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") +
+                                            hipGetErrorString(g));
+        }
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return ASG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+asg_status asg_device_math_noalloc(int32_t fn, const double* x, double* y, int64_t n) {
+    if (fn < ASG_MATH_EXP || fn > ASG_MATH_EXP2_F32 || n < 0 || (n > 0 && (x == nullptr || y == nullptr))) {
+        return fail(ASG_ERR_INVALID, "bad argument");
+    }
+    if (n > kNoAllocMaxValues) {
+        return fail(ASG_ERR_CAPACITY, "asg_device_math_noalloc: n = " + std::to_string(n) +
+                                          " exceeds its scratch of " +
+                                          std::to_string(kNoAllocMaxValues) + " values");
+    }
+    if (n == 0) return ASG_OK;
+    double* d = nullptr;
+    const asg_status st = scratch_of_current_device(&d);
+    if (st != ASG_OK) return st;
+    return guarded([&]() -> asg_status {
+        HIP_CHECK(hipMemcpy(d, x, static_cast<size_t>(n) * sizeof(double), hipMemcpyHostToDevice));
+        const int block = 256;
+        const int64_t grid = std::min<int64_t>((n + block - 1) / block, 65536);
+        hipLaunchKernelGGL(asg::k_math, dim3(static_cast<unsigned>(grid)), dim3(block), 0, nullptr,
+                           fn, d, d + kNoAllocMaxValues, n);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipMemcpy(y, d + kNoAllocMaxValues, static_cast<size_t>(n) * sizeof(double),
+                            hipMemcpyDeviceToHost));
+        return ASG_OK;
+    });
 }
 
 asg_status asg_device_math(int32_t fn, const double* x, double* y, int64_t n) {

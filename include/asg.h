@@ -485,6 +485,15 @@ enum {
  * 2^y). */
 asg_status asg_device_math(int32_t fn, const double* x, double* y, int64_t n);
 
+/* asg_device_math without allocating: no counterpart in the reference. For
+ * callers that cannot allocate device memory (code under graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc): the inputs and
+ * outputs are staged through a file-scope device array of 2 x 2^20 doubles
+ * (16 MiB, held by every GPU the library runs on until the process ends).
+ * n above 2^20 returns ASG_ERR_CAPACITY. Calls that overlap on one GPU race
+ * for the array; the last call's contents stay in it. */
+asg_status asg_device_math_noalloc(int32_t fn, const double* x, double* y, int64_t n);
+
 #ifdef __cplusplus
 }
 #endif
