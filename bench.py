@@ -364,6 +364,11 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="default schedule, eager launches")
     ap.add_argument("--fp32-only", action="store_true",
                     help="run only the batched FP32 sweep (for kernel profiles)")
+    ap.add_argument("--fp64-only", action="store_true",
+                    help="run only the batched FP64 sweep's 5 s run, profiled (for kernel "
+                         "profiles)")
+    ap.add_argument("--sparse-only", action="store_true",
+                    help="run only the Sparse FP64 leg (for kernel profiles)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy kernel of MIB MiB five times "
                          "(the reference point for HBM byte counters) and exit")
@@ -394,12 +399,22 @@ def main():
     from astrosimgpu_amd import _capi as capi
     from astrosimgpu_amd import sweep
 
+    if args.sparse_only:
+        print(json.dumps({"sparse_fp64": sparse_fp64(10, 10 * args.sparse_seconds)}), flush=True)
+        return
+
     t_build = time.time()
     specs, models = build_sweep(args.networks, ACC_TRANSIENT_BS * 100.0, ACC_TIMED_BS * 100.0)
     n_total = sum(m.n_neurons for m in models)
     na_total = sum(m.n_astro for m in models)
     log(f"built {len(models)} networks ({n_total} neurons, {na_total} astrocytes) "
         f"in {time.time() - t_build:.1f} s")
+
+    if args.fp64_only:
+        e64, _, _, kt64 = accuracy_run(models, capi.ASG_FP64, profile=True)
+        print(json.dumps({"batched_fp64": n_total * ACC_TIMED_BS * STEPS_PER_BENCH_STEP / e64,
+                          "kernel_us_per_window_fp64": per_window_us(kt64)}), flush=True)
+        return
 
     # Headline: batched FP32, autotuned schedule, graph replay.
     log("batched FP32 ...")
