@@ -230,6 +230,18 @@ struct DevNet {
     uint32_t sic_live_p[2];  // FP32 networks: the same, one word per window
                              // parity (the close empties the other parity's)
     int32_t open;          // 1 from a window's cell pass to its close
+    // Window clock: the step-periodic phases of the window that starts at
+    // `step` (set_clock), kept beside it so that the cell pass divides
+    // nothing: every wavefront would otherwise repeat the same 64-bit
+    // divisions.
+    int32_t clk_row;          // step % S
+    int32_t clk_sic_arrive;   // first_hit(step, d_a2n, sic_interval)
+    int32_t clk_sic_send;     // first_hit(step, 0, sic_interval)
+    int32_t clk_nz_next[2];   // per neuron population: first noise-interval
+                              // change after the window's first step
+    int32_t clk_anz_next;     // the same for the astrocytes' noise
+    int64_t clk_anz_idx;      // step / astrocyte steps_per_noise
+    int64_t clk_rec_next, clk_rec_slot;  // astrocyte recording cursor
     // bit T mod (hist_mask + 1): the SIC exchange arriving at step T had a
     // nonzero factor, i.e. its ring_sic row was written; 0 means every
     // neuron's SIC is 0 (a power-of-two ring of at least S + 32 bits)
@@ -247,19 +259,12 @@ __device__ __forceinline__ uint32_t live_arrivals(const DevNet& n, int64_t T0) {
 }
 
 // Recording cadence of one window: `next` is the window offset of the next
-// recorded step, `slot` its ring slot; advance() moves to the one after.
+// recorded step, `slot` its ring slot (from the window clock); advance()
+// moves to the one after.
 struct RecCursor {
     int64_t next, slot;
-    __device__ __forceinline__ RecCursor(const DevNet& n, int64_t T0) {
-        next = n.rec_n == 0 ? INT64_MAX : 0;
-        slot = 0;
-        if (n.rec_n != 0) {
-            const int64_t d = T0 - n.rec_start;
-            const int64_t q = d <= 0 ? 0 : (d + n.rec_every - 1) / n.rec_every;
-            next = n.rec_start + q * n.rec_every - T0;
-            slot = q % n.rec_cap;
-        }
-    }
+    __device__ __forceinline__ explicit RecCursor(const DevNet& n)
+        : next(n.clk_rec_next), slot(n.clk_rec_slot) {}
     __device__ __forceinline__ void advance(const DevNet& n) {
         next += n.rec_every;
         slot = slot + 1 == n.rec_cap ? 0 : slot + 1;
@@ -330,6 +335,32 @@ __device__ __forceinline__ int64_t first_hit(int64_t T0, int64_t origin, int64_t
     return r == 0 ? 0 : period - r;
 }
 
+// The window clock of the window that starts at step T (the DevNet clk_
+// fields). Written by one lane per network, where `step` changes: the
+// close of a window and the per-call launch that sets the run targets
+// (after which any host-side change of the step or of the recording
+// settings is taken up).
+__device__ inline void set_clock(DevNet& n, int64_t T) {
+    n.clk_row = static_cast<int32_t>(T % n.S);
+    n.clk_sic_arrive = static_cast<int32_t>(first_hit(T, n.d_a2n, n.sic_interval));
+    n.clk_sic_send = static_cast<int32_t>(first_hit(T, 0, n.sic_interval));
+    for (int p = 0; p < 2; ++p) {
+        const int64_t period = n.in[p].noise_std > 0.0 ? n.in[p].steps_per_noise : 1;
+        n.clk_nz_next[p] = static_cast<int32_t>(first_hit(T + 1, 0, period) + 1);
+    }
+    const int64_t spn = n.ap.in.steps_per_noise;
+    n.clk_anz_idx = spn > 0 ? T / spn : 0;
+    n.clk_anz_next = spn > 0 ? static_cast<int32_t>(first_hit(T + 1, 0, spn) + 1) : INT32_MAX;
+    n.clk_rec_next = INT64_MAX;
+    n.clk_rec_slot = 0;
+    if (n.rec_n != 0) {
+        const int64_t d = T - n.rec_start;
+        const int64_t q = d <= 0 ? 0 : (d + n.rec_every - 1) / n.rec_every;
+        n.clk_rec_next = n.rec_start + q * n.rec_every - T;
+        n.clk_rec_slot = q % n.rec_cap;
+    }
+}
+
 __device__ inline void add_i64(int64_t* p, int64_t v) {
     atomicAdd(reinterpret_cast<unsigned long long*>(p), static_cast<unsigned long long>(v));
 }
@@ -346,7 +377,6 @@ __device__ __forceinline__ uint64_t neuron_window(const DevNet& n, uint32_t i, i
     const NeuronK p = n.pk[pi];
     const InputK in = n.in[pi];
     const int S = n.S;
-    const int d_a2n = n.d_a2n;
     const int sic_interval = n.sic_interval;
     const int substeps = n.substeps;
     const double hs = n.h, half_h = n.half_h, h_sixth = n.h_sixth;
@@ -358,8 +388,8 @@ __device__ __forceinline__ uint64_t neuron_window(const DevNet& n, uint32_t i, i
     const int32_t* __restrict__ drive_n = n.drive_n;
     const double* __restrict__ drive_noise = n.drive_noise;
 
-    int row = static_cast<int>(T0 % S);  // ring row of the current step
-    int64_t sic_next = first_hit(T0, d_a2n, sic_interval);
+    int row = n.clk_row;  // ring row of the current step
+    int64_t sic_next = n.clk_sic_arrive;  // first_hit(T0, d_a2n, sic_interval)
     const uint32_t live_in = live_arrivals(n, T0);
     const bool sic_live_n = live_in & 1u;
     double V = 0, w = 0, gex = 0, dgex = 0, gin = 0, dgin = 0, Isic = 0, Vr = 0, b = 0;
@@ -371,7 +401,7 @@ __device__ __forceinline__ uint64_t neuron_window(const DevNet& n, uint32_t i, i
     // The drive holds one noise value per noise interval of the window:
     // value nz_j from window step nz_next - steps_per_noise on.
     const int64_t nz_period = has_noise ? in.steps_per_noise : 1;
-    int64_t nz_next = first_hit(T0 + 1, 0, nz_period) + 1;
+    int64_t nz_next = n.clk_nz_next[pi];  // first_hit(T0 + 1, 0, nz_period) + 1
     uint32_t nz_j = 0;
     if (valid) {
         V = n.V[i];
@@ -500,15 +530,15 @@ __device__ __forceinline__ void astro_window(const DevNet& n, uint32_t a, int64_
     double Ca = n.Ca[a], IP3 = n.IP3[a], hh = n.hh[a];
     const double Ca_tot = n.Ca_tot[a], IP3_0 = n.IP3_0[a], tau = n.tau_IP3[a],
                  delta = n.delta_IP3[a];
-    int row = static_cast<int>(T0 % S);  // ring row of the current step
+    int row = n.clk_row;  // ring row of the current step
     double v_n = ring_astro[static_cast<size_t>(row) * LA + a];
     // noise index T / steps_per_noise, redrawn at the window start and at
     // every multiple of steps_per_noise
     const int64_t spn = ap.in.steps_per_noise;
-    int64_t noise_idx = T0 / spn;
-    int64_t noise_next = first_hit(T0 + 1, 0, spn) + 1;
-    int64_t sic_next = first_hit(T0, 0, sic_interval);
-    RecCursor rec(n, T0);
+    int64_t noise_idx = n.clk_anz_idx;    // T0 / spn
+    int64_t noise_next = n.clk_anz_next;  // first_hit(T0 + 1, 0, spn) + 1
+    int64_t sic_next = n.clk_sic_send;    // first_hit(T0, 0, sic_interval)
+    RecCursor rec(n);
     double noise = 0.0;
     uint32_t live_bits = 0;
     // Poisson stream T * kStreamStride + a: its key advances by a constant
@@ -582,7 +612,6 @@ __device__ __forceinline__ uint64_t neuron_window_f32(const DevNet& n, uint32_t 
     const NeuronK pd = n.pk[pi];
     const InputK in = n.in[pi];
     const int S = n.S;
-    const int d_a2n = n.d_a2n;
     const int sic_interval = n.sic_interval;
     const int substeps = n.substeps;
     const double hs = n.h, half_h = n.half_h, h_sixth = n.h_sixth;
@@ -594,8 +623,8 @@ __device__ __forceinline__ uint64_t neuron_window_f32(const DevNet& n, uint32_t 
     const int32_t* __restrict__ drive_n = n.drive_n;
     const float* __restrict__ drive_noise = n.fdrive_noise;
 
-    int row = static_cast<int>(T0 % S);  // ring row of the current step
-    int64_t sic_next = first_hit(T0, d_a2n, sic_interval);
+    int row = n.clk_row;  // ring row of the current step
+    int64_t sic_next = n.clk_sic_arrive;  // first_hit(T0, d_a2n, sic_interval)
     const uint32_t live_in = live_arrivals(n, T0);
     const bool sic_live_n = live_in & 1u;
     // FP64 registers; the conductances are rounded to FP32 at the end of
@@ -610,7 +639,7 @@ __device__ __forceinline__ uint64_t neuron_window_f32(const DevNet& n, uint32_t 
     // The drive holds one noise value per noise interval of the window:
     // value nz_j from window step nz_next - steps_per_noise on.
     const int64_t nz_period = has_noise ? in.steps_per_noise : 1;
-    int64_t nz_next = first_hit(T0 + 1, 0, nz_period) + 1;
+    int64_t nz_next = n.clk_nz_next[pi];  // first_hit(T0 + 1, 0, nz_period) + 1
     uint32_t nz_j = 0;
     if (valid) {
         V = n.V[i];
@@ -755,15 +784,15 @@ __device__ __forceinline__ bool astro_window_f32(const DevNet& n, uint32_t a, in
     double Ca = n.Ca[a], IP3 = n.IP3[a], hh = n.hh[a];
     const float Ca_tot = n.fCa_tot[a], IP3_0 = n.fIP3_0[a], inv_tau = n.finv_tau[a],
                 delta = n.fdelta[a];
-    int row = static_cast<int>(T0 % S);  // ring row of the current step
+    int row = n.clk_row;  // ring row of the current step
     int64_t v_n = ring_astro[static_cast<size_t>(row) * LA + a];
     // noise index T / steps_per_noise, redrawn at the window start and at
     // every multiple of steps_per_noise
     const int64_t spn = ain.steps_per_noise;
-    int64_t noise_idx = T0 / spn;
-    int64_t noise_next = first_hit(T0 + 1, 0, spn) + 1;
-    int64_t sic_next = first_hit(T0, 0, sic_interval);
-    RecCursor rec(n, T0);
+    int64_t noise_idx = n.clk_anz_idx;    // T0 / spn
+    int64_t noise_next = n.clk_anz_next;  // first_hit(T0 + 1, 0, spn) + 1
+    int64_t sic_next = n.clk_sic_send;    // first_hit(T0, 0, sic_interval)
+    RecCursor rec(n);
     float noise = 0.0f;
     uint32_t live_bits = 0;
     // Poisson stream T * kStreamStride + a: its key advances by a constant
@@ -912,6 +941,7 @@ __device__ inline void close_window_with(DevNet& n, uint32_t sic_live) {
     }
     n.c_steps += W;
     n.step = T0 + W;
+    set_clock(n, T0 + W);
     n.open = 0;
 }
 
@@ -1986,7 +2016,9 @@ __global__ __launch_bounds__(256) void k_set_targets(DevNet* nets, const int32_t
                                                      int32_t n_active, const int64_t* targets) {
     const int j = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     if (j < n_active) {
-        nets[active[j]].target = targets[j];
+        DevNet& n = nets[active[j]];
+        n.target = targets[j];
+        set_clock(n, n.step);
     }
 }
 
