@@ -122,6 +122,8 @@ constexpr int64_t kSicWaves = 4096; // SIC wavefronts that fill the device (256 
 constexpr int64_t kGroupWaves = 16384;  // per-step wavefronts beyond which steps are grouped
 constexpr int kDeliverWaves = 512;  // delivery wavefronts sharing out the spiking blocks
 constexpr int kSicWorkers = 0;      // FP32 SIC wavefronts (0: one per FP32 neuron block)
+constexpr int kMaxLanes = 2;        // FP32 lanes (network groups on streams of their own)
+constexpr int kLanes = 2;           // lanes by default
 constexpr size_t kMaskBudget = size_t(256) << 20;  // delivery masks per network
 constexpr size_t kMinChunks = 64;
 
@@ -2557,18 +2559,28 @@ struct Engine {
     CellBlock* d_blocks = nullptr;
     CellBlock* d_nblocks = nullptr;   // neuron blocks only
     int32_t* d_active = nullptr;
+    // FP32 lanes: the FP32 networks are split into `lanes` groups of about
+    // equal cell count, each with its own lists and stream, so that one
+    // group's cell pass (bound by instruction issue) overlaps another's post
+    // pass (bound by memory latency). The groups share no network, so
+    // nothing orders them. The FP64 networks and lane 0 use `stream`.
+    hipStream_t lane_stream[kMaxLanes] = {};
+    hipEvent_t ev_fork = nullptr;
+    hipEvent_t ev_join[kMaxLanes] = {};
+    int sched_lanes = kLanes;  // requested
+    int lanes = 1;             // of the current tables
     // The window's delivery lists: neuron blocks that spiked, appended by
     // k_cells: the FP64 networks' from entry 0, counted in d_work_n[0]
-    // (zeroed by the FP64 networks' k_drive); the FP32 networks' in two
-    // lists by window parity p, from entry nblock_range[1] + p * (FP32
-    // neuron blocks), counted in d_work_n[1 + p] (zeroed by the post pass of
-    // the window before).
+    // (zeroed by the FP64 networks' k_drive); each FP32 lane's in two lists
+    // by window parity p (work32), counted in d_work_n[1 + 2 lane + p]
+    // (zeroed by the lane's post pass of the window before).
     CellBlock* d_work = nullptr;
     int32_t* d_work_n = nullptr;
-    // The FP32 post pass's per-window state and its SIC items, the neuron
-    // blocks of the networks with a live SIC exchange (by parity: two lists
-    // of the FP32 neuron blocks' count), and per slot the first of its
-    // neuron blocks in the FP32 range of d_nblocks and their count.
+    // The FP32 post pass's per-window state (2 per lane, by parity) and its
+    // SIC items, the neuron blocks of the networks with a live SIC exchange
+    // (per lane and parity, a list of the lane's neuron blocks' count), and
+    // per slot the first of its neuron blocks in its lane's range of
+    // d_nblocks and their count.
     PostState* d_post = nullptr;
     int32_t* d_sic_item = nullptr;
     size_t sic_item_cap = 0;
@@ -2606,10 +2618,13 @@ struct Engine {
     // Window sequences replayed from captured graphs (when use_graphs and
     // not profiling); cleared whenever the tables change.
     bool use_graphs = false;
+    // One graph per stream (per lane), replayed each on its own stream
+    // between the fork and the join, so that the lanes overlap as eager
+    // launches do.
     struct GraphEntry {
         int64_t windows;
         int Wg, G, wpb, SG, parity;  // parity of the first window
-        hipGraphExec_t exec;
+        hipGraphExec_t exec[kMaxLanes];
     };
     // Windows launched so far: FP32 windows alternate their lists by the
     // parity of this count, across calls (a graph replays the parities it
@@ -2625,13 +2640,13 @@ struct Engine {
     int64_t timed_windows = 0, pending_windows = 0;
     // [0] cells, [1] compaction, [2] delivery pass, [3] ordered sums, [4] drive
     double kernel_ms[5] = {0, 0, 0, 0, 0};
-    // d_blocks holds two ranges, [cell_range[r], cell_range[r + 1]): the
-    // FP64 networks' blocks, then the FP32 networks'.
-    int cell_range[3] = {0, 0, 0};
-    // d_nblocks and d_active likewise: the FP64 networks' neuron blocks
-    // (active slots), then the FP32's.
-    int nblock_range[3] = {0, 0, 0};
-    int active_range[3] = {0, 0, 0};
+    // d_blocks holds ranges [cell_range[r], cell_range[r + 1]): r = 0 the
+    // FP64 networks' blocks, r = 1 + l those of FP32 lane l (l < lanes).
+    int cell_range[kMaxLanes + 2] = {};
+    // d_nblocks and d_active likewise: neuron blocks (active slots) of the
+    // FP64 networks, then of each FP32 lane.
+    int nblock_range[kMaxLanes + 2] = {};
+    int active_range[kMaxLanes + 2] = {};
 
     explicit Engine(int max_n, int64_t budget_bytes) : max_nets(max_n), budget(budget_bytes) {
         HIP_CHECK(hipGetDevice(&device));
@@ -2649,10 +2664,19 @@ struct Engine {
         HIP_CHECK(hipMalloc(&d_offsets, sizeof(int64_t) * m));
         // The delivery list counts and the post pass's state exist for every
         // batch: k_post empties them even when no network has neurons.
-        HIP_CHECK(hipMalloc(&d_work_n, 3 * sizeof(int32_t)));
-        HIP_CHECK(hipMemsetAsync(d_work_n, 0, 3 * sizeof(int32_t), stream));
-        HIP_CHECK(hipMalloc(&d_post, 2 * sizeof(PostState)));
-        HIP_CHECK(hipMemsetAsync(d_post, 0, 2 * sizeof(PostState), stream));
+        constexpr size_t n_counts = 1 + 2 * kMaxLanes;
+        HIP_CHECK(hipMalloc(&d_work_n, n_counts * sizeof(int32_t)));
+        HIP_CHECK(hipMemsetAsync(d_work_n, 0, n_counts * sizeof(int32_t), stream));
+        HIP_CHECK(hipMalloc(&d_post, 2 * kMaxLanes * sizeof(PostState)));
+        HIP_CHECK(hipMemsetAsync(d_post, 0, 2 * kMaxLanes * sizeof(PostState), stream));
+        lane_stream[0] = stream;
+        for (int l = 1; l < kMaxLanes; ++l) {
+            HIP_CHECK(hipStreamCreateWithFlags(&lane_stream[l], hipStreamNonBlocking));
+        }
+        HIP_CHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+        for (int l = 0; l < kMaxLanes; ++l) {
+            HIP_CHECK(hipEventCreateWithFlags(&ev_join[l], hipEventDisableTiming));
+        }
         HIP_CHECK(hipMalloc(&d_nbr, 2 * sizeof(int32_t) * m));
         h_nbr.assign(2 * m, 0);
         HIP_CHECK(hipMemcpyAsync(d_nbr, h_nbr.data(), 2 * sizeof(int32_t) * m,
@@ -2686,6 +2710,13 @@ struct Engine {
         (void)hipHostFree(h_offsets);
         (void)hipHostFree(h_stage_step);
         (void)hipHostFree(h_stage_src);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        for (int l = 0; l < kMaxLanes; ++l) {
+            if (ev_join[l]) (void)hipEventDestroy(ev_join[l]);
+        }
+        for (int l = 1; l < kMaxLanes; ++l) {
+            if (lane_stream[l]) (void)hipStreamDestroy(lane_stream[l]);
+        }
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -2748,22 +2779,51 @@ struct Engine {
         HIP_CHECK(hipMalloc(&p, cap * sizeof(T)));
     }
 
+    // FP32 lane of every slot: the FP32 networks by cell blocks, largest
+    // first, each to the lane with the fewest blocks so far. One lane while
+    // profiling, so that the per-kernel events see every FP32 kernel.
+    std::vector<int> assign_lanes() {
+        std::vector<int> lane_of(static_cast<size_t>(max_nets), 0);
+        std::vector<std::pair<int64_t, int>> w;
+        for (int sidx = 0; sidx < max_nets; ++sidx) {
+            const NetSlot& s = slots[sidx];
+            if (!s.used || s.dn.precision != ASG_FP32) continue;
+            const int64_t nb = (s.dn.n_neur + kCellBlock - 1) / kCellBlock +
+                               (s.dn.n_astro + kCellBlock - 1) / kCellBlock;
+            w.emplace_back(-nb, sidx);
+        }
+        lanes = profiling ? 1 : std::max(1, std::min<int>(sched_lanes, static_cast<int>(w.size())));
+        std::sort(w.begin(), w.end());
+        int64_t load[kMaxLanes] = {};
+        for (const auto& e : w) {
+            int best = 0;
+            for (int l = 1; l < lanes; ++l) {
+                if (load[l] < load[best]) best = l;
+            }
+            lane_of[static_cast<size_t>(e.second)] = best;
+            load[best] -= e.first;
+        }
+        return lane_of;
+    }
+
     void rebuild_tables() {
         std::vector<CellBlock> blocks, nblocks;
         std::vector<int32_t> active;
         Wg = kWindowCap;
         std::fill(h_nbr.begin(), h_nbr.end(), 0);
-        // Blocks in two contiguous ranges, one per cell kernel: the FP64
-        // networks' blocks, then the FP32 networks'. Within a network the
+        const std::vector<int> lane_of = assign_lanes();
+        // Blocks in contiguous ranges, one per cell launch: the FP64
+        // networks' blocks, then each FP32 lane's. Within a network the
         // neuron blocks come first; its first block is its lead block.
-        for (int range = 0; range < 2; ++range) {
+        for (int range = 0; range < 1 + lanes; ++range) {
             for (int sidx = 0; sidx < max_nets; ++sidx) {
                 const NetSlot& s = slots[sidx];
-                if (!s.used || (s.dn.precision == ASG_FP32) != (range == 1)) continue;
+                if (!s.used || (s.dn.precision == ASG_FP32) != (range > 0)) continue;
+                if (range > 0 && lane_of[static_cast<size_t>(sidx)] != range - 1) continue;
                 active.push_back(sidx);
                 Wg = std::min(Wg, s.W);
-                if (range == 1) {
-                    h_nbr[2 * sidx] = static_cast<int32_t>(nblocks.size()) - nblock_range[1];
+                if (range > 0) {
+                    h_nbr[2 * sidx] = static_cast<int32_t>(nblocks.size()) - nblock_range[range];
                     h_nbr[2 * sidx + 1] =
                         static_cast<int32_t>((s.dn.n_neur + kCellBlock - 1) / kCellBlock);
                 }
@@ -2796,7 +2856,7 @@ struct Engine {
             HIP_CHECK(hipMemcpyAsync(d_nblocks, nblocks.data(), nblocks.size() * sizeof(CellBlock),
                                      hipMemcpyHostToDevice, stream));
             // each neuron block is listed at most once per window (FP32: in
-            // the lists of the window's parity)
+            // its lane's lists of the window's parity)
             grow(d_work, work_cap, nblocks.size() + (nblocks.size() - nblock_range[1]));
             grow(d_sic_item, sic_item_cap, 2 * (nblocks.size() - nblock_range[1]));
         }
@@ -2875,43 +2935,53 @@ struct Engine {
         return static_cast<size_t>(64 * Wg) * (sizeof(double) + sizeof(uint32_t));
     }
 
-    // The drive of the next window of the networks of precision range
-    // `range` as a launch of its own; the FP64 range also empties its
-    // delivery list count.
+    // The stream of range `range` (0 the FP64 networks, 1 + l FP32 lane l).
+    hipStream_t range_stream(int range) const {
+        return range == 0 ? stream : lane_stream[range - 1];
+    }
+
+    // The drive of the next window of the networks of range `range` as a
+    // launch of its own; the FP64 range also empties its delivery list
+    // count.
     void launch_drive(int range) {
         const int count = nblock_range[range + 1] - nblock_range[range];
         if (count == 0) return;
-        hipLaunchKernelGGL(k_drive, dim3(count), dim3(64), drive_lds_bytes(), stream, d_nets,
+        hipLaunchKernelGGL(k_drive, dim3(count), dim3(64), drive_lds_bytes(), range_stream(range),
+                           d_nets,
                            d_nblocks + nblock_range[range], count, Wg,
                            range == 0 ? d_work_n : nullptr);
     }
 
-    // The FP32 delivery list, its count and the post pass's state of window
-    // parity p.
-    CellBlock* work32(int p) const {
-        return d_work + nblock_range[1] + p * (nblock_range[2] - nblock_range[1]);
+    // FP32 lane l's delivery list, its count, the post pass's state and the
+    // SIC items of window parity p.
+    int lane_nblocks(int l) const { return nblock_range[2 + l] - nblock_range[1 + l]; }
+    CellBlock* work32(int l, int p) const {
+        return d_work + nblock_range[1] + 2 * (nblock_range[1 + l] - nblock_range[1]) +
+               p * lane_nblocks(l);
     }
-    int32_t* work_n32(int p) const { return d_work_n + 1 + p; }
-    int32_t* sic_items32(int p) const {
-        return d_sic_item + p * (nblock_range[2] - nblock_range[1]);
+    int32_t* work_n32(int l, int p) const { return d_work_n + 1 + 2 * l + p; }
+    PostState* post32(int l, int p) const { return d_post + 2 * l + p; }
+    int32_t* sic_items32(int l, int p) const {
+        return d_sic_item + 2 * (nblock_range[1 + l] - nblock_range[1]) + p * lane_nblocks(l);
     }
 
-    // Pass 3 of the FP32 networks with the window's close and the next
-    // window's drive (window parity p).
-    void launch_post(int p) {
-        const int n32 = active_range[2] - active_range[1];
+    // Pass 3 of FP32 lane l with the window's close and the next window's
+    // drive (window parity p).
+    void launch_post(int l, int p) {
+        const int n32 = active_range[2 + l] - active_range[1 + l];
         if (n32 == 0) return;
-        const int count = nblock_range[2] - nblock_range[1];
+        const int count = lane_nblocks(l);
         const int C = (n32 + 63) / 64;
         const int D = std::min(count, sched_deliver_waves);
         // SIC wavefronts: by default one per neuron block, so that every
         // live block has one however many networks are live (those beyond
         // the window's items exit at once)
         const int E = sched_sic_workers > 0 ? std::min(count, sched_sic_workers) : count;
-        hipLaunchKernelGGL(k_post, dim3(C + D + E + count), dim3(64), drive_lds_bytes(), stream,
-                           d_nets, d_nblocks + nblock_range[1], count, Wg, C, D, E, work32(p),
-                           work_n32(p), d_post + p, sic_items32(p), d_active + active_range[1],
-                           n32, p, work_n32(p ^ 1), d_post + (p ^ 1));
+        hipLaunchKernelGGL(k_post, dim3(C + D + E + count), dim3(64), drive_lds_bytes(),
+                           lane_stream[l], d_nets, d_nblocks + nblock_range[1 + l], count, Wg, C,
+                           D, E, work32(l, p), work_n32(l, p), post32(l, p), sic_items32(l, p),
+                           d_active + active_range[1 + l], n32, p, work_n32(l, p ^ 1),
+                           post32(l, p ^ 1));
     }
 
     // Window steps per marking / summing wavefront: as requested, or (0) the
@@ -2921,28 +2991,31 @@ struct Engine {
         return static_cast<int64_t>(n_blocks) * Wg > kGroupWaves ? Wg : 1;
     }
 
-    // The cell pass of precision range `range` (FP32: window parity p).
+    // The cell pass of range `range` (0 the FP64 networks; 1 + l FP32 lane
+    // l, window parity p).
     template <int P>
     void launch_cells(int range, int32_t Wg_, int wpb, int p) {
         const int count = cell_range[range + 1] - cell_range[range];
         if (count == 0) return;
         const CellBlock* blocks = d_blocks + cell_range[range];
         const dim3 grid((count + wpb - 1) / wpb);
-        CellBlock* work = P == ASG_FP32 ? work32(p) : d_work;
-        int32_t* work_n = P == ASG_FP32 ? work_n32(p) : d_work_n;
-        PostState* post = d_post + p;
-        int32_t* sic_item = sic_items32(p);
+        const int l = P == ASG_FP32 ? range - 1 : 0;
+        CellBlock* work = P == ASG_FP32 ? work32(l, p) : d_work;
+        int32_t* work_n = P == ASG_FP32 ? work_n32(l, p) : d_work_n;
+        PostState* post = post32(l, p);
+        int32_t* sic_item = sic_items32(l, p);
+        hipStream_t st = range_stream(range);
         switch (wpb) {
             case 4:
-                hipLaunchKernelGGL((k_cells<P, 4>), grid, dim3(4 * kCellBlock), 0, stream, d_nets,
+                hipLaunchKernelGGL((k_cells<P, 4>), grid, dim3(4 * kCellBlock), 0, st, d_nets,
                                    blocks, count, Wg_, work, work_n, p, post, sic_item, d_nbr);
                 break;
             case 2:
-                hipLaunchKernelGGL((k_cells<P, 2>), grid, dim3(2 * kCellBlock), 0, stream, d_nets,
+                hipLaunchKernelGGL((k_cells<P, 2>), grid, dim3(2 * kCellBlock), 0, st, d_nets,
                                    blocks, count, Wg_, work, work_n, p, post, sic_item, d_nbr);
                 break;
             default:
-                hipLaunchKernelGGL((k_cells<P, 1>), grid, dim3(kCellBlock), 0, stream, d_nets,
+                hipLaunchKernelGGL((k_cells<P, 1>), grid, dim3(kCellBlock), 0, st, d_nets,
                                    blocks, count, Wg_, work, work_n, p, post, sic_item, d_nbr);
                 break;
         }
@@ -2961,43 +3034,71 @@ struct Engine {
         return g;
     }
 
-    // The launches of `windows` windows on the stream (eager, or recorded
-    // into a graph when the stream is capturing); with `marks`, one window
-    // in prof_every is bracketed by events (6 per timed window). Returns the
+    // Lanes beyond the first wait for what `stream` holds so far (fork), and
+    // `stream` for what they hold (join).
+    void fork_lanes() {
+        if (lanes > 1) {
+            HIP_CHECK(hipEventRecord(ev_fork, stream));
+            for (int l = 1; l < lanes; ++l) {
+                HIP_CHECK(hipStreamWaitEvent(lane_stream[l], ev_fork, 0));
+            }
+        }
+    }
+    void join_lanes() {
+        for (int l = 1; l < lanes; ++l) {
+            HIP_CHECK(hipEventRecord(ev_join[l], lane_stream[l]));
+            HIP_CHECK(hipStreamWaitEvent(stream, ev_join[l], 0));
+        }
+    }
+
+    // The launches of `windows` windows, the first of sequence number seq0
+    // (eager, or recorded into a graph when the streams are capturing):
+    // every stream's (only < 0), or only those of lane `only`'s stream (for
+    // the FP64 networks: lane 0's). With `marks`, one window in prof_every
+    // is bracketed by events on `stream` (6 per timed window). Returns the
     // windows timed.
     //
     // FP64 networks: drive, cells, compaction, delivery (marking + SIC),
-    // ordered sums (whose lead wavefront closes the window). FP32 networks:
+    // ordered sums (whose lead wavefront closes the window). FP32 lanes:
     // cells, then k_post (delivery, SIC of the live networks, the next
     // window's drive, close); the first window's drive is launched before
-    // the loop.
-    int64_t launch_windows(int64_t windows, int G, int wpb, int SG, bool marks) {
+    // the loop. Each lane's launches go to its own stream.
+    int64_t launch_windows(int64_t windows, int G, int wpb, int SG, bool marks, int64_t seq0,
+                           int only = -1) {
         int64_t timed = 0;
-        if (windows > 0) {
-            launch_drive(1);
+        if (windows <= 0) {
+            return 0;
         }
-        const int n64 = active_range[1];
+        auto on = [&](int l) { return only < 0 || only == l; };
+        for (int l = 0; l < lanes; ++l) {
+            if (on(l)) launch_drive(1 + l);
+        }
+        const int n64 = on(0) ? active_range[1] : 0;
         for (int64_t wdx = 0; wdx < windows; ++wdx) {
-            const int p = static_cast<int>(win_seq++ & 1);  // the window's parity
+            const int p = static_cast<int>((seq0 + wdx) & 1);  // the window's parity
             const bool timed_here = marks && (prof_seq++ % prof_every) == 0;
             auto mark = [&](int k) {
                 if (timed_here) HIP_CHECK(hipEventRecord(events[6 * timed + k], stream));
             };
             mark(0);
-            launch_drive(0);
+            if (n64 > 0) launch_drive(0);
             mark(1);
-            launch_cells<ASG_FP64>(0, Wg, wpb, p);
-            launch_cells<ASG_FP32>(1, Wg, wpb, p);
+            if (n64 > 0) launch_cells<ASG_FP64>(0, Wg, wpb, p);
+            for (int l = 0; l < lanes; ++l) {
+                if (on(l)) launch_cells<ASG_FP32>(1 + l, Wg, wpb, p);
+            }
             mark(2);
             if (n64 > 0) {
                 hipLaunchKernelGGL(k_compact, dim3(n64, Wg), dim3(64), 0, stream, d_nets,
                                    d_active);
             }
             mark(3);
-            launch_deliver64(G, Wg, SG);
-            launch_post(p);
+            if (n64 > 0) launch_deliver64(G, Wg, SG);
+            for (int l = 0; l < lanes; ++l) {
+                if (on(l)) launch_post(l, p);
+            }
             mark(4);
-            const int sum_items = cell_range[1] * ((Wg + SG - 1) / SG);
+            const int sum_items = n64 > 0 ? cell_range[1] * ((Wg + SG - 1) / SG) : 0;
             if (sum_items > 0) {
                 hipLaunchKernelGGL(k_sum, dim3((sum_items + 3) / 4), dim3(256), 0, stream, d_nets,
                                    d_blocks, sum_items, Wg, SG);
@@ -3008,31 +3109,43 @@ struct Engine {
         return timed;
     }
 
-    // Graph of `windows` windows for the current tables and schedule,
-    // captured on first use.
-    hipGraphExec_t graph_for(int64_t windows, int G, int wpb, int SG) {
+    // Graphs of `windows` windows from the current window sequence number,
+    // for the current tables and schedule, captured on first use.
+    const GraphEntry& graph_for(int64_t windows, int G, int wpb, int SG) {
         const int parity = static_cast<int>(win_seq & 1);
         for (const GraphEntry& g : graphs) {
             if (g.windows == windows && g.G == G && g.wpb == wpb && g.SG == SG && g.Wg == Wg &&
                 g.parity == parity) {
-                win_seq += windows;  // what launching them would have counted
-                return g.exec;
+                return g;
             }
         }
-        hipGraph_t graph = nullptr;
-        HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        launch_windows(windows, G, wpb, SG, false);
-        HIP_CHECK(hipStreamEndCapture(stream, &graph));
-        hipGraphExec_t exec = nullptr;
-        const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        HIP_CHECK(e);
-        graphs.push_back(GraphEntry{windows, Wg, G, wpb, SG, parity, exec});
-        return exec;
+        GraphEntry ge{windows, Wg, G, wpb, SG, parity, {}};
+        try {
+            for (int l = 0; l < lanes; ++l) {
+                hipGraph_t graph = nullptr;
+                HIP_CHECK(hipStreamBeginCapture(lane_stream[l], hipStreamCaptureModeThreadLocal));
+                launch_windows(windows, G, wpb, SG, false, win_seq, l);
+                HIP_CHECK(hipStreamEndCapture(lane_stream[l], &graph));
+                const hipError_t e = hipGraphInstantiate(&ge.exec[l], graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                HIP_CHECK(e);
+            }
+        } catch (...) {
+            for (hipGraphExec_t x : ge.exec) {
+                if (x) (void)hipGraphExecDestroy(x);
+            }
+            throw;
+        }
+        graphs.push_back(ge);
+        return graphs.back();
     }
 
     void clear_graphs() {
-        for (GraphEntry& g : graphs) (void)hipGraphExecDestroy(g.exec);
+        for (GraphEntry& g : graphs) {
+            for (hipGraphExec_t x : g.exec) {
+                if (x) (void)hipGraphExecDestroy(x);
+            }
+        }
         graphs.clear();
     }
 
@@ -3064,12 +3177,22 @@ struct Engine {
         if (profiling) {
             // per-kernel events need eager launches
             ensure_events(windows);
-            pending_windows = launch_windows(windows, G, sched_cell_waves, SG, true);
+            fork_lanes();
+            pending_windows = launch_windows(windows, G, sched_cell_waves, SG, true, win_seq);
+            join_lanes();
         } else if (use_graphs && windows > 0) {
-            HIP_CHECK(hipGraphLaunch(graph_for(windows, G, sched_cell_waves, SG), stream));
+            const GraphEntry& g = graph_for(windows, G, sched_cell_waves, SG);
+            fork_lanes();
+            for (int l = 0; l < lanes; ++l) {
+                HIP_CHECK(hipGraphLaunch(g.exec[l], lane_stream[l]));
+            }
+            join_lanes();
         } else {
-            launch_windows(windows, G, sched_cell_waves, SG, false);
+            fork_lanes();
+            launch_windows(windows, G, sched_cell_waves, SG, false, win_seq);
+            join_lanes();
         }
+        win_seq += windows;
         HIP_CHECK(hipGetLastError());
         pending = true;
         if (!async) wait();
@@ -3083,11 +3206,11 @@ struct Engine {
     // schedule changes launch geometry only, never results, so tuning
     // advances the simulation like any other call.
     struct Sched {
-        int wpb, g, sg, dw, sw;
+        int wpb, g, sg, dw, sw, lanes;
     };
     Sched current_schedule() const {
         return Sched{sched_cell_waves, sched_sic_group, sched_step_group, sched_deliver_waves,
-                     sched_sic_workers};
+                     sched_sic_workers, sched_lanes};
     }
     void apply_schedule(const Sched& c) {
         sched_cell_waves = c.wpb;
@@ -3095,6 +3218,14 @@ struct Engine {
         sched_step_group = c.sg;
         sched_deliver_waves = c.dw;
         sched_sic_workers = c.sw;
+        set_lanes(c.lanes);
+    }
+    // A new lane count takes effect with the next table rebuild.
+    void set_lanes(int l) {
+        if (l != sched_lanes) {
+            sched_lanes = l;
+            dirty = true;
+        }
     }
 
     // Times every candidate; applies and returns the fastest (median ms per
@@ -3105,6 +3236,10 @@ struct Engine {
         for (int r = 0; r < rounds; ++r) {
             for (size_t c = 0; c < cand.size(); ++c) {
                 apply_schedule(cand[c]);
+                if (dirty) {
+                    // new tables (and graphs): built by an untimed call
+                    step(static_cast<int64_t>(wpt) * Wg, true);
+                }
                 HIP_CHECK(hipEventRecord(e0, stream));
                 step(static_cast<int64_t>(wpt) * Wg, true);
                 HIP_CHECK(hipEventRecord(e1, stream));
@@ -3143,6 +3278,15 @@ struct Engine {
         float best_t = 0.0f;
         try {
             std::vector<Sched> cand;
+            if (active_range[1 + lanes] - active_range[1] > 1) {  // FP32 networks to split
+                for (int l = 1; l <= kMaxLanes; ++l) {
+                    Sched c = current_schedule();
+                    c.lanes = l;
+                    cand.push_back(c);
+                }
+                best_t = tune_among(cand, wpt, rounds, e0, e1);
+                cand.clear();
+            }
             for (int wpb : {1, 2, 4}) {
                 Sched c = current_schedule();
                 c.wpb = wpb;
@@ -3161,7 +3305,7 @@ struct Engine {
                 }
                 best_t = tune_among(cand, wpt, rounds, e0, e1);
             }
-            if (active_range[2] > active_range[1]) {  // FP32 networks
+            if (active_range[1 + lanes] > active_range[1]) {  // FP32 networks
                 cand.clear();
                 // Not the SIC wavefront count: the SIC load grows with the
                 // activity (networks whose astrocytes pass the threshold),
@@ -3782,13 +3926,13 @@ asg_status asg_batch_set_graphs(asg_batch* batch, int32_t on) {
 asg_status asg_batch_set_schedule(asg_batch* batch, const asg_schedule* sched) {
     if (batch == nullptr || sched == nullptr) return fail(ASG_ERR_INVALID, "bad argument");
     const int w = sched->cell_waves, g = sched->sic_group, sg = sched->step_group;
-    const int dw = sched->deliver_waves, sw = sched->sic_workers;
+    const int dw = sched->deliver_waves, sw = sched->sic_workers, ln = sched->lanes;
     if ((w != 1 && w != 2 && w != 4) || (g != 0 && g != 1 && g != 2 && g != 4 && g != 8 && g != 16) ||
         sg < 0 || sg > asg::kWindowCap || (dw != 0 && (dw < 64 || dw > 4096)) ||
-        (sw != 0 && (sw < 64 || sw > 8192))) {
+        (sw != 0 && (sw < 64 || sw > 8192)) || ln < 0 || ln > asg::kMaxLanes) {
         return fail(ASG_ERR_INVALID,
                     "cell_waves must be 1, 2 or 4; sic_group 0, 1, 2, 4, 8 or 16; step_group 0-16; "
-                    "deliver_waves 0 or 64-4096; sic_workers 0 or 64-8192");
+                    "deliver_waves 0 or 64-4096; sic_workers 0 or 64-8192; lanes 0, 1 or 2");
     }
     return guarded([&]() -> asg_status {
         batch->eng->wait();
@@ -3797,6 +3941,7 @@ asg_status asg_batch_set_schedule(asg_batch* batch, const asg_schedule* sched) {
         batch->eng->sched_step_group = sg;
         batch->eng->sched_deliver_waves = dw == 0 ? asg::kDeliverWaves : dw;
         batch->eng->sched_sic_workers = sw;
+        batch->eng->set_lanes(ln == 0 ? asg::kLanes : ln);
         return ASG_OK;
     });
 }
@@ -3808,6 +3953,7 @@ asg_status asg_batch_get_schedule(asg_batch* batch, asg_schedule* out) {
     out->step_group = batch->eng->sched_step_group;
     out->deliver_waves = batch->eng->sched_deliver_waves;
     out->sic_workers = batch->eng->sched_sic_workers;
+    out->lanes = batch->eng->sched_lanes;
     out->ms_per_window = batch->eng->tuned_ms_per_window;
     return ASG_OK;
 }
@@ -3843,6 +3989,9 @@ asg_status set_profiling(Engine& e, int32_t every) {
     if (every < 0) return fail(ASG_ERR_INVALID, "profiling period must be >= 0");
     return guarded([&]() -> asg_status {
         e.wait();
+        if (e.profiling != (every != 0)) {
+            e.dirty = true;  // profiling runs the FP32 networks in one lane
+        }
         e.profiling = every != 0;
         e.prof_every = every > 0 ? every : 1;
         e.prof_seq = 0;

@@ -416,8 +416,10 @@ int64_t asg_batch_bytes_in_use(const asg_batch* batch);
  * device), step group of the FP64 networks (window steps per spike-marking
  * and per summing wavefront, 1-16; 0 = the whole window for large batches,
  * 1 otherwise), the wavefronts that share out the delivery list (64-4096,
- * 0 = 512) and the FP32 post pass's SIC wavefronts (64-8192; 0 = one per
- * neuron block, the default).
+ * 0 = 512), the FP32 post pass's SIC wavefronts (64-8192; 0 = one per
+ * neuron block, the default) and the FP32 lanes (1 or 2; 0 = 2): groups of
+ * the FP32 networks of about equal cell count whose windows run on streams
+ * of their own, so that one group's cell pass overlaps another's post pass.
  * Results do not depend on it. ms_per_window is the autotuner's measured
  * time (output only). */
 typedef struct asg_schedule {
@@ -426,18 +428,21 @@ typedef struct asg_schedule {
     int32_t step_group;
     int32_t deliver_waves;
     int32_t sic_workers;
+    int32_t lanes;
     float ms_per_window;
 } asg_schedule;
 asg_status asg_batch_set_schedule(asg_batch* batch, const asg_schedule* sched);
 asg_status asg_batch_get_schedule(asg_batch* batch, asg_schedule* out);
 
-/* Autotuner: tunes the schedule one knob at a time (cell waves, then the
- * FP64 networks' SIC and step groups when the batch has FP64 networks, then
- * the delivery wavefront count when it has FP32 networks), timing
- * each candidate as step calls of windows_per_try windows, `rounds` times
- * round-robin, and keeping the smallest median (written to *chosen if not
- * NULL). The networks advance by the windows run, exactly as step calls of
- * that length would advance them. */
+/* Autotuner: tunes the schedule one knob at a time (the FP32 lanes when the
+ * batch has two or more FP32 networks, cell waves, then the FP64 networks'
+ * SIC and step groups when the batch has FP64 networks, then the delivery
+ * wavefront count when it has FP32 networks), timing each candidate as step
+ * calls of windows_per_try windows, `rounds` times round-robin, and keeping
+ * the smallest median (written to *chosen if not NULL); a candidate that
+ * changes the lanes first runs one untimed call of the same length. The
+ * networks advance by the windows run, exactly as step calls of that length
+ * would advance them. */
 asg_status asg_batch_autotune(asg_batch* batch, int32_t windows_per_try, int32_t rounds,
                               asg_schedule* chosen);
 

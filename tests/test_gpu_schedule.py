@@ -68,6 +68,9 @@ def test_every_schedule_gives_the_same_bits():
     for dw in (64, 512, 4096):
         for sw in (64, 2048, 8192):
             _same(ref, _run(lambda b, d=dw, e=sw: b.set_schedule(1, 0, 0, d, e)))
+    # the FP32 networks in one lane, and split over two streams
+    for lanes in (1, 2):
+        _same(ref, _run(lambda b, n=lanes: b.set_schedule(1, 0, 0, 0, 0, n)))
 
 
 def test_graph_replays_equal_eager_launches():
@@ -77,6 +80,11 @@ def test_graph_replays_equal_eager_launches():
     print("[gpu case] graphs", flush=True)
     ref = _run(lambda b: None, chunks=(1000, 1000, 1000))
     _same(ref, _run(lambda b: b.set_graphs(True), chunks=(1000, 1000, 1000)))
+
+    def one_lane_graphs(b):
+        b.set_schedule(1, 0, 0, 0, 0, 1)
+        b.set_graphs(True)
+    _same(ref, _run(one_lane_graphs, chunks=(1000, 1000, 1000)))
     # uneven chunks: several graphs (one per window count)
     ref = _run(lambda b: None)
     _same(ref, _run(lambda b: b.set_graphs(True)))
@@ -103,6 +111,7 @@ def test_autotune_then_step_equals_plain_run():
     assert chosen["step_group"] >= 1
     assert chosen["deliver_waves"] in (512, 1024, 2048)
     assert chosen["sic_workers"] == 0
+    assert chosen["lanes"] in (1, 2)
     assert chosen["ms_per_window"] > 0.0
     done = b.net_step(slots[0])
     assert 0 < done < 4000 and all(b.net_step(s) == done for s in slots)
