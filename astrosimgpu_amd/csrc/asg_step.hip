@@ -1807,13 +1807,14 @@ __global__ __launch_bounds__(64) void k_post(DevNet* nets, const CellBlock* nblo
             other_post->sic_items = 0;
         }
     } else if (e < C + D) {
-        const int items = *work_n;
+        // each of the lane's neuron blocks is listed at most once a window
+        const int items = min(*work_n, n_nblocks);
         for (int j = e - C; j < items; j += D) {
             const CellBlock cb = work[j];
             deliver_fixed(nets[cb.net], cb.first, lane);
         }
     } else if (e < C + D + E) {
-        const uint32_t items = post->sic_items;
+        const uint32_t items = min(post->sic_items, static_cast<uint32_t>(n_nblocks));
         for (uint32_t j = static_cast<uint32_t>(e - C - D); j < items; j += static_cast<uint32_t>(E)) {
             const CellBlock cb = nblocks[sic_item[j]];
             DevNet& n = nets[cb.net];
@@ -2551,6 +2552,9 @@ void build_slot(NetSlot& s, const asg_net_desc& d, int64_t total, hipStream_t st
 // A set of networks stepped together. asg_net wraps an engine with one slot.
 struct Engine {
     hipStream_t stream = nullptr;
+    // New networks' buffers are filled on a stream of their own, so that an
+    // add need not wait for a step call in flight.
+    hipStream_t up_stream = nullptr;
     int device = 0;
     int max_nets = 0;
     int64_t budget = 0;
@@ -2651,6 +2655,7 @@ struct Engine {
     explicit Engine(int max_n, int64_t budget_bytes) : max_nets(max_n), budget(budget_bytes) {
         HIP_CHECK(hipGetDevice(&device));
         HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        HIP_CHECK(hipStreamCreateWithFlags(&up_stream, hipStreamNonBlocking));
         slots.resize(static_cast<size_t>(max_n));
         const size_t m = static_cast<size_t>(max_n);
         HIP_CHECK(hipMalloc(&d_nets, sizeof(DevNet) * m));
@@ -2717,6 +2722,7 @@ struct Engine {
         for (int l = 1; l < kMaxLanes; ++l) {
             if (lane_stream[l]) (void)hipStreamDestroy(lane_stream[l]);
         }
+        if (up_stream) (void)hipStreamDestroy(up_stream);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -2867,6 +2873,15 @@ struct Engine {
         }
         HIP_CHECK(hipMemcpyAsync(d_nbr, h_nbr.data(), h_nbr.size() * sizeof(int32_t),
                                  hipMemcpyHostToDevice, stream));
+        // Every list starts empty under the new tables. A lane's post pass
+        // empties the lists of the parity after its own, so after a call the
+        // lists of its last window's parity still hold that window's counts;
+        // that is harmless while the lane runs on (its next window has the
+        // other parity), not for a lane that stops and later runs again on
+        // other networks: its stale entries would name blocks of networks
+        // that may be gone, and its appends would run past its lists.
+        HIP_CHECK(hipMemsetAsync(d_work_n, 0, (1 + 2 * kMaxLanes) * sizeof(int32_t), stream));
+        HIP_CHECK(hipMemsetAsync(d_post, 0, 2 * kMaxLanes * sizeof(PostState), stream));
         HIP_CHECK(hipStreamSynchronize(stream));
         dirty = false;
         clear_graphs();
@@ -2884,15 +2899,21 @@ struct Engine {
         const DerivedConn* none = nullptr;
         const Plan pl = plan_of(d, none);
         if (budget > 0 && bytes_in_use() + static_cast<int64_t>(pl.bytes) > budget) return -2;
-        wait();
+        // No wait for a step call in flight: the free slot is in none of the
+        // tables it runs on (a removed slot's last call was drained by the
+        // remove), so its device descriptor and buffers are not read until
+        // the next table rebuild, by which time the uploads have finished.
         NetSlot& s = slots[free_slot];
         try {
-            build_slot(s, d, total, stream);
+            build_slot(s, d, total, up_stream);
+            HIP_CHECK(hipMemcpyAsync(d_nets + free_slot, &s.dn, sizeof(DevNet),
+                                     hipMemcpyHostToDevice, up_stream));
+            HIP_CHECK(hipStreamSynchronize(up_stream));
         } catch (...) {
+            (void)hipStreamSynchronize(up_stream);
             release(s);
             throw;
         }
-        upload(free_slot);
         dirty = true;
         return free_slot;
     }

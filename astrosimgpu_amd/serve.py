@@ -138,17 +138,20 @@ class Server:
         ready: list = []         # built, waiting for room
         running: dict = {}       # slot -> job
         done: list = []
-        peak_running = 0
-        peak_bytes = 0
+        peak = [0, 0]            # running jobs, device bytes
         i = 0
-        while len(done) < len(jobs):
+        in_flight = False        # a chunk launched and not yet drained
+
+        def intake():
+            """Host work that needs no finished chunk: arrivals to the
+            builders, built networks to the queue, admission (a network's
+            upload does not wait for the chunk in flight)."""
+            nonlocal i, building
             now = clock()
-            # arrivals -> builders
             while i < len(pending) and pending[i].arrival_s <= now:
                 j = pending[i]
                 building.append((j, self.pool.submit(self.builder, j)))
                 i += 1
-            # finished builds -> ready
             still = []
             for j, fut in building:
                 if fut.done():
@@ -171,34 +174,48 @@ class Server:
                 j.slot = slot
                 j.admitted_s = clock()
                 running[slot] = j
-            peak_running = max(peak_running, len(running))
-            peak_bytes = max(peak_bytes, self.batch.bytes_in_use())
-            if not running:
-                if building:
-                    cf.wait([f for _, f in building], return_when=cf.FIRST_COMPLETED)
-                elif i < len(pending):
-                    sleep(max(0.0, pending[i].arrival_s - clock()))
-                elif ready:
-                    raise RuntimeError(f"job {ready[0].id} does not fit an empty batch")
+            peak[0] = max(peak[0], len(running))
+            peak[1] = max(peak[1], self.batch.bytes_in_use())
+
+        while len(done) < len(jobs):
+            intake()
+            if in_flight:
+                # the chunk's rasters (waits for the device), then the jobs
+                # it finished leave and free their memory
+                drained = self.batch.spikes_all()
+                in_flight = False
+                finished = []
+                for slot, j in running.items():
+                    if slot not in drained:
+                        continue  # admitted while the chunk ran: not in it
+                    st, src = drained[slot]
+                    j.spike_steps.append(st)
+                    j.spike_src.append(src)
+                    if self.batch.net_step(slot) >= j.total_steps:
+                        finished.append(slot)
+                for slot in finished:
+                    j = running.pop(slot)
+                    self.batch.remove(slot)
+                    j.done_s = clock()
+                    j.model = None
+                    done.append(j)
+                    if progress is not None:
+                        progress(len(done), len(jobs))
+                if finished:
+                    intake()  # waiting jobs into the room just freed
+            if running:
+                # the next chunk for every running job; the host takes in
+                # arrivals and uploads new jobs while it runs
+                self.batch.step(self.chunk_steps, wait=False)
+                in_flight = True
                 continue
-            # one chunk for every running job
-            self.batch.step(self.chunk_steps)
-            finished = []
-            drained = self.batch.spikes_all()
-            for slot, j in running.items():
-                st, src = drained[slot]
-                j.spike_steps.append(st)
-                j.spike_src.append(src)
-                if self.batch.net_step(slot) >= j.total_steps:
-                    finished.append(slot)
-            for slot in finished:
-                j = running.pop(slot)
-                self.batch.remove(slot)
-                j.done_s = clock()
-                j.model = None
-                done.append(j)
-                if progress is not None:
-                    progress(len(done), len(jobs))
+            if building:
+                cf.wait([f for _, f in building], return_when=cf.FIRST_COMPLETED)
+            elif i < len(pending):
+                sleep(max(0.0, pending[i].arrival_s - clock()))
+            elif ready:
+                raise RuntimeError(f"job {ready[0].id} does not fit an empty batch")
+        peak_running, peak_bytes = peak
         makespan = clock()
         ns = sum(j.n * j.total_steps for j in done)
         lat = np.array([j.latency_s for j in done])

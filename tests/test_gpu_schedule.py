@@ -73,6 +73,33 @@ def test_every_schedule_gives_the_same_bits():
         _same(ref, _run(lambda b, n=lanes: b.set_schedule(1, 0, 0, 0, 0, n)))
 
 
+def test_a_lane_that_stops_and_runs_again_starts_from_empty_lists():
+    """Calls alternate between one lane and two, each an odd number of
+    10-step windows, so the second lane runs again on the window parity of
+    its last window before it stopped (whose lists that lane's post pass had
+    not emptied): the bits of a run that never changed lanes."""
+    print("[gpu case] lanes stop and resume", flush=True)
+    chunks = (30, 10, 30, 10, 30, 10, 30, 50)
+    total = sum(chunks)
+    ref = _run(lambda b: None, chunks=chunks, total=total)
+    models = _models()
+    b = A.Batch(8)
+    slots = [b.add(m, total) for m in models]
+    parts = {s: ([], []) for s in slots}
+    for k, c in enumerate(chunks):
+        b.set_schedule(1, 0, 0, 0, 0, 2 if k % 2 == 0 else 1)
+        b.step(c)
+        for s in slots:
+            st, src = b.spikes(s)
+            parts[s][0].append(st)
+            parts[s][1].append(src)
+        b.state(slots[1], "V")
+    out = [(np.concatenate(parts[s][0]), np.concatenate(parts[s][1]), b.state(s, "V"),
+            b.state(s, "Ca")) for s in slots]
+    b.close()
+    _same(ref, out)
+
+
 def test_graph_replays_equal_eager_launches():
     """The same 1000-step call replayed from one graph several times, with
     eager work (raster drains, state reads) and syncs between the replays;

@@ -57,8 +57,11 @@ class FakeBatch:
         del self.nets[slot]
         self.log.append(("remove", slot, self.bytes_in_use()))
 
-    def step(self, n):
+    def step(self, n, wait=True):
+        # the chunk covers the networks present at the call; ones added
+        # while it "runs" (before the drain) start with the next chunk
         self.clock.t += n * 2e-5
+        self.log.append(("step", len(self.nets), self.bytes_in_use()))
         for v in self.nets.values():
             v[0] = min(v[1], v[0] + n)
 
@@ -143,6 +146,7 @@ def test_a_job_larger_than_the_budget_is_not_admitted_ahead_of_order():
     rep = srv.run(jobs, clock=clk, sleep=clk.sleep)
     srv.close()
     adds = [slot for ev, slot, _ in fb.log if ev == "add"]
+    assert all(j.total_steps > 0 and fb.log for j in jobs)
     assert len(adds) == 4 and rep.jobs == 4
     ids = [j.id for j in sorted(jobs, key=lambda j: j.admitted_s)]
     assert ids == [0, 1, 2, 3]
@@ -160,3 +164,24 @@ def test_a_job_that_cannot_fit_an_empty_batch_is_an_error():
         raise AssertionError("expected an error")
     finally:
         srv.close()
+
+
+def test_jobs_admitted_while_a_chunk_runs_join_the_next_one():
+    """The loop launches a chunk, then takes in arrivals and admits jobs
+    while it runs: every job still runs to its total in whole chunks (its
+    last one cut at the total), and no chunk is launched with no job."""
+    base = config_path("paper_sparse")
+    jobs = _jobs()
+    clk = FakeClock()
+    fb = FakeBatch(16, 1 << 40, clk)
+    srv = S.Server(base, chunk_steps=300, batch=fb, executor=InlineExecutor())
+    rep = srv.run(jobs, clock=clk, sleep=clk.sleep)
+    srv.close()
+    assert rep.jobs == len(jobs)
+    assert all(running > 0 for ev, running, _ in fb.log if ev == "step")
+    for j in jobs:
+        assert j.arrival_s <= j.admitted_s < j.done_s
+    # chunks overlapped admission: some job was admitted between a chunk's
+    # launch and its drain (an add logged right after a step)
+    evs = [ev for ev, _, _ in fb.log]
+    assert any(a == "step" and b == "add" for a, b in zip(evs, evs[1:]))
