@@ -55,10 +55,13 @@
 #include <string>
 #include <vector>
 
+#include <rocprim/device/device_segmented_radix_sort.hpp>
+
 #include "asg.h"
 #include "amath.h"
 #include "cell_math.h"
 #include "cell_math_f32.h"
+#include "raster_order.h"
 #include "rng.h"
 
 namespace asg {
@@ -2037,19 +2040,31 @@ __global__ __launch_bounds__(256) void k_raster_heads(const DevNet* nets, const 
 }
 
 // Packs the raster entries of active network blockIdx.y (heads[y] of them)
-// at offsets[y] of one staging buffer and empties that network's raster.
+// at offsets[y] of one staging buffer and empties that network's raster:
+// as (step, source) pairs, or, with out_key, as keys (step - bases[y]) *
+// 2^32 + source, which sort like the pairs (the host checks that every step
+// offset fits 32 bits).
 __global__ __launch_bounds__(256) void k_raster_gather(DevNet* nets, const int32_t* active,
                                                        const int64_t* heads,
                                                        const int64_t* offsets, int64_t* out_step,
-                                                       uint32_t* out_src) {
+                                                       uint32_t* out_src, const int64_t* bases,
+                                                       uint64_t* out_key) {
     DevNet& n = nets[active[blockIdx.y]];
     const int64_t cnt = heads[blockIdx.y];
     const int64_t off = offsets[blockIdx.y];
     const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-    for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < cnt;
-         e += stride) {
-        out_step[off + e] = n.r_step[e];
-        out_src[off + e] = n.r_src[e];
+    if (out_key != nullptr) {
+        const int64_t base = bases[blockIdx.y];
+        for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < cnt;
+             e += stride) {
+            out_key[off + e] = (static_cast<uint64_t>(n.r_step[e] - base) << 32) | n.r_src[e];
+        }
+    } else {
+        for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < cnt;
+             e += stride) {
+            out_step[off + e] = n.r_step[e];
+            out_src[off + e] = n.r_src[e];
+        }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0 && cnt > 0) {
         n.r_head = 0;  // every kernel that appends has finished (same stream)
@@ -2329,6 +2344,7 @@ struct NetSlot {
     size_t bytes = 0;
     int W = 1;
     int64_t step = 0;
+    int64_t drained_to = 0;       // the raster holds entries of steps [drained_to, step)
     int64_t total = 0;            // run length; INT64_MAX for asg_net
     void* rec_alloc = nullptr;    // recording ring (owned separately: it can be replaced)
     int64_t rec_read = 0;         // records handed to the host so far
@@ -2541,6 +2557,7 @@ void build_slot(NetSlot& s, const asg_net_desc& d, int64_t total, hipStream_t st
     s.bytes = pl.bytes;
     s.W = static_cast<int>(pl.Wmax);
     s.step = 0;
+    s.drained_to = 0;
     s.total = total;
     s.used = true;
 }
@@ -2603,14 +2620,26 @@ struct Engine {
     int64_t* d_heads = nullptr;
     int32_t* h_flags = nullptr;
     int32_t* d_flags = nullptr;
-    int64_t* h_offsets = nullptr;
+    int64_t* h_offsets = nullptr;      // max_nets + 1: segment bounds of the staging buffer
     int64_t* d_offsets = nullptr;
+    int64_t* h_bases = nullptr;        // max_nets: step of key offset 0 per network
+    int64_t* d_bases = nullptr;
+    // The drain's sort: packed keys of every entry, sorted per network on
+    // the device (rocPRIM's segmented radix sort), and its scratch.
+    uint64_t* d_stage_key = nullptr;
+    uint64_t* d_stage_key2 = nullptr;
+    uint64_t* h_stage_key = nullptr;
+    void* d_sort_temp = nullptr;
+    size_t sort_temp_cap = 0;
     int64_t* d_stage_step = nullptr;
     uint32_t* d_stage_src = nullptr;
     int64_t* h_stage_step = nullptr;
     uint32_t* h_stage_src = nullptr;
-    size_t stage_cap = 0;
+    size_t stage_cap = 0;       // of the key arrays
+    size_t stage_pair_cap = 0;  // of the (step, source) arrays
     bool raster_dirty = false;         // steps ran since the last drain
+    // scratch of order_raster
+    std::vector<uint64_t> sort_keys, sort_tmp;
     // Launch schedule (results do not depend on it): cell-kernel wavefronts
     // per workgroup (1, 2, 4) and SIC group (1-16 steps, 0 = by batch size).
     int sched_cell_waves = 1;
@@ -2662,11 +2691,13 @@ struct Engine {
         HIP_CHECK(hipHostMalloc(&h_targets, sizeof(int64_t) * m, hipHostMallocDefault));
         HIP_CHECK(hipHostMalloc(&h_heads, sizeof(int64_t) * m, hipHostMallocDefault));
         HIP_CHECK(hipHostMalloc(&h_flags, sizeof(int32_t) * m, hipHostMallocDefault));
-        HIP_CHECK(hipHostMalloc(&h_offsets, sizeof(int64_t) * m, hipHostMallocDefault));
+        HIP_CHECK(hipHostMalloc(&h_offsets, sizeof(int64_t) * (m + 1), hipHostMallocDefault));
+        HIP_CHECK(hipHostMalloc(&h_bases, sizeof(int64_t) * m, hipHostMallocDefault));
         HIP_CHECK(hipMalloc(&d_targets, sizeof(int64_t) * m));
         HIP_CHECK(hipMalloc(&d_heads, sizeof(int64_t) * m));
         HIP_CHECK(hipMalloc(&d_flags, sizeof(int32_t) * m));
-        HIP_CHECK(hipMalloc(&d_offsets, sizeof(int64_t) * m));
+        HIP_CHECK(hipMalloc(&d_offsets, sizeof(int64_t) * (m + 1)));
+        HIP_CHECK(hipMalloc(&d_bases, sizeof(int64_t) * m));
         // The delivery list counts and the post pass's state exist for every
         // batch: k_post empties them even when no network has neurons.
         constexpr size_t n_counts = 1 + 2 * kMaxLanes;
@@ -2704,6 +2735,10 @@ struct Engine {
         (void)hipFree(d_heads);
         (void)hipFree(d_flags);
         (void)hipFree(d_offsets);
+        (void)hipFree(d_bases);
+        (void)hipFree(d_stage_key);
+        (void)hipFree(d_stage_key2);
+        (void)hipFree(d_sort_temp);
         (void)hipFree(d_post);
         (void)hipFree(d_sic_item);
         (void)hipFree(d_nbr);
@@ -2713,6 +2748,8 @@ struct Engine {
         (void)hipHostFree(h_heads);
         (void)hipHostFree(h_flags);
         (void)hipHostFree(h_offsets);
+        (void)hipHostFree(h_bases);
+        (void)hipHostFree(h_stage_key);
         (void)hipHostFree(h_stage_step);
         (void)hipHostFree(h_stage_src);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
@@ -3354,6 +3391,25 @@ struct Engine {
     // Move the device rasters of all networks into their host queues: one
     // launch + copy for the heads and overflow bits, one launch packing every
     // network's new entries into the staging buffer, one copy back.
+    // Grows a device array and its pinned host copy to `count` elements.
+    template <typename T>
+    static void grow_pair(T*& dev, T*& host, size_t count) {
+        (void)hipFree(dev);
+        (void)hipHostFree(host);
+        dev = nullptr;
+        host = nullptr;
+        HIP_CHECK(hipMalloc(&dev, count * sizeof(T)));
+        HIP_CHECK(hipHostMalloc(&host, count * sizeof(T), hipHostMallocDefault));
+    }
+
+    // Moves every active network's new raster entries into its host queue,
+    // in the reference's order (step, then source). The entries are packed
+    // as keys (step offset from the step the network was last drained to,
+    // then source) and sorted per network on the device, so the host only
+    // unpacks them; FP32 delivery appends a window's entries in the order
+    // its wavefronts reserve them, and this is where the order is restored.
+    // Should a network have run 2^32 steps since its last drain, the pairs
+    // come over as they are and the host sorts them instead.
     void drain_all() {
         wait();
         if (!raster_dirty || n_active == 0) return;
@@ -3364,93 +3420,95 @@ struct Engine {
         HIP_CHECK(hipMemcpyAsync(h_flags, d_flags, sizeof(int32_t) * n_active,
                                  hipMemcpyDeviceToHost, stream));
         HIP_CHECK(hipStreamSynchronize(stream));
-        int64_t total = 0, most = 0;
+        int64_t total = 0, most = 0, span = 1;
         for (int j = 0; j < n_active; ++j) {
+            NetSlot& s = slots[h_active[j]];
             h_offsets[j] = total;
             total += h_heads[j];
             most = std::max(most, h_heads[j]);
-            slots[h_active[j]].overflow = h_flags[j];
+            s.overflow = h_flags[j];
+            h_bases[j] = s.drained_to;
+            span = std::max(span, s.step - s.drained_to);
+            s.drained_to = s.step;
         }
+        h_offsets[n_active] = total;
         raster_dirty = false;
         if (total == 0) return;
-        if (static_cast<size_t>(total) > stage_cap) {
+        const bool keyed = span < (int64_t(1) << 32) && total < (int64_t(1) << 32);
+        if (keyed && static_cast<size_t>(total) > stage_cap) {
             const size_t cap = std::max(static_cast<size_t>(total), 2 * stage_cap);
-            (void)hipFree(d_stage_step);
-            (void)hipFree(d_stage_src);
-            (void)hipHostFree(h_stage_step);
-            (void)hipHostFree(h_stage_src);
-            d_stage_step = nullptr;
-            d_stage_src = nullptr;
-            h_stage_step = nullptr;
-            h_stage_src = nullptr;
             stage_cap = 0;
-            HIP_CHECK(hipMalloc(&d_stage_step, cap * sizeof(int64_t)));
-            HIP_CHECK(hipMalloc(&d_stage_src, cap * sizeof(uint32_t)));
-            HIP_CHECK(hipHostMalloc(&h_stage_step, cap * sizeof(int64_t), hipHostMallocDefault));
-            HIP_CHECK(hipHostMalloc(&h_stage_src, cap * sizeof(uint32_t), hipHostMallocDefault));
+            grow_pair(d_stage_key, h_stage_key, cap);
+            (void)hipFree(d_stage_key2);
+            d_stage_key2 = nullptr;
+            HIP_CHECK(hipMalloc(&d_stage_key2, cap * sizeof(uint64_t)));
             stage_cap = cap;
         }
-        HIP_CHECK(hipMemcpyAsync(d_offsets, h_offsets, sizeof(int64_t) * n_active,
+        if (!keyed && static_cast<size_t>(total) > stage_pair_cap) {
+            const size_t cap = std::max(static_cast<size_t>(total), 2 * stage_pair_cap);
+            stage_pair_cap = 0;
+            grow_pair(d_stage_step, h_stage_step, cap);
+            grow_pair(d_stage_src, h_stage_src, cap);
+            stage_pair_cap = cap;
+        }
+        HIP_CHECK(hipMemcpyAsync(d_offsets, h_offsets, sizeof(int64_t) * (n_active + 1),
+                                 hipMemcpyHostToDevice, stream));
+        HIP_CHECK(hipMemcpyAsync(d_bases, h_bases, sizeof(int64_t) * n_active,
                                  hipMemcpyHostToDevice, stream));
         const int64_t gx = std::min<int64_t>(64, (most + 255) / 256);
         hipLaunchKernelGGL(k_raster_gather, dim3(static_cast<uint32_t>(gx), n_active), dim3(256),
                            0, stream, d_nets, d_active, d_heads, d_offsets, d_stage_step,
-                           d_stage_src);
+                           d_stage_src, d_bases, keyed ? d_stage_key : nullptr);
         HIP_CHECK(hipGetLastError());
-        HIP_CHECK(hipMemcpyAsync(h_stage_step, d_stage_step, sizeof(int64_t) * total,
-                                 hipMemcpyDeviceToHost, stream));
-        HIP_CHECK(hipMemcpyAsync(h_stage_src, d_stage_src, sizeof(uint32_t) * total,
-                                 hipMemcpyDeviceToHost, stream));
+        if (keyed) {
+            int bits = 1;
+            while (bits < 32 && (span >> bits) != 0) ++bits;
+            const unsigned end_bit = 32u + static_cast<unsigned>(bits);
+            const unsigned items = static_cast<unsigned>(total);
+            const unsigned segs = static_cast<unsigned>(n_active);
+            size_t need = 0;
+            HIP_CHECK(rocprim::segmented_radix_sort_keys(nullptr, need, d_stage_key, d_stage_key2,
+                                                         items, segs, d_offsets, d_offsets + 1,
+                                                         0u, end_bit, stream));
+            if (need > sort_temp_cap) {
+                (void)hipFree(d_sort_temp);
+                d_sort_temp = nullptr;
+                sort_temp_cap = 0;
+                HIP_CHECK(hipMalloc(&d_sort_temp, need));
+                sort_temp_cap = need;
+            }
+            HIP_CHECK(rocprim::segmented_radix_sort_keys(d_sort_temp, need, d_stage_key,
+                                                         d_stage_key2, items, segs, d_offsets,
+                                                         d_offsets + 1, 0u, end_bit, stream));
+            HIP_CHECK(hipMemcpyAsync(h_stage_key, d_stage_key2, sizeof(uint64_t) * total,
+                                     hipMemcpyDeviceToHost, stream));
+        } else {
+            HIP_CHECK(hipMemcpyAsync(h_stage_step, d_stage_step, sizeof(int64_t) * total,
+                                     hipMemcpyDeviceToHost, stream));
+            HIP_CHECK(hipMemcpyAsync(h_stage_src, d_stage_src, sizeof(uint32_t) * total,
+                                     hipMemcpyDeviceToHost, stream));
+        }
         HIP_CHECK(hipStreamSynchronize(stream));
         for (int j = 0; j < n_active; ++j) {
             const int64_t nr = h_heads[j];
             if (nr == 0) continue;
             NetSlot& s = slots[h_active[j]];
-            const int64_t* ps = h_stage_step + h_offsets[j];
-            const uint32_t* pr = h_stage_src + h_offsets[j];
             const size_t at = s.q_step.size();
-            s.q_step.insert(s.q_step.end(), ps, ps + nr);
-            s.q_src.insert(s.q_src.end(), pr, pr + nr);
-            if (s.dn.precision == ASG_FP32) {
-                // FP32 delivery appends a window's entries in the order its
-                // wavefronts reserve them: restore the reference's order,
-                // step then source. Windows arrive in order, so entries are
-                // out of place only within a window: an insertion sort costs
-                // about one comparison per entry. Windows with many spikes
-                // make it quadratic; past a budget of moves the range is
-                // sorted outright instead.
-                sort_raster(s.q_step, s.q_src, at);
-            }
-        }
-    }
-
-    // Orders q_step/q_src[at ..] by step, then source.
-    static void sort_raster(std::vector<int64_t>& q_step, std::vector<uint32_t>& q_src, size_t at) {
-        int64_t* qs = q_step.data();
-        uint32_t* qr = q_src.data();
-        const size_t end = q_step.size();
-        size_t budget = 16 * (end - at) + 1024;  // element moves before giving up
-        for (size_t a = at + 1; a < end; ++a) {
-            const int64_t ks = qs[a];
-            const uint32_t kr = qr[a];
-            size_t b = a;
-            while (b > at && (qs[b - 1] > ks || (qs[b - 1] == ks && qr[b - 1] > kr))) {
-                qs[b] = qs[b - 1];
-                qr[b] = qr[b - 1];
-                --b;
-            }
-            qs[b] = ks;
-            qr[b] = kr;
-            budget -= std::min(budget, a - b);
-            if (budget == 0) {
-                std::vector<std::pair<int64_t, uint32_t>> e(end - at);
-                for (size_t q = at; q < end; ++q) e[q - at] = {qs[q], qr[q]};
-                std::sort(e.begin(), e.end());
-                for (size_t q = at; q < end; ++q) {
-                    qs[q] = e[q - at].first;
-                    qr[q] = e[q - at].second;
+            s.q_step.resize(at + static_cast<size_t>(nr));
+            s.q_src.resize(at + static_cast<size_t>(nr));
+            int64_t* qs = s.q_step.data() + at;
+            uint32_t* qr = s.q_src.data() + at;
+            if (keyed) {
+                const uint64_t* k = h_stage_key + h_offsets[j];
+                const int64_t base = h_bases[j];
+                for (int64_t e = 0; e < nr; ++e) {
+                    qs[e] = base + static_cast<int64_t>(k[e] >> 32);
+                    qr[e] = static_cast<uint32_t>(k[e]);
                 }
-                return;
+            } else {
+                std::memcpy(qs, h_stage_step + h_offsets[j], sizeof(int64_t) * nr);
+                std::memcpy(qr, h_stage_src + h_offsets[j], sizeof(uint32_t) * nr);
+                order_raster(qs, qr, static_cast<size_t>(nr), sort_keys, sort_tmp);
             }
         }
     }

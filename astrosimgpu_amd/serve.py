@@ -241,14 +241,30 @@ def _progress_printer(t0):
 
 
 def calibrate(base_config: str, count: int, precision: int, chunk_steps: int, builders: int,
-              seed: int = 99) -> float:
-    """Capacity in neuron-steps/s: `count` jobs of the serving distribution all
-    present at t = 0, run to completion with continuous batching."""
+              seed: int = 99, chunks: int = 5) -> float:
+    """Capacity in neuron-steps/s: `count` jobs of the serving distribution
+    resident in one batch, advanced `chunks` chunks of `chunk_steps` steps
+    (after one untimed chunk) with every chunk's rasters drained, as the
+    serving loop advances its running jobs. A steady-state figure: no job
+    finishes during the timed chunks, so no tail of a few long jobs left
+    alone on the device counts against it."""
     jobs = job_stream(count, rate_per_s=1e9, seed=seed, precision=precision)
-    srv = Server(base_config, chunk_steps=chunk_steps, builders=builders)
-    rep = srv.run(jobs)
-    srv.close()
-    return rep.neuron_steps_per_s
+    with cf.ThreadPoolExecutor(max_workers=max(1, builders)) as pool:
+        models = list(pool.map(lambda j: build_job(base_config, j), jobs))
+    batch = Batch(max(count, 1))
+    try:
+        for m in models:
+            batch.add(m, (chunks + 1) * chunk_steps)
+        batch.step(chunk_steps)
+        batch.spikes_all_raw()
+        t0 = time.perf_counter()
+        for _ in range(chunks):
+            batch.step(chunk_steps)
+            batch.spikes_all_raw()
+        seconds = time.perf_counter() - t0
+    finally:
+        batch.close()
+    return sum(m.n_neurons for m in models) * chunks * chunk_steps / seconds
 
 
 def main(argv=None):
@@ -267,13 +283,22 @@ def main(argv=None):
     ap.add_argument("--max-slots", type=int, default=512)
     ap.add_argument("--budget-gb", type=float, default=0.0, help="0 = free device memory")
     ap.add_argument("--config", default=DEFAULT_CONFIG)
+    ap.add_argument("--calibrate-only", action="store_true",
+                    help="print the capacity measured with --calibration-jobs resident jobs "
+                         "and exit")
     args = ap.parse_args(argv)
     prec = capi.ASG_FP32 if args.precision == "fp32" else capi.ASG_FP64
+    if args.calibrate_only:
+        cap = calibrate(args.config, args.calibration_jobs, prec, args.chunk_steps, args.builders)
+        print(json.dumps({"capacity_neuron_steps_per_s": round(cap, 1),
+                          "resident_jobs": args.calibration_jobs,
+                          "chunk_steps": args.chunk_steps}), flush=True)
+        return
     mean_ns = _mean_job_neuron_steps()
     capacity = None
     rate = args.rate
     if rate <= 0.0:
-        print(f"calibrating with {args.calibration_jobs} jobs present at once ...",
+        print(f"calibrating with {args.calibration_jobs} jobs resident at once ...",
               file=sys.stderr, flush=True)
         capacity = calibrate(args.config, args.calibration_jobs, prec, args.chunk_steps,
                              args.builders)
