@@ -164,6 +164,9 @@ struct DevNet {
     int64_t step;          // first step of the next window
     int64_t target;        // step this call stops at
     int64_t win_T0;        // first step of the current window
+    int32_t win_sic_send;  // first window step of the current window that sends
+                           // a SIC exchange (window offset; may be >= its length)
+    int32_t win_pad;
     uint64_t seed;
     double dt, h, half_h, h_sixth;
     NeuronK pk[2];
@@ -196,6 +199,7 @@ struct DevNet {
     float *fgex, *fdgex, *fgin, *fdgin, *fIsic;
     float *fCa_tot, *fIP3_0, *finv_tau, *fdelta;
     float *fring_sic, *ffac;
+    uint32_t* ffac_nz;     // [n_astro]: bit k = entry k of the astrocyte's ffac line is nonzero
     // FP32 networks' exc, inh and n2a rings in fixed point (value * 2^32):
     // delivered by 64-bit atomic adds, whose integer sums do not depend on
     // their order
@@ -786,6 +790,10 @@ __device__ __forceinline__ bool astro_window_f32(const DevNet& n, uint32_t a, in
                 h_sixth = static_cast<float>(n.h_sixth);
     int64_t* __restrict__ ring_astro = n.qring_astro;
     float* __restrict__ fac = n.ffac;
+    // bit k: entry k of this astrocyte's factor line is nonzero
+    uint32_t* __restrict__ fac_nz = n.ffac_nz;
+    const uint32_t nz_prev = fac_nz[a];
+    uint32_t sent = 0;  // bits of the window's steps that send an exchange
     double Ca = n.Ca[a], IP3 = n.IP3[a], hh = n.hh[a];
     const float Ca_tot = n.fCa_tot[a], IP3_0 = n.fIP3_0[a], inv_tau = n.finv_tau[a],
                 delta = n.fdelta[a];
@@ -844,7 +852,12 @@ __device__ __forceinline__ bool astro_window_f32(const DevNet& n, uint32_t a, in
         if (k == sic_next) {
             sic_next += sic_interval;
             const float f = sic_factor_mixed(Ca, SIC_th, SIC_scale);
-            fac[static_cast<size_t>(k) * LA + a] = f;
+            sent |= 1u << k;
+            // a zero over a zero is not stored: the line is written where
+            // it changes (an astrocyte below threshold writes nothing)
+            if (f != 0.0f || ((nz_prev >> k) & 1u)) {
+                fac[static_cast<size_t>(a) * 16 + k] = f;
+            }
             live_bits |= static_cast<uint32_t>(f != 0.0f) << k;
         }
         row = row_n;
@@ -852,21 +865,31 @@ __device__ __forceinline__ bool astro_window_f32(const DevNet& n, uint32_t a, in
     n.Ca[a] = Ca;
     n.IP3[a] = IP3;
     n.hh[a] = hh;
+    const uint32_t nz = (nz_prev & ~sent) | live_bits;
+    if (nz != nz_prev) {
+        fac_nz[a] = nz;
+    }
     return mark_sic_live(const_cast<uint32_t*>(&n.sic_live_p[par]), live_bits, W);
 }
 
-// FP32 SIC of one (neuron, window step): sum over (astrocyte, multiplicity)
-// of m * w_a2n * factor, compensated.
+// FP32 SIC of one neuron for every window step that sends an exchange:
+// sum over (astrocyte, multiplicity) of m * w_a2n * factor, compensated.
+// The factors are stored astrocyte-major, 16 to a 64-byte line (ffac[a * 16
+// + k] = factor of window step k), so one pair costs one line for all the
+// window's steps rather than one line per step. Entries of steps that send
+// are current (the astrocyte pass stores every new factor except a zero over
+// a zero); the others are not read.
 template <int GM>
-__device__ __forceinline__ void deliver_sic_f32(const DevNet& n, uint32_t t, int g0, int G,
+__device__ __forceinline__ void deliver_sic_f32(const DevNet& n, uint32_t t, int G,
                                                 uint32_t sic_live) {
+    static_assert(GM <= 16, "one line of factors per astrocyte");
     const size_t LN = n.ldn;
     const int64_t T0 = n.win_T0;
-    uint32_t send = 0;  // bit j: window step g0 + j sends an exchange
-    for (int64_t j = first_hit(T0 + g0, 0, n.sic_interval); j < G; j += n.sic_interval) {
+    uint32_t send = 0;  // bit j: window step j sends an exchange
+    for (int j = n.win_sic_send; j < G; j += n.sic_interval) {
         send |= 1u << j;
     }
-    const uint32_t live = send & (sic_live >> g0);
+    const uint32_t live = send & sic_live;
     float acc[GM], comp[GM];
 #pragma unroll
     for (int j = 0; j < GM; ++j) {
@@ -876,20 +899,36 @@ __device__ __forceinline__ void deliver_sic_f32(const DevNet& n, uint32_t t, int
     if (live != 0) {
         const uint32_t* __restrict__ src = n.sic_src;
         const uint32_t* __restrict__ mult = n.sic_mult;
-        const float* __restrict__ fac = n.ffac + static_cast<size_t>(g0) * n.lda;
-        const size_t LA = n.lda;
+        const float4* __restrict__ fac = reinterpret_cast<const float4*>(n.ffac);
         const float w_a2n = static_cast<float>(n.w_a2n);
+        // the 16-byte parts of a line that hold live steps (uniform: one
+        // network per wavefront)
+        const uint32_t parts = (live & 0xFu ? 1u : 0u) | (live & 0xF0u ? 2u : 0u) |
+                               (live & 0xF00u ? 4u : 0u) | (live & 0xF000u ? 8u : 0u);
         for (uint32_t q = 0; q < n.sic_width; ++q) {
             const uint32_t m = mult[static_cast<size_t>(q) * LN + t];
             if (m == 0) {
                 break;
             }
             const uint32_t a = src[static_cast<size_t>(q) * LN + t];
+            const float4* row = fac + static_cast<size_t>(a) * 4;
+            float v[16];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                float4 x = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (4 * c < GM && ((parts >> c) & 1u)) {
+                    x = row[c];
+                }
+                v[4 * c] = x.x;
+                v[4 * c + 1] = x.y;
+                v[4 * c + 2] = x.z;
+                v[4 * c + 3] = x.w;
+            }
             const float fm = static_cast<float>(m);
 #pragma unroll
             for (int j = 0; j < GM; ++j) {
                 if ((live >> j) & 1u) {
-                    const float fa = fac[j * LA + a];
+                    const float fa = v[j];
                     if (fa != 0.0f) {
                         comp_add(acc[j], comp[j], fm * (w_a2n * fa));
                     }
@@ -900,7 +939,7 @@ __device__ __forceinline__ void deliver_sic_f32(const DevNet& n, uint32_t t, int
     // A live exchange writes every neuron's slot; the others are neither
     // written nor read (sic_hist).
     const int S = n.S;
-    const int row0 = static_cast<int>((T0 + g0 + n.d_a2n) % S);
+    const int row0 = static_cast<int>((T0 + n.d_a2n) % S);
 #pragma unroll
     for (int j = 0; j < GM; ++j) {
         const float v = acc[j] + comp[j];
@@ -1181,6 +1220,7 @@ __global__ __launch_bounds__(kCellBlock * WPB) void k_cells(DevNet* nets, const 
     if (cb.lead && lane == 0) {
         n.win_T0 = T0;
         n.win_W = W;
+        n.win_sic_send = n.clk_sic_send;
         n.open = 1;
     }
     if (W == 0) {
@@ -1823,7 +1863,7 @@ __global__ __launch_bounds__(64) void k_post(DevNet* nets, const CellBlock* nblo
             DevNet& n = nets[cb.net];
             const uint32_t t = cb.first + lane;
             if (t < n.n_neur) {
-                deliver_sic_f32<kWindowCap>(n, t, 0, n.win_W, n.sic_live_p[par]);
+                deliver_sic_f32<kWindowCap>(n, t, n.win_W, n.sic_live_p[par]);
             }
         }
     } else if (e - C - D - E < n_nblocks) {
@@ -2319,7 +2359,8 @@ Plan plan_of(const asg_net_desc& d, const DerivedConn* dc) {
     const bool f32 = d.precision == ASG_FP32;
     // rings: FP64 exc/inh/SIC and n2a; FP32 fixed-point exc/inh and n2a, FP32 SIC
     b += f32 ? p.S * p.ldn * (8 + 8 + 4) + p.S * p.lda * 8 : p.S * p.ldn * 3 * 8 + p.S * p.lda * 8;
-    b += p.Wmax * p.lda * fb + p.hist_bits / 8;
+    // SIC factors: FP64 [Wmax][astrocytes]; FP32 one 16-float line per astrocyte
+    b += (f32 ? 16 * p.lda * 4 + p.lda * 4 : p.Wmax * p.lda * 8) + p.hist_bits / 8;
     b += (d.n_exc + 1 + p.n_exc_syn + d.n_inh + 1 + p.n_inh_syn) * 4;
     b += (2 * (d.n_exc + 1) + 2 * p.n2a_dedup) * 4;
     b += 2 * p.sic_pairs * 4;
@@ -2490,7 +2531,8 @@ void build_slot(NetSlot& s, const asg_net_desc& d, int64_t total, hipStream_t st
         n.qring_inh = dev_zero<int64_t>(s, pl.S * pl.ldn, st);
         n.fring_sic = dev_zero<float>(s, pl.S * pl.ldn, st);
         n.qring_astro = dev_zero<int64_t>(s, pl.S * pl.lda, st);
-        n.ffac = dev_zero<float>(s, pl.Wmax * pl.lda, st);
+        n.ffac = dev_zero<float>(s, 16 * pl.lda, st);
+        n.ffac_nz = dev_zero<uint32_t>(s, pl.lda, st);
         HIP_CHECK(hipStreamSynchronize(st));
     } else {
         n.V = dev_copy(s, V0.data(), N, st);
