@@ -10,8 +10,10 @@ round(0.2 N) astrocytes, IP3 weight w_n2a uniform in [0.10, 0.40], seed
 
   value        batched FP32 (FP32 conductances, rings in fixed point, the slow
                state V, w, Ca, IP3, h carried in FP64): all 256 networks in
-               one asg_batch, schedule picked by the autotuner, each call's
-               windows replayed from a captured hipGraph. Neuron-steps/s =
+               one asg_batch, schedule picked by the autotuner, then each
+               call's windows replayed from captured hipGraphs or launched
+               eagerly, whichever a short probe of both finds faster (the
+               line says which). Neuron-steps/s =
                sum over networks of N x time steps / wall seconds of the
                timed bench steps. One bench step = 1,000 time steps (0.1 s
                simulated) of every network plus draining their spike
@@ -72,6 +74,7 @@ SPARSE_CONFIG = os.path.join(HERE, "tests", "golden", "configs", "paper_sparse.j
 REF_BIN = os.path.join(HERE, "oracle", "_ref", "astrosimgpu_ref")
 HBM_PEAK_GBS = 8000.0
 STEPS_PER_BENCH_STEP = 1000
+PROBE_STEPS = 3  # bench steps per launch-mode probe
 WINDOW = 10  # min(d_e, d_i, d_a2n) of use_case.json and paper_sparse.json = 1 ms = 10 steps
 ACC_TRANSIENT_BS, ACC_TIMED_BS = 5, 50  # accuracy run: 0.5 s + 5 s, in bench steps
 SPARSE_SEEDS = (1, 2, 3)
@@ -362,6 +365,10 @@ def main():
     ap.add_argument("--cpu-every", type=int, default=16)
     ap.add_argument("--no-sparse", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="default schedule, eager launches")
+    ap.add_argument("--lanes", type=int, default=0, choices=(0, 1, 2),
+                    help="with --no-tune: FP32 lanes of the schedule (0 = the default, 2); "
+                         "1 puts every FP32 kernel over the whole batch, as the profiled "
+                         "run does, for kernel traces")
     ap.add_argument("--fp32-only", action="store_true",
                     help="run only the batched FP32 sweep (for kernel profiles)")
     ap.add_argument("--fp64-only", action="store_true",
@@ -418,21 +425,39 @@ def main():
 
     # Headline: batched FP32, autotuned schedule, graph replay.
     log("batched FP32 ...")
-    tune_steps = 0 if args.no_tune else 40 * 3 * 20 * WINDOW  # <= 40 candidates x 3 rounds x 20 windows
+    # <= 40 candidates x 3 rounds x 20 windows (x 2 for the lanes' untimed
+    # calls), then the launch probe
+    tune_steps = 0 if args.no_tune else (44 * 3 * 20 * WINDOW + 4 * (PROBE_STEPS + 1) * 1000)
     batch, slots = new_batch(models, capi.ASG_FP32,
                              tune_steps + (args.warmup + args.steps + 10) * 1000, record=False)
     schedule = None
+    if args.no_tune and args.lanes:
+        batch.set_schedule(lanes=args.lanes)
     # The transient first: the activity the schedule is tuned on (how many
     # networks have a live SIC exchange, how many blocks spike) should be
     # the timed run's.
     for _ in range(args.warmup):
         batch.step(STEPS_PER_BENCH_STEP)
         batch.spikes_all_raw()
+    launch = "eager"
+    probe = None
     if not args.no_tune:
         batch.set_graphs(True)
         schedule = batch.autotune(windows_per_try=20, rounds=3)
         batch.spikes_all_raw()
         log(f"  autotuned schedule {schedule}")
+        # Graph replay or eager launches of the tuned schedule, whichever is
+        # faster on a short probe of each (alternating, twice).
+        probe = {"graphs": [], "eager": []}
+        for _ in range(2):
+            for mode in ("graphs", "eager"):
+                batch.set_graphs(mode == "graphs")
+                t, _ = timed_steps(batch, slots, 1, PROBE_STEPS, False)
+                probe[mode].append(1000.0 * t / PROBE_STEPS)
+        probe = {k: round(min(v), 3) for k, v in probe.items()}
+        launch = "graphs" if probe["graphs"] <= probe["eager"] else "eager"
+        batch.set_graphs(launch == "graphs")
+        log(f"  launch probe ms per bench step {probe}: {launch}")
     e32, _ = timed_steps(batch, slots, 0, args.steps, False)
     timed_ns = n_total * args.steps * STEPS_PER_BENCH_STEP
     value = timed_ns / e32
@@ -567,13 +592,15 @@ def main():
             "workload": f"{len(models)}-network sweep (N log-uniform in [200, 5000], 80/20 E/I, "
                         "0.2 N astrocytes, w_n2a U[0.10, 0.40], in-degree 100), batched FP32 "
                         "path (FP32 conductances, fixed-point rings, V/w/Ca/IP3/h carried in "
-                        f"FP64), {'autotuned schedule, hipGraph replay, ' if schedule else ''}"
+                        f"FP64), {'autotuned schedule, ' if schedule else ''}{launch} launches, "
                         f"{args.warmup} bench steps untimed + {args.steps} timed, 1000 time "
                         "steps (dt 0.1 ms) per bench step; accuracy over 0.5 s + 5 s",
             "networks": len(models), "neurons": n_total, "astrocytes": na_total,
             "time_steps_per_bench_step": STEPS_PER_BENCH_STEP,
         },
         "schedule": schedule,
+        "launch": launch,
+        "launch_probe_ms_per_step": probe,
         "plain_fp64": None if plain64 is None else
         {"value": round(plain64, 1), "unit": "neuron-steps/s", "networks_timed": len(sub)},
         "batched_fp64": None if e64 is None else {"value": round(e64, 1), "unit": "neuron-steps/s"},
