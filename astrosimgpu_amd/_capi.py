@@ -105,7 +105,7 @@ class AstroScalingResult(C.Structure):
 class Schedule(C.Structure):
     _fields_ = [("cell_waves", C.c_int32), ("sic_group", C.c_int32), ("step_group", C.c_int32),
                 ("deliver_waves", C.c_int32), ("sic_workers", C.c_int32),
-                ("lanes", C.c_int32), ("ms_per_window", C.c_float)]
+                ("lanes", C.c_int32), ("teams", C.c_int32), ("ms_per_window", C.c_float)]
 
 
 class CalciumSummary(C.Structure):

@@ -125,8 +125,9 @@ constexpr int64_t kSicWaves = 4096; // SIC wavefronts that fill the device (256 
 constexpr int64_t kGroupWaves = 16384;  // per-step wavefronts beyond which steps are grouped
 constexpr int kDeliverWaves = 512;  // delivery wavefronts sharing out the spiking blocks
 constexpr int kSicWorkers = 0;      // FP32 SIC wavefronts (0: one per FP32 neuron block)
-constexpr int kMaxLanes = 2;        // FP32 lanes (network groups on streams of their own)
+constexpr int kMaxLanes = 3;        // FP32 lanes (network groups on streams of their own)
 constexpr int kLanes = 2;           // lanes by default
+constexpr int kMaxTeams = 1024;     // teams of the persistent FP32 launch
 constexpr size_t kMaskBudget = size_t(256) << 20;  // delivery masks per network
 constexpr size_t kMinChunks = 64;
 
@@ -1198,22 +1199,14 @@ __global__ __launch_bounds__(64) void k_drive(DevNet* nets, const CellBlock* nbl
 // between two sets of these lists and live words by parity `par` (the host
 // passes the set of the window's parity), so that the post pass can empty
 // the other set while it reads this one.
-template <int P, int WPB>
-__global__ __launch_bounds__(kCellBlock * WPB) void k_cells(DevNet* nets, const CellBlock* blocks,
-                                                            int32_t count, int32_t Wg,
-                                                            CellBlock* work, int32_t* work_n,
-                                                            int32_t par, PostState* post,
-                                                            int32_t* sic_item, const int32_t* nbr) {
-    // The wavefront's block index is uniform across its lanes: said so
-    // explicitly, so the block and network descriptors stay in scalar
-    // registers.
-    const int32_t b = __builtin_amdgcn_readfirstlane(static_cast<int32_t>(blockIdx.x) * WPB +
-                                                     static_cast<int32_t>(threadIdx.x >> 6));
-    if (b >= count) {
-        return;
-    }
+// The cell pass of one 64-cell block, run by one whole wavefront (the
+// block is uniform across its lanes).
+template <int P>
+__device__ __forceinline__ void cells_item(DevNet* nets, const CellBlock& cb, int32_t Wg,
+                                           CellBlock* work, int32_t* work_n, int32_t par,
+                                           PostState* post, int32_t* sic_item,
+                                           const int32_t* nbr) {
     const uint32_t lane = threadIdx.x & 63;
-    const CellBlock cb = blocks[b];
     DevNet& n = nets[cb.net];
     const int W = window_len(n, Wg);
     const int64_t T0 = n.step;
@@ -1257,6 +1250,36 @@ __global__ __launch_bounds__(kCellBlock * WPB) void k_cells(DevNet* nets, const 
             astro_window(n, i, T0, W);
         }
     }
+}
+
+// Window pass 1: every cell of every network advances W steps. One kernel per
+// precision, so neither carries the other's registers; neurons and
+// astrocytes share the launch, so the short astrocyte blocks overlap the
+// neuron blocks.
+// WPB wavefronts (64-cell blocks) per workgroup, a launch-schedule knob.
+// A neuron block that spiked in the window appends itself to its
+// precision's delivery list `work`, counted in *work_n. An FP32 network
+// whose SIC exchange becomes live in the window lists its neuron blocks
+// (nbr[2 * slot + 1] of them from index nbr[2 * slot] of the FP32 neuron
+// blocks) as the post pass's SIC items in sic_item. FP32 windows alternate
+// between two sets of these lists and live words by parity `par` (the host
+// passes the set of the window's parity), so that the post pass can empty
+// the other set while it reads this one.
+template <int P, int WPB>
+__global__ __launch_bounds__(kCellBlock * WPB) void k_cells(DevNet* nets, const CellBlock* blocks,
+                                                            int32_t count, int32_t Wg,
+                                                            CellBlock* work, int32_t* work_n,
+                                                            int32_t par, PostState* post,
+                                                            int32_t* sic_item, const int32_t* nbr) {
+    // The wavefront's block index is uniform across its lanes: said so
+    // explicitly, so the block and network descriptors stay in scalar
+    // registers.
+    const int32_t b = __builtin_amdgcn_readfirstlane(static_cast<int32_t>(blockIdx.x) * WPB +
+                                                     static_cast<int32_t>(threadIdx.x >> 6));
+    if (b >= count) {
+        return;
+    }
+    cells_item<P>(nets, blocks[b], Wg, work, work_n, par, post, sic_item, nbr);
 }
 
 
@@ -1826,16 +1849,15 @@ __global__ __launch_bounds__(256) void k_deliver(DevNet* nets, const CellBlock* 
 // The close writes only what no other workgroup of the launch reads (the
 // live word it reads is this parity's, which nothing writes now), so the
 // workgroups need no ordering among themselves.
-__global__ __launch_bounds__(64) void k_post(DevNet* nets, const CellBlock* nblocks,
-                                             int32_t n_nblocks, int32_t Wg, int32_t C, int32_t D,
-                                             int32_t E, const CellBlock* work,
-                                             const int32_t* work_n, const PostState* post,
-                                             const int32_t* sic_item, const int32_t* active,
-                                             int32_t n_active, int32_t par,
-                                             int32_t* other_work_n, PostState* other_post) {
-    extern __shared__ double drive_lds[];
-    const int e = static_cast<int>(blockIdx.x);
-    const uint32_t lane = threadIdx.x;
+// Item e of the post pass (the layout above), run by one whole wavefront.
+__device__ __forceinline__ void post_item(int e, DevNet* nets, const CellBlock* nblocks,
+                                          int32_t n_nblocks, int32_t Wg, int32_t C, int32_t D,
+                                          int32_t E, const CellBlock* work,
+                                          const int32_t* work_n, const PostState* post,
+                                          const int32_t* sic_item, const int32_t* active,
+                                          int32_t n_active, int32_t par, int32_t* other_work_n,
+                                          PostState* other_post, double* drive_lds) {
+    const uint32_t lane = threadIdx.x & 63;
     if (e < C) {
         const int j = e * 64 + static_cast<int>(lane);
         if (j < n_active) {
@@ -1869,6 +1891,148 @@ __global__ __launch_bounds__(64) void k_post(DevNet* nets, const CellBlock* nblo
     } else if (e - C - D - E < n_nblocks) {
         const CellBlock cb = nblocks[e - C - D - E];
         drive_block(nets[cb.net], cb, Wg, drive_lds);
+    }
+}
+
+__global__ __launch_bounds__(64) void k_post(DevNet* nets, const CellBlock* nblocks,
+                                             int32_t n_nblocks, int32_t Wg, int32_t C, int32_t D,
+                                             int32_t E, const CellBlock* work,
+                                             const int32_t* work_n, const PostState* post,
+                                             const int32_t* sic_item, const int32_t* active,
+                                             int32_t n_active, int32_t par,
+                                             int32_t* other_work_n, PostState* other_post) {
+    extern __shared__ double drive_lds[];
+    post_item(static_cast<int>(blockIdx.x), nets, nblocks, n_nblocks, Wg, C, D, E, work, work_n,
+              post, sic_item, active, n_active, par, other_work_n, other_post, drive_lds);
+}
+
+// ------------------------------------------------ persistent FP32 launch ---
+//
+// One cooperative launch runs all of a call's windows of the FP32 networks.
+// The networks are split into teams (grouped as the lanes of the per-kernel
+// launches are); team t owns `wgs` one-wavefront workgroups, which run the
+// call's first drive and then, window by window, the team's cell pass and
+// its post pass, with a barrier among the team's workgroups after each.
+// Networks exchange nothing, so no team ever waits for another: one team's
+// post pass (latency-bound delivery chains) runs beside another's cell pass
+// (bound by instruction issue).
+struct Team {
+    int32_t cell_first, cells;      // its blocks in the block table
+    int32_t nblock_first, nblocks;  // its neuron blocks in the neuron-block table
+    int32_t active_first, nets;     // its networks in the active list
+    int32_t wgs;                    // its workgroups
+    int32_t pad;
+};
+
+// Longest wait at a team barrier before the launch gives up: 2 s of the
+// 100 MHz real-time counter (ASG_TEAM_WAIT_US overrides it, for tests of
+// the failure path). One pass of a team takes milliseconds at most.
+constexpr uint64_t kTeamWaitTicks = 200000000ull;
+
+// Barrier among a team's workgroups, in the pattern of a grid-wide barrier:
+// every arrival adds one to the team's counter (which only grows during a
+// launch) and the barrier is passed once it reaches `target`; agent-scope
+// release before the arrival and acquire after it make every member's
+// stores before the barrier visible to every member after it. A wait longer
+// than wait_ticks sets *failed and *host_failed; a member that finds
+// *failed set stops waiting too. Returns false then, and the workgroup
+// leaves the kernel, so every wavefront exits.
+__device__ __forceinline__ bool team_barrier(uint32_t* counter, uint32_t target, int32_t* failed,
+                                             int32_t* host_failed, uint64_t wait_ticks) {
+    __syncthreads();
+    int32_t ok = 1;
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+        while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__hip_atomic_load(failed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                ok = 0;
+                break;
+            }
+            if (__builtin_amdgcn_s_memrealtime() - t0 > wait_ticks) {
+                __hip_atomic_store(failed, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(host_failed, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+    return __builtin_amdgcn_readfirstlane(ok) != 0;
+}
+
+// The persistent launch: workgroup g is member wg_team[g].y of team
+// wg_team[g].x. The team's lists are those a lane with the same blocks
+// would use (offsets from nb0, the FP64 networks' neuron block count), and
+// its windows alternate them by parity from par0. bar holds a counter per
+// team, 64 bytes apart, zero at the launch.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_persist(DevNet* nets, const CellBlock* blocks,
+                                                const CellBlock* nblocks, const Team* teams,
+                                                const int2* wg_team, const int32_t* active,
+                                                const int32_t* nbr, CellBlock* work,
+                                                int32_t* work_n, PostState* post,
+                                                int32_t* sic_item, int32_t nb0, int32_t Wg,
+                                                int32_t windows, int32_t par0, int32_t dwaves,
+                                                int32_t swaves, uint32_t* bar, int32_t* failed,
+                                                int32_t* host_failed, uint64_t wait_ticks) {
+    extern __shared__ double drive_lds[];
+    const int t = __builtin_amdgcn_readfirstlane(wg_team[blockIdx.x].x);
+    const int r = __builtin_amdgcn_readfirstlane(wg_team[blockIdx.x].y);
+    const Team tm = teams[t];
+    const int T = tm.wgs;
+    uint32_t* counter = bar + 16 * t;
+    uint32_t target = 0;
+    for (int j = r; j < tm.nblocks; j += T) {  // the call's first drive
+        const CellBlock cb = nblocks[tm.nblock_first + j];
+        drive_block(nets[cb.net], cb, Wg, drive_lds);
+    }
+    target += static_cast<uint32_t>(T);
+    if (!team_barrier(counter, target, failed, host_failed, wait_ticks)) {
+        return;
+    }
+    const int rel = tm.nblock_first - nb0;
+    CellBlock* work_t = work + nb0 + 2 * rel;
+    int32_t* sic_t = sic_item + 2 * rel;
+    int32_t* work_n_t = work_n + 1 + 2 * t;
+    PostState* post_t = post + 2 * t;
+    const CellBlock* nblocks_t = nblocks + tm.nblock_first;
+    const int C = (tm.nets + 63) / 64;
+    const int D = min(tm.nblocks, dwaves);
+    const int E = swaves > 0 ? min(tm.nblocks, swaves) : tm.nblocks;
+    const int items = C + D + E + tm.nblocks;
+    for (int w = 0; w < windows; ++w) {
+        const int p = (par0 + w) & 1;
+        for (int b = r; b < tm.cells; b += T) {
+            cells_item<ASG_FP32>(nets, blocks[tm.cell_first + b], Wg, work_t + p * tm.nblocks,
+                                 work_n_t + p, p, post_t + p, sic_t + p * tm.nblocks, nbr);
+        }
+        target += static_cast<uint32_t>(T);
+        if (!team_barrier(counter, target, failed, host_failed, wait_ticks)) {
+            return;
+        }
+        for (int e = r; e < items; e += T) {
+            post_item(e, nets, nblocks_t, tm.nblocks, Wg, C, D, E, work_t + p * tm.nblocks,
+                      work_n_t + p, post_t + p, sic_t + p * tm.nblocks, active + tm.active_first,
+                      tm.nets, p, work_n_t + (p ^ 1), post_t + (p ^ 1), drive_lds);
+        }
+        target += static_cast<uint32_t>(T);
+        if (!team_barrier(counter, target, failed, host_failed, wait_ticks)) {
+            return;
+        }
+    }
+}
+
+// Zeroes the persistent launch's barrier counters and failure word.
+__global__ __launch_bounds__(256) void k_persist_prep(uint32_t* bar, int32_t n, int32_t* failed) {
+    const int j = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (j < n) {
+        bar[j] = 0;
+    }
+    if (j == 0) {
+        *failed = 0;
     }
 }
 
@@ -2650,6 +2814,7 @@ struct Engine {
     int32_t* d_nbr = nullptr;
     std::vector<int32_t> h_nbr;
     size_t blocks_cap = 0, nblocks_cap = 0, active_cap = 0, work_cap = 0;
+    size_t work_n_cap = 0, post_cap = 0;
     int n_blocks = 0, n_nblocks = 0, n_active = 0, Wg = 1;
     std::vector<int32_t> h_active;     // host copy of d_active
     bool dirty = true;
@@ -2717,11 +2882,28 @@ struct Engine {
     double kernel_ms[5] = {0, 0, 0, 0, 0};
     // d_blocks holds ranges [cell_range[r], cell_range[r + 1]): r = 0 the
     // FP64 networks' blocks, r = 1 + l those of FP32 lane l (l < lanes).
-    int cell_range[kMaxLanes + 2] = {};
+    std::vector<int> cell_range = std::vector<int>(kMaxLanes + 2, 0);
     // d_nblocks and d_active likewise: neuron blocks (active slots) of the
     // FP64 networks, then of each FP32 lane.
-    int nblock_range[kMaxLanes + 2] = {};
-    int active_range[kMaxLanes + 2] = {};
+    std::vector<int> nblock_range = std::vector<int>(kMaxLanes + 2, 0);
+    std::vector<int> active_range = std::vector<int>(kMaxLanes + 2, 0);
+    // The persistent launch (sched_teams > 0): the FP32 networks' groups of
+    // the tables are then teams (see k_persist) rather than stream lanes,
+    // with their descriptors, each workgroup's (team, member), a barrier
+    // counter per team, and the failure words (device; pinned host, which a
+    // timed-out barrier sets and wait() reads).
+    int sched_teams = 0;
+    bool persistent = false;   // the current tables are team tables
+    int max_resident = 0;      // workgroups of k_persist the device holds at once
+    Team* d_teams = nullptr;
+    int2* d_wg_team = nullptr;
+    uint32_t* d_bar = nullptr;
+    int32_t* d_pfail = nullptr;
+    int32_t* h_pfail = nullptr;
+    int32_t* h_pfail_dev = nullptr;  // h_pfail as the device addresses it
+    size_t teams_cap = 0, wg_cap = 0, bar_cap = 0;
+    int n_teams = 0, persist_grid = 0;
+    uint64_t team_wait_ticks = kTeamWaitTicks;
 
     explicit Engine(int max_n, int64_t budget_bytes) : max_nets(max_n), budget(budget_bytes) {
         HIP_CHECK(hipGetDevice(&device));
@@ -2742,11 +2924,20 @@ struct Engine {
         HIP_CHECK(hipMalloc(&d_bases, sizeof(int64_t) * m));
         // The delivery list counts and the post pass's state exist for every
         // batch: k_post empties them even when no network has neurons.
-        constexpr size_t n_counts = 1 + 2 * kMaxLanes;
-        HIP_CHECK(hipMalloc(&d_work_n, n_counts * sizeof(int32_t)));
-        HIP_CHECK(hipMemsetAsync(d_work_n, 0, n_counts * sizeof(int32_t), stream));
-        HIP_CHECK(hipMalloc(&d_post, 2 * kMaxLanes * sizeof(PostState)));
-        HIP_CHECK(hipMemsetAsync(d_post, 0, 2 * kMaxLanes * sizeof(PostState), stream));
+        work_n_cap = 1 + 2 * kMaxLanes;
+        post_cap = 2 * kMaxLanes;
+        HIP_CHECK(hipMalloc(&d_work_n, work_n_cap * sizeof(int32_t)));
+        HIP_CHECK(hipMemsetAsync(d_work_n, 0, work_n_cap * sizeof(int32_t), stream));
+        HIP_CHECK(hipMalloc(&d_post, post_cap * sizeof(PostState)));
+        HIP_CHECK(hipMemsetAsync(d_post, 0, post_cap * sizeof(PostState), stream));
+        HIP_CHECK(hipMalloc(&d_pfail, sizeof(int32_t)));
+        HIP_CHECK(hipMemsetAsync(d_pfail, 0, sizeof(int32_t), stream));
+        HIP_CHECK(hipHostMalloc(&h_pfail, sizeof(int32_t), hipHostMallocDefault));
+        *h_pfail = 0;
+        HIP_CHECK(hipHostGetDevicePointer(reinterpret_cast<void**>(&h_pfail_dev), h_pfail, 0));
+        if (const char* us = std::getenv("ASG_TEAM_WAIT_US")) {
+            team_wait_ticks = std::strtoull(us, nullptr, 10) * 100;  // 100 MHz counter
+        }
         lane_stream[0] = stream;
         for (int l = 1; l < kMaxLanes; ++l) {
             HIP_CHECK(hipStreamCreateWithFlags(&lane_stream[l], hipStreamNonBlocking));
@@ -2784,6 +2975,11 @@ struct Engine {
         (void)hipFree(d_post);
         (void)hipFree(d_sic_item);
         (void)hipFree(d_nbr);
+        (void)hipFree(d_teams);
+        (void)hipFree(d_wg_team);
+        (void)hipFree(d_bar);
+        (void)hipFree(d_pfail);
+        (void)hipHostFree(h_pfail);
         (void)hipFree(d_stage_step);
         (void)hipFree(d_stage_src);
         (void)hipHostFree(h_targets);
@@ -2817,6 +3013,12 @@ struct Engine {
         if (pending) {
             HIP_CHECK(hipStreamSynchronize(stream));
             pending = false;
+            if (*h_pfail != 0) {
+                *h_pfail = 0;
+                throw HipError{hipErrorLaunchTimeOut,
+                               "persistent launch: a team barrier waited longer than 2 s and the "
+                               "launch stopped; the networks' state is that of an unfinished window"};
+            }
             if (pending_windows > 0) {
                 // event intervals: drive, cells, compaction, delivery, sums
                 constexpr int kSlot[5] = {4, 0, 1, 2, 3};
@@ -2877,9 +3079,13 @@ struct Engine {
                                (s.dn.n_astro + kCellBlock - 1) / kCellBlock;
             w.emplace_back(-nb, sidx);
         }
-        lanes = profiling ? 1 : std::max(1, std::min<int>(sched_lanes, static_cast<int>(w.size())));
+        const int n32 = static_cast<int>(w.size());
+        persistent = !profiling && sched_teams > 0 && n32 > 0;
+        lanes = profiling    ? 1
+                : persistent ? std::max(1, std::min(sched_teams, n32))
+                             : std::max(1, std::min(sched_lanes, n32));
         std::sort(w.begin(), w.end());
-        int64_t load[kMaxLanes] = {};
+        std::vector<int64_t> load(static_cast<size_t>(lanes), 0);
         for (const auto& e : w) {
             int best = 0;
             for (int l = 1; l < lanes; ++l) {
@@ -2891,12 +3097,149 @@ struct Engine {
         return lane_of;
     }
 
+    // Descriptors and workgroups of the teams (the FP32 groups 1 + t of the
+    // tables) for the persistent launch. The grid is what the device holds
+    // at once (the occupancy API; a cooperative launch needs every
+    // workgroup resident), shared among the teams by their blocks per
+    // window (cell blocks plus neuron blocks, whose post items -- drive,
+    // delivery, SIC -- scale with them), at least one each and no more than
+    // a team has items in its larger pass. Workgroups are dispatched to the
+    // eight XCDs round-robin by index, so a team takes its members from the
+    // indices of one XCD while it can: its lists, rings and counters then
+    // stay in one L2.
+    void build_teams() {
+        int on = 0, cus = 0;
+        HIP_CHECK(hipDeviceGetAttribute(&on, hipDeviceAttributeCooperativeLaunch, device));
+        if (!on) {
+            throw HipError{hipErrorNotSupported, "the device does not support cooperative launches"};
+        }
+        HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+        int per_cu = 0;
+        HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void*>(k_persist), 64, drive_lds_bytes()));
+        max_resident = per_cu * cus;
+        n_teams = lanes;
+        if (n_teams > max_resident) {
+            throw HipError{hipErrorInvalidValue, "more persistent teams than resident workgroups"};
+        }
+        std::vector<Team> tm(static_cast<size_t>(n_teams));
+        std::vector<int64_t> weight(tm.size()), useful(tm.size());
+        int64_t wsum = 0, usum = 0;
+        for (int t = 0; t < n_teams; ++t) {
+            Team& x = tm[static_cast<size_t>(t)];
+            x.cell_first = cell_range[1 + t];
+            x.cells = cell_range[2 + t] - cell_range[1 + t];
+            x.nblock_first = nblock_range[1 + t];
+            x.nblocks = nblock_range[2 + t] - nblock_range[1 + t];
+            x.active_first = active_range[1 + t];
+            x.nets = active_range[2 + t] - active_range[1 + t];
+            x.pad = 0;
+            const int post_items = (x.nets + 63) / 64 + 3 * x.nblocks;
+            weight[static_cast<size_t>(t)] = x.cells + x.nblocks;
+            useful[static_cast<size_t>(t)] = std::max(1, std::max(x.cells, post_items));
+            wsum += weight[static_cast<size_t>(t)];
+            usum += useful[static_cast<size_t>(t)];
+        }
+        const int64_t G = std::min<int64_t>(max_resident, usum);
+        int64_t given = 0;
+        for (size_t t = 0; t < tm.size(); ++t) {
+            const int64_t share = wsum > 0 ? G * weight[t] / wsum : 1;
+            tm[t].wgs = static_cast<int32_t>(std::max<int64_t>(1, std::min(share, useful[t])));
+            given += tm[t].wgs;
+        }
+        // hand out what rounding left, largest weight first, while any team
+        // can use more; take back any excess from the largest teams
+        std::vector<int> order(tm.size());
+        for (size_t t = 0; t < tm.size(); ++t) order[t] = static_cast<int>(t);
+        std::sort(order.begin(), order.end(), [&](int a, int b) {
+            return weight[static_cast<size_t>(a)] > weight[static_cast<size_t>(b)];
+        });
+        for (bool more = true; given < G && more;) {
+            more = false;
+            for (int t : order) {
+                if (given >= G) break;
+                if (tm[static_cast<size_t>(t)].wgs < useful[static_cast<size_t>(t)]) {
+                    ++tm[static_cast<size_t>(t)].wgs;
+                    ++given;
+                    more = true;
+                }
+            }
+        }
+        while (given > max_resident) {
+            for (int t : order) {
+                if (given <= max_resident) break;
+                if (tm[static_cast<size_t>(t)].wgs > 1) {
+                    --tm[static_cast<size_t>(t)].wgs;
+                    --given;
+                }
+            }
+        }
+        persist_grid = static_cast<int>(given);
+        // members: indices x, x + 8, ... belong to XCD x
+        constexpr int kXcds = 8;
+        std::vector<std::vector<int>> free_of(kXcds);
+        for (int g = persist_grid - 1; g >= 0; --g) free_of[static_cast<size_t>(g % kXcds)].push_back(g);
+        std::vector<int2> wg(static_cast<size_t>(persist_grid));
+        std::sort(order.begin(), order.end(), [&](int a, int b) {
+            return tm[static_cast<size_t>(a)].wgs > tm[static_cast<size_t>(b)].wgs;
+        });
+        for (int t : order) {
+            int rank = 0;
+            while (rank < tm[static_cast<size_t>(t)].wgs) {
+                size_t x = 0;
+                for (size_t y = 1; y < free_of.size(); ++y) {
+                    if (free_of[y].size() > free_of[x].size()) x = y;
+                }
+                while (rank < tm[static_cast<size_t>(t)].wgs && !free_of[x].empty()) {
+                    wg[static_cast<size_t>(free_of[x].back())] = int2{t, rank++};
+                    free_of[x].pop_back();
+                }
+            }
+        }
+        grow(d_teams, teams_cap, tm.size());
+        grow(d_wg_team, wg_cap, wg.size());
+        grow(d_bar, bar_cap, 16 * tm.size());
+        HIP_CHECK(hipMemcpyAsync(d_teams, tm.data(), tm.size() * sizeof(Team),
+                                 hipMemcpyHostToDevice, stream));
+        HIP_CHECK(hipMemcpyAsync(d_wg_team, wg.data(), wg.size() * sizeof(int2),
+                                 hipMemcpyHostToDevice, stream));
+    }
+
+    // Windows per persistent launch at most: each team counter then stays
+    // below (1 + 2 * kPersistWindows) * max_resident < 2^32.
+    static constexpr int64_t kPersistWindows = int64_t(1) << 16;
+
+    // The FP32 networks' `windows` windows from window sequence number seq0
+    // as persistent launches on `stream`.
+    void launch_persistent(int64_t windows, int64_t seq0) {
+        const int32_t nb0 = nblock_range[1];
+        const int32_t dw = sched_deliver_waves, sw = sched_sic_workers, wg = Wg;
+        const int32_t bars = 16 * n_teams;
+        for (int64_t done = 0; done < windows; done += kPersistWindows) {
+            int32_t win = static_cast<int32_t>(std::min(kPersistWindows, windows - done));
+            int32_t par0 = static_cast<int32_t>((seq0 + done) & 1);
+            hipLaunchKernelGGL(k_persist_prep, dim3((bars + 255) / 256), dim3(256), 0, stream,
+                               d_bar, bars, d_pfail);
+            void* args[] = {&d_nets, &d_blocks, &d_nblocks, &d_teams, &d_wg_team, &d_active,
+                            &d_nbr, &d_work, &d_work_n, &d_post, &d_sic_item,
+                            const_cast<int32_t*>(&nb0), const_cast<int32_t*>(&wg), &win, &par0,
+                            const_cast<int32_t*>(&dw), const_cast<int32_t*>(&sw), &d_bar,
+                            &d_pfail, &h_pfail_dev, &team_wait_ticks};
+            HIP_CHECK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_persist),
+                                                 dim3(persist_grid), dim3(64), args,
+                                                 static_cast<unsigned>(drive_lds_bytes()), stream));
+        }
+    }
+
     void rebuild_tables() {
         std::vector<CellBlock> blocks, nblocks;
         std::vector<int32_t> active;
         Wg = kWindowCap;
         std::fill(h_nbr.begin(), h_nbr.end(), 0);
         const std::vector<int> lane_of = assign_lanes();
+        cell_range.assign(static_cast<size_t>(lanes) + 2, 0);
+        nblock_range.assign(static_cast<size_t>(lanes) + 2, 0);
+        active_range.assign(static_cast<size_t>(lanes) + 2, 0);
         // Blocks in contiguous ranges, one per cell launch: the FP64
         // networks' blocks, then each FP32 lane's. Within a network the
         // neuron blocks come first; its first block is its lead block.
@@ -2959,8 +3302,11 @@ struct Engine {
         // other parity), not for a lane that stops and later runs again on
         // other networks: its stale entries would name blocks of networks
         // that may be gone, and its appends would run past its lists.
-        HIP_CHECK(hipMemsetAsync(d_work_n, 0, (1 + 2 * kMaxLanes) * sizeof(int32_t), stream));
-        HIP_CHECK(hipMemsetAsync(d_post, 0, 2 * kMaxLanes * sizeof(PostState), stream));
+        grow(d_work_n, work_n_cap, 1 + 2 * static_cast<size_t>(lanes));
+        grow(d_post, post_cap, 2 * static_cast<size_t>(lanes));
+        if (persistent) build_teams();
+        HIP_CHECK(hipMemsetAsync(d_work_n, 0, work_n_cap * sizeof(int32_t), stream));
+        HIP_CHECK(hipMemsetAsync(d_post, 0, post_cap * sizeof(PostState), stream));
         HIP_CHECK(hipStreamSynchronize(stream));
         dirty = false;
         clear_graphs();
@@ -3136,16 +3482,19 @@ struct Engine {
 
     // Lanes beyond the first wait for what `stream` holds so far (fork), and
     // `stream` for what they hold (join).
+    // Streams of the FP32 lanes: none beyond `stream` under persistent
+    // launches, whose groups are teams within one launch.
+    int stream_lanes() const { return persistent ? 1 : lanes; }
     void fork_lanes() {
-        if (lanes > 1) {
+        if (stream_lanes() > 1) {
             HIP_CHECK(hipEventRecord(ev_fork, stream));
-            for (int l = 1; l < lanes; ++l) {
+            for (int l = 1; l < stream_lanes(); ++l) {
                 HIP_CHECK(hipStreamWaitEvent(lane_stream[l], ev_fork, 0));
             }
         }
     }
     void join_lanes() {
-        for (int l = 1; l < lanes; ++l) {
+        for (int l = 1; l < stream_lanes(); ++l) {
             HIP_CHECK(hipEventRecord(ev_join[l], lane_stream[l]));
             HIP_CHECK(hipStreamWaitEvent(stream, ev_join[l], 0));
         }
@@ -3164,16 +3513,16 @@ struct Engine {
     // window's drive, close); the first window's drive is launched before
     // the loop. Each lane's launches go to its own stream.
     int64_t launch_windows(int64_t windows, int G, int wpb, int SG, bool marks, int64_t seq0,
-                           int only = -1) {
+                           int only = -1, bool fp32 = true) {
         int64_t timed = 0;
         if (windows <= 0) {
             return 0;
         }
-        auto on = [&](int l) { return only < 0 || only == l; };
+        const int n64 = (only < 0 || only == 0) ? active_range[1] : 0;
+        auto on = [&](int l) { return fp32 && (only < 0 || only == l); };
         for (int l = 0; l < lanes; ++l) {
             if (on(l)) launch_drive(1 + l);
         }
-        const int n64 = on(0) ? active_range[1] : 0;
         for (int64_t wdx = 0; wdx < windows; ++wdx) {
             const int p = static_cast<int>((seq0 + wdx) & 1);  // the window's parity
             const bool timed_here = marks && (prof_seq++ % prof_every) == 0;
@@ -3274,7 +3623,12 @@ struct Engine {
         raster_dirty = true;
         const int G = sic_group_for(sched_sic_group);
         const int SG = step_group_for(sched_step_group);
-        if (profiling) {
+        if (persistent) {
+            // the FP64 networks of a mixed batch by their kernels, then the
+            // FP32 teams in one launch
+            launch_windows(windows, G, sched_cell_waves, SG, false, win_seq, -1, false);
+            launch_persistent(windows, win_seq);
+        } else if (profiling) {
             // per-kernel events need eager launches
             ensure_events(windows);
             fork_lanes();
@@ -3306,11 +3660,11 @@ struct Engine {
     // schedule changes launch geometry only, never results, so tuning
     // advances the simulation like any other call.
     struct Sched {
-        int wpb, g, sg, dw, sw, lanes;
+        int wpb, g, sg, dw, sw, lanes, teams;
     };
     Sched current_schedule() const {
         return Sched{sched_cell_waves, sched_sic_group, sched_step_group, sched_deliver_waves,
-                     sched_sic_workers, sched_lanes};
+                     sched_sic_workers, sched_lanes, sched_teams};
     }
     void apply_schedule(const Sched& c) {
         sched_cell_waves = c.wpb;
@@ -3319,11 +3673,18 @@ struct Engine {
         sched_deliver_waves = c.dw;
         sched_sic_workers = c.sw;
         set_lanes(c.lanes);
+        set_teams(c.teams);
     }
-    // A new lane count takes effect with the next table rebuild.
+    // A new lane or team count takes effect with the next table rebuild.
     void set_lanes(int l) {
         if (l != sched_lanes) {
             sched_lanes = l;
+            dirty = true;
+        }
+    }
+    void set_teams(int t) {
+        if (t != sched_teams) {
+            sched_teams = t;
             dirty = true;
         }
     }
@@ -3378,10 +3739,20 @@ struct Engine {
         float best_t = 0.0f;
         try {
             std::vector<Sched> cand;
-            if (active_range[1 + lanes] - active_range[1] > 1) {  // FP32 networks to split
-                for (int l = 1; l <= kMaxLanes; ++l) {
+            const int n32 = active_range[1 + lanes] - active_range[1];
+            if (n32 > 0) {
+                // how the FP32 windows are launched: per-kernel launches on
+                // 1..kMaxLanes streams, or one persistent launch of teams
+                for (int l = 1; l <= std::min(kMaxLanes, n32); ++l) {
                     Sched c = current_schedule();
                     c.lanes = l;
+                    c.teams = 0;
+                    cand.push_back(c);
+                }
+                for (int t : {4, 16, 64, 256}) {
+                    if (t > n32 && t != 4) continue;
+                    Sched c = current_schedule();
+                    c.teams = std::min(t, n32);
                     cand.push_back(c);
                 }
                 best_t = tune_among(cand, wpt, rounds, e0, e1);
@@ -4048,12 +4419,15 @@ asg_status asg_batch_set_schedule(asg_batch* batch, const asg_schedule* sched) {
     if (batch == nullptr || sched == nullptr) return fail(ASG_ERR_INVALID, "bad argument");
     const int w = sched->cell_waves, g = sched->sic_group, sg = sched->step_group;
     const int dw = sched->deliver_waves, sw = sched->sic_workers, ln = sched->lanes;
+    const int tm = sched->teams;
     if ((w != 1 && w != 2 && w != 4) || (g != 0 && g != 1 && g != 2 && g != 4 && g != 8 && g != 16) ||
         sg < 0 || sg > asg::kWindowCap || (dw != 0 && (dw < 64 || dw > 4096)) ||
-        (sw != 0 && (sw < 64 || sw > 8192)) || ln < 0 || ln > asg::kMaxLanes) {
+        (sw != 0 && (sw < 64 || sw > 8192)) || ln < 0 || ln > asg::kMaxLanes || tm < 0 ||
+        tm > asg::kMaxTeams) {
         return fail(ASG_ERR_INVALID,
                     "cell_waves must be 1, 2 or 4; sic_group 0, 1, 2, 4, 8 or 16; step_group 0-16; "
-                    "deliver_waves 0 or 64-4096; sic_workers 0 or 64-8192; lanes 0, 1 or 2");
+                    "deliver_waves 0 or 64-4096; sic_workers 0 or 64-8192; lanes 0-3; "
+                    "teams 0-1024");
     }
     return guarded([&]() -> asg_status {
         batch->eng->wait();
@@ -4063,6 +4437,7 @@ asg_status asg_batch_set_schedule(asg_batch* batch, const asg_schedule* sched) {
         batch->eng->sched_deliver_waves = dw == 0 ? asg::kDeliverWaves : dw;
         batch->eng->sched_sic_workers = sw;
         batch->eng->set_lanes(ln == 0 ? asg::kLanes : ln);
+        batch->eng->set_teams(tm);
         return ASG_OK;
     });
 }
@@ -4075,6 +4450,7 @@ asg_status asg_batch_get_schedule(asg_batch* batch, asg_schedule* out) {
     out->deliver_waves = batch->eng->sched_deliver_waves;
     out->sic_workers = batch->eng->sched_sic_workers;
     out->lanes = batch->eng->sched_lanes;
+    out->teams = batch->eng->sched_teams;
     out->ms_per_window = batch->eng->tuned_ms_per_window;
     return ASG_OK;
 }

@@ -329,7 +329,7 @@ class Batch:
     def _schedule_dict(sch) -> dict:
         return {"cell_waves": sch.cell_waves, "sic_group": sch.sic_group,
                 "step_group": sch.step_group, "deliver_waves": sch.deliver_waves,
-                "sic_workers": sch.sic_workers, "lanes": sch.lanes,
+                "sic_workers": sch.sic_workers, "lanes": sch.lanes, "teams": sch.teams,
                 "ms_per_window": sch.ms_per_window}
 
     def autotune(self, windows_per_try: int = 20, rounds: int = 3) -> dict:
@@ -345,9 +345,13 @@ class Batch:
         return self._schedule_dict(sch)
 
     def set_schedule(self, cell_waves: int = 1, sic_group: int = 0, step_group: int = 0,
-                     deliver_waves: int = 0, sic_workers: int = 0, lanes: int = 0) -> None:
+                     deliver_waves: int = 0, sic_workers: int = 0, lanes: int = 0,
+                     teams: int = 0) -> None:
+        """Launch geometry of the windows (results do not depend on it);
+        teams > 0 runs each call's FP32 windows in one persistent launch of
+        that many teams of networks."""
         sch = capi.Schedule(int(cell_waves), int(sic_group), int(step_group), int(deliver_waves),
-                            int(sic_workers), int(lanes), 0.0)
+                            int(sic_workers), int(lanes), int(teams), 0.0)
         check(lib().asg_batch_set_schedule(self._h, C.byref(sch)))
 
     def set_graphs(self, on: bool) -> None:

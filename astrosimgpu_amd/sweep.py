@@ -82,9 +82,13 @@ def _record_plan(model: Model, record_astro_max: int = 100):
 
 
 def run_batched(models, precision: int, chunk_steps: int = 10000, budget_bytes: int = 0,
-                profile: bool = False) -> RunResult:
-    """All networks in one batch; returns after every network finished."""
+                profile: bool = False, schedule: dict | None = None) -> RunResult:
+    """All networks in one batch (launch schedule `schedule`, keyword
+    arguments of Batch.set_schedule, if given); returns after every network
+    finished."""
     batch = Batch(len(models), budget_bytes)
+    if schedule:
+        batch.set_schedule(**schedule)
     slots = []
     totals = []
     for m in models:

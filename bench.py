@@ -365,10 +365,13 @@ def main():
     ap.add_argument("--cpu-every", type=int, default=16)
     ap.add_argument("--no-sparse", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="default schedule, eager launches")
-    ap.add_argument("--lanes", type=int, default=0, choices=(0, 1, 2),
+    ap.add_argument("--lanes", type=int, default=0, choices=(0, 1, 2, 3),
                     help="with --no-tune: FP32 lanes of the schedule (0 = the default, 2); "
                          "1 puts every FP32 kernel over the whole batch, as the profiled "
                          "run does, for kernel traces")
+    ap.add_argument("--teams", type=int, default=0,
+                    help="with --no-tune: run each call's FP32 windows in one persistent "
+                         "launch of this many teams of networks (0 = per-kernel launches)")
     ap.add_argument("--fp32-only", action="store_true",
                     help="run only the batched FP32 sweep (for kernel profiles)")
     ap.add_argument("--fp64-only", action="store_true",
@@ -431,8 +434,8 @@ def main():
     batch, slots = new_batch(models, capi.ASG_FP32,
                              tune_steps + (args.warmup + args.steps + 10) * 1000, record=False)
     schedule = None
-    if args.no_tune and args.lanes:
-        batch.set_schedule(lanes=args.lanes)
+    if args.no_tune and (args.lanes or args.teams):
+        batch.set_schedule(lanes=args.lanes, teams=args.teams)
     # The transient first: the activity the schedule is tuned on (how many
     # networks have a live SIC exchange, how many blocks spike) should be
     # the timed run's.

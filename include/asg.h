@@ -417,11 +417,18 @@ int64_t asg_batch_bytes_in_use(const asg_batch* batch);
  * and per summing wavefront, 1-16; 0 = the whole window for large batches,
  * 1 otherwise), the wavefronts that share out the delivery list (64-4096,
  * 0 = 512), the FP32 post pass's SIC wavefronts (64-8192; 0 = one per
- * neuron block, the default) and the FP32 lanes (1 or 2; 0 = 2): groups of
- * the FP32 networks of about equal cell count whose windows run on streams
- * of their own, so that one group's cell pass overlaps another's post pass.
- * Results do not depend on it. ms_per_window is the autotuner's measured
- * time (output only). */
+ * neuron block, the default), the FP32 lanes (1-3; 0 = 2): groups of the
+ * FP32 networks of about equal cell count whose windows run on streams of
+ * their own, so that one group's cell pass overlaps another's post pass,
+ * and the FP32 teams (0 = off; 1-1024): with teams, each step call runs all
+ * of its windows of the FP32 networks in one persistent cooperative launch
+ * (grid sized by the occupancy API), the networks split into that many teams
+ * (at most one per network), each with its own workgroups and its own
+ * barrier between passes, so teams never wait for one another; lanes and
+ * graph replay then do not apply to the FP32 networks. A barrier that waits
+ * longer than 2 s stops the launch, and the step call (or the next call
+ * that waits for it) returns ASG_ERR_DEVICE. Results do not depend on the
+ * schedule. ms_per_window is the autotuner's measured time (output only). */
 typedef struct asg_schedule {
     int32_t cell_waves;
     int32_t sic_group;
@@ -429,13 +436,15 @@ typedef struct asg_schedule {
     int32_t deliver_waves;
     int32_t sic_workers;
     int32_t lanes;
+    int32_t teams;
     float ms_per_window;
 } asg_schedule;
 asg_status asg_batch_set_schedule(asg_batch* batch, const asg_schedule* sched);
 asg_status asg_batch_get_schedule(asg_batch* batch, asg_schedule* out);
 
-/* Autotuner: tunes the schedule one knob at a time (the FP32 lanes when the
- * batch has two or more FP32 networks, cell waves, then the FP64 networks'
+/* Autotuner: tunes the schedule one knob at a time (how the FP32 networks'
+ * windows are launched -- 1-3 lanes, or a persistent launch of 4, 16, 64 or
+ * 256 teams -- when the batch has FP32 networks, cell waves, then the FP64 networks'
  * SIC and step groups when the batch has FP64 networks, then the delivery
  * wavefront count when it has FP32 networks), timing each candidate as step
  * calls of windows_per_try windows, `rounds` times round-robin, and keeping

@@ -73,6 +73,88 @@ def test_every_schedule_gives_the_same_bits():
         _same(ref, _run(lambda b, n=lanes: b.set_schedule(1, 0, 0, 0, 0, n)))
 
 
+def test_persistent_teams_give_the_same_bits():
+    """Each call's FP32 windows in one persistent cooperative launch: the two
+    FP32 networks in one team or one team each (a count above the network
+    count is capped), behind the mixed batch's FP64 networks' own kernels,
+    with the delivery and SIC wavefront counts varied and graph replay asked
+    for (the FP64 networks' graphs; the persistent launch is not captured)."""
+    print("[gpu case] persistent teams", flush=True)
+    ref = _run(lambda b: None)
+    for teams in (1, 2, 5):
+        _same(ref, _run(lambda b, t=teams: b.set_schedule(teams=t)))
+    for dw, sw in ((64, 64), (4096, 8192)):
+        _same(ref, _run(lambda b, d=dw, e=sw: b.set_schedule(1, 0, 0, d, e, 0, 2)))
+
+    def teams_and_graphs(b):
+        b.set_schedule(teams=2)
+        b.set_graphs(True)
+    _same(ref, _run(teams_and_graphs))
+
+
+def test_switching_between_teams_lanes_and_graphs_keeps_the_bits():
+    """Calls alternate among a persistent launch of two teams, two stream
+    lanes with graphs, one team and one lane, each call an odd number of
+    windows (list parities carry over between launch modes)."""
+    print("[gpu case] teams / lanes switches", flush=True)
+    chunks = (30, 10, 30, 10, 30, 10, 30, 50)
+    total = sum(chunks)
+    ref = _run(lambda b: None, chunks=chunks, total=total)
+    modes = [dict(teams=2), dict(lanes=2), dict(teams=1), dict(lanes=1)]
+    models = _models()
+    b = A.Batch(8)
+    slots = [b.add(m, total) for m in models]
+    parts = {s: ([], []) for s in slots}
+    for k, c in enumerate(chunks):
+        b.set_schedule(**modes[k % len(modes)])
+        b.set_graphs(k % 4 == 1)
+        b.step(c)
+        for s in slots:
+            st, src = b.spikes(s)
+            parts[s][0].append(st)
+            parts[s][1].append(src)
+        b.state(slots[1], "V")
+    out = [(np.concatenate(parts[s][0]), np.concatenate(parts[s][1]), b.state(s, "V"),
+            b.state(s, "Ca")) for s in slots]
+    b.close()
+    _same(ref, out)
+
+
+def test_persistent_sweep_equals_per_kernel_launches():
+    """A 40-network FP32 sweep (200-5000 neurons), 0.3 s: the persistent
+    launch with 8 teams and with one team per network gives every network's
+    raster and recorded Ca of the default per-kernel launches, bit for bit."""
+    print("[gpu case] persistent sweep", flush=True)
+    from astrosimgpu_amd import sweep
+    specs = sweep.sweep_specs(count=40, gen_seed=5)
+    models = [sweep.build(config_path("use_case"), s, 200.0, 100.0) for s in specs]
+    ref = sweep.run_batched(models, capi.ASG_FP32, chunk_steps=1000)
+    for teams in (8, 40):
+        got = sweep.run_batched(models, capi.ASG_FP32, chunk_steps=1000,
+                                schedule=dict(teams=teams))
+        for k in range(len(models)):
+            assert np.array_equal(ref.spikes[k], got.spikes[k]), (teams, k)
+            assert np.array_equal(ref.traces[k][1].view(np.uint64),
+                                  got.traces[k][1].view(np.uint64)), (teams, k)
+
+
+def test_a_team_barrier_that_times_out_fails_the_call(monkeypatch):
+    """The persistent launch's barrier wait is bounded: with the bound set to
+    zero from the host the first barrier gives up, every workgroup leaves
+    the kernel, and the call returns ASG_ERR_DEVICE naming the barrier."""
+    print("[gpu case] team barrier time-out", flush=True)
+    monkeypatch.setenv("ASG_TEAM_WAIT_US", "0")
+    b = A.Batch(8)
+    for m in _models():
+        b.add(m, 3000)
+    b.set_schedule(teams=2)
+    with pytest.raises(A.AsgError) as e:
+        b.step(500)
+    assert e.value.status == capi.ASG_ERR_DEVICE
+    assert "team barrier" in str(e.value)
+    b.close()
+
+
 def test_a_lane_that_stops_and_runs_again_starts_from_empty_lists():
     """Calls alternate between one lane and two, each an odd number of
     10-step windows, so the second lane runs again on the window parity of
@@ -138,7 +220,8 @@ def test_autotune_then_step_equals_plain_run():
     assert chosen["step_group"] >= 1
     assert chosen["deliver_waves"] in (512, 1024, 2048)
     assert chosen["sic_workers"] == 0
-    assert chosen["lanes"] in (1, 2)
+    assert chosen["lanes"] in (1, 2, 3)
+    assert chosen["teams"] in (0, 2)  # two FP32 networks: at most two teams
     assert chosen["ms_per_window"] > 0.0
     done = b.net_step(slots[0])
     assert 0 < done < 4000 and all(b.net_step(s) == done for s in slots)
