@@ -39,9 +39,10 @@ round(0.2 N) astrocytes, IP3 weight w_n2a uniform in [0.10, 0.40], seed
                thread rate, and the C1 / C2 configurations on all cores and on
                one thread.
   sparse_fp64  the reference's 'Sparse' network (10,000 neurons, 10,000
-               astrocytes, paper_sparse.json), FP64, seeds 1-3 side by side,
-               1 s transient + --sparse-seconds timed, with its cell kernel's
-               roofline.
+               astrocytes, paper_sparse.json), FP64, seeds 1-3 in one batch
+               (graph replay), 1 s transient + --sparse-seconds timed; beside
+               it the drop-in path (one asg_net per seed, eager) and seed 1's
+               cell-kernel roofline.
   astro_scaling the isolated astrocyte update at 1e7 cells, 1,000 steps.
   serving      a bounded sample of the serving loop (--serve-jobs jobs,
                Poisson arrivals at 0.8 x the capacity measured first).
@@ -207,34 +208,55 @@ def run_plain(models, total_steps: int):
 
 
 def sparse_fp64(transient_bench_steps: int, timed_bench_steps: int):
-    """The Sparse network, FP64, seeds 1-3 resident together, each on its
-    own stream; then seed 1 alone with per-kernel event timing for the
-    roofline of its cell kernel."""
+    """The Sparse network, FP64, seeds 1-3 resident together in one batch
+    (the FP64 batch is bit-identical to each network run alone), windows
+    replayed from captured graphs; the drop-in path (each seed its own
+    asg_net on its own stream, eager launches) timed the same way beside it;
+    then seed 1 alone with per-kernel event timing for the roofline of its
+    cell kernel."""
     import astrosimgpu_amd as A
     models = [A.Model(SPARSE_CONFIG, seed=s) for s in SPARSE_SEEDS]
-    nets = [A.DeviceNetwork(m) for m in models]
+    total = (transient_bench_steps + timed_bench_steps) * STEPS_PER_BENCH_STEP
+    batch = A.Batch(len(models))
+    slots = [batch.add(m, total) for m in models]
+    batch.set_graphs(True)
     spikes = {s: [] for s in SPARSE_SEEDS}
 
-    def bench_step():
-        for net in nets:
-            net.step(STEPS_PER_BENCH_STEP, wait=False)
-        for s, net in zip(SPARSE_SEEDS, nets):
-            net.sync()
-            st, _ = net.spikes()
-            spikes[s].append(st)
+    def batch_step():
+        batch.step(STEPS_PER_BENCH_STEP)
+        drained = batch.spikes_all()
+        for s, slot in zip(SPARSE_SEEDS, slots):
+            spikes[s].append(drained[slot][0])
 
     for _ in range(transient_bench_steps):
-        bench_step()
+        batch_step()
     t0 = time.perf_counter()
     for _ in range(timed_bench_steps):
-        bench_step()
+        batch_step()
     elapsed = time.perf_counter() - t0
+    batch.close()
     first = transient_bench_steps * STEPS_PER_BENCH_STEP
     steps = timed_bench_steps * STEPS_PER_BENCH_STEP
     rates = {}
     for s in SPARSE_SEEDS:
         st = np.concatenate(spikes[s])
         rates[str(s)] = round(int((st >= first).sum()) * 1000.0 / (steps * 0.1 * 10000), 4)
+    # the drop-in path: one asg_net per seed, each on its own stream
+    nets = [A.DeviceNetwork(m) for m in models]
+
+    def nets_step():
+        for net in nets:
+            net.step(STEPS_PER_BENCH_STEP, wait=False)
+        for net in nets:
+            net.sync()
+            net.spikes()
+
+    for _ in range(min(transient_bench_steps, 5)):
+        nets_step()
+    t1 = time.perf_counter()
+    for _ in range(timed_bench_steps):
+        nets_step()
+    elapsed_nets = time.perf_counter() - t1
     for net in nets:
         net.close()
     # seed 1 alone, profiled: its cell kernel's bytes per window / time
@@ -249,6 +271,8 @@ def sparse_fp64(transient_bench_steps: int, timed_bench_steps: int):
     by = cell_bytes(m.n_neurons, m.n_astro, WINDOW, False, 1 if noise else 0)
     us = kt["cells"] * 1000.0 / max(kt["windows"], 1)
     out = {"value": round(3 * 10000 * steps / elapsed, 1), "unit": "neuron-steps/s",
+           "launch": "the three seeds in one FP64 batch, graph replay",
+           "drop_in_asg_net_value": round(3 * 10000 * steps / elapsed_nets, 1),
            "seconds_timed": steps / 10000.0, "mean_rate_hz": rates,
            "kernel_us_per_window_seed1": per_window_us(kt),
            "roofline": {"kernel": "k_cells<FP64> (one Sparse network, one window of 10 steps)",
