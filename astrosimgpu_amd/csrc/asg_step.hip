@@ -610,32 +610,64 @@ __device__ __forceinline__ void astro_window(const DevNet& n, uint32_t a, int64_
 
 // ----------------------------------------------------- FP32 cell windows ---
 
+// Global-memory access of the FP32 windows: a base address uniform across
+// the wavefront (scalar registers) plus a 32-bit byte offset per lane, so
+// each load or store is one instruction with no 64-bit address arithmetic.
+// A network's arrays fit (plan_of refuses an FP32 network whose rings,
+// drive or factor lines exceed 4 GiB).
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+
+template <typename T>
+__device__ __forceinline__ gptr<T> uniform_global(T* p) {
+    const uint64_t v = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v >> 32));
+    return (gptr<T>)(reinterpret_cast<T*>((static_cast<uint64_t>(hi) << 32) | lo));
+}
+
+template <typename T>
+__device__ __forceinline__ gptr<T> at_byte(gptr<T> base, uint32_t byte_off) {
+    return (gptr<T>)((__attribute__((address_space(1))) char*)base + byte_off);
+}
+
+__device__ __forceinline__ int64_t uniform_i64(int64_t v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
+    const uint32_t hi =
+        __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(static_cast<uint64_t>(v) >> 32));
+    return static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo);
+}
+
 __device__ __forceinline__ uint64_t neuron_window_f32(const DevNet& n, uint32_t i, int64_t T0,
                                                       int W) {
     // FP32 path: mul-add pairs may fuse (it has no host build to match)
     #pragma clang fp contract(fast)
     const uint32_t N = n.n_neur;
-    const size_t LN = n.ldn;  // row stride of the [rows][neurons] arrays
     const bool valid = i < N;
     const int pi = (i < n.n_exc) ? 0 : 1;
     const NeuronKf pf = n.pkf[pi];
     const NeuronK pd = n.pk[pi];
     const InputK in = n.in[pi];
-    const int S = n.S;
-    const int sic_interval = n.sic_interval;
-    const int substeps = n.substeps;
+    // the window's uniform quantities, in scalar registers
+    const uint32_t LN4 = __builtin_amdgcn_readfirstlane(n.ldn) * 4u;  // a [.][neurons] row
+    const uint32_t LN8 = 2u * LN4;
+    const uint32_t S8 = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(n.S)) * LN8;
+    const int sic_interval = __builtin_amdgcn_readfirstlane(n.sic_interval);
+    const int substeps = __builtin_amdgcn_readfirstlane(n.substeps);
     const double hs = n.h, half_h = n.half_h, h_sixth = n.h_sixth;
-    const uint32_t words = n.words;
-    int64_t* __restrict__ ring_exc = n.qring_exc;
-    int64_t* __restrict__ ring_inh = n.qring_inh;
-    const float* __restrict__ ring_sic = n.fring_sic;
-    uint64_t* __restrict__ spk_bits = n.spk_bits;
-    const int32_t* __restrict__ drive_n = n.drive_n;
-    const float* __restrict__ drive_noise = n.fdrive_noise;
+    const uint32_t words = __builtin_amdgcn_readfirstlane(n.words);
+    gptr<int64_t> ring_exc = uniform_global(n.qring_exc);
+    gptr<int64_t> ring_inh = uniform_global(n.qring_inh);
+    gptr<const float> ring_sic = uniform_global(static_cast<const float*>(n.fring_sic));
+    gptr<uint64_t> spk_bits = uniform_global(n.spk_bits);
+    gptr<const int32_t> drive_n = uniform_global(static_cast<const int32_t*>(n.drive_n));
+    gptr<const float> drive_noise = uniform_global(static_cast<const float*>(n.fdrive_noise));
 
-    int row = n.clk_row;  // ring row of the current step
-    int64_t sic_next = n.clk_sic_arrive;  // first_hit(T0, d_a2n, sic_interval)
-    const uint32_t live_in = live_arrivals(n, T0);
+    // byte offsets: of the current step's ring row, and of this lane's cell
+    uint32_t row8 = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(n.clk_row)) * LN8;
+    const uint32_t i8 = i * 8u, i4 = i * 4u;
+    int64_t sic_next = uniform_i64(n.clk_sic_arrive);  // first_hit(T0, d_a2n, sic_interval)
+    const uint32_t live_in = __builtin_amdgcn_readfirstlane(live_arrivals(n, T0));
     const bool sic_live_n = live_in & 1u;
     // FP64 registers; the conductances are rounded to FP32 at the end of
     // every step, V and w stay FP64
@@ -650,7 +682,7 @@ __device__ __forceinline__ uint64_t neuron_window_f32(const DevNet& n, uint32_t 
     // value nz_j from window step nz_next - steps_per_noise on.
     const int64_t nz_period = has_noise ? in.steps_per_noise : 1;
     int64_t nz_next = n.clk_nz_next[pi];  // first_hit(T0 + 1, 0, nz_period) + 1
-    uint32_t nz_j = 0;
+    uint32_t nz_off = i4;                 // byte offset of the current noise value
     if (valid) {
         V = n.V[i];
         w = n.w[i];
@@ -662,41 +694,39 @@ __device__ __forceinline__ uint64_t neuron_window_f32(const DevNet& n, uint32_t 
         refr = n.refr[i];
         Vr = n.V_reset[i];
         b = n.b[i];
-        const size_t s0 = static_cast<size_t>(row) * LN + i;
-        ex_n = ring_exc[s0];
-        ih_n = ring_inh[s0];
-        if (sic_live_n) sic_n = ring_sic[s0];
-        if (has_poiss) ev_n = drive_n[i];
-        if (has_noise) noise_n = drive_noise[i];  // the window's first noise interval
+        ex_n = *at_byte(ring_exc, row8 + i8);
+        ih_n = *at_byte(ring_inh, row8 + i8);
+        if (sic_live_n) sic_n = *at_byte(ring_sic, row8 / 2u + i4);
+        if (has_poiss) ev_n = *at_byte(drive_n, i4);
+        if (has_noise) noise_n = *at_byte(drive_noise, i4);  // the window's first noise interval
     }
     const uint32_t lane = threadIdx.x & 63;
     uint64_t spiked_any = 0;  // OR of the window's spike words (uniform)
     for (int k = 0; k < W; ++k) {
         bool spiked = false;
-        const int row_n = ring_next(row, S);
+        const uint32_t rown8 = row8 + LN8 == S8 ? 0u : row8 + LN8;
         const bool sic_step = k == sic_next;
         if (sic_step) {
             sic_next += sic_interval;
         }
         const bool next_live = (live_in >> (k + 1)) & 1u;
         if (valid) {
-            const size_t slot = static_cast<size_t>(row) * LN + i;
+            const uint32_t slot8 = row8 + i8;
             const int64_t ex = ex_n, ih = ih_n;
             const float sic = sic_n;
             const int ev = ev_n;
             const float noise = noise_n;
             if (k + 1 < W) {
-                const size_t nx = static_cast<size_t>(row_n) * LN + i;
-                ex_n = ring_exc[nx];
-                ih_n = ring_inh[nx];
-                sic_n = next_live ? ring_sic[nx] : 0.0f;
-                const size_t dk1 = static_cast<size_t>(k + 1) * LN + i;
-                if (has_poiss) ev_n = drive_n[dk1];
+                const uint32_t nx8 = rown8 + i8;
+                ex_n = *at_byte(ring_exc, nx8);
+                ih_n = *at_byte(ring_inh, nx8);
+                sic_n = next_live ? *at_byte(ring_sic, rown8 / 2u + i4) : 0.0f;
+                if (has_poiss) ev_n = *at_byte(drive_n, static_cast<uint32_t>(k + 1) * LN4 + i4);
                 if (has_noise && k + 1 == nz_next) {
                     // a new noise interval: its value is the next one stored
                     nz_next += nz_period;
-                    ++nz_j;
-                    noise_n = drive_noise[static_cast<size_t>(nz_j) * LN + i];
+                    nz_off += LN4;
+                    noise_n = *at_byte(drive_noise, nz_off);
                 }
             }
             double exc_in = 0.0, inh_in = 0.0;
@@ -707,7 +737,7 @@ __device__ __forceinline__ uint64_t neuron_window_f32(const DevNet& n, uint32_t 
                 } else {
                     inh_in -= v;
                 }
-                ring_exc[slot] = 0;
+                *at_byte(ring_exc, slot8) = 0;
             }
             if (ih != 0) {
                 const double v = static_cast<double>(ih) * kFixUnit;
@@ -716,7 +746,7 @@ __device__ __forceinline__ uint64_t neuron_window_f32(const DevNet& n, uint32_t 
                 } else {
                     inh_in += v;
                 }
-                ring_inh[slot] = 0;
+                *at_byte(ring_inh, slot8) = 0;
             }
             if (sic_step) {
                 Isic = sic;
@@ -751,10 +781,10 @@ __device__ __forceinline__ uint64_t neuron_window_f32(const DevNet& n, uint32_t 
         }
         const uint64_t mask = __ballot(spiked);
         if (lane == 0 && (i & ~63u) < N) {
-            spk_bits[static_cast<size_t>(k) * words + (i >> 6)] = mask;
+            *at_byte(spk_bits, (static_cast<uint32_t>(k) * words + (i >> 6)) * 8u) = mask;
         }
         spiked_any |= mask;
-        row = row_n;
+        row8 = rown8;
     }
     if (valid) {
         n.V[i] = V;
@@ -776,21 +806,23 @@ __device__ __forceinline__ bool astro_window_f32(const DevNet& n, uint32_t a, in
     // FP32 path: mul-add pairs may fuse (it has no host build to match)
     #pragma clang fp contract(fast)
     const uint32_t NA = n.n_astro;
-    const size_t LA = n.lda;  // row stride of the [rows][astrocytes] arrays
     if (a >= NA) {
         return false;
     }
     const AstroKf kf = n.akf;
     const InputK ain = n.ap.in;
     const double SIC_th = n.ap.SIC_th, SIC_scale = n.ap.SIC_scale;
-    const int S = n.S;
-    const int sic_interval = n.sic_interval;
-    const int substeps = n.substeps;
+    const int sic_interval = __builtin_amdgcn_readfirstlane(n.sic_interval);
+    const int substeps = __builtin_amdgcn_readfirstlane(n.substeps);
     const uint64_t seed = n.seed;
     const float hs = static_cast<float>(n.h), half_h = static_cast<float>(n.half_h),
                 h_sixth = static_cast<float>(n.h_sixth);
-    int64_t* __restrict__ ring_astro = n.qring_astro;
-    float* __restrict__ fac = n.ffac;
+    // uniform bases and 32-bit byte offsets (see at_byte)
+    gptr<int64_t> ring_astro = uniform_global(n.qring_astro);
+    gptr<float> fac = uniform_global(n.ffac);
+    const uint32_t LA8 = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(n.lda)) * 8u;
+    const uint32_t S8 = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(n.S)) * LA8;
+    const uint32_t a8 = a * 8u, line = a * 64u;  // this astrocyte's ring column, factor line
     // bit k: entry k of this astrocyte's factor line is nonzero
     uint32_t* __restrict__ fac_nz = n.ffac_nz;
     const uint32_t nz_prev = fac_nz[a];
@@ -798,30 +830,30 @@ __device__ __forceinline__ bool astro_window_f32(const DevNet& n, uint32_t a, in
     double Ca = n.Ca[a], IP3 = n.IP3[a], hh = n.hh[a];
     const float Ca_tot = n.fCa_tot[a], IP3_0 = n.fIP3_0[a], inv_tau = n.finv_tau[a],
                 delta = n.fdelta[a];
-    int row = n.clk_row;  // ring row of the current step
-    int64_t v_n = ring_astro[static_cast<size_t>(row) * LA + a];
+    // byte offset of the ring row of the current step
+    uint32_t row8 = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(n.clk_row)) * LA8;
+    int64_t v_n = *at_byte(ring_astro, row8 + a8);
     // noise index T / steps_per_noise, redrawn at the window start and at
     // every multiple of steps_per_noise
     const int64_t spn = ain.steps_per_noise;
-    int64_t noise_idx = n.clk_anz_idx;    // T0 / spn
-    int64_t noise_next = n.clk_anz_next;  // first_hit(T0 + 1, 0, spn) + 1
-    int64_t sic_next = n.clk_sic_send;    // first_hit(T0, 0, sic_interval)
+    int64_t noise_idx = uniform_i64(n.clk_anz_idx);    // T0 / spn
+    int64_t noise_next = uniform_i64(n.clk_anz_next);  // first_hit(T0 + 1, 0, spn) + 1
+    int64_t sic_next = uniform_i64(n.clk_sic_send);    // first_hit(T0, 0, sic_interval)
     RecCursor rec(n);
     float noise = 0.0f;
     uint32_t live_bits = 0;
     // Poisson stream T * kStreamStride + a: its key advances by a constant
     uint64_t pkey = rng_key(seed ^ kSaltAstroPoisson, static_cast<uint64_t>(T0) * kStreamStride + a);
     for (int k = 0; k < W; ++k, pkey += kStreamStride * kStreamMul) {
-        const int row_n = ring_next(row, S);
-        const size_t slot = static_cast<size_t>(row) * LA + a;
+        const uint32_t rown8 = row8 + LA8 == S8 ? 0u : row8 + LA8;
         const int64_t v = v_n;
         if (k + 1 < W) {
-            v_n = ring_astro[static_cast<size_t>(row_n) * LA + a];
+            v_n = *at_byte(ring_astro, rown8 + a8);
         }
         float ip3_in = 0.0f;
         if (v != 0) {
             ip3_in += static_cast<float>(static_cast<double>(v) * kFixUnit);
-            ring_astro[slot] = 0;
+            *at_byte(ring_astro, row8 + a8) = 0;
         }
         if (ain.poiss_rate > 0.0) {
             const int ev = rng_poisson_fresh_keyed(pkey, ain.lambda, ain.limit);
@@ -857,11 +889,11 @@ __device__ __forceinline__ bool astro_window_f32(const DevNet& n, uint32_t a, in
             // a zero over a zero is not stored: the line is written where
             // it changes (an astrocyte below threshold writes nothing)
             if (f != 0.0f || ((nz_prev >> k) & 1u)) {
-                fac[static_cast<size_t>(a) * 16 + k] = f;
+                *at_byte(fac, line + static_cast<uint32_t>(k) * 4u) = f;
             }
             live_bits |= static_cast<uint32_t>(f != 0.0f) << k;
         }
-        row = row_n;
+        row8 = rown8;
     }
     n.Ca[a] = Ca;
     n.IP3[a] = IP3;
@@ -2541,6 +2573,20 @@ Plan plan_of(const asg_net_desc& d, const DerivedConn* dc) {
     return p;
 }
 
+// The FP32 windows address a network's rings, drive and factor lines by
+// 32-bit byte offsets from their bases: each such array must stay below
+// 4 GiB (about 13 million neurons or astrocytes at the shipped delays).
+std::string check_offsets(const asg_net_desc& d) {
+    if (d.precision != ASG_FP32) return "";
+    const Plan p = plan_of(d, nullptr);
+    const size_t limit = size_t(1) << 32;
+    if (p.S * p.ldn * 8 >= limit || p.S * p.lda * 8 >= limit || p.lda * 64 >= limit ||
+        p.Wmax * p.ldn * 4 >= limit) {
+        return "FP32 network too large: a ring, drive or SIC-factor array would exceed 4 GiB";
+    }
+    return "";
+}
+
 // One network's device allocation and its host mirror.
 struct NetSlot {
     bool used = false;
@@ -4152,7 +4198,8 @@ int64_t asg_query_footprint(const asg_net_desc* desc) {
 
 asg_status asg_net_create(const asg_net_desc* desc, asg_net** out) {
     if (desc == nullptr || out == nullptr) return fail(ASG_ERR_INVALID, "NULL argument");
-    const std::string err = asg::validate(*desc);
+    std::string err = asg::validate(*desc);
+    if (err.empty()) err = asg::check_offsets(*desc);
     if (!err.empty()) return fail(ASG_ERR_INVALID, err);
     return guarded([&]() -> asg_status {
         auto net = std::make_unique<asg_net>();
@@ -4302,7 +4349,8 @@ asg_status asg_batch_add(asg_batch* batch, const asg_net_desc* desc, int64_t tot
     if (batch == nullptr || desc == nullptr || slot == nullptr || total_steps < 0) {
         return fail(ASG_ERR_INVALID, "bad argument");
     }
-    const std::string err = asg::validate(*desc);
+    std::string err = asg::validate(*desc);
+    if (err.empty()) err = asg::check_offsets(*desc);
     if (!err.empty()) return fail(ASG_ERR_INVALID, err);
     return guarded([&]() -> asg_status {
         const int s = batch->eng->add(*desc, total_steps);
