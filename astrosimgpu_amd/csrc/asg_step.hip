@@ -238,6 +238,14 @@ struct DevNet {
     uint32_t sic_live;     // bit k: some astrocyte has a nonzero SIC factor at
                            // window step k (set by k_cells, cleared at the close)
     uint32_t sic_live_p[2];  // FP32 networks: the same, one word per window
+    // FP32 networks whose inhibitory neurons begin inside a 64-neuron block
+    // (n_exc % 64 != 0): that block's cell pass is two wavefronts, one per
+    // population. The inhibitory one's spike words ([Wmax]) are spk_extra,
+    // which delivery ORs into the block's; split_state (per window parity)
+    // collects bit 0 / bit 1 as each part finishes and bit 2 if either
+    // spiked, so that exactly one of them lists the block.
+    uint64_t* spk_extra;
+    uint32_t split_state[2];
                              // parity (the close empties the other parity's)
     int32_t open;          // 1 from a window's cell pass to its close
     // Window clock: the step-periodic phases of the window that starts at
@@ -638,13 +646,17 @@ __device__ __forceinline__ int64_t uniform_i64(int64_t v) {
     return static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo);
 }
 
+// The W steps of the lanes of population pi (0 excitatory, 1 inhibitory;
+// uniform across the wavefront, so that the population's constants stay in
+// scalar registers) of one neuron block. Lane k ORs window step k's spike
+// word into spk_word (the caller stores the words). Returns the OR of the
+// window's spike words.
 __device__ __forceinline__ uint64_t neuron_window_f32(const DevNet& n, uint32_t i, int64_t T0,
-                                                      int W) {
+                                                      int W, int pi, uint64_t& spk_word) {
     // FP32 path: mul-add pairs may fuse (it has no host build to match)
     #pragma clang fp contract(fast)
     const uint32_t N = n.n_neur;
-    const bool valid = i < N;
-    const int pi = (i < n.n_exc) ? 0 : 1;
+    const bool valid = i < N && (i < n.n_exc) == (pi == 0);
     const NeuronKf pf = n.pkf[pi];
     const NeuronK pd = n.pk[pi];
     const InputK in = n.in[pi];
@@ -655,11 +667,9 @@ __device__ __forceinline__ uint64_t neuron_window_f32(const DevNet& n, uint32_t 
     const int sic_interval = __builtin_amdgcn_readfirstlane(n.sic_interval);
     const int substeps = __builtin_amdgcn_readfirstlane(n.substeps);
     const double hs = n.h, half_h = n.half_h, h_sixth = n.h_sixth;
-    const uint32_t words = __builtin_amdgcn_readfirstlane(n.words);
     gptr<int64_t> ring_exc = uniform_global(n.qring_exc);
     gptr<int64_t> ring_inh = uniform_global(n.qring_inh);
     gptr<const float> ring_sic = uniform_global(static_cast<const float*>(n.fring_sic));
-    gptr<uint64_t> spk_bits = uniform_global(n.spk_bits);
     gptr<const int32_t> drive_n = uniform_global(static_cast<const int32_t*>(n.drive_n));
     gptr<const float> drive_noise = uniform_global(static_cast<const float*>(n.fdrive_noise));
 
@@ -702,6 +712,15 @@ __device__ __forceinline__ uint64_t neuron_window_f32(const DevNet& n, uint32_t 
     }
     const uint32_t lane = threadIdx.x & 63;
     uint64_t spiked_any = 0;  // OR of the window's spike words (uniform)
+    // No store inside the step loop: on this GPU loads and stores complete
+    // in order, so a store issued in a step (conditional, so uncounted by
+    // the compiler) would make the next step's prefetched loads wait for it.
+    // The ring slots that held input are cleared after the loop (bit k of
+    // clr_ex / clr_ih: window step k's slot), and lane k keeps step k's spike
+    // word for the caller's store after the loop. Nothing else touches this
+    // window's ring rows before the post pass.
+    const uint32_t row0_8 = row8;
+    uint32_t clr_ex = 0, clr_ih = 0;
     for (int k = 0; k < W; ++k) {
         bool spiked = false;
         const uint32_t rown8 = row8 + LN8 == S8 ? 0u : row8 + LN8;
@@ -711,7 +730,6 @@ __device__ __forceinline__ uint64_t neuron_window_f32(const DevNet& n, uint32_t 
         }
         const bool next_live = (live_in >> (k + 1)) & 1u;
         if (valid) {
-            const uint32_t slot8 = row8 + i8;
             const int64_t ex = ex_n, ih = ih_n;
             const float sic = sic_n;
             const int ev = ev_n;
@@ -737,7 +755,7 @@ __device__ __forceinline__ uint64_t neuron_window_f32(const DevNet& n, uint32_t 
                 } else {
                     inh_in -= v;
                 }
-                *at_byte(ring_exc, slot8) = 0;
+                clr_ex |= 1u << k;
             }
             if (ih != 0) {
                 const double v = static_cast<double>(ih) * kFixUnit;
@@ -746,7 +764,7 @@ __device__ __forceinline__ uint64_t neuron_window_f32(const DevNet& n, uint32_t 
                 } else {
                     inh_in += v;
                 }
-                *at_byte(ring_inh, slot8) = 0;
+                clr_ih |= 1u << k;
             }
             if (sic_step) {
                 Isic = sic;
@@ -780,11 +798,17 @@ __device__ __forceinline__ uint64_t neuron_window_f32(const DevNet& n, uint32_t 
             dgin = round_f32(dgin);
         }
         const uint64_t mask = __ballot(spiked);
-        if (lane == 0 && (i & ~63u) < N) {
-            *at_byte(spk_bits, (static_cast<uint32_t>(k) * words + (i >> 6)) * 8u) = mask;
-        }
+        spk_word = lane == static_cast<uint32_t>(k) ? (spk_word | mask) : spk_word;
         spiked_any |= mask;
         row8 = rown8;
+    }
+    for (uint32_t m = clr_ex; m != 0; m &= m - 1) {
+        const uint32_t r8 = row0_8 + static_cast<uint32_t>(__ffs(m) - 1) * LN8;
+        *at_byte(ring_exc, (r8 >= S8 ? r8 - S8 : r8) + i8) = 0;
+    }
+    for (uint32_t m = clr_ih; m != 0; m &= m - 1) {
+        const uint32_t r8 = row0_8 + static_cast<uint32_t>(__ffs(m) - 1) * LN8;
+        *at_byte(ring_inh, (r8 >= S8 ? r8 - S8 : r8) + i8) = 0;
     }
     if (valid) {
         n.V[i] = V;
@@ -839,6 +863,9 @@ __device__ __forceinline__ bool astro_window_f32(const DevNet& n, uint32_t a, in
     int64_t noise_idx = uniform_i64(n.clk_anz_idx);    // T0 / spn
     int64_t noise_next = uniform_i64(n.clk_anz_next);  // first_hit(T0 + 1, 0, spn) + 1
     int64_t sic_next = uniform_i64(n.clk_sic_send);    // first_hit(T0, 0, sic_interval)
+    // ring slots that held input, cleared after the loop (see neuron_window_f32)
+    const uint32_t row0_8 = row8;
+    uint32_t clr = 0;
     RecCursor rec(n);
     float noise = 0.0f;
     uint32_t live_bits = 0;
@@ -853,10 +880,14 @@ __device__ __forceinline__ bool astro_window_f32(const DevNet& n, uint32_t a, in
         float ip3_in = 0.0f;
         if (v != 0) {
             ip3_in += static_cast<float>(static_cast<double>(v) * kFixUnit);
-            *at_byte(ring_astro, row8 + a8) = 0;
+            clr |= 1u << k;
         }
         if (ain.poiss_rate > 0.0) {
-            const int ev = rng_poisson_fresh_keyed(pkey, ain.lambda, ain.limit);
+            // Knuth's loop only: an FP32 network with lambda > 30 here (the
+            // reference's rounded normal) is refused at creation
+            // (check_fp32_limits), so the normal's FP64 log and cos need no
+            // registers in this kernel
+            const int ev = rng_poisson_knuth_keyed(pkey, ain.limit);
             if (ev > 0) {
                 ip3_in += static_cast<float>(ain.poiss_weight * static_cast<double>(ev));
             }
@@ -894,6 +925,10 @@ __device__ __forceinline__ bool astro_window_f32(const DevNet& n, uint32_t a, in
             live_bits |= static_cast<uint32_t>(f != 0.0f) << k;
         }
         row8 = rown8;
+    }
+    for (uint32_t m = clr; m != 0; m &= m - 1) {
+        const uint32_t r8 = row0_8 + static_cast<uint32_t>(__ffs(m) - 1) * LA8;
+        *at_byte(ring_astro, (r8 >= S8 ? r8 - S8 : r8) + a8) = 0;
     }
     n.Ca[a] = Ca;
     n.IP3[a] = IP3;
@@ -1253,9 +1288,36 @@ __device__ __forceinline__ void cells_item(DevNet* nets, const CellBlock& cb, in
     }
     const uint32_t i = cb.first + lane;
     if constexpr (P == ASG_FP32) {
-        if (cb.kind == 0) {
-            if (neuron_window_f32(n, i, T0, W) != 0 && lane == 0) {
-                work[atomicAdd(work_n, 1)] = cb;
+        if (cb.kind != 1) {
+            // one population per wavefront (its constants in scalar
+            // registers): kind 0 the population of the block's first neuron,
+            // kind 2 the inhibitory part of the block where they begin
+            const uint32_t n_exc = __builtin_amdgcn_readfirstlane(n.n_exc);
+            const int pi = cb.kind == 2 ? 1 : (cb.first < n_exc ? 0 : 1);
+            uint64_t spk_word = 0;
+            const uint64_t any = neuron_window_f32(n, i, T0, W, pi, spk_word);
+            const bool split = cb.first < n_exc && cb.first + 63 >= n_exc && n_exc < n.n_neur;
+            if (lane < static_cast<uint32_t>(W) && cb.first < n.n_neur) {
+                if (cb.kind == 2) {
+                    n.spk_extra[lane] = spk_word;
+                } else {
+                    n.spk_bits[static_cast<size_t>(lane) * n.words + (cb.first >> 6)] = spk_word;
+                }
+            }
+            bool list = any != 0;
+            if (split) {
+                // the part that finishes second lists the block if either
+                // part spiked; the atomic's old value is all it reads
+                uint32_t old = 0;
+                if (lane == 0) {
+                    old = atomicOr(&n.split_state[par],
+                                   (cb.kind == 2 ? 2u : 1u) | (any != 0 ? 4u : 0u));
+                }
+                old = __builtin_amdgcn_readfirstlane(old);
+                list = (old & (cb.kind == 2 ? 1u : 2u)) != 0 && ((old & 4u) != 0 || any != 0);
+            }
+            if (list && lane == 0) {
+                work[atomicAdd(work_n, 1)] = CellBlock{cb.net, 0, cb.first, cb.lead};
             }
         } else {
             // the wavefront whose update made the exchange live lists the
@@ -1297,8 +1359,11 @@ __device__ __forceinline__ void cells_item(DevNet* nets, const CellBlock& cb, in
 // between two sets of these lists and live words by parity `par` (the host
 // passes the set of the window's parity), so that the post pass can empty
 // the other set while it reads this one.
+// The FP32 instantiation is held to five wavefronts per SIMD (96 VGPRs,
+// which its neuron and astrocyte paths fit without spilling).
 template <int P, int WPB>
-__global__ __launch_bounds__(kCellBlock * WPB) void k_cells(DevNet* nets, const CellBlock* blocks,
+__global__ __launch_bounds__(kCellBlock * WPB)
+__attribute__((amdgpu_waves_per_eu(P == ASG_FP32 ? 5 : 1))) void k_cells(DevNet* nets, const CellBlock* blocks,
                                                             int32_t count, int32_t Wg,
                                                             CellBlock* work, int32_t* work_n,
                                                             int32_t par, PostState* post,
@@ -1707,9 +1772,14 @@ __device__ __forceinline__ void deliver_fixed(DevNet& n, uint32_t first, uint32_
     const int64_t T0 = n.win_T0;
     const int S = n.S;
     const size_t LN = n.ldn, LA = n.lda;
-    // lane k < W holds the block's spike word of window step k
-    const uint64_t l_word =
-        static_cast<int>(lane) < W ? n.spk_bits[static_cast<size_t>(lane) * n.words + (first >> 6)] : 0;
+    // lane k < W holds the block's spike word of window step k (with the
+    // inhibitory part's, in the block where the inhibitory neurons begin)
+    const bool split = first < NE && first + 63 >= NE && NE < N;
+    uint64_t l_word = 0;
+    if (static_cast<int>(lane) < W) {
+        l_word = n.spk_bits[static_cast<size_t>(lane) * n.words + (first >> 6)];
+        if (split) l_word |= n.spk_extra[lane];
+    }
     uint32_t mine = 0;  // bit k: this lane's neuron spiked at window step k
     for (int k2 = 0; k2 < W; ++k2) {
         mine |= static_cast<uint32_t>((lane_u64(l_word, k2) >> lane) & 1ULL) << k2;
@@ -1898,6 +1968,7 @@ __device__ __forceinline__ void post_item(int e, DevNet* nets, const CellBlock* 
                 close_window_with(n, n.sic_live_p[par]);
             }
             n.sic_live_p[par ^ 1] = 0;
+            n.split_state[par ^ 1] = 0;
         }
         if (j == 0) {
             *other_work_n = 0;
@@ -2261,6 +2332,8 @@ __global__ __launch_bounds__(256) void k_set_targets(DevNet* nets, const int32_t
         DevNet& n = nets[active[j]];
         n.target = targets[j];
         set_clock(n, n.step);
+        n.split_state[0] = 0;
+        n.split_state[1] = 0;
     }
 }
 
@@ -2573,11 +2646,19 @@ Plan plan_of(const asg_net_desc& d, const DerivedConn* dc) {
     return p;
 }
 
-// The FP32 windows address a network's rings, drive and factor lines by
-// 32-bit byte offsets from their bases: each such array must stay below
-// 4 GiB (about 13 million neurons or astrocytes at the shipped delays).
-std::string check_offsets(const asg_net_desc& d) {
+// Limits of the FP32 path. Its windows address a network's rings, drive
+// and factor lines by 32-bit byte offsets from their bases, so each such
+// array must stay below 4 GiB (about 13 million neurons or astrocytes at the
+// shipped delays). Its cell kernel draws the astrocytes' Poisson input by
+// Knuth's loop only, so their expected events per step must not exceed 30
+// (above that the reference switches to a rounded normal: a Poisson rate
+// over 300 kHz at dt = 0.1 ms, which the FP64 path runs).
+std::string check_fp32_limits(const asg_net_desc& d) {
     if (d.precision != ASG_FP32) return "";
+    if (d.input_astro.poiss_rate * d.dt * 1e-3 > 30.0) {
+        return "FP32 path: astrocyte Poisson input above 30 events per step is not supported "
+               "(run this network in FP64)";
+    }
     const Plan p = plan_of(d, nullptr);
     const size_t limit = size_t(1) << 32;
     if (p.S * p.ldn * 8 >= limit || p.S * p.lda * 8 >= limit || p.lda * 64 >= limit ||
@@ -2783,6 +2864,7 @@ void build_slot(NetSlot& s, const asg_net_desc& d, int64_t total, hipStream_t st
     n.sic_src = dev_copy(s, dc.sic_src.data(), dc.sic_src.size(), st);
     n.sic_mult = dev_copy(s, dc.sic_mult.data(), dc.sic_mult.size(), st);
     n.spk_bits = dev_zero<uint64_t>(s, pl.Wmax * pl.words, st);
+    n.spk_extra = dev_zero<uint64_t>(s, pl.Wmax, st);
     n.spk_woff = dev_zero<uint32_t>(s, pl.Wmax * pl.words, st);
     n.spk_list = dev_zero<uint32_t>(s, pl.Wmax * N, st);
     n.spk_n = dev_zero<uint32_t>(s, pl.Wmax * 2, st);
@@ -3306,6 +3388,12 @@ struct Engine {
                     blocks.push_back(CellBlock{sidx, 0, f, lead});
                     nblocks.push_back(CellBlock{sidx, 0, f, lead});
                     lead = 0;
+                    // FP32: the block where the inhibitory neurons begin
+                    // gets a second cell wavefront for them (kind 2)
+                    const uint32_t ne = s.dn.n_exc;
+                    if (range > 0 && f < ne && f + kCellBlock > ne && ne < s.dn.n_neur) {
+                        blocks.push_back(CellBlock{sidx, 2, f, 0});
+                    }
                 }
                 for (uint32_t f = 0; f < s.dn.n_astro; f += kCellBlock) {
                     blocks.push_back(CellBlock{sidx, 1, f, lead});
@@ -4199,7 +4287,7 @@ int64_t asg_query_footprint(const asg_net_desc* desc) {
 asg_status asg_net_create(const asg_net_desc* desc, asg_net** out) {
     if (desc == nullptr || out == nullptr) return fail(ASG_ERR_INVALID, "NULL argument");
     std::string err = asg::validate(*desc);
-    if (err.empty()) err = asg::check_offsets(*desc);
+    if (err.empty()) err = asg::check_fp32_limits(*desc);
     if (!err.empty()) return fail(ASG_ERR_INVALID, err);
     return guarded([&]() -> asg_status {
         auto net = std::make_unique<asg_net>();
@@ -4350,7 +4438,7 @@ asg_status asg_batch_add(asg_batch* batch, const asg_net_desc* desc, int64_t tot
         return fail(ASG_ERR_INVALID, "bad argument");
     }
     std::string err = asg::validate(*desc);
-    if (err.empty()) err = asg::check_offsets(*desc);
+    if (err.empty()) err = asg::check_fp32_limits(*desc);
     if (!err.empty()) return fail(ASG_ERR_INVALID, err);
     return guarded([&]() -> asg_status {
         const int s = batch->eng->add(*desc, total_steps);

@@ -96,6 +96,20 @@ ASG_HD int rng_poisson_fresh_keyed(std::uint64_t key, double lambda, double limi
     return count;
 }
 
+// Knuth's loop of rng_poisson_fresh_keyed alone, for lambda <= 30 (the
+// product of fresh uniforms until it is <= limit = e^-lambda; 0 when
+// lambda <= 0, as limit >= 1 then).
+ASG_HD int rng_poisson_knuth_keyed(std::uint64_t key, double limit) {
+    std::uint64_t counter = 0;
+    double product = rng_uniform_keyed(key, counter++);
+    int count = 0;
+    while (product > limit) {
+        ++count;
+        product *= rng_uniform_keyed(key, counter++);
+    }
+    return count;
+}
+
 ASG_HD int rng_poisson_fresh(std::uint64_t seed, std::uint64_t stream, double lambda,
                              double limit) {
     return rng_poisson_fresh_keyed(rng_key(seed, stream), lambda, limit);

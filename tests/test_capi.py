@@ -149,6 +149,24 @@ def test_invalid_descriptions_rejected_before_the_device():
     assert A.lib().asg_net_create(None, C.byref(h)) == capi.ASG_ERR_INVALID
 
 
+def test_fp32_limits_rejected_before_the_device():
+    """The FP32 path refuses what it does not run: astrocyte Poisson input
+    above 30 events per step (the reference's rounded-normal regime, which
+    the FP64 path runs) and arrays beyond its 32-bit byte offsets."""
+    m = A.Model(config_path("quick"))
+    h = C.c_void_p()
+    d = capi.NetDesc.from_buffer_copy(m.desc.contents)
+    d.precision = capi.ASG_FP32
+    d.input_astro.poiss_rate = 30.0 / (d.dt * 1e-3) * 1.01
+    assert A.lib().asg_net_create(C.byref(d), C.byref(h)) == capi.ASG_ERR_INVALID
+    assert b"30 events per step" in A.lib().asg_last_error()
+    d = capi.NetDesc.from_buffer_copy(m.desc.contents)
+    d.precision = capi.ASG_FP32
+    d.exc.delay_steps = 1 << 22  # rings of 4M rows: beyond 4 GiB
+    assert A.lib().asg_net_create(C.byref(d), C.byref(h)) == capi.ASG_ERR_INVALID
+    assert b"4 GiB" in A.lib().asg_last_error()
+
+
 def test_footprint_of_sparse_network_fits_one_gpu():
     m = A.Model(config_path("paper_sparse"))
     fp = m.footprint()
