@@ -1941,13 +1941,15 @@ __global__ __launch_bounds__(256) void k_deliver(DevNet* nets, const CellBlock* 
 //    emptied for the next window (nothing here reads them);
 //  * [C, C + D): the delivery wavefronts, sharing out the list of neuron
 //    blocks that spiked (deliver_fixed);
-//  * [C + D, C + D + E): SIC wavefronts, sharing out the neuron blocks of
-//    the networks the cell pass listed as having a live exchange (a network
-//    whose astrocytes all have a zero SIC factor has nothing to sum);
-//  * [C + D + E, C + D + E + n_nblocks): the drive of the next window, one
+//  * [C + D, C + D + n_nblocks): the drive of the next window, one
 //    wavefront per neuron block -- it depends on nothing this window
 //    computes, and fills the device while the delivery chains wait on
-//    memory.
+//    memory; its wavefronts are the launch's longest, so they are
+//    dispatched right after the delivery ones;
+//  * [C + D + n_nblocks, C + D + n_nblocks + E): SIC wavefronts, sharing
+//    out the neuron blocks of the networks the cell pass listed as having a
+//    live exchange (a network whose astrocytes all have a zero SIC factor
+//    has nothing to sum).
 // The close writes only what no other workgroup of the launch reads (the
 // live word it reads is this parity's, which nothing writes now), so the
 // workgroups need no ordering among themselves.
@@ -1981,9 +1983,13 @@ __device__ __forceinline__ void post_item(int e, DevNet* nets, const CellBlock* 
             const CellBlock cb = work[j];
             deliver_fixed(nets[cb.net], cb.first, lane);
         }
-    } else if (e < C + D + E) {
+    } else if (e < C + D + n_nblocks) {
+        const CellBlock cb = nblocks[e - C - D];
+        drive_block(nets[cb.net], cb, Wg, drive_lds);
+    } else if (e < C + D + n_nblocks + E) {
         const uint32_t items = min(post->sic_items, static_cast<uint32_t>(n_nblocks));
-        for (uint32_t j = static_cast<uint32_t>(e - C - D); j < items; j += static_cast<uint32_t>(E)) {
+        for (uint32_t j = static_cast<uint32_t>(e - C - D - n_nblocks); j < items;
+             j += static_cast<uint32_t>(E)) {
             const CellBlock cb = nblocks[sic_item[j]];
             DevNet& n = nets[cb.net];
             const uint32_t t = cb.first + lane;
@@ -1991,9 +1997,6 @@ __device__ __forceinline__ void post_item(int e, DevNet* nets, const CellBlock* 
                 deliver_sic_f32<kWindowCap>(n, t, n.win_W, n.sic_live_p[par]);
             }
         }
-    } else if (e - C - D - E < n_nblocks) {
-        const CellBlock cb = nblocks[e - C - D - E];
-        drive_block(nets[cb.net], cb, Wg, drive_lds);
     }
 }
 
