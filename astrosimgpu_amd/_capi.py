@@ -176,6 +176,10 @@ PROTOTYPES = {
     "asg_batch_create": (C.c_int, [C.c_int32, C.c_int64, PP]),
     "asg_batch_destroy": (C.c_int, [P]),
     "asg_batch_add": (C.c_int, [P, C.POINTER(NetDesc), C.c_int64, C.POINTER(C.c_int32)]),
+    "asg_net_prepare": (C.c_int, [C.POINTER(NetDesc), C.POINTER(P)]),
+    "asg_prepared_free": (None, [P]),
+    "asg_batch_add_prepared": (C.c_int, [P, C.POINTER(NetDesc), P, C.c_int64,
+                                         C.POINTER(C.c_int32)]),
     "asg_batch_remove": (C.c_int, [P, C.c_int32]),
     "asg_batch_step": (C.c_int, [P, C.c_int64]),
     "asg_batch_step_async": (C.c_int, [P, C.c_int64]),

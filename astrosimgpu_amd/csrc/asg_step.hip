@@ -50,6 +50,7 @@
 #include <cstdlib>
 #include <cstddef>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -2672,10 +2673,70 @@ std::string check_fp32_limits(const asg_net_desc& d) {
 }
 
 // One network's device allocation and its host mirror.
+// Device blocks that networks' arrays are carved from. A removed network's
+// blocks wait here for the next added network, so that admission in a
+// serving loop calls no hipMalloc (slow, tens of microseconds to
+// milliseconds) or hipFree (which may wait for the device) in steady state.
+// At most kPoolCap bytes are kept; a failed hipMalloc first returns the
+// cached blocks to the device and tries again.
+struct BlockPool {
+    static constexpr size_t kGranule = size_t(2) << 20;
+    static constexpr size_t kPoolCap = size_t(16) << 30;
+    std::multimap<size_t, void*> free_blocks;  // size -> block
+    size_t cached = 0;
+
+    static size_t rounded(size_t need) {
+        return (std::max<size_t>(need, 1) + kGranule - 1) / kGranule * kGranule;
+    }
+
+    // A block of at least rounded(need) bytes: a cached one of at most twice
+    // that, or a new one. Returns the block and its size.
+    std::pair<void*, size_t> acquire(size_t need) {
+        const size_t size = rounded(need);
+        auto it = free_blocks.lower_bound(size);
+        if (it != free_blocks.end() && it->first <= 2 * size) {
+            const std::pair<void*, size_t> b{it->second, it->first};
+            cached -= it->first;
+            free_blocks.erase(it);
+            return b;
+        }
+        void* p = nullptr;
+        if (hipMalloc(&p, size) != hipSuccess) {
+            (void)hipGetLastError();
+            flush();
+            HIP_CHECK(hipMalloc(&p, size));
+        }
+        return {p, size};
+    }
+    void give_back(void* p, size_t size) {
+        free_blocks.emplace(size, p);
+        cached += size;
+        while (cached > kPoolCap && !free_blocks.empty()) {
+            auto big = std::prev(free_blocks.end());
+            cached -= big->first;
+            (void)hipFree(big->second);
+            free_blocks.erase(big);
+        }
+    }
+    void flush() {
+        for (auto& b : free_blocks) (void)hipFree(b.second);
+        free_blocks.clear();
+        cached = 0;
+    }
+};
+
 struct NetSlot {
     bool used = false;
     DevNet dn{};                  // host copy (device addresses)
-    std::vector<void*> allocs;
+    // The slot's arrays are carved, 256-byte aligned, from blocks of the
+    // pool (one block sized by the network's planned footprint, as a rule),
+    // each zeroed once when taken.
+    BlockPool* pool = nullptr;
+    std::vector<std::pair<void*, size_t>> blocks;
+    char* carve = nullptr;        // next free byte of the current block
+    size_t carve_left = 0;
+    size_t block_hint = 0;        // size asked for the first block
+    hipStream_t carve_stream = nullptr;
     size_t bytes = 0;
     int W = 1;
     int64_t step = 0;
@@ -2689,13 +2750,24 @@ struct NetSlot {
     int32_t overflow = 0;         // overflow bits read at the last drain
 };
 
+// `count` elements of T from the slot's arena, zero-filled.
 template <typename T>
 T* dev_alloc(NetSlot& s, size_t count) {
-    void* p = nullptr;
-    if (count == 0) count = 1;
-    HIP_CHECK(hipMalloc(&p, count * sizeof(T)));
-    s.allocs.push_back(p);
-    return static_cast<T*>(p);
+    const size_t bytes = (std::max<size_t>(count, 1) * sizeof(T) + 255) / 256 * 256;
+    if (bytes > s.carve_left) {
+        const size_t want = std::max(bytes, s.block_hint);
+        const std::pair<void*, size_t> b = s.pool->acquire(want);
+        s.block_hint = 0;
+        s.blocks.push_back(b);
+        s.bytes += BlockPool::rounded(want);  // what the slot asked for (see Engine::add)
+        HIP_CHECK(hipMemsetAsync(b.first, 0, b.second, s.carve_stream));
+        s.carve = static_cast<char*>(b.first);
+        s.carve_left = b.second;
+    }
+    T* p = reinterpret_cast<T*>(s.carve);
+    s.carve += bytes;
+    s.carve_left -= bytes;
+    return p;
 }
 
 template <typename T>
@@ -2707,27 +2779,35 @@ T* dev_copy(NetSlot& s, const T* host, size_t count, hipStream_t st) {
     return p;
 }
 
-template <typename T>
-T* dev_fill(NetSlot& s, T value, size_t count, hipStream_t st) {
-    std::vector<T> h(std::max<size_t>(count, 1), value);
-    T* p = dev_alloc<T>(s, count);
-    HIP_CHECK(hipMemcpyAsync(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
-    HIP_CHECK(hipStreamSynchronize(st));
+__global__ __launch_bounds__(256) void k_fill_f64(double* p, double value, int64_t count) {
+    const int64_t j = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (j < count) {
+        p[j] = value;
+    }
+}
+
+double* dev_fill(NetSlot& s, double value, size_t count, hipStream_t st) {
+    double* p = dev_alloc<double>(s, count);
+    if (count > 0) {
+        hipLaunchKernelGGL(k_fill_f64, dim3(static_cast<unsigned>((count + 255) / 256)), dim3(256),
+                           0, st, p, value, static_cast<int64_t>(count));
+    }
     return p;
 }
 
+// The arena's blocks are zeroed when taken.
 template <typename T>
-T* dev_zero(NetSlot& s, size_t count, hipStream_t st) {
-    T* p = dev_alloc<T>(s, count);
-    HIP_CHECK(hipMemsetAsync(p, 0, std::max<size_t>(count, 1) * sizeof(T), st));
-    return p;
+T* dev_zero(NetSlot& s, size_t count, hipStream_t) {
+    return dev_alloc<T>(s, count);
 }
 
 void release(NetSlot& s) {
-    for (void* p : s.allocs) {
-        (void)hipFree(p);
+    for (const auto& b : s.blocks) {
+        s.pool->give_back(b.first, b.second);
     }
-    s.allocs.clear();
+    s.blocks.clear();
+    s.carve = nullptr;
+    s.carve_left = 0;
     if (s.rec_alloc != nullptr) {
         (void)hipFree(s.rec_alloc);
         s.rec_alloc = nullptr;
@@ -2740,9 +2820,33 @@ void release(NetSlot& s) {
     s.overflow = 0;
 }
 
-void build_slot(NetSlot& s, const asg_net_desc& d, int64_t total, hipStream_t st) {
-    const DerivedConn dc = derive(d);
-    const Plan pl = plan_of(d, &dc);
+// The host-side part of admitting a network: the derived connectivity and
+// the plan. No device call, so any host thread can make it.
+struct Prepared {
+    DerivedConn dc;
+    Plan pl;
+};
+
+Prepared prepare(const asg_net_desc& d) {
+    Prepared p;
+    p.dc = derive(d);
+    p.pl = plan_of(d, &p.dc);
+    return p;
+}
+
+// `prep` is prepare(d) made earlier (or null: made here).
+void build_slot(NetSlot& s, const asg_net_desc& d, int64_t total, hipStream_t st,
+                const Prepared* prep = nullptr) {
+    Prepared local;
+    if (prep == nullptr) {
+        local = prepare(d);
+        prep = &local;
+    }
+    const DerivedConn& dc = prep->dc;
+    const Plan& pl = prep->pl;
+    s.block_hint = pl.bytes;
+    s.carve_stream = st;
+    s.bytes = 0;
     DevNet& n = s.dn;
     n = DevNet{};
     n.n_exc = d.n_exc;
@@ -2891,7 +2995,6 @@ void build_slot(NetSlot& s, const asg_net_desc& d, int64_t total, hipStream_t st
     n.r_step = dev_alloc<int64_t>(s, pl.r_cap);
     n.r_src = dev_alloc<uint32_t>(s, pl.r_cap);
     HIP_CHECK(hipStreamSynchronize(st));
-    s.bytes = pl.bytes;
     s.W = static_cast<int>(pl.Wmax);
     s.step = 0;
     s.drained_to = 0;
@@ -2912,6 +3015,7 @@ struct Engine {
     int device = 0;
     int max_nets = 0;
     int64_t budget = 0;
+    BlockPool pool;                    // the slots' device blocks
     std::vector<NetSlot> slots;
     DevNet* d_nets = nullptr;
     CellBlock* d_blocks = nullptr;
@@ -3041,6 +3145,7 @@ struct Engine {
         HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         HIP_CHECK(hipStreamCreateWithFlags(&up_stream, hipStreamNonBlocking));
         slots.resize(static_cast<size_t>(max_n));
+        for (NetSlot& s : slots) s.pool = &pool;
         const size_t m = static_cast<size_t>(max_n);
         HIP_CHECK(hipMalloc(&d_nets, sizeof(DevNet) * m));
         HIP_CHECK(hipHostMalloc(&h_targets, sizeof(int64_t) * m, hipHostMallocDefault));
@@ -3089,6 +3194,7 @@ struct Engine {
         clear_graphs();
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
         for (NetSlot& s : slots) release(s);
+        pool.flush();
         (void)hipFree(d_nets);
         (void)hipFree(d_blocks);
         (void)hipFree(d_nblocks);
@@ -3449,7 +3555,7 @@ struct Engine {
         clear_graphs();
     }
 
-    int add(const asg_net_desc& d, int64_t total) {
+    int add(const asg_net_desc& d, int64_t total, const Prepared* prep = nullptr) {
         int free_slot = -1;
         for (int i = 0; i < max_nets; ++i) {
             if (!slots[i].used) {
@@ -3459,15 +3565,18 @@ struct Engine {
         }
         if (free_slot < 0) return -1;
         const DerivedConn* none = nullptr;
-        const Plan pl = plan_of(d, none);
-        if (budget > 0 && bytes_in_use() + static_cast<int64_t>(pl.bytes) > budget) return -2;
+        const Plan pl = prep != nullptr ? prep->pl : plan_of(d, none);
+        // A slot counts the rounded sizes it asks the pool for (a reused
+        // block can be larger; the pool keeps what it caches bounded).
+        const int64_t need = static_cast<int64_t>(BlockPool::rounded(pl.bytes));
+        if (budget > 0 && bytes_in_use() + need > budget) return -2;
         // No wait for a step call in flight: the free slot is in none of the
         // tables it runs on (a removed slot's last call was drained by the
         // remove), so its device descriptor and buffers are not read until
         // the next table rebuild, by which time the uploads have finished.
         NetSlot& s = slots[free_slot];
         try {
-            build_slot(s, d, total, up_stream);
+            build_slot(s, d, total, up_stream, prep);
             HIP_CHECK(hipMemcpyAsync(d_nets + free_slot, &s.dn, sizeof(DevNet),
                                      hipMemcpyHostToDevice, up_stream));
             HIP_CHECK(hipStreamSynchronize(up_stream));
@@ -4224,6 +4333,10 @@ struct asg_batch {
     std::unique_ptr<Engine> eng;
 };
 
+struct asg_prepared {
+    asg::Prepared p;
+};
+
 namespace {
 
 const char* overflow_message(int bits) {
@@ -4284,7 +4397,7 @@ const char* asg_version(void) { return "astrosimgpu-amd 0.1 (gfx950)"; }
 
 int64_t asg_query_footprint(const asg_net_desc* desc) {
     if (desc == nullptr) return -1;
-    return static_cast<int64_t>(asg::plan_of(*desc, nullptr).bytes);
+    return static_cast<int64_t>(asg::BlockPool::rounded(asg::plan_of(*desc, nullptr).bytes));
 }
 
 asg_status asg_net_create(const asg_net_desc* desc, asg_net** out) {
@@ -4445,6 +4558,38 @@ asg_status asg_batch_add(asg_batch* batch, const asg_net_desc* desc, int64_t tot
     if (!err.empty()) return fail(ASG_ERR_INVALID, err);
     return guarded([&]() -> asg_status {
         const int s = batch->eng->add(*desc, total_steps);
+        if (s == -1) return fail(ASG_ERR_CAPACITY, "no free slot");
+        if (s == -2) return fail(ASG_ERR_CAPACITY, "device memory budget exceeded");
+        *slot = s;
+        return ASG_OK;
+    });
+}
+
+asg_status asg_net_prepare(const asg_net_desc* desc, asg_prepared** out) {
+    if (desc == nullptr || out == nullptr) return fail(ASG_ERR_INVALID, "NULL argument");
+    std::string err = asg::validate(*desc);
+    if (err.empty()) err = asg::check_fp32_limits(*desc);
+    if (!err.empty()) return fail(ASG_ERR_INVALID, err);
+    try {
+        auto prep = std::make_unique<asg_prepared>();
+        prep->p = asg::prepare(*desc);
+        *out = prep.release();
+        return ASG_OK;
+    } catch (const std::bad_alloc&) {
+        return fail(ASG_ERR_NOMEM, "host allocation failed");
+    }
+}
+
+void asg_prepared_free(asg_prepared* prep) { delete prep; }
+
+asg_status asg_batch_add_prepared(asg_batch* batch, const asg_net_desc* desc,
+                                  const asg_prepared* prep, int64_t total_steps, int32_t* slot) {
+    if (batch == nullptr || desc == nullptr || prep == nullptr || slot == nullptr ||
+        total_steps < 0) {
+        return fail(ASG_ERR_INVALID, "bad argument");
+    }
+    return guarded([&]() -> asg_status {
+        const int s = batch->eng->add(*desc, total_steps, &prep->p);
         if (s == -1) return fail(ASG_ERR_CAPACITY, "no free slot");
         if (s == -2) return fail(ASG_ERR_CAPACITY, "device memory budget exceeded");
         *slot = s;

@@ -88,7 +88,23 @@ class Model:
     def footprint(self) -> int:
         return int(lib().asg_query_footprint(self.desc))
 
+    def prepare(self) -> None:
+        """The host-side part of admitting this network to a batch
+        (asg_net_prepare: validation, derived connectivity, device plan),
+        done now, on the calling thread; the next Batch.add / try_add then
+        only allocates and uploads. Change nothing in the desc afterwards."""
+        self.release_prepared()
+        handle = C.c_void_p()
+        check(lib().asg_net_prepare(self.desc, C.byref(handle)))
+        self._prepared = handle
+
+    def release_prepared(self) -> None:
+        if getattr(self, "_prepared", None):
+            lib().asg_prepared_free(self._prepared)
+            self._prepared = None
+
     def close(self):
+        self.release_prepared()
         if getattr(self, "_h", None):
             lib().asg_model_free(self._h)
             self._h = None
@@ -203,19 +219,30 @@ class Batch:
         self._rec_n = {}
         self._drain = None  # staging arrays of spikes_all, kept between calls
 
+    def _add(self, model: Model, total_steps: int, slot) -> int:
+        prep = getattr(model, "_prepared", None)
+        if prep:
+            return lib().asg_batch_add_prepared(self._h, model.desc, prep, int(total_steps),
+                                                C.byref(slot))
+        return lib().asg_batch_add(self._h, model.desc, int(total_steps), C.byref(slot))
+
     def add(self, model: Model, total_steps: int) -> int:
+        """Admit a network (with the work of Model.prepare done beforehand
+        if it was called)."""
         slot = C.c_int32()
-        check(lib().asg_batch_add(self._h, model.desc, int(total_steps), C.byref(slot)))
+        check(self._add(model, total_steps, slot))
+        model.release_prepared()
         self.models[slot.value] = model
         return slot.value
 
     def try_add(self, model: Model, total_steps: int) -> Optional[int]:
         """Like add, but returns None when the batch has no room now."""
         slot = C.c_int32()
-        st = lib().asg_batch_add(self._h, model.desc, int(total_steps), C.byref(slot))
+        st = self._add(model, total_steps, slot)
         if st == capi.ASG_ERR_CAPACITY:
             return None
         check(st)
+        model.release_prepared()
         self.models[slot.value] = model
         return slot.value
 

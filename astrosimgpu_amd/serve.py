@@ -92,6 +92,9 @@ def build_job(base_config: str, job: Job, in_degree: float = 100.0) -> Model:
                            "connectivity.generator": "per_row",
                            "simulation.record_astro": False})
     m.desc.contents.precision = job.precision
+    # the host side of admission (derived connectivity, plan) here, on the
+    # builder's thread, rather than on the loop's thread between chunks
+    m.prepare()
     return m
 
 

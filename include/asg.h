@@ -351,7 +351,9 @@ asg_status asg_net_set_recording(asg_net* net, int64_t start_step, int32_t every
 asg_status asg_net_read_trace(asg_net* net, double* out, int64_t* steps, int64_t max_records,
                               int64_t* n_out);
 
-/* Bytes of device memory asg_net_create would allocate for `desc`. */
+/* Bytes of device memory asg_net_create would take for `desc` (its planned
+ * arrays, rounded up to the 2 MiB blocks they are carved from): what batch
+ * admission counts against the budget. */
 int64_t asg_query_footprint(const asg_net_desc* desc);
 
 /* ------------------------------------------------------------------------ */
@@ -370,6 +372,21 @@ asg_status asg_batch_destroy(asg_batch* batch);
  * id. ASG_ERR_CAPACITY if the footprint does not fit the budget now. */
 asg_status asg_batch_add(asg_batch* batch, const asg_net_desc* desc, int64_t total_steps,
                          int32_t* slot);
+
+/* Admission in two parts, for serving loops (no counterpart in the
+ * reference). asg_net_prepare does the host-side work of asg_batch_add --
+ * validation, the deduplicated neuron->astrocyte rows and the SIC gather
+ * layout, the device plan -- with no device call, so it can run on any host
+ * thread (a job's builder) while the device thread keeps stepping.
+ * asg_batch_add_prepared then only allocates and uploads: it is
+ * asg_batch_add for a desc that was prepared (the same desc, its host arrays
+ * unchanged), with the same results. Free the handle with
+ * asg_prepared_free once the network is added (or not needed). */
+typedef struct asg_prepared asg_prepared;
+asg_status asg_net_prepare(const asg_net_desc* desc, asg_prepared** out);
+void asg_prepared_free(asg_prepared* prep);
+asg_status asg_batch_add_prepared(asg_batch* batch, const asg_net_desc* desc,
+                                  const asg_prepared* prep, int64_t total_steps, int32_t* slot);
 
 /* Free a network's device memory (finished or not). */
 asg_status asg_batch_remove(asg_batch* batch, int32_t slot);

@@ -167,6 +167,22 @@ def test_fp32_limits_rejected_before_the_device():
     assert b"4 GiB" in A.lib().asg_last_error()
 
 
+def test_prepare_runs_without_a_device_and_validates():
+    """asg_net_prepare (the host half of batch admission) needs no device
+    and refuses what asg_batch_add refuses."""
+    m = A.Model(config_path("quick"))
+    h = C.c_void_p()
+    assert A.lib().asg_net_prepare(m.desc, C.byref(h)) == capi.ASG_OK and h.value
+    A.lib().asg_prepared_free(h)
+    m.prepare()
+    m.prepare()  # again: the earlier handle is freed
+    m.release_prepared()
+    d = capi.NetDesc.from_buffer_copy(m.desc.contents)
+    d.substeps = 0
+    assert A.lib().asg_net_prepare(C.byref(d), C.byref(h)) == capi.ASG_ERR_INVALID
+    assert b"substeps" in A.lib().asg_last_error()
+
+
 def test_footprint_of_sparse_network_fits_one_gpu():
     m = A.Model(config_path("paper_sparse"))
     fp = m.footprint()
