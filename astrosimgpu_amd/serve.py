@@ -282,7 +282,8 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp32")
     ap.add_argument("--chunk-steps", type=int, default=2000)
-    ap.add_argument("--builders", type=int, default=4)
+    ap.add_argument("--builders", type=int, default=8,
+                    help="host threads that build and prepare jobs' networks")
     ap.add_argument("--max-slots", type=int, default=512)
     ap.add_argument("--budget-gb", type=float, default=0.0, help="0 = free device memory")
     ap.add_argument("--config", default=DEFAULT_CONFIG)

@@ -581,10 +581,10 @@ def main():
     if args.serve_jobs > 0 and not args.fp32_only:
         log(f"serving sample, {args.serve_jobs} jobs ...")
         from astrosimgpu_amd import serve
-        cap = serve.calibrate(serve.DEFAULT_CONFIG, 48, capi.ASG_FP32, 2000, 4)
+        cap = serve.calibrate(serve.DEFAULT_CONFIG, 48, capi.ASG_FP32, 2000, 8)
         rate = 0.8 * cap / serve._mean_job_neuron_steps()
         jobs = serve.job_stream(args.serve_jobs, rate, seed=7)
-        srv = serve.Server(serve.DEFAULT_CONFIG, chunk_steps=2000, builders=4)
+        srv = serve.Server(serve.DEFAULT_CONFIG, chunk_steps=2000, builders=8)
         rep_ = srv.run(jobs)
         srv.close()
         serving = {"jobs": rep_.jobs, "rate_per_s": round(rate, 3),
