@@ -3478,17 +3478,16 @@ struct Engine {
         nblock_range.assign(static_cast<size_t>(lanes) + 2, 0);
         active_range.assign(static_cast<size_t>(lanes) + 2, 0);
         // Blocks in contiguous ranges, one per cell launch: the FP64
-        // networks' blocks, then each FP32 lane's. In the FP64 range each
-        // network's neuron blocks come before its astrocyte blocks. An FP32
-        // range lists all its astrocyte blocks first: an FP32 astrocyte
-        // wavefront runs longer than a neuron one (its counter-stream draw
-        // every step), and a launch of more wavefronts than the device
-        // holds at once finishes sooner with the longest dispatched first
-        // (42.8 against 45.0 us per window on the 256-network sweep).
-        // Each network's first neuron block (its first astrocyte block if
-        // it has no neurons) is its lead block.
+        // networks' blocks, then each FP32 lane's. A range lists all its
+        // astrocyte blocks before its neuron blocks: an astrocyte wavefront
+        // runs longer than a neuron one (a counter-stream draw every step),
+        // and a launch of more wavefronts than the device holds at once
+        // finishes sooner with the longest dispatched first (on the
+        // 256-network sweep, FP32 42.8 against 45.0 us per window, FP64
+        // 113.3 against 114.2). Each network's first neuron block (its
+        // first astrocyte block if it has no neurons) is its lead block.
         for (int range = 0; range < 1 + lanes; ++range) {
-            std::vector<CellBlock> neuron_part;  // FP32: after the astrocyte blocks
+            std::vector<CellBlock> neuron_part;  // after the astrocyte blocks
             for (int sidx = 0; sidx < max_nets; ++sidx) {
                 const NetSlot& s = slots[sidx];
                 if (!s.used || (s.dn.precision == ASG_FP32) != (range > 0)) continue;
@@ -3500,17 +3499,16 @@ struct Engine {
                     h_nbr[2 * sidx + 1] =
                         static_cast<int32_t>((s.dn.n_neur + kCellBlock - 1) / kCellBlock);
                 }
-                std::vector<CellBlock>& nb_dst = range > 0 ? neuron_part : blocks;
                 uint32_t lead = 1;
                 for (uint32_t f = 0; f < s.dn.n_neur; f += kCellBlock) {
-                    nb_dst.push_back(CellBlock{sidx, 0, f, lead});
+                    neuron_part.push_back(CellBlock{sidx, 0, f, lead});
                     nblocks.push_back(CellBlock{sidx, 0, f, lead});
                     lead = 0;
                     // FP32: the block where the inhibitory neurons begin
                     // gets a second cell wavefront for them (kind 2)
                     const uint32_t ne = s.dn.n_exc;
                     if (range > 0 && f < ne && f + kCellBlock > ne && ne < s.dn.n_neur) {
-                        nb_dst.push_back(CellBlock{sidx, 2, f, 0});
+                        neuron_part.push_back(CellBlock{sidx, 2, f, 0});
                     }
                 }
                 for (uint32_t f = 0; f < s.dn.n_astro; f += kCellBlock) {
