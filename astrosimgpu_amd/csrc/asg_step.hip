@@ -2388,6 +2388,11 @@ __global__ void k_math(int32_t fn, const double* x, double* y, int64_t n) {
     for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
          i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
         const double v = x[i];
+        if (fn == ASG_MATH_DIV) {
+            // pairs (a, b): a / b by div_rn, b passed through (n is even)
+            y[i] = (i & 1) ? v : div_rn(v, x[i + 1], 1.0 / x[i + 1]);
+            continue;
+        }
         double r;
         switch (fn) {
             case ASG_MATH_EXP: r = asg_exp(v); break;
@@ -2423,6 +2428,9 @@ NeuronK make_neuron_k(const asg_neuron_params& s, double h) {
     p.E_ex = s.E_ex;
     p.E_in = s.E_in;
     p.I_e = s.I_e;
+    p.inv_C_m = 1.0 / s.C_m;
+    p.inv_Delta_T = 1.0 / s.Delta_T;
+    p.inv_tau_w = 1.0 / s.tau_w;
     p.gL_DeltaT = s.g_L * s.Delta_T;
     constexpr double kE = 2.7182818284590452353602874713527;
     p.psc_ex = kE / s.tau_syn_ex;
@@ -4855,7 +4863,8 @@ asg_status scratch_of_current_device(double** out) {
 extern "C" {
 
 asg_status asg_device_math_noalloc(int32_t fn, const double* x, double* y, int64_t n) {
-    if (fn < ASG_MATH_EXP || fn > ASG_MATH_EXP2_F32 || n < 0 || (n > 0 && (x == nullptr || y == nullptr))) {
+    if (fn < ASG_MATH_EXP || fn > ASG_MATH_DIV || n < 0 || (n > 0 && (x == nullptr || y == nullptr)) ||
+        (fn == ASG_MATH_DIV && n % 2 != 0)) {
         return fail(ASG_ERR_INVALID, "bad argument");
     }
     if (n > kNoAllocMaxValues) {
@@ -4881,7 +4890,8 @@ asg_status asg_device_math_noalloc(int32_t fn, const double* x, double* y, int64
 }
 
 asg_status asg_device_math(int32_t fn, const double* x, double* y, int64_t n) {
-    if (fn < ASG_MATH_EXP || fn > ASG_MATH_EXP2_F32 || n < 0 || (n > 0 && (x == nullptr || y == nullptr))) {
+    if (fn < ASG_MATH_EXP || fn > ASG_MATH_DIV || n < 0 || (n > 0 && (x == nullptr || y == nullptr)) ||
+        (fn == ASG_MATH_DIV && n % 2 != 0)) {
         return fail(ASG_ERR_INVALID, "bad argument");
     }
     if (n == 0) return ASG_OK;

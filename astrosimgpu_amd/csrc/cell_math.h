@@ -12,10 +12,27 @@
 
 namespace asg {
 
+// a / b rounded to nearest, from y = 1/b rounded to nearest (b a positive
+// normal constant with a normal reciprocal). q0 = a y is within 1.5 ulp of
+// a/b; one correction q1 = q0 + (a - b q0) y makes it faithful, and for a
+// faithful q the remainder a - b q is exact and q + (a - b q) y rounds to
+// RN(a/b) (Markstein's theorem: y within half an ulp of 1/b, q faithful,
+// no overflow or underflow; here |a| is 0 or far from both). A zero or
+// non-finite a takes q0 = a y, which is a / b's +-0, +-inf or NaN. Eight
+// full-rate instructions in place of the eleven of an IEEE division with
+// its quarter-rate reciprocal.
+ASG_HD double div_rn(double a, double b, double y) {
+    const double q0 = a * y;
+    const double q1 = fma(fma(-b, q0, a), y, q0);
+    const double q2 = fma(fma(-b, q1, a), y, q1);
+    return (a == 0.0 || !__builtin_isfinite(a)) ? q0 : q2;
+}
+
 // AstroConstants (astrocyte_kernel.hpp:29-40) with the constant products of
 // astro_derivatives folded in.
 struct AstroK {
     double ratio_ER_cyt;
+    double inv_ratio_ER_cyt;  // 1 / ratio_ER_cyt rounded to nearest
     double Kd_IP3_1, Kd_IP3_2, Kd_act;
     double ratio_rate_IP3R;   // ratio_ER_cyt * rate_IP3R
     double ratio_rate_L;      // ratio_ER_cyt * rate_L
@@ -28,6 +45,7 @@ struct AstroK {
 inline AstroK make_astro_k(const asg_astro_constants& c) {
     AstroK k{};
     k.ratio_ER_cyt = c.ratio_ER_cyt;
+    k.inv_ratio_ER_cyt = 1.0 / c.ratio_ER_cyt;
     k.Kd_IP3_1 = c.Kd_IP3_1;
     k.Kd_IP3_2 = c.Kd_IP3_2;
     k.Kd_act = c.Kd_act;
@@ -40,10 +58,13 @@ inline AstroK make_astro_k(const asg_astro_constants& c) {
     return k;
 }
 
-// astro_derivatives, astrocyte_kernel.hpp:47-68.
-ASG_HD void astro_rhs(const AstroK& c, double Ca_tot, double IP3_0, double tau_IP3, double Ca,
-                      double IP3, double h, double noise, double& dCa, double& dIP3, double& dh) {
-    const double Ca_ER = (Ca_tot - Ca) / c.ratio_ER_cyt;
+// astro_derivatives, astrocyte_kernel.hpp:47-68, with its divisions by
+// ratio_ER_cyt and tau_IP3 as div_rn (inv_tau = 1 / tau_IP3 rounded to
+// nearest; the same roundings).
+ASG_HD void astro_rhs(const AstroK& c, double Ca_tot, double IP3_0, double tau_IP3,
+                      double inv_tau, double Ca, double IP3, double h, double noise, double& dCa,
+                      double& dIP3, double& dh) {
+    const double Ca_ER = div_rn(Ca_tot - Ca, c.ratio_ER_cyt, c.inv_ratio_ER_cyt);
     const double m_inf = IP3 / (IP3 + c.Kd_IP3_1);
     const double n_inf = Ca / (Ca + c.Kd_act);
     const double m3 = m_inf * m_inf * m_inf;
@@ -55,7 +76,7 @@ ASG_HD void astro_rhs(const AstroK& c, double Ca_tot, double IP3_0, double tau_I
     const double alpha_h = c.k_Kd_inh * (IP3 + c.Kd_IP3_1) / (IP3 + c.Kd_IP3_2);
     const double beta_h = c.k_IP3R * Ca;
     dCa = J_channel - J_pump + J_leak + noise;
-    dIP3 = (IP3_0 - IP3) / tau_IP3;
+    dIP3 = div_rn(IP3_0 - IP3, tau_IP3, inv_tau);
     dh = alpha_h * (1.0 - h) - beta_h * h;
 }
 
@@ -68,14 +89,15 @@ ASG_HD void astro_step(const AstroK& c, double Ca_tot, double IP3_0, double tau_
     if (ip3_input != 0.0) {
         IP3 += delta_IP3 * ip3_input;
     }
+    const double inv_tau = 1.0 / tau_IP3;
     for (int s = 0; s < substeps; ++s) {
         double k1C, k1I, k1h, k2C, k2I, k2h, k3C, k3I, k3h, k4C, k4I, k4h;
-        astro_rhs(c, Ca_tot, IP3_0, tau_IP3, Ca, IP3, h, noise, k1C, k1I, k1h);
-        astro_rhs(c, Ca_tot, IP3_0, tau_IP3, Ca + half_h * k1C, IP3 + half_h * k1I,
+        astro_rhs(c, Ca_tot, IP3_0, tau_IP3, inv_tau, Ca, IP3, h, noise, k1C, k1I, k1h);
+        astro_rhs(c, Ca_tot, IP3_0, tau_IP3, inv_tau, Ca + half_h * k1C, IP3 + half_h * k1I,
                   h + half_h * k1h, noise, k2C, k2I, k2h);
-        astro_rhs(c, Ca_tot, IP3_0, tau_IP3, Ca + half_h * k2C, IP3 + half_h * k2I,
+        astro_rhs(c, Ca_tot, IP3_0, tau_IP3, inv_tau, Ca + half_h * k2C, IP3 + half_h * k2I,
                   h + half_h * k2h, noise, k3C, k3I, k3h);
-        astro_rhs(c, Ca_tot, IP3_0, tau_IP3, Ca + h_step * k3C, IP3 + h_step * k3I,
+        astro_rhs(c, Ca_tot, IP3_0, tau_IP3, inv_tau, Ca + h_step * k3C, IP3 + h_step * k3I,
                   h + h_step * k3h, noise, k4C, k4I, k4h);
         Ca += h_sixth * (k1C + 2.0 * k2C + 2.0 * k3C + k4C);
         IP3 += h_sixth * (k1I + 2.0 * k2I + 2.0 * k3I + k4I);
@@ -100,21 +122,24 @@ ASG_HD double sic_factor(double Ca, double SIC_th, double SIC_scale) {
 // precomputed by the same functions: exp(-h/tau_syn), e/tau_syn, g_L*Delta_T.
 struct NeuronK {
     double C_m, neg_g_L, E_L, V_th, Delta_T, a, tau_w, V_peak, E_ex, E_in, I_e;
+    double inv_C_m, inv_Delta_T, inv_tau_w;  // 1 / C_m, ... rounded to nearest
     double gL_DeltaT;           // g_L * Delta_T
     double psc_ex, psc_in;      // e / tau_syn_ex, e / tau_syn_in
     double ex_decay, in_decay;  // exp(-h / tau_syn_ex), exp(-h / tau_syn_in)
     int refr_steps;             // int(t_ref / h + 0.5)
 };
 
-// NeuronPopulation::derivatives, neuron.cpp:85-99.
+// NeuronPopulation::derivatives, neuron.cpp:85-99, with its three divisions
+// by constants as div_rn (the same roundings).
 ASG_HD void adex_rhs(const NeuronK& p, double V, double w, double g_ex, double g_in, double I_ext,
                      double& dV, double& dw) {
     const double Vc = (p.V_peak < V) ? p.V_peak : V;
-    const double I_spike = p.gL_DeltaT * asg_exp((Vc - p.V_th) / p.Delta_T);
+    const double I_spike = p.gL_DeltaT * asg_exp(div_rn(Vc - p.V_th, p.Delta_T, p.inv_Delta_T));
     const double I_syn_ex = g_ex * (Vc - p.E_ex);
     const double I_syn_in = g_in * (Vc - p.E_in);
-    dV = (p.neg_g_L * (Vc - p.E_L) + I_spike - I_syn_ex - I_syn_in - w + p.I_e + I_ext) / p.C_m;
-    dw = (p.a * (Vc - p.E_L) - w) / p.tau_w;
+    dV = div_rn(p.neg_g_L * (Vc - p.E_L) + I_spike - I_syn_ex - I_syn_in - w + p.I_e + I_ext,
+                p.C_m, p.inv_C_m);
+    dw = div_rn(p.a * (Vc - p.E_L) - w, p.tau_w, p.inv_tau_w);
 }
 
 // The substep loop of NeuronPopulation::update, neuron.cpp:171-229. Returns

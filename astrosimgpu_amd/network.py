@@ -414,8 +414,10 @@ def _read_trace(call, n_rec: int, max_records: int):
 
 def device_math(fn: str, x: np.ndarray) -> np.ndarray:
     """Evaluate exp/log/sin/cos/sqrt of the device step, or the FP32 path's
-    2^y of the spike current ("exp2_f32"), on the device."""
-    codes = {"exp": 0, "log": 1, "sin": 2, "cos": 3, "sqrt": 4, "exp2_f32": 5}
+    2^y of the spike current ("exp2_f32"), on the device; "div": x holds
+    (a, b) pairs and the result's even entries are a / b by the FP64 step's
+    division through the reciprocal of b."""
+    codes = {"exp": 0, "log": 1, "sin": 2, "cos": 3, "sqrt": 4, "exp2_f32": 5, "div": 6}
     x = np.ascontiguousarray(x, np.float64)
     y = np.empty_like(x)
     check(lib().asg_device_math(codes[fn], x.ctypes.data_as(capi.DBLP),

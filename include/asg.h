@@ -507,7 +507,10 @@ enum {
     ASG_MATH_SIN = 2,
     ASG_MATH_COS = 3,
     ASG_MATH_SQRT = 4,
-    ASG_MATH_EXP2_F32 = 5 /* the FP32 path's 2^y of the AdEx spike current */
+    ASG_MATH_EXP2_F32 = 5, /* the FP32 path's 2^y of the AdEx spike current */
+    ASG_MATH_DIV = 6       /* the FP64 step's division by a constant through its
+                              reciprocal: x holds n / 2 pairs (a, b) (n even),
+                              y[2i] = a / b, y[2i + 1] = b */
 };
 
 /* Evaluates one of the step's functions on the device for n host inputs
