@@ -3039,6 +3039,13 @@ struct Engine {
     hipEvent_t ev_join[kMaxLanes] = {};
     int sched_lanes = kLanes;  // requested
     int lanes = 1;             // of the current tables
+    // FP64 groups: the FP64 networks split like the FP32 lanes (as many as
+    // the lane count, each on lane stream g), so that one group's cell pass
+    // overlaps another's latency-bound compaction, delivery, sums and
+    // drive. Group g's blocks, neuron blocks and networks are [c64[g],
+    // c64[g + 1]), [nb64[g], nb64[g + 1]) and [a64[g], a64[g + 1]) of range 0.
+    int lanes64 = 1;
+    std::vector<int> c64, nb64, a64;
     // The window's delivery lists: neuron blocks that spiked, appended by
     // k_cells: the FP64 networks' from entry 0, counted in d_work_n[0]
     // (zeroed by the FP64 networks' k_drive); each FP32 lane's in two lists
@@ -3329,16 +3336,31 @@ struct Engine {
         lanes = profiling    ? 1
                 : persistent ? std::max(1, std::min(sched_teams, n32))
                              : std::max(1, std::min(sched_lanes, n32));
-        std::sort(w.begin(), w.end());
-        std::vector<int64_t> load(static_cast<size_t>(lanes), 0);
-        for (const auto& e : w) {
-            int best = 0;
-            for (int l = 1; l < lanes; ++l) {
-                if (load[l] < load[best]) best = l;
-            }
-            lane_of[static_cast<size_t>(e.second)] = best;
-            load[best] -= e.first;
+        // the FP64 networks' groups, balanced the same way
+        std::vector<std::pair<int64_t, int>> w64;
+        for (int sidx = 0; sidx < max_nets; ++sidx) {
+            const NetSlot& s = slots[sidx];
+            if (!s.used || s.dn.precision == ASG_FP32) continue;
+            const int64_t nb = (s.dn.n_neur + kCellBlock - 1) / kCellBlock +
+                               (s.dn.n_astro + kCellBlock - 1) / kCellBlock;
+            w64.emplace_back(-nb, sidx);
         }
+        const int n64 = static_cast<int>(w64.size());
+        lanes64 = profiling || persistent ? 1 : std::max(1, std::min(sched_lanes, n64));
+        auto spread = [&](std::vector<std::pair<int64_t, int>>& v, int k) {
+            std::sort(v.begin(), v.end());
+            std::vector<int64_t> load(static_cast<size_t>(k), 0);
+            for (const auto& e : v) {
+                int best = 0;
+                for (int l = 1; l < k; ++l) {
+                    if (load[l] < load[best]) best = l;
+                }
+                lane_of[static_cast<size_t>(e.second)] = best;
+                load[best] -= e.first;
+            }
+        };
+        spread(w, lanes);
+        spread(w64, lanes64);
         return lane_of;
     }
 
@@ -3494,12 +3516,18 @@ struct Engine {
         // 256-network sweep, FP32 42.8 against 45.0 us per window, FP64
         // 113.3 against 114.2). Each network's first neuron block (its
         // first astrocyte block if it has no neurons) is its lead block.
+        // The FP64 range is its groups' blocks one group after another.
+        c64.assign(static_cast<size_t>(lanes64) + 1, 0);
+        nb64.assign(static_cast<size_t>(lanes64) + 1, 0);
+        a64.assign(static_cast<size_t>(lanes64) + 1, 0);
         for (int range = 0; range < 1 + lanes; ++range) {
+          const int groups = range == 0 ? lanes64 : 1;
+          for (int grp = 0; grp < groups; ++grp) {
             std::vector<CellBlock> neuron_part;  // after the astrocyte blocks
             for (int sidx = 0; sidx < max_nets; ++sidx) {
                 const NetSlot& s = slots[sidx];
                 if (!s.used || (s.dn.precision == ASG_FP32) != (range > 0)) continue;
-                if (range > 0 && lane_of[static_cast<size_t>(sidx)] != range - 1) continue;
+                if (lane_of[static_cast<size_t>(sidx)] != (range > 0 ? range - 1 : grp)) continue;
                 active.push_back(sidx);
                 Wg = std::min(Wg, s.W);
                 if (range > 0) {
@@ -3525,6 +3553,12 @@ struct Engine {
                 }
             }
             blocks.insert(blocks.end(), neuron_part.begin(), neuron_part.end());
+            if (range == 0) {
+                c64[grp + 1] = static_cast<int>(blocks.size());
+                nb64[grp + 1] = static_cast<int>(nblocks.size());
+                a64[grp + 1] = static_cast<int>(active.size());
+            }
+          }
             cell_range[range + 1] = static_cast<int>(blocks.size());
             nblock_range[range + 1] = static_cast<int>(nblocks.size());
             active_range[range + 1] = static_cast<int>(active.size());
@@ -3561,7 +3595,7 @@ struct Engine {
         // other parity), not for a lane that stops and later runs again on
         // other networks: its stale entries would name blocks of networks
         // that may be gone, and its appends would run past its lists.
-        grow(d_work_n, work_n_cap, 1 + 2 * static_cast<size_t>(lanes));
+        grow(d_work_n, work_n_cap, 1 + 2 * static_cast<size_t>(lanes) + (lanes64 - 1));
         grow(d_post, post_cap, 2 * static_cast<size_t>(lanes));
         if (persistent) build_teams();
         HIP_CHECK(hipMemsetAsync(d_work_n, 0, work_n_cap * sizeof(int32_t), stream));
@@ -3616,27 +3650,81 @@ struct Engine {
     // k_deliver of the FP64 networks for SIC group size G (its accumulators
     // are registers).
     template <int G>
-    void launch_deliver_g(int32_t Wg_, int MG) {
-        const int count = nblock_range[1];
+    void launch_deliver_g(int32_t Wg_, int MG, int grp) {
+        const int count = nb64[grp + 1] - nb64[grp];
         if (count == 0) return;
         // delivery wavefronts (before the SIC ones) over the listed blocks
         const int groups = (Wg_ + MG - 1) / MG;
         const int D = std::min(count * groups, sched_deliver_waves);
         const int items = count * ((Wg_ + G - 1) / G) + D;
-        hipLaunchKernelGGL(k_deliver<G>, dim3((items + 3) / 4), dim3(256), 0, stream, d_nets,
-                           d_nblocks, items, Wg_, D, MG, d_work, d_work_n);
+        hipLaunchKernelGGL(k_deliver<G>, dim3((items + 3) / 4), dim3(256), 0, lane_stream[grp],
+                           d_nets, d_nblocks + nb64[grp], items, Wg_, D, MG, d_work + nb64[grp],
+                           work_n64(grp));
     }
 
-    // Pass 3 of the FP64 networks: marking in groups of MG steps and the SIC
+    // Pass 3 of FP64 group grp: marking in groups of MG steps and the SIC
     // sums of every block.
-    void launch_deliver64(int G, int32_t Wg_, int MG) {
+    void launch_deliver64(int G, int32_t Wg_, int MG, int grp) {
         switch (G) {
-            case 1: launch_deliver_g<1>(Wg_, MG); break;
-            case 2: launch_deliver_g<2>(Wg_, MG); break;
-            case 4: launch_deliver_g<4>(Wg_, MG); break;
-            case 8: launch_deliver_g<8>(Wg_, MG); break;
-            default: launch_deliver_g<16>(Wg_, MG); break;
+            case 1: launch_deliver_g<1>(Wg_, MG, grp); break;
+            case 2: launch_deliver_g<2>(Wg_, MG, grp); break;
+            case 4: launch_deliver_g<4>(Wg_, MG, grp); break;
+            case 8: launch_deliver_g<8>(Wg_, MG, grp); break;
+            default: launch_deliver_g<16>(Wg_, MG, grp); break;
         }
+    }
+
+    // FP64 group grp's delivery list count (its list is d_work + nb64[grp]).
+    int32_t* work_n64(int grp) const {
+        return grp == 0 ? d_work_n : d_work_n + 1 + 2 * lanes + (grp - 1);
+    }
+
+    // FP64 group grp's window launches (on lane stream grp): the drive
+    // (whose first block empties the group's list count), the cell pass,
+    // the compaction of its spike words and its ordered sums.
+    void launch_drive64(int grp) {
+        const int count = nb64[grp + 1] - nb64[grp];
+        if (count == 0) return;
+        hipLaunchKernelGGL(k_drive, dim3(count), dim3(64), drive_lds_bytes(), lane_stream[grp],
+                           d_nets, d_nblocks + nb64[grp], count, Wg, work_n64(grp));
+    }
+    void launch_cells64(int grp, int32_t Wg_, int wpb, int p) {
+        const int count = c64[grp + 1] - c64[grp];
+        if (count == 0) return;
+        const CellBlock* blocks = d_blocks + c64[grp];
+        const dim3 grid((count + wpb - 1) / wpb);
+        CellBlock* work = d_work + nb64[grp];
+        int32_t* work_n = work_n64(grp);
+        hipStream_t st = lane_stream[grp];
+        // post / sic_item are not used by the FP64 cell pass
+        switch (wpb) {
+            case 4:
+                hipLaunchKernelGGL((k_cells<ASG_FP64, 4>), grid, dim3(4 * kCellBlock), 0, st,
+                                   d_nets, blocks, count, Wg_, work, work_n, p, d_post, d_sic_item,
+                                   d_nbr);
+                break;
+            case 2:
+                hipLaunchKernelGGL((k_cells<ASG_FP64, 2>), grid, dim3(2 * kCellBlock), 0, st,
+                                   d_nets, blocks, count, Wg_, work, work_n, p, d_post, d_sic_item,
+                                   d_nbr);
+                break;
+            default:
+                hipLaunchKernelGGL((k_cells<ASG_FP64, 1>), grid, dim3(kCellBlock), 0, st, d_nets,
+                                   blocks, count, Wg_, work, work_n, p, d_post, d_sic_item, d_nbr);
+                break;
+        }
+    }
+    void launch_compact64(int grp) {
+        const int n = a64[grp + 1] - a64[grp];
+        if (n == 0) return;
+        hipLaunchKernelGGL(k_compact, dim3(n, Wg), dim3(64), 0, lane_stream[grp], d_nets,
+                           d_active + a64[grp]);
+    }
+    void launch_sum64(int grp, int SG) {
+        const int items = (c64[grp + 1] - c64[grp]) * ((Wg + SG - 1) / SG);
+        if (items == 0) return;
+        hipLaunchKernelGGL(k_sum, dim3((items + 3) / 4), dim3(256), 0, lane_stream[grp], d_nets,
+                           d_blocks + c64[grp], items, Wg, SG);
     }
 
     size_t drive_lds_bytes() const {
@@ -3648,16 +3736,13 @@ struct Engine {
         return range == 0 ? stream : lane_stream[range - 1];
     }
 
-    // The drive of the next window of the networks of range `range` as a
-    // launch of its own; the FP64 range also empties its delivery list
-    // count.
+    // The drive of the call's first window of FP32 lane `range` - 1 as a
+    // launch of its own (later windows are driven by the post pass).
     void launch_drive(int range) {
         const int count = nblock_range[range + 1] - nblock_range[range];
         if (count == 0) return;
         hipLaunchKernelGGL(k_drive, dim3(count), dim3(64), drive_lds_bytes(), range_stream(range),
-                           d_nets,
-                           d_nblocks + nblock_range[range], count, Wg,
-                           range == 0 ? d_work_n : nullptr);
+                           d_nets, d_nblocks + nblock_range[range], count, Wg, nullptr);
     }
 
     // FP32 lane l's delivery list, its count, the post pass's state and the
@@ -3746,7 +3831,7 @@ struct Engine {
     // `stream` for what they hold (join).
     // Streams of the FP32 lanes: none beyond `stream` under persistent
     // launches, whose groups are teams within one launch.
-    int stream_lanes() const { return persistent ? 1 : lanes; }
+    int stream_lanes() const { return persistent ? 1 : std::max(lanes, lanes64); }
     void fork_lanes() {
         if (stream_lanes() > 1) {
             HIP_CHECK(hipEventRecord(ev_fork, stream));
@@ -3764,12 +3849,12 @@ struct Engine {
 
     // The launches of `windows` windows, the first of sequence number seq0
     // (eager, or recorded into a graph when the streams are capturing):
-    // every stream's (only < 0), or only those of lane `only`'s stream (for
-    // the FP64 networks: lane 0's). With `marks`, one window in prof_every
+    // every stream's (only < 0), or only those of lane stream `only` (FP32
+    // lane `only` and FP64 group `only`). With `marks`, one window in prof_every
     // is bracketed by events on `stream` (6 per timed window). Returns the
     // windows timed.
     //
-    // FP64 networks: drive, cells, compaction, delivery (marking + SIC),
+    // FP64 groups: drive, cells, compaction, delivery (marking + SIC),
     // ordered sums (whose lead wavefront closes the window). FP32 lanes:
     // cells, then k_post (delivery, SIC of the live networks, the next
     // window's drive, close); the first window's drive is launched before
@@ -3780,7 +3865,7 @@ struct Engine {
         if (windows <= 0) {
             return 0;
         }
-        const int n64 = (only < 0 || only == 0) ? active_range[1] : 0;
+        auto on64 = [&](int g) { return g < lanes64 && (only < 0 || only == g); };
         auto on = [&](int l) { return fp32 && (only < 0 || only == l); };
         for (int l = 0; l < lanes; ++l) {
             if (on(l)) launch_drive(1 + l);
@@ -3792,27 +3877,30 @@ struct Engine {
                 if (timed_here) HIP_CHECK(hipEventRecord(events[6 * timed + k], stream));
             };
             mark(0);
-            if (n64 > 0) launch_drive(0);
+            for (int g = 0; g < lanes64; ++g) {
+                if (on64(g)) launch_drive64(g);
+            }
             mark(1);
-            if (n64 > 0) launch_cells<ASG_FP64>(0, Wg, wpb, p);
+            for (int g = 0; g < lanes64; ++g) {
+                if (on64(g)) launch_cells64(g, Wg, wpb, p);
+            }
             for (int l = 0; l < lanes; ++l) {
                 if (on(l)) launch_cells<ASG_FP32>(1 + l, Wg, wpb, p);
             }
             mark(2);
-            if (n64 > 0) {
-                hipLaunchKernelGGL(k_compact, dim3(n64, Wg), dim3(64), 0, stream, d_nets,
-                                   d_active);
+            for (int g = 0; g < lanes64; ++g) {
+                if (on64(g)) launch_compact64(g);
             }
             mark(3);
-            if (n64 > 0) launch_deliver64(G, Wg, SG);
+            for (int g = 0; g < lanes64; ++g) {
+                if (on64(g)) launch_deliver64(G, Wg, SG, g);
+            }
             for (int l = 0; l < lanes; ++l) {
                 if (on(l)) launch_post(l, p);
             }
             mark(4);
-            const int sum_items = n64 > 0 ? cell_range[1] * ((Wg + SG - 1) / SG) : 0;
-            if (sum_items > 0) {
-                hipLaunchKernelGGL(k_sum, dim3((sum_items + 3) / 4), dim3(256), 0, stream, d_nets,
-                                   d_blocks, sum_items, Wg, SG);
+            for (int g = 0; g < lanes64; ++g) {
+                if (on64(g)) launch_sum64(g, SG);
             }
             mark(5);
             if (timed_here) ++timed;
@@ -3832,7 +3920,7 @@ struct Engine {
         }
         GraphEntry ge{windows, Wg, G, wpb, SG, parity, {}};
         try {
-            for (int l = 0; l < lanes; ++l) {
+            for (int l = 0; l < stream_lanes(); ++l) {
                 hipGraph_t graph = nullptr;
                 HIP_CHECK(hipStreamBeginCapture(lane_stream[l], hipStreamCaptureModeThreadLocal));
                 launch_windows(windows, G, wpb, SG, false, win_seq, l);
@@ -3899,7 +3987,7 @@ struct Engine {
         } else if (use_graphs && windows > 0) {
             const GraphEntry& g = graph_for(windows, G, sched_cell_waves, SG);
             fork_lanes();
-            for (int l = 0; l < lanes; ++l) {
+            for (int l = 0; l < stream_lanes(); ++l) {
                 HIP_CHECK(hipGraphLaunch(g.exec[l], lane_stream[l]));
             }
             join_lanes();

@@ -162,10 +162,14 @@ def timed_steps(batch, slots, warmup: int, steps: int, keep_spikes: bool):
     return elapsed, spk
 
 
-def accuracy_run(models, precision, profile=False):
-    """0.5 s + 5 s of the whole batch, eager launches, default schedule:
-    timed seconds of the 5 s, spikes, Ca traces, per-window kernel times."""
+def accuracy_run(models, precision, profile=False, lanes=0):
+    """0.5 s + 5 s of the whole batch, eager launches, default schedule
+    (or `lanes` lanes): timed seconds of the 5 s, spikes, Ca traces, and
+    with `profile` per-window kernel times (a profiled run launches its
+    windows in one lane, so its time is not the batch's throughput)."""
     batch, slots = new_batch(models, precision, (ACC_TRANSIENT_BS + ACC_TIMED_BS) * 1000)
+    if lanes:
+        batch.set_schedule(lanes=lanes)
     if profile:
         batch.set_profiling(True, every=PROFILE_EVERY)
     elapsed, spk = timed_steps(batch, slots, ACC_TRANSIENT_BS, ACC_TIMED_BS, True)
@@ -390,8 +394,9 @@ def main():
     ap.add_argument("--no-sparse", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="default schedule, eager launches")
     ap.add_argument("--lanes", type=int, default=0, choices=(0, 1, 2, 3),
-                    help="with --no-tune: FP32 lanes of the schedule (0 = the default, 2); "
-                         "1 puts every FP32 kernel over the whole batch, as the profiled "
+                    help="with --no-tune: lanes of the schedule (0 = the default, 2; the "
+                         "FP32 lanes, and the FP64 groups of --fp64-only's timed run); "
+                         "1 puts every kernel over the whole batch, as the profiled "
                          "run does, for kernel traces")
     ap.add_argument("--teams", type=int, default=0,
                     help="with --no-tune: run each call's FP32 windows in one persistent "
@@ -445,7 +450,8 @@ def main():
         f"in {time.time() - t_build:.1f} s")
 
     if args.fp64_only:
-        e64, _, _, kt64 = accuracy_run(models, capi.ASG_FP64, profile=True)
+        e64, _, _, _ = accuracy_run(models, capi.ASG_FP64, lanes=args.lanes)
+        _, _, _, kt64 = accuracy_run(models, capi.ASG_FP64, profile=True)
         print(json.dumps({"batched_fp64": n_total * ACC_TIMED_BS * STEPS_PER_BENCH_STEP / e64,
                           "kernel_us_per_window_fp64": per_window_us(kt64)}), flush=True)
         return
@@ -501,7 +507,7 @@ def main():
     e64 = kt64 = plain64 = sub = None
     if not args.fp32_only:
         log("accuracy, 0.5 s + 5 s: batched FP64 ...")
-        e64, spk64, tr64, kt64 = accuracy_run(models, capi.ASG_FP64, profile=True)
+        e64, spk64, tr64, _ = accuracy_run(models, capi.ASG_FP64)
         log("  batched FP32 ...")
         a32, spk32, tr32, _ = accuracy_run(models, capi.ASG_FP32)
         acc_ns = n_total * ACC_TIMED_BS * STEPS_PER_BENCH_STEP
@@ -519,7 +525,9 @@ def main():
             saved.append((d.neuron[0].V_m_init, d.neuron[1].V_m_init))
             for k in (0, 1):
                 d.neuron[k].V_m_init = d.neuron[k].V_m_init + 1e-6
-        _, spkp, trp, _ = accuracy_run(models, capi.ASG_FP64)
+        # (its kernel times, in one lane, are the FP64 per-window figures:
+        # the same work as the run above)
+        _, spkp, trp, kt64 = accuracy_run(models, capi.ASG_FP64, profile=True)
         for m, (v0, v1) in zip(models, saved):
             m.desc.contents.neuron[0].V_m_init = v0
             m.desc.contents.neuron[1].V_m_init = v1

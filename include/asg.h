@@ -434,10 +434,11 @@ int64_t asg_batch_bytes_in_use(const asg_batch* batch);
  * and per summing wavefront, 1-16; 0 = the whole window for large batches,
  * 1 otherwise), the wavefronts that share out the delivery list (64-4096,
  * 0 = 512), the FP32 post pass's SIC wavefronts (64-8192; 0 = one per
- * neuron block, the default), the FP32 lanes (1-3; 0 = 2): groups of the
- * FP32 networks of about equal cell count whose windows run on streams of
- * their own, so that one group's cell pass overlaps another's post pass,
- * and the FP32 teams (0 = off; 1-1024): with teams, each step call runs all
+ * neuron block, the default), the lanes (1-3; 0 = 2): groups of the FP32
+ * networks of about equal cell count whose windows run on streams of their
+ * own, so that one group's cell pass overlaps another's post pass, and as
+ * many groups of the FP64 networks, so that one group's cell pass overlaps
+ * another's compaction, delivery and sums; and the FP32 teams (0 = off; 1-1024): with teams, each step call runs all
  * of its windows of the FP32 networks in one persistent cooperative launch
  * (grid sized by the occupancy API), the networks split into that many teams
  * (at most one per network), each with its own workgroups and its own

@@ -16,20 +16,20 @@ SPECS = [("quick", 3, capi.ASG_FP64), ("use_case", 4, capi.ASG_FP32), ("random_p
          ("quick", 6, capi.ASG_FP32)]
 
 
-def _models():
+def _models(all_fp64=False):
     ms = []
     for cfg, seed, prec in SPECS:
         m = A.Model(config_path(cfg), seed=seed)
-        m.desc.contents.precision = prec
+        m.desc.contents.precision = capi.ASG_FP64 if all_fp64 else prec
         ms.append(m)
     return ms
 
 
-def _run(configure, chunks=(700, 1000, 1000, 300), total=3000, drain_all=None):
+def _run(configure, chunks=(700, 1000, 1000, 300), total=3000, drain_all=None, all_fp64=False):
     """Every chunk's spikes are drained (slot by slot, or all slots in one
     call with that capacity) and a state array read between the calls
     (eager work between graph replays)."""
-    models = _models()
+    models = _models(all_fp64)
     b = A.Batch(8)
     slots = [b.add(m, total) for m in models]
     configure(b)
@@ -71,6 +71,36 @@ def test_every_schedule_gives_the_same_bits():
     # the FP32 networks in one lane, and split over two streams
     for lanes in (1, 2):
         _same(ref, _run(lambda b, n=lanes: b.set_schedule(1, 0, 0, 0, 0, n)))
+
+
+def test_fp64_groups_give_the_same_bits():
+    """An FP64-only batch split into one, two or three groups of networks,
+    each on a stream of its own, eager and from graphs, and with the group
+    count changed between calls: the bits of the one-group run."""
+    print("[gpu case] FP64 groups", flush=True)
+    ref = _run(lambda b: b.set_schedule(lanes=1), all_fp64=True)
+    for lanes in (2, 3):
+        _same(ref, _run(lambda b, n=lanes: b.set_schedule(lanes=n), all_fp64=True))
+
+        def with_graphs(b, n=lanes):
+            b.set_schedule(lanes=n)
+            b.set_graphs(True)
+        _same(ref, _run(with_graphs, all_fp64=True))
+    models = _models(True)
+    b = A.Batch(8)
+    slots = [b.add(m, 3000) for m in models]
+    parts = {s: ([], []) for s in slots}
+    for k, c in enumerate((700, 1000, 1000, 300)):
+        b.set_schedule(lanes=(1, 3, 2, 1)[k])
+        b.step(c)
+        for s in slots:
+            st, src = b.spikes(s)
+            parts[s][0].append(st)
+            parts[s][1].append(src)
+    out = [(np.concatenate(parts[s][0]), np.concatenate(parts[s][1]), b.state(s, "V"),
+            b.state(s, "Ca")) for s in slots]
+    b.close()
+    _same(ref, out)
 
 
 def test_persistent_teams_give_the_same_bits():
