@@ -3689,30 +3689,10 @@ struct Engine {
                            d_nets, d_nblocks + nb64[grp], count, Wg, work_n64(grp));
     }
     void launch_cells64(int grp, int32_t Wg_, int wpb, int p) {
-        const int count = c64[grp + 1] - c64[grp];
-        if (count == 0) return;
-        const CellBlock* blocks = d_blocks + c64[grp];
-        const dim3 grid((count + wpb - 1) / wpb);
-        CellBlock* work = d_work + nb64[grp];
-        int32_t* work_n = work_n64(grp);
-        hipStream_t st = lane_stream[grp];
         // post / sic_item are not used by the FP64 cell pass
-        switch (wpb) {
-            case 4:
-                hipLaunchKernelGGL((k_cells<ASG_FP64, 4>), grid, dim3(4 * kCellBlock), 0, st,
-                                   d_nets, blocks, count, Wg_, work, work_n, p, d_post, d_sic_item,
-                                   d_nbr);
-                break;
-            case 2:
-                hipLaunchKernelGGL((k_cells<ASG_FP64, 2>), grid, dim3(2 * kCellBlock), 0, st,
-                                   d_nets, blocks, count, Wg_, work, work_n, p, d_post, d_sic_item,
-                                   d_nbr);
-                break;
-            default:
-                hipLaunchKernelGGL((k_cells<ASG_FP64, 1>), grid, dim3(kCellBlock), 0, st, d_nets,
-                                   blocks, count, Wg_, work, work_n, p, d_post, d_sic_item, d_nbr);
-                break;
-        }
+        launch_cells_k<ASG_FP64>(d_blocks + c64[grp], c64[grp + 1] - c64[grp], Wg_, wpb, p,
+                                 d_work + nb64[grp], work_n64(grp), d_post, d_sic_item,
+                                 lane_stream[grp]);
     }
     void launch_compact64(int grp) {
         const int n = a64[grp + 1] - a64[grp];
@@ -3784,20 +3764,14 @@ struct Engine {
         return static_cast<int64_t>(n_blocks) * Wg > kGroupWaves ? Wg : 1;
     }
 
-    // The cell pass of range `range` (0 the FP64 networks; 1 + l FP32 lane
-    // l, window parity p).
+    // One cell-pass launch: `count` blocks from `blocks` (window parity p),
+    // appending the blocks that spiked to work / *work_n, on stream st.
     template <int P>
-    void launch_cells(int range, int32_t Wg_, int wpb, int p) {
-        const int count = cell_range[range + 1] - cell_range[range];
+    void launch_cells_k(const CellBlock* blocks, int count, int32_t Wg_, int wpb, int p,
+                        CellBlock* work, int32_t* work_n, PostState* post, int32_t* sic_item,
+                        hipStream_t st) {
         if (count == 0) return;
-        const CellBlock* blocks = d_blocks + cell_range[range];
         const dim3 grid((count + wpb - 1) / wpb);
-        const int l = P == ASG_FP32 ? range - 1 : 0;
-        CellBlock* work = P == ASG_FP32 ? work32(l, p) : d_work;
-        int32_t* work_n = P == ASG_FP32 ? work_n32(l, p) : d_work_n;
-        PostState* post = post32(l, p);
-        int32_t* sic_item = sic_items32(l, p);
-        hipStream_t st = range_stream(range);
         switch (wpb) {
             case 4:
                 hipLaunchKernelGGL((k_cells<P, 4>), grid, dim3(4 * kCellBlock), 0, st, d_nets,
@@ -3812,6 +3786,13 @@ struct Engine {
                                    blocks, count, Wg_, work, work_n, p, post, sic_item, d_nbr);
                 break;
         }
+    }
+
+    // The cell pass of FP32 lane l (window parity p).
+    void launch_cells32(int l, int32_t Wg_, int wpb, int p) {
+        launch_cells_k<ASG_FP32>(d_blocks + cell_range[1 + l], cell_range[2 + l] - cell_range[1 + l],
+                                 Wg_, wpb, p, work32(l, p), work_n32(l, p), post32(l, p),
+                                 sic_items32(l, p), lane_stream[l]);
     }
 
     // The SIC group the schedule asks for, or (0) the largest power of two
@@ -3885,7 +3866,7 @@ struct Engine {
                 if (on64(g)) launch_cells64(g, Wg, wpb, p);
             }
             for (int l = 0; l < lanes; ++l) {
-                if (on(l)) launch_cells<ASG_FP32>(1 + l, Wg, wpb, p);
+                if (on(l)) launch_cells32(l, Wg, wpb, p);
             }
             mark(2);
             for (int g = 0; g < lanes64; ++g) {
