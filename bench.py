@@ -211,7 +211,7 @@ def run_plain(models, total_steps: int):
     return ns, elapsed
 
 
-def sparse_fp64(transient_bench_steps: int, timed_bench_steps: int):
+def sparse_fp64(transient_bench_steps: int, timed_bench_steps: int, lanes: int = 0):
     """The Sparse network, FP64, seeds 1-3 resident together in one batch
     (the FP64 batch is bit-identical to each network run alone), windows
     replayed from captured graphs; the drop-in path (each seed its own
@@ -223,6 +223,8 @@ def sparse_fp64(transient_bench_steps: int, timed_bench_steps: int):
     total = (transient_bench_steps + timed_bench_steps) * STEPS_PER_BENCH_STEP
     batch = A.Batch(len(models))
     slots = [batch.add(m, total) for m in models]
+    if lanes:
+        batch.set_schedule(lanes=lanes)
     batch.set_graphs(True)
     spikes = {s: [] for s in SPARSE_SEEDS}
 
@@ -395,7 +397,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="default schedule, eager launches")
     ap.add_argument("--lanes", type=int, default=0, choices=(0, 1, 2, 3),
                     help="with --no-tune: lanes of the schedule (0 = the default, 2; the "
-                         "FP32 lanes, and the FP64 groups of --fp64-only's timed run); "
+                         "FP32 lanes, and the FP64 groups of --fp64-only's timed run and of "
+                         "--sparse-only's batch); "
                          "1 puts every kernel over the whole batch, as the profiled "
                          "run does, for kernel traces")
     ap.add_argument("--teams", type=int, default=0,
@@ -439,7 +442,8 @@ def main():
     from astrosimgpu_amd import sweep
 
     if args.sparse_only:
-        print(json.dumps({"sparse_fp64": sparse_fp64(10, 10 * args.sparse_seconds)}), flush=True)
+        print(json.dumps({"sparse_fp64": sparse_fp64(10, 10 * args.sparse_seconds, args.lanes)}),
+              flush=True)
         return
 
     t_build = time.time()
