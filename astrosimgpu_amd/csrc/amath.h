@@ -73,7 +73,22 @@ ASG_HD double exp_specialcase(double tmp, std::uint64_t sbits, std::uint64_t ki)
     return 0x1p-1022 * y;
 }
 
-ASG_HD double asg_exp(double x) {
+// exp's 2 KiB table as read by asg_exp: the table itself (indexed in place,
+// on the host and the device), or a copy at a pointer (for instance in a
+// workgroup's LDS, where lookups on a dependency chain wait less than on the
+// global copy). Host code must index the table in place: the table is a
+// device variable, and a pointer to it taken in host code points at its
+// host-side stand-in, which holds zeros.
+struct ExpTabDirect {
+    ASG_HD std::uint64_t operator()(std::uint64_t i) const { return ASG_EXP_TAB[i]; }
+};
+struct ExpTabAt {
+    const std::uint64_t* p;
+    ASG_HD std::uint64_t operator()(std::uint64_t i) const { return p[i]; }
+};
+
+template <class Tab>
+ASG_HD double asg_exp_with(double x, Tab tab) {
     constexpr double kInvLn2N = 0x1.71547652b82fep0 * 128.0;
     constexpr double kNegLn2hiN = -0x1.62e42fefa0000p-8;
     constexpr double kNegLn2loN = -0x1.cf79abc9e3b3ap-47;
@@ -108,8 +123,8 @@ ASG_HD double asg_exp(double x) {
     const double r = fma_(kd, kNegLn2loN, fma_(kd, kNegLn2hiN, x));
     const std::uint64_t idx = 2 * (ki % 128);
     const std::uint64_t top = ki << 45;
-    const double tail = as_f64(ASG_EXP_TAB[idx]);
-    const std::uint64_t sbits = ASG_EXP_TAB[idx + 1] + top;
+    const double tail = as_f64(tab(idx));
+    const std::uint64_t sbits = tab(idx + 1) + top;
     const double r2 = r * r;
     const double tmp = fma_(r2 * r2, fma_(r, C5, C4), fma_(r2, fma_(r, C3, C2), tail + r));
     if (abstop == 0) {
@@ -118,6 +133,8 @@ ASG_HD double asg_exp(double x) {
     const double scale = as_f64(sbits);
     return fma_(scale, tmp, scale);
 }
+
+ASG_HD double asg_exp(double x) { return asg_exp_with(x, ExpTabDirect{}); }
 
 // ---------------------------------------------------------------- log ----
 

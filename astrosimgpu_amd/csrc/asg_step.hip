@@ -386,7 +386,8 @@ __device__ inline void add_i64(int64_t* p, int64_t v) {
 
 // --------------------------------------------------------------- kernels ---
 
-__device__ __forceinline__ uint64_t neuron_window(const DevNet& n, uint32_t i, int64_t T0, int W) {
+__device__ __forceinline__ uint64_t neuron_window(const DevNet& n, uint32_t i, int64_t T0, int W,
+                                                  const uint64_t* exp_tab) {
     // Everything the loop reads is copied into registers once; the rings are
     // reached through restrict pointers so their stores cannot force reloads.
     const uint32_t N = n.n_neur;
@@ -510,7 +511,7 @@ __device__ __forceinline__ uint64_t neuron_window(const DevNet& n, uint32_t i, i
                 dgin += inh_in * p.psc_in;
             }
             spiked = adex_step(p, Vr, b, I_ext, hs, half_h, h_sixth, substeps, V, w, gex, dgex,
-                               gin, dgin, refr);
+                               gin, dgin, refr, exp_tab);
         }
         const uint64_t mask = __ballot(spiked);
         if (lane == 0 && (i & ~63u) < N) {
@@ -1273,7 +1274,7 @@ template <int P>
 __device__ __forceinline__ void cells_item(DevNet* nets, const CellBlock& cb, int32_t Wg,
                                            CellBlock* work, int32_t* work_n, int32_t par,
                                            PostState* post, int32_t* sic_item,
-                                           const int32_t* nbr) {
+                                           const int32_t* nbr, const uint64_t* exp_tab) {
     const uint32_t lane = threadIdx.x & 63;
     DevNet& n = nets[cb.net];
     const int W = window_len(n, Wg);
@@ -1338,7 +1339,7 @@ __device__ __forceinline__ void cells_item(DevNet* nets, const CellBlock& cb, in
         }
     } else {
         if (cb.kind == 0) {
-            if (neuron_window(n, i, T0, W) != 0 && lane == 0) {
+            if (neuron_window(n, i, T0, W, exp_tab) != 0 && lane == 0) {
                 work[atomicAdd(work_n, 1)] = cb;
             }
         } else {
@@ -1374,10 +1375,24 @@ __attribute__((amdgpu_waves_per_eu(P == ASG_FP32 ? 5 : 1))) void k_cells(DevNet*
     // registers.
     const int32_t b = __builtin_amdgcn_readfirstlane(static_cast<int32_t>(blockIdx.x) * WPB +
                                                      static_cast<int32_t>(threadIdx.x >> 6));
-    if (b >= count) {
-        return;
+    if constexpr (P == ASG_FP64) {
+        // exp's table in LDS: its lookups sit on the RK4 stages' dependency
+        // chain, and an LDS read returns sooner than the global table's
+        __shared__ uint64_t exp_tab[256];
+        for (uint32_t j = threadIdx.x; j < 256; j += blockDim.x) {
+            exp_tab[j] = ASG_EXP_TAB[j];
+        }
+        __syncthreads();
+        if (b >= count) {
+            return;
+        }
+        cells_item<P>(nets, blocks[b], Wg, work, work_n, par, post, sic_item, nbr, exp_tab);
+    } else {
+        if (b >= count) {
+            return;
+        }
+        cells_item<P>(nets, blocks[b], Wg, work, work_n, par, post, sic_item, nbr, nullptr);
     }
-    cells_item<P>(nets, blocks[b], Wg, work, work_n, par, post, sic_item, nbr);
 }
 
 
@@ -2114,7 +2129,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 4))) void
         const int p = (par0 + w) & 1;
         for (int b = r; b < tm.cells; b += T) {
             cells_item<ASG_FP32>(nets, blocks[tm.cell_first + b], Wg, work_t + p * tm.nblocks,
-                                 work_n_t + p, p, post_t + p, sic_t + p * tm.nblocks, nbr);
+                                 work_n_t + p, p, post_t + p, sic_t + p * tm.nblocks, nbr, nullptr);
         }
         target += static_cast<uint32_t>(T);
         if (!team_barrier(counter, target, failed, host_failed, wait_ticks)) {

@@ -130,11 +130,13 @@ struct NeuronK {
 };
 
 // NeuronPopulation::derivatives, neuron.cpp:85-99, with its three divisions
-// by constants as div_rn (the same roundings).
+// by constants as div_rn (the same roundings); exp's table from exp_tab (a
+// copy of ASG_EXP_TAB).
 ASG_HD void adex_rhs(const NeuronK& p, double V, double w, double g_ex, double g_in, double I_ext,
-                     double& dV, double& dw) {
+                     const std::uint64_t* exp_tab, double& dV, double& dw) {
     const double Vc = (p.V_peak < V) ? p.V_peak : V;
-    const double I_spike = p.gL_DeltaT * asg_exp(div_rn(Vc - p.V_th, p.Delta_T, p.inv_Delta_T));
+    const double I_spike =
+        p.gL_DeltaT * asg_exp_with(div_rn(Vc - p.V_th, p.Delta_T, p.inv_Delta_T), ExpTabAt{exp_tab});
     const double I_syn_ex = g_ex * (Vc - p.E_ex);
     const double I_syn_in = g_in * (Vc - p.E_in);
     dV = div_rn(p.neg_g_L * (Vc - p.E_L) + I_spike - I_syn_ex - I_syn_in - w + p.I_e + I_ext,
@@ -142,12 +144,14 @@ ASG_HD void adex_rhs(const NeuronK& p, double V, double w, double g_ex, double g
     dw = div_rn(p.a * (Vc - p.E_L) - w, p.tau_w, p.inv_tau_w);
 }
 
-// The substep loop of NeuronPopulation::update, neuron.cpp:171-229. Returns
+// The substep loop of NeuronPopulation::update, neuron.cpp:171-229 (exp's
+// table from exp_tab, a copy of ASG_EXP_TAB). Returns
 // true if the cell spiked in this step (at most once, as the reference's
 // `spiked` flag).
 ASG_HD bool adex_step(const NeuronK& p, double V_reset, double b, double I_ext, double h_step,
                       double half_h, double h_sixth, int substeps, double& V, double& w,
-                      double& g_ex, double& dg_ex, double& g_in, double& dg_in, int& refr) {
+                      double& g_ex, double& dg_ex, double& g_in, double& dg_in, int& refr,
+                      const std::uint64_t* exp_tab) {
     bool spiked = false;
     for (int s = 0; s < substeps; ++s) {
         const double g_ex_next = p.ex_decay * (g_ex + h_step * dg_ex);
@@ -157,16 +161,19 @@ ASG_HD bool adex_step(const NeuronK& p, double V_reset, double b, double I_ext, 
         if (refr > 0) {
             V = V_reset;
             double dV, dw;
-            adex_rhs(p, V, w, g_ex, g_in, I_ext, dV, dw);
+            adex_rhs(p, V, w, g_ex, g_in, I_ext, exp_tab, dV, dw);
             w += h_step * dw;
         } else {
             const double g_ex_mid = 0.5 * (g_ex + g_ex_next);
             const double g_in_mid = 0.5 * (g_in + g_in_next);
             double k1V, k1w, k2V, k2w, k3V, k3w, k4V, k4w;
-            adex_rhs(p, V, w, g_ex, g_in, I_ext, k1V, k1w);
-            adex_rhs(p, V + half_h * k1V, w + half_h * k1w, g_ex_mid, g_in_mid, I_ext, k2V, k2w);
-            adex_rhs(p, V + half_h * k2V, w + half_h * k2w, g_ex_mid, g_in_mid, I_ext, k3V, k3w);
-            adex_rhs(p, V + h_step * k3V, w + h_step * k3w, g_ex_next, g_in_next, I_ext, k4V, k4w);
+            adex_rhs(p, V, w, g_ex, g_in, I_ext, exp_tab, k1V, k1w);
+            adex_rhs(p, V + half_h * k1V, w + half_h * k1w, g_ex_mid, g_in_mid, I_ext, exp_tab,
+                     k2V, k2w);
+            adex_rhs(p, V + half_h * k2V, w + half_h * k2w, g_ex_mid, g_in_mid, I_ext, exp_tab,
+                     k3V, k3w);
+            adex_rhs(p, V + h_step * k3V, w + h_step * k3w, g_ex_next, g_in_next, I_ext, exp_tab,
+                     k4V, k4w);
             V += h_sixth * (k1V + 2.0 * k2V + 2.0 * k3V + k4V);
             w += h_sixth * (k1w + 2.0 * k2w + 2.0 * k3w + k4w);
         }
