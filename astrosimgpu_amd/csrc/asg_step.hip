@@ -3172,8 +3172,12 @@ struct Engine {
 
     explicit Engine(int max_n, int64_t budget_bytes) : max_nets(max_n), budget(budget_bytes) {
         HIP_CHECK(hipGetDevice(&device));
+        // The other streams (uploads, lanes) are created when first needed:
+        // streams share the device's few hardware queues in the order they
+        // are created, so an engine that creates only the streams it uses
+        // (one, for a single network) leaves the other queues to other
+        // engines, and several single-network engines run side by side.
         HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-        HIP_CHECK(hipStreamCreateWithFlags(&up_stream, hipStreamNonBlocking));
         slots.resize(static_cast<size_t>(max_n));
         for (NetSlot& s : slots) s.pool = &pool;
         const size_t m = static_cast<size_t>(max_n);
@@ -3205,9 +3209,6 @@ struct Engine {
             team_wait_ticks = std::strtoull(us, nullptr, 10) * 100;  // 100 MHz counter
         }
         lane_stream[0] = stream;
-        for (int l = 1; l < kMaxLanes; ++l) {
-            HIP_CHECK(hipStreamCreateWithFlags(&lane_stream[l], hipStreamNonBlocking));
-        }
         HIP_CHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
         for (int l = 0; l < kMaxLanes; ++l) {
             HIP_CHECK(hipEventCreateWithFlags(&ev_join[l], hipEventDisableTiming));
@@ -3640,13 +3641,19 @@ struct Engine {
         // remove), so its device descriptor and buffers are not read until
         // the next table rebuild, by which time the uploads have finished.
         NetSlot& s = slots[free_slot];
+        // an idle engine uploads on its own stream; one with a step call in
+        // flight on a stream of its own, so the upload overlaps the call
+        if (pending && up_stream == nullptr) {
+            HIP_CHECK(hipStreamCreateWithFlags(&up_stream, hipStreamNonBlocking));
+        }
+        hipStream_t up = pending ? up_stream : stream;
         try {
-            build_slot(s, d, total, up_stream, prep);
+            build_slot(s, d, total, up, prep);
             HIP_CHECK(hipMemcpyAsync(d_nets + free_slot, &s.dn, sizeof(DevNet),
-                                     hipMemcpyHostToDevice, up_stream));
-            HIP_CHECK(hipStreamSynchronize(up_stream));
+                                     hipMemcpyHostToDevice, up));
+            HIP_CHECK(hipStreamSynchronize(up));
         } catch (...) {
-            (void)hipStreamSynchronize(up_stream);
+            (void)hipStreamSynchronize(up);
             release(s);
             throw;
         }
@@ -3828,6 +3835,14 @@ struct Engine {
     // Streams of the FP32 lanes: none beyond `stream` under persistent
     // launches, whose groups are teams within one launch.
     int stream_lanes() const { return persistent ? 1 : std::max(lanes, lanes64); }
+    // Creates the lane streams the current tables use that do not exist yet.
+    void ensure_lane_streams() {
+        for (int l = 1; l < stream_lanes(); ++l) {
+            if (lane_stream[l] == nullptr) {
+                HIP_CHECK(hipStreamCreateWithFlags(&lane_stream[l], hipStreamNonBlocking));
+            }
+        }
+    }
     void fork_lanes() {
         if (stream_lanes() > 1) {
             HIP_CHECK(hipEventRecord(ev_fork, stream));
@@ -3950,6 +3965,7 @@ struct Engine {
         if (n <= 0) return;
         if (dirty) rebuild_tables();
         if (n_active == 0) return;
+        ensure_lane_streams();
         int64_t windows = 0;
         // The run target is the only descriptor field the host changes
         // between calls: all of them go down in one copy and one launch.

@@ -247,24 +247,32 @@ def sparse_fp64(transient_bench_steps: int, timed_bench_steps: int, lanes: int =
     for s in SPARSE_SEEDS:
         st = np.concatenate(spikes[s])
         rates[str(s)] = round(int((st >= first).sum()) * 1000.0 / (steps * 0.1 * 10000), 4)
-    # the drop-in path: one asg_net per seed, each on its own stream
-    nets = [A.DeviceNetwork(m) for m in models]
-
-    def nets_step():
+    # the drop-in path: one asg_net per seed, each on its own stream, with
+    # eager launches and then with its window sequences replayed from graphs
+    def drop_in(graphs):
+        nets = [A.DeviceNetwork(m) for m in models]
         for net in nets:
-            net.step(STEPS_PER_BENCH_STEP, wait=False)
-        for net in nets:
-            net.sync()
-            net.spikes()
+            net.set_graphs(graphs)
 
-    for _ in range(min(transient_bench_steps, 5)):
-        nets_step()
-    t1 = time.perf_counter()
-    for _ in range(timed_bench_steps):
-        nets_step()
-    elapsed_nets = time.perf_counter() - t1
-    for net in nets:
-        net.close()
+        def nets_step():
+            for net in nets:
+                net.step(STEPS_PER_BENCH_STEP, wait=False)
+            for net in nets:
+                net.sync()
+                net.spikes()
+
+        for _ in range(min(transient_bench_steps, 5)):
+            nets_step()
+        t1 = time.perf_counter()
+        for _ in range(timed_bench_steps):
+            nets_step()
+        elapsed = time.perf_counter() - t1
+        for net in nets:
+            net.close()
+        return elapsed
+
+    elapsed_nets = drop_in(False)
+    elapsed_nets_graphs = drop_in(True)
     # seed 1 alone, profiled: its cell kernel's bytes per window / time
     m = models[0]
     net = A.DeviceNetwork(m)
@@ -279,6 +287,7 @@ def sparse_fp64(transient_bench_steps: int, timed_bench_steps: int, lanes: int =
     out = {"value": round(3 * 10000 * steps / elapsed, 1), "unit": "neuron-steps/s",
            "launch": "the three seeds in one FP64 batch, graph replay",
            "drop_in_asg_net_value": round(3 * 10000 * steps / elapsed_nets, 1),
+           "drop_in_asg_net_graphs_value": round(3 * 10000 * steps / elapsed_nets_graphs, 1),
            "seconds_timed": steps / 10000.0, "mean_rate_hz": rates,
            "kernel_us_per_window_seed1": per_window_us(kt),
            "roofline": {"kernel": "k_cells<FP64> (one Sparse network, one window of 10 steps)",
