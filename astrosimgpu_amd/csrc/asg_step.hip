@@ -3172,12 +3172,15 @@ struct Engine {
 
     explicit Engine(int max_n, int64_t budget_bytes) : max_nets(max_n), budget(budget_bytes) {
         HIP_CHECK(hipGetDevice(&device));
-        // The other streams (uploads, lanes) are created when first needed:
-        // streams share the device's few hardware queues in the order they
-        // are created, so an engine that creates only the streams it uses
-        // (one, for a single network) leaves the other queues to other
-        // engines, and several single-network engines run side by side.
+        // Streams share the device's few hardware queues in the order they
+        // are created. A batch creates its upload and lane streams here; a
+        // single-network engine creates them only when first needed (it
+        // normally uses its main stream alone), so that it leaves the other
+        // queues to other engines and several of them run side by side.
         HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        if (max_n > 1) {
+            HIP_CHECK(hipStreamCreateWithFlags(&up_stream, hipStreamNonBlocking));
+        }
         slots.resize(static_cast<size_t>(max_n));
         for (NetSlot& s : slots) s.pool = &pool;
         const size_t m = static_cast<size_t>(max_n);
@@ -3209,6 +3212,9 @@ struct Engine {
             team_wait_ticks = std::strtoull(us, nullptr, 10) * 100;  // 100 MHz counter
         }
         lane_stream[0] = stream;
+        if (max_n > 1) {
+            ensure_lane_streams(kMaxLanes);
+        }
         HIP_CHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
         for (int l = 0; l < kMaxLanes; ++l) {
             HIP_CHECK(hipEventCreateWithFlags(&ev_join[l], hipEventDisableTiming));
@@ -3641,12 +3647,13 @@ struct Engine {
         // remove), so its device descriptor and buffers are not read until
         // the next table rebuild, by which time the uploads have finished.
         NetSlot& s = slots[free_slot];
-        // an idle engine uploads on its own stream; one with a step call in
-        // flight on a stream of its own, so the upload overlaps the call
+        // uploads go on the upload stream, so that they overlap a step call
+        // in flight; a single-network engine without one uploads on its
+        // main stream while idle and creates one otherwise
         if (pending && up_stream == nullptr) {
             HIP_CHECK(hipStreamCreateWithFlags(&up_stream, hipStreamNonBlocking));
         }
-        hipStream_t up = pending ? up_stream : stream;
+        hipStream_t up = up_stream != nullptr ? up_stream : stream;
         try {
             build_slot(s, d, total, up, prep);
             HIP_CHECK(hipMemcpyAsync(d_nets + free_slot, &s.dn, sizeof(DevNet),
@@ -3835,9 +3842,9 @@ struct Engine {
     // Streams of the FP32 lanes: none beyond `stream` under persistent
     // launches, whose groups are teams within one launch.
     int stream_lanes() const { return persistent ? 1 : std::max(lanes, lanes64); }
-    // Creates the lane streams the current tables use that do not exist yet.
-    void ensure_lane_streams() {
-        for (int l = 1; l < stream_lanes(); ++l) {
+    // Creates the lane streams below `count` that do not exist yet.
+    void ensure_lane_streams(int count) {
+        for (int l = 1; l < count; ++l) {
             if (lane_stream[l] == nullptr) {
                 HIP_CHECK(hipStreamCreateWithFlags(&lane_stream[l], hipStreamNonBlocking));
             }
@@ -3965,7 +3972,7 @@ struct Engine {
         if (n <= 0) return;
         if (dirty) rebuild_tables();
         if (n_active == 0) return;
-        ensure_lane_streams();
+        ensure_lane_streams(stream_lanes());
         int64_t windows = 0;
         // The run target is the only descriptor field the host changes
         // between calls: all of them go down in one copy and one launch.
